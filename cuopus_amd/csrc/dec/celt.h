@@ -1,0 +1,1530 @@
+// celt.h -- CELT fixed-point decoder (mono, 48 kHz, 960-sample mode).
+//
+// Restates libopus 1.5.2's celt_decode_with_ec (vendor/celt/celt_decoder.c:
+// 970-1369) and its closure: coarse/fine energy (quant_bands.c:428-543,
+// laplace.c:88-135), tf/spread/dynalloc/trim (celt_decoder.c:460-497,
+// 1206-1250), bit allocation (rate.c:245-646), PVQ band decoding
+// (bands.c:700-1672 with quant_partition made iterative; vq.c; cwrs.c:463-541),
+// anti-collapse (bands.c:268-370), synthesis (celt_decoder.c:382-458,
+// bands.c:196-267, mdct.c:242-342, kiss_fft.c:48-567), comb post-filter
+// (celt.c:98-256), deemphasis (celt_decoder.c:279-377) and loss concealment
+// (celt_decoder.c:515-969).
+//
+// Layout: the persistent per-chain state is CeltState; per-frame temporaries
+// live in CeltScratch, a plain struct the caller places in HBM (or LDS) so no
+// large private arrays are spilled to scratch memory.
+#pragma once
+#include "entdec.h"
+#include "tables_gen.h"
+
+namespace cuo {
+
+constexpr int kDecBuf = 2048;
+constexpr int kOverlap = 120;
+constexpr int kLpcOrder = 24;
+constexpr int kNbBands = 21;
+constexpr int kMaxPeriod = 1024;
+constexpr int kCombMinPeriod = 15;
+
+enum { SPREAD_NONE = 0, SPREAD_LIGHT = 1, SPREAD_NORMAL = 2, SPREAD_AGGRESSIVE = 3 };
+
+struct CeltState {
+    int32_t start, end;
+    // ---- cleared by reset (OPUS_RESET_STATE) from here on
+    uint32_t rng;
+    int32_t error;
+    int32_t last_pitch_index;
+    int32_t loss_duration;
+    int32_t skip_plc;
+    int32_t pf_period, pf_period_old;
+    int32_t pf_gain, pf_gain_old;   // opus_val16 values
+    int32_t pf_tapset, pf_tapset_old;
+    int32_t prefilter_and_fold;
+    int32_t preemph_mem;
+    int32_t mem[kDecBuf + kOverlap];
+    int16_t lpc[kLpcOrder];
+    int16_t oldBandE[2 * kNbBands];
+    int16_t oldLogE[2 * kNbBands];
+    int16_t oldLogE2[2 * kNbBands];
+    int16_t backgroundLogE[2 * kNbBands];
+};
+
+struct CeltScratch {
+    int16_t X[960];
+    int32_t freq[960];
+    int16_t norm[8 * 78];
+    int16_t tmp[176];
+    int32_t iy[176];
+    int32_t pulses[kNbBands], fine_quant[kNbBands], fine_priority[kNbBands];
+    int32_t cap[kNbBands], offsets[kNbBands], tf_res[kNbBands];
+    int32_t bits1[kNbBands], bits2[kNbBands], thresh[kNbBands], trim_offset[kNbBands];
+    uint8_t collapse[kNbBands];
+    // loss concealment
+    int16_t exc[kMaxPeriod + kLpcOrder];
+    int16_t fir_tmp[kMaxPeriod];
+    int16_t lp_pitch[kDecBuf >> 1];
+    int32_t xcorr[kMaxPeriod];
+    int32_t etmp[kOverlap];
+};
+
+// ------------------------------------------------------------------ reset
+CUO_FN void celt_reset(CeltState *st)
+{
+    st->rng = 0; st->error = 0; st->last_pitch_index = 0; st->loss_duration = 0;
+    st->pf_period = st->pf_period_old = 0; st->pf_gain = st->pf_gain_old = 0;
+    st->pf_tapset = st->pf_tapset_old = 0; st->prefilter_and_fold = 0; st->preemph_mem = 0;
+    for (int i = 0; i < kDecBuf + kOverlap; i++) st->mem[i] = 0;
+    for (int i = 0; i < kLpcOrder; i++) st->lpc[i] = 0;
+    for (int i = 0; i < 2 * kNbBands; i++) {
+        st->oldBandE[i] = 0;
+        st->oldLogE[i] = st->oldLogE2[i] = -(28 << kDbShift);
+        st->backgroundLogE[i] = 0;
+    }
+    st->skip_plc = 1;
+}
+
+CUO_FN void celt_init(CeltState *st)
+{
+    st->start = 0;
+    st->end = kNbBands;
+    celt_reset(st);
+}
+
+// ------------------------------------------------------------------ small helpers
+CUO_FN int32_t inner_prod16(const int16_t *x, const int16_t *y, int n)
+{
+    int32_t s = 0;
+    for (int i = 0; i < n; i++) s = mac16_16(s, x[i], y[i]);
+    return s;
+}
+
+CUO_FN void renormalise_vector(int16_t *X, int N, int16_t gain)
+{
+    int32_t E = 1 + inner_prod16(X, X, N);
+    int k = celt_ilog2(E) >> 1;
+    int32_t t = vshr32(E, 2 * (k - 7));
+    int16_t g = (int16_t)mul16_16_p15(celt_rsqrt_norm(t), gain);
+    for (int i = 0; i < N; i++) X[i] = (int16_t)pshr32(mul16_16(g, X[i]), k + 1);
+}
+
+CUO_FN int16_t bitexact_cos(int16_t x)
+{
+    int32_t tmp = (4096 + (int32_t)x * x) >> 13;
+    int16_t x2 = (int16_t)tmp;
+    x2 = (int16_t)((32767 - x2) + frac_mul16(x2, (-7651 + frac_mul16(x2, (8277 + frac_mul16(-626, x2))))));
+    return (int16_t)(1 + x2);
+}
+
+CUO_FN int bitexact_log2tan(int isin, int icos)
+{
+    int lc = ilog((uint32_t)icos), ls = ilog((uint32_t)isin);
+    icos <<= 15 - lc;
+    isin <<= 15 - ls;
+    return (ls - lc) * (1 << 11) + frac_mul16(isin, frac_mul16(isin, -2597) + 7932)
+           - frac_mul16(icos, frac_mul16(icos, -2597) + 7932);
+}
+
+// ------------------------------------------------------------------ rate helpers (rate.h)
+CUO_FN const uint8_t *pulse_cache(int band, int LMp1)
+{
+    return tab::kCacheBits + tab::kCacheIndex[LMp1 * kNbBands + band];
+}
+CUO_FN int get_pulses(int i) { return i < 8 ? i : (8 + (i & 7)) << ((i >> 3) - 1); }
+CUO_FN int bits2pulses(int band, int LM, int bits)
+{
+    const uint8_t *cache = pulse_cache(band, LM + 1);
+    int lo = 0, hi = cache[0];
+    bits--;
+    for (int i = 0; i < 6; i++) {
+        int mid = (lo + hi + 1) >> 1;
+        if ((int)cache[mid] >= bits) hi = mid; else lo = mid;
+    }
+    if (bits - (lo == 0 ? -1 : (int)cache[lo]) <= (int)cache[hi] - bits) return lo;
+    return hi;
+}
+CUO_FN int pulses2bits(int band, int LM, int pulses)
+{
+    const uint8_t *cache = pulse_cache(band, LM + 1);
+    return pulses == 0 ? 0 : cache[pulses] + 1;
+}
+
+// ------------------------------------------------------------------ coarse/fine energy
+CUO_FN int laplace_decode(RangeDec *d, uint32_t fs, int decay)
+{
+    int val = 0;
+    uint32_t fl = 0;
+    uint32_t fm = ec_decode_bin(d, 15);
+    if (fm >= fs) {
+        val++;
+        fl = fs;
+        fs = ((32768 - 2 * 16 - fs) * (uint32_t)(int32_t)(16384 - decay) >> 15) + 1;
+        while (fs > 1 && fm >= fl + 2 * fs) {
+            fs *= 2;
+            fl += fs;
+            fs = ((fs - 2) * (uint32_t)(int32_t)decay) >> 15;
+            fs += 1;
+            val++;
+        }
+        if (fs <= 1) {
+            int di = (int)((fm - fl) >> 1);
+            val += di;
+            fl += 2 * (uint32_t)di;
+        }
+        if (fm < fl + fs) val = -val; else fl += fs;
+    }
+    uint32_t fh = fl + fs < 32768 ? fl + fs : 32768;
+    ec_update(d, fl, fh, 32768);
+    return val;
+}
+
+CUO_FN void unquant_coarse_energy(int start, int end, int16_t *oldE, int intra, RangeDec *d, int LM)
+{
+    const uint8_t *prob = tab::kEProbModel + (LM * 2 + intra) * 42;
+    int32_t prev = 0;
+    int16_t coef, beta;
+    if (intra) { coef = 0; beta = tab::kBetaIntra; }
+    else { beta = tab::kBetaCoef[LM]; coef = tab::kPredCoef[LM]; }
+    int32_t budget = (int32_t)d->storage * 8;
+    for (int i = start; i < end; i++) {
+        int qi;
+        int32_t tell = ec_tell(d);
+        if (budget - tell >= 15) {
+            int pi = 2 * imin(i, 20);
+            qi = laplace_decode(d, (uint32_t)prob[pi] << 7, prob[pi + 1] << 6);
+        } else if (budget - tell >= 2) {
+            qi = ec_icdf(d, tab::kSmallEnergyIcdf, 2);
+            qi = (qi >> 1) ^ -(qi & 1);
+        } else if (budget - tell >= 1) {
+            qi = -ec_bit_logp(d, 1);
+        } else {
+            qi = -1;
+        }
+        int32_t q = shl32(qi, kDbShift);
+        oldE[i] = max16(-(9 << kDbShift), oldE[i]);
+        int32_t tmp = pshr32(mul16_16(coef, oldE[i]), 8) + prev + shl32(q, 7);
+        tmp = max32(-(28 << (kDbShift + 7)), tmp);
+        oldE[i] = (int16_t)pshr32(tmp, 7);
+        prev = prev + shl32(q, 7) - mul16_16(beta, pshr32(q, 8));
+    }
+}
+
+CUO_FN void unquant_fine_energy(int start, int end, int16_t *oldE, const int32_t *fine_quant, RangeDec *d)
+{
+    for (int i = start; i < end; i++) {
+        if (fine_quant[i] <= 0) continue;
+        int q2 = (int)ec_bits(d, fine_quant[i]);
+        int16_t offset = (int16_t)(((shl32(q2, kDbShift) + (1 << (kDbShift - 1))) >> fine_quant[i]) - (1 << (kDbShift - 1)));
+        oldE[i] = (int16_t)(oldE[i] + offset);
+    }
+}
+
+CUO_FN void unquant_energy_finalise(int start, int end, int16_t *oldE, const int32_t *fine_quant,
+                                    const int32_t *fine_priority, int bits_left, RangeDec *d)
+{
+    for (int prio = 0; prio < 2; prio++) {
+        for (int i = start; i < end && bits_left >= 1; i++) {
+            if (fine_quant[i] >= 8 || fine_priority[i] != prio) continue;
+            int q2 = (int)ec_bits(d, 1);
+            int16_t offset = (int16_t)((int16_t)(shl16(q2, kDbShift) - (1 << (kDbShift - 1))) >> (fine_quant[i] + 1));
+            oldE[i] = (int16_t)(oldE[i] + offset);
+            bits_left--;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ tf / allocation
+CUO_FN void tf_decode(int start, int end, int isTransient, int32_t *tf_res, int LM, RangeDec *d)
+{
+    uint32_t budget = d->storage * 8;
+    uint32_t tell = (uint32_t)ec_tell(d);
+    int logp = isTransient ? 2 : 4;
+    int tf_select_rsv = LM > 0 && tell + logp + 1 <= budget;
+    budget -= tf_select_rsv;
+    int tf_changed = 0, curr = 0;
+    for (int i = start; i < end; i++) {
+        if (tell + logp <= budget) {
+            curr ^= ec_bit_logp(d, logp);
+            tell = (uint32_t)ec_tell(d);
+            tf_changed |= curr;
+        }
+        tf_res[i] = curr;
+        logp = isTransient ? 4 : 5;
+    }
+    int tf_select = 0;
+    const int8_t *tsel = tab::kTfSelect + 8 * LM;
+    if (tf_select_rsv && tsel[4 * isTransient + 0 + tf_changed] != tsel[4 * isTransient + 2 + tf_changed])
+        tf_select = ec_bit_logp(d, 1);
+    for (int i = start; i < end; i++) tf_res[i] = tsel[4 * isTransient + 2 * tf_select + tf_res[i]];
+}
+
+// clt_compute_allocation + interp_bits2pulses, mono decoder (rate.c:245-646)
+CUO_FN int compute_allocation(int start, int end, const int32_t *offsets, const int32_t *cap, int alloc_trim,
+                              int32_t total, int32_t *balance_out, int32_t *bits, int32_t *ebits,
+                              int32_t *fine_priority, int LM, RangeDec *d, CeltScratch *sc)
+{
+    const int16_t *eB = tab::kEBands;
+    int32_t *bits1 = sc->bits1, *bits2 = sc->bits2, *thresh = sc->thresh, *trim_offset = sc->trim_offset;
+    const int C = 1;
+    total = max32(total, 0);
+    int skip_start = start;
+    int skip_rsv = total >= 1 << kBitRes ? 1 << kBitRes : 0;
+    total -= skip_rsv;
+    for (int j = start; j < end; j++) {
+        thresh[j] = imax(C << kBitRes, (3 * (eB[j + 1] - eB[j]) << LM << kBitRes) >> 4);
+        trim_offset[j] = C * (eB[j + 1] - eB[j]) * (alloc_trim - 5 - LM) * (end - j - 1) * (1 << (LM + kBitRes)) >> 6;
+        if ((eB[j + 1] - eB[j]) << LM == 1) trim_offset[j] -= C << kBitRes;
+    }
+    int lo = 1, hi = tab::kNbAllocVectors - 1;
+    do {
+        int done = 0, psum = 0, mid = (lo + hi) >> 1;
+        for (int j = end; j-- > start;) {
+            int N = eB[j + 1] - eB[j];
+            int bitsj = C * N * tab::kAllocVectors[mid * kNbBands + j] << LM >> 2;
+            if (bitsj > 0) bitsj = imax(0, bitsj + trim_offset[j]);
+            bitsj += offsets[j];
+            if (bitsj >= thresh[j] || done) { done = 1; psum += imin(bitsj, cap[j]); }
+            else if (bitsj >= C << kBitRes) psum += C << kBitRes;
+        }
+        if (psum > total) hi = mid - 1; else lo = mid + 1;
+    } while (lo <= hi);
+    hi = lo--;
+    for (int j = start; j < end; j++) {
+        int N = eB[j + 1] - eB[j];
+        int bits1j = C * N * tab::kAllocVectors[lo * kNbBands + j] << LM >> 2;
+        int bits2j = hi >= tab::kNbAllocVectors ? cap[j] : C * N * tab::kAllocVectors[hi * kNbBands + j] << LM >> 2;
+        if (bits1j > 0) bits1j = imax(0, bits1j + trim_offset[j]);
+        if (bits2j > 0) bits2j = imax(0, bits2j + trim_offset[j]);
+        if (lo > 0) bits1j += offsets[j];
+        bits2j += offsets[j];
+        if (offsets[j] > 0) skip_start = j;
+        bits2j = imax(0, bits2j - bits1j);
+        bits1[j] = bits1j;
+        bits2[j] = bits2j;
+    }
+    // ---- interp_bits2pulses
+    const int alloc_floor = C << kBitRes;
+    const int logM = LM << kBitRes;
+    lo = 0; hi = 1 << 6;
+    int32_t psum;
+    for (int i = 0; i < 6; i++) {
+        int mid = (lo + hi) >> 1;
+        psum = 0;
+        int done = 0;
+        for (int j = end; j-- > start;) {
+            int tmp = bits1[j] + (mid * (int32_t)bits2[j] >> 6);
+            if (tmp >= thresh[j] || done) { done = 1; psum += imin(tmp, cap[j]); }
+            else if (tmp >= alloc_floor) psum += alloc_floor;
+        }
+        if (psum > total) hi = mid; else lo = mid;
+    }
+    psum = 0;
+    int done = 0;
+    for (int j = end; j-- > start;) {
+        int tmp = bits1[j] + ((int32_t)lo * bits2[j] >> 6);
+        if (tmp < thresh[j] && !done) tmp = tmp >= alloc_floor ? alloc_floor : 0;
+        else done = 1;
+        tmp = imin(tmp, cap[j]);
+        bits[j] = tmp;
+        psum += tmp;
+    }
+    int codedBands;
+    for (codedBands = end;; codedBands--) {
+        int j = codedBands - 1;
+        if (j <= skip_start) { total += skip_rsv; break; }
+        int32_t left = total - psum;
+        int32_t percoeff = left / (eB[codedBands] - eB[start]);
+        left -= (eB[codedBands] - eB[start]) * percoeff;
+        int rem = imax(left - (eB[j] - eB[start]), 0);
+        int band_width = eB[codedBands] - eB[j];
+        int band_bits = (int)(bits[j] + percoeff * band_width + rem);
+        if (band_bits >= imax(thresh[j], alloc_floor + (1 << kBitRes))) {
+            if (ec_bit_logp(d, 1)) break;
+            psum += 1 << kBitRes;
+            band_bits -= 1 << kBitRes;
+        }
+        psum -= bits[j];
+        if (band_bits >= alloc_floor) { psum += alloc_floor; bits[j] = alloc_floor; }
+        else bits[j] = 0;
+    }
+    // mono: no intensity / dual stereo symbols
+    int32_t left = total - psum;
+    int32_t percoeff = left / (eB[codedBands] - eB[start]);
+    left -= (eB[codedBands] - eB[start]) * percoeff;
+    for (int j = start; j < codedBands; j++) bits[j] += (int)percoeff * (eB[j + 1] - eB[j]);
+    for (int j = start; j < codedBands; j++) {
+        int tmp = (int)imin(left, eB[j + 1] - eB[j]);
+        bits[j] += tmp;
+        left -= tmp;
+    }
+    int32_t balance = 0;
+    int j;
+    for (j = start; j < codedBands; j++) {
+        int N0 = eB[j + 1] - eB[j];
+        int N = N0 << LM;
+        int32_t bit = (int32_t)bits[j] + balance;
+        int32_t excess;
+        if (N > 1) {
+            excess = max32(bit - cap[j], 0);
+            bits[j] = bit - excess;
+            int den = C * N;
+            int NClogN = den * (tab::kLogN[j] + logM);
+            int offset = (NClogN >> 1) - den * 21;
+            if (N == 2) offset += den << kBitRes >> 2;
+            if (bits[j] + offset < den * 2 << kBitRes) offset += NClogN >> 2;
+            else if (bits[j] + offset < den * 3 << kBitRes) offset += NClogN >> 3;
+            ebits[j] = imax(0, bits[j] + offset + (den << (kBitRes - 1)));
+            ebits[j] = (ebits[j] / den) >> kBitRes;
+            if (C * ebits[j] > (bits[j] >> kBitRes)) ebits[j] = bits[j] >> kBitRes;
+            ebits[j] = imin(ebits[j], 8);
+            fine_priority[j] = ebits[j] * (den << kBitRes) >= bits[j] + offset;
+            bits[j] -= C * ebits[j] << kBitRes;
+        } else {
+            excess = max32(0, bit - (C << kBitRes));
+            bits[j] = bit - excess;
+            ebits[j] = 0;
+            fine_priority[j] = 1;
+        }
+        if (excess > 0) {
+            int extra_fine = imin(excess >> kBitRes, 8 - ebits[j]);
+            ebits[j] += extra_fine;
+            int extra_bits = extra_fine * C << kBitRes;
+            fine_priority[j] = extra_bits >= excess - balance;
+            excess -= extra_bits;
+        }
+        balance = excess;
+    }
+    *balance_out = balance;
+    for (; j < end; j++) {
+        ebits[j] = bits[j] >> kBitRes;
+        bits[j] = 0;
+        fine_priority[j] = ebits[j] < 1;
+    }
+    return codedBands;
+}
+
+// ------------------------------------------------------------------ PVQ
+CUO_FN uint32_t pvq_u(int n, int k)
+{
+    int r = imin(n, k), c = imax(n, k);
+    return tab::kPvqU[tab::kPvqURow[r] + c];
+}
+CUO_FN uint32_t pvq_v(int n, int k) { return pvq_u(n, k) + pvq_u(n, k + 1); }
+
+// cwrsi: codeword index -> pulse vector, returns sum of squares (cwrs.c:463-537)
+CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
+{
+    int32_t yy = 0;
+    uint32_t p;
+    int s, k0;
+    int16_t val;
+    while (n > 2) {
+        uint32_t q;
+        if (k >= n) {
+            const uint32_t *row = tab::kPvqU + tab::kPvqURow[n];
+            p = row[k + 1];
+            s = -(i >= p);
+            i -= p & (uint32_t)s;
+            k0 = k;
+            q = row[n];
+            if (q > i) {
+                k = n;
+                do p = tab::kPvqU[tab::kPvqURow[--k] + n]; while (p > i);
+            } else {
+                for (p = row[k]; p > i; p = row[k]) k--;
+            }
+            i -= p;
+            val = (int16_t)((k0 - k + s) ^ s);
+            *y++ = val;
+            yy = mac16_16(yy, val, val);
+        } else {
+            p = tab::kPvqU[tab::kPvqURow[k] + n];
+            q = tab::kPvqU[tab::kPvqURow[k + 1] + n];
+            if (p <= i && i < q) {
+                i -= p;
+                *y++ = 0;
+            } else {
+                s = -(i >= q);
+                i -= q & (uint32_t)s;
+                k0 = k;
+                do p = tab::kPvqU[tab::kPvqURow[--k] + n]; while (p > i);
+                i -= p;
+                val = (int16_t)((k0 - k + s) ^ s);
+                *y++ = val;
+                yy = mac16_16(yy, val, val);
+            }
+        }
+        n--;
+    }
+    p = 2 * (uint32_t)k + 1;
+    s = -(i >= p);
+    i -= p & (uint32_t)s;
+    k0 = k;
+    k = (int)((i + 1) >> 1);
+    if (k) i -= 2 * (uint32_t)k - 1;
+    val = (int16_t)((k0 - k + s) ^ s);
+    *y++ = val;
+    yy = mac16_16(yy, val, val);
+    s = -(int)i;
+    val = (int16_t)((k + s) ^ s);
+    *y = val;
+    yy = mac16_16(yy, val, val);
+    return yy;
+}
+
+CUO_FN void exp_rotation1(int16_t *X, int len, int stride, int16_t c, int16_t s)
+{
+    int16_t ms = (int16_t)-s;
+    int16_t *Xp = X;
+    for (int i = 0; i < len - stride; i++) {
+        int16_t x1 = Xp[0], x2 = Xp[stride];
+        Xp[stride] = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
+        *Xp++ = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
+    }
+    Xp = &X[len - 2 * stride - 1];
+    for (int i = len - 2 * stride - 1; i >= 0; i--) {
+        int16_t x1 = Xp[0], x2 = Xp[stride];
+        Xp[stride] = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
+        *Xp-- = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
+    }
+}
+
+CUO_FN void exp_rotation(int16_t *X, int len, int dir, int stride, int K, int spread)
+{
+    if (2 * K >= len || spread == SPREAD_NONE) return;
+    const int factor = spread == 1 ? 15 : (spread == 2 ? 10 : 5);
+    int16_t gain = (int16_t)celt_div((int32_t)mul16_16(kQ15One, len), (int32_t)(len + factor * K));
+    int16_t theta = (int16_t)(mul16_16_q15(gain, gain) >> 1);
+    int16_t c = celt_cos_norm((int32_t)theta);
+    int16_t s = celt_cos_norm((int32_t)(int16_t)sub16(kQ15One, theta));
+    int stride2 = 0;
+    if (len >= 8 * stride) {
+        stride2 = 1;
+        while ((stride2 * stride2 + stride2) * stride + (stride >> 2) < len) stride2++;
+    }
+    len = len / stride;
+    for (int i = 0; i < stride; i++) {
+        if (dir < 0) {
+            if (stride2) exp_rotation1(X + i * len, len, stride2, s, c);
+            exp_rotation1(X + i * len, len, 1, c, s);
+        } else {
+            exp_rotation1(X + i * len, len, 1, c, (int16_t)-s);
+            if (stride2) exp_rotation1(X + i * len, len, stride2, s, (int16_t)-c);
+        }
+    }
+}
+
+CUO_FN unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDec *d, int16_t gain, int32_t *iy)
+{
+    int32_t Ryy = cwrsi(N, K, ec_uint(d, pvq_v(N, K)), iy);
+    // normalise_residual
+    int k = celt_ilog2(Ryy) >> 1;
+    int32_t t = vshr32(Ryy, 2 * (k - 7));
+    int16_t g = (int16_t)mul16_16_p15(celt_rsqrt_norm(t), gain);
+    for (int i = 0; i < N; i++) X[i] = (int16_t)pshr32(mul16_16(g, iy[i]), k + 1);
+    exp_rotation(X, N, -1, B, K, spread);
+    // extract_collapse_mask
+    if (B <= 1) return 1;
+    int N0 = N / B;
+    unsigned cm = 0;
+    for (int i = 0; i < B; i++) {
+        unsigned tmp = 0;
+        for (int j = 0; j < N0; j++) tmp |= (unsigned)iy[i * N0 + j];
+        cm |= (unsigned)(tmp != 0) << i;
+    }
+    return cm;
+}
+
+CUO_FN void haar1(int16_t *X, int N0, int stride)
+{
+    N0 >>= 1;
+    for (int i = 0; i < stride; i++)
+        for (int j = 0; j < N0; j++) {
+            int32_t t1 = mul16_16(23170, X[stride * 2 * j + i]);
+            int32_t t2 = mul16_16(23170, X[stride * (2 * j + 1) + i]);
+            X[stride * 2 * j + i] = (int16_t)pshr32(t1 + t2, 15);
+            X[stride * (2 * j + 1) + i] = (int16_t)pshr32(t1 - t2, 15);
+        }
+}
+
+CUO_FN void deinterleave_hadamard(int16_t *X, int N0, int stride, int hadamard, int16_t *tmp)
+{
+    int N = N0 * stride;
+    if (hadamard) {
+        const int16_t *ordery = tab::kOrderyTable + stride - 2;
+        for (int i = 0; i < stride; i++)
+            for (int j = 0; j < N0; j++) tmp[ordery[i] * N0 + j] = X[j * stride + i];
+    } else {
+        for (int i = 0; i < stride; i++)
+            for (int j = 0; j < N0; j++) tmp[i * N0 + j] = X[j * stride + i];
+    }
+    for (int i = 0; i < N; i++) X[i] = tmp[i];
+}
+
+CUO_FN void interleave_hadamard(int16_t *X, int N0, int stride, int hadamard, int16_t *tmp)
+{
+    int N = N0 * stride;
+    if (hadamard) {
+        const int16_t *ordery = tab::kOrderyTable + stride - 2;
+        for (int i = 0; i < stride; i++)
+            for (int j = 0; j < N0; j++) tmp[j * stride + i] = X[ordery[i] * N0 + j];
+    } else {
+        for (int i = 0; i < stride; i++)
+            for (int j = 0; j < N0; j++) tmp[j * stride + i] = X[i * N0 + j];
+    }
+    for (int i = 0; i < N; i++) X[i] = tmp[i];
+}
+
+struct BandCtx {
+    RangeDec *ec;
+    int band;
+    int spread;
+    int tf_change;
+    int32_t remaining_bits;
+    uint32_t seed;
+    int32_t *iy;
+};
+
+static constexpr int16_t kExp2Table8[8] = {16384, 17866, 19483, 21247, 23170, 25267, 27554, 30048};
+
+CUO_FN int compute_qn(int N, int b, int offset, int pulse_cap)
+{
+    int N2 = 2 * N - 1;
+    int qb = (b + N2 * offset) / N2;
+    qb = imin(b - pulse_cap - (4 << kBitRes), qb);
+    qb = imin(8 << kBitRes, qb);
+    if (qb < (1 << kBitRes >> 1)) return 1;
+    int qn = kExp2Table8[qb & 0x7] >> (14 - (qb >> kBitRes));
+    return (qn + 1) >> 1 << 1;
+}
+
+// One level of the mono theta split (bands.c:700-903 with stereo = 0).
+struct SplitOut { int imid, iside, delta, itheta, qalloc; };
+
+CUO_FN SplitOut compute_theta_mono(BandCtx *ctx, int N, int *b, int B, int B0, int LM, int *fill)
+{
+    SplitOut o;
+    int i = ctx->band;
+    int pulse_cap = tab::kLogN[i] + LM * (1 << kBitRes);
+    int offset = (pulse_cap >> 1) - 4;
+    int qn = compute_qn(N, *b, offset, pulse_cap);
+    int32_t tell = (int32_t)ec_tell_frac(ctx->ec);
+    int itheta = 0;
+    if (qn != 1) {
+        if (B0 > 1) {
+            itheta = (int)ec_uint(ctx->ec, qn + 1);
+        } else {
+            int fs = 1, fl = 0;
+            int ft = ((qn >> 1) + 1) * ((qn >> 1) + 1);
+            int fm = (int)ec_decode(ctx->ec, ft);
+            if (fm < ((qn >> 1) * ((qn >> 1) + 1) >> 1)) {
+                itheta = (int)(isqrt32(8 * (uint32_t)fm + 1) - 1) >> 1;
+                fs = itheta + 1;
+                fl = itheta * (itheta + 1) >> 1;
+            } else {
+                itheta = (2 * (qn + 1) - (int)isqrt32(8 * (uint32_t)(ft - fm - 1) + 1)) >> 1;
+                fs = qn + 1 - itheta;
+                fl = ft - ((qn + 1 - itheta) * (qn + 2 - itheta) >> 1);
+            }
+            ec_update(ctx->ec, fl, fl + fs, ft);
+        }
+        itheta = (int)((int32_t)itheta * 16384 / qn);
+    }
+    int qalloc = (int32_t)ec_tell_frac(ctx->ec) - tell;
+    *b -= qalloc;
+    if (itheta == 0) {
+        o.imid = 32767; o.iside = 0; *fill &= (1 << B) - 1; o.delta = -16384;
+    } else if (itheta == 16384) {
+        o.imid = 0; o.iside = 32767; *fill &= ((1 << B) - 1) << B; o.delta = 16384;
+    } else {
+        o.imid = bitexact_cos((int16_t)itheta);
+        o.iside = bitexact_cos((int16_t)(16384 - itheta));
+        o.delta = frac_mul16((N - 1) << 7, bitexact_log2tan(o.iside, o.imid));
+    }
+    o.itheta = itheta;
+    o.qalloc = qalloc;
+    return o;
+}
+
+// quant_partition (decoder, mono) as an explicit-stack iteration: each frame
+// holds the arguments of one recursive call plus what the call keeps between
+// its two children (bands.c:943-1108).
+struct QpFrame {
+    int16_t *X, *lowband, *Y, *lowband2;
+    int N, b, B, B0, LM, fill;
+    int16_t gain, mid, side;
+    int stage, mfirst, mbits, sbits, itheta, rebalance;
+    unsigned cm;
+};
+
+CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B0, int16_t *lowband0,
+                                int LM0, int16_t gain0, int fill0)
+{
+    QpFrame stk[6];
+    int sp = 0;
+    unsigned ret = 0;
+    {
+        QpFrame &f = stk[sp++];
+        f.X = X0; f.N = N0; f.b = b0; f.B = B0; f.lowband = lowband0; f.LM = LM0; f.gain = gain0;
+        f.fill = fill0; f.stage = 0;
+    }
+    while (sp > 0) {
+        QpFrame &f = stk[sp - 1];
+        if (f.stage == 0) {
+            const uint8_t *cache = pulse_cache(ctx->band, f.LM + 1);
+            if (f.LM != -1 && f.b > cache[cache[0]] + 12 && f.N > 2) {
+                f.B0 = f.B;
+                f.N >>= 1;
+                f.Y = f.X + f.N;
+                f.LM -= 1;
+                if (f.B == 1) f.fill = (f.fill & 1) | (f.fill << 1);
+                f.B = (f.B + 1) >> 1;
+                SplitOut so = compute_theta_mono(ctx, f.N, &f.b, f.B, f.B0, f.LM, &f.fill);
+                f.mid = (int16_t)so.imid;
+                f.side = (int16_t)so.iside;
+                int delta = so.delta;
+                f.itheta = so.itheta;
+                if (f.B0 > 1 && (f.itheta & 0x3fff)) {
+                    if (f.itheta > 8192) delta -= delta >> (4 - f.LM);
+                    else delta = imin(0, delta + (f.N << kBitRes >> (5 - f.LM)));
+                }
+                f.mbits = imax(0, imin(f.b, (f.b - delta) / 2));
+                f.sbits = f.b - f.mbits;
+                ctx->remaining_bits -= so.qalloc;
+                f.lowband2 = f.lowband ? f.lowband + f.N : nullptr;
+                f.rebalance = ctx->remaining_bits;
+                f.mfirst = f.mbits >= f.sbits;
+                f.stage = 1;
+                QpFrame &c = stk[sp++];
+                c.stage = 0; c.N = f.N; c.B = f.B; c.LM = f.LM;
+                if (f.mfirst) {
+                    c.X = f.X; c.b = f.mbits; c.lowband = f.lowband;
+                    c.gain = (int16_t)mul16_16_p15(f.gain, f.mid); c.fill = f.fill;
+                } else {
+                    c.X = f.Y; c.b = f.sbits; c.lowband = f.lowband2;
+                    c.gain = (int16_t)mul16_16_p15(f.gain, f.side); c.fill = f.fill >> f.B;
+                }
+                continue;
+            }
+            // leaf: PVQ or folding
+            int q = bits2pulses(ctx->band, f.LM, f.b);
+            int curr_bits = pulses2bits(ctx->band, f.LM, q);
+            ctx->remaining_bits -= curr_bits;
+            while (ctx->remaining_bits < 0 && q > 0) {
+                ctx->remaining_bits += curr_bits;
+                q--;
+                curr_bits = pulses2bits(ctx->band, f.LM, q);
+                ctx->remaining_bits -= curr_bits;
+            }
+            unsigned cm = 0;
+            if (q != 0) {
+                cm = alg_unquant(f.X, f.N, get_pulses(q), ctx->spread, f.B, ctx->ec, f.gain, ctx->iy);
+            } else {
+                unsigned cm_mask = (unsigned)(1UL << f.B) - 1;
+                int fill = f.fill & (int)cm_mask;
+                if (!fill) {
+                    for (int j = 0; j < f.N; j++) f.X[j] = 0;
+                } else {
+                    if (f.lowband == nullptr) {
+                        for (int j = 0; j < f.N; j++) {
+                            ctx->seed = lcg_rand(ctx->seed);
+                            f.X[j] = (int16_t)((int32_t)ctx->seed >> 20);
+                        }
+                        cm = cm_mask;
+                    } else {
+                        for (int j = 0; j < f.N; j++) {
+                            ctx->seed = lcg_rand(ctx->seed);
+                            int16_t tmp = (ctx->seed & 0x8000) ? 4 : -4;  // QCONST16(1/256, 10)
+                            f.X[j] = (int16_t)(f.lowband[j] + tmp);
+                        }
+                        cm = (unsigned)fill;
+                    }
+                    renormalise_vector(f.X, f.N, f.gain);
+                }
+            }
+            ret = cm;
+            sp--;
+            continue;
+        }
+        if (f.stage == 1) {
+            if (f.mfirst) {
+                f.cm = ret;
+                int reb = f.mbits - (f.rebalance - ctx->remaining_bits);
+                if (reb > 3 << kBitRes && f.itheta != 0) f.sbits += reb - (3 << kBitRes);
+            } else {
+                f.cm = ret << (f.B0 >> 1);
+                int reb = f.sbits - (f.rebalance - ctx->remaining_bits);
+                if (reb > 3 << kBitRes && f.itheta != 16384) f.mbits += reb - (3 << kBitRes);
+            }
+            f.stage = 2;
+            QpFrame &c = stk[sp++];
+            c.stage = 0; c.N = f.N; c.B = f.B; c.LM = f.LM;
+            if (f.mfirst) {
+                c.X = f.Y; c.b = f.sbits; c.lowband = f.lowband2;
+                c.gain = (int16_t)mul16_16_p15(f.gain, f.side); c.fill = f.fill >> f.B;
+            } else {
+                c.X = f.X; c.b = f.mbits; c.lowband = f.lowband;
+                c.gain = (int16_t)mul16_16_p15(f.gain, f.mid); c.fill = f.fill;
+            }
+            continue;
+        }
+        // stage 2: both children done
+        if (f.mfirst) f.cm |= ret << (f.B0 >> 1); else f.cm |= ret;
+        ret = f.cm;
+        sp--;
+    }
+    return ret;
+}
+
+static constexpr uint8_t kBitInterleave[16] = {0, 1, 1, 1, 2, 3, 3, 3, 2, 3, 3, 3, 2, 3, 3, 3};
+static constexpr uint8_t kBitDeinterleave[16] = {0x00, 0x03, 0x0C, 0x0F, 0x30, 0x33, 0x3C, 0x3F,
+                                                 0xC0, 0xC3, 0xCC, 0xCF, 0xF0, 0xF3, 0xFC, 0xFF};
+
+// quant_band, decoder, mono (bands.c:1109-1234)
+CUO_FN unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_t *lowband, int LM,
+                           int16_t *lowband_out, int16_t gain, int16_t *lowband_scratch, int fill, int16_t *tmp)
+{
+    int N0 = N, N_B = N, B0 = B, time_divide = 0, recombine = 0;
+    int longBlocks = B0 == 1;
+    int tf_change = ctx->tf_change;
+    N_B = N_B / B;
+    if (N == 1) {
+        int sign = 0;
+        if (ctx->remaining_bits >= 1 << kBitRes) {
+            sign = (int)ec_bits(ctx->ec, 1);
+            ctx->remaining_bits -= 1 << kBitRes;
+        }
+        X[0] = sign ? -16384 : 16384;
+        if (lowband_out) lowband_out[0] = (int16_t)(X[0] >> 4);
+        return 1;
+    }
+    if (tf_change > 0) recombine = tf_change;
+    if (lowband_scratch && lowband && (recombine || ((N_B & 1) == 0 && tf_change < 0) || B0 > 1)) {
+        for (int j = 0; j < N; j++) lowband_scratch[j] = lowband[j];
+        lowband = lowband_scratch;
+    }
+    for (int k = 0; k < recombine; k++) {
+        if (lowband) haar1(lowband, N >> k, 1 << k);
+        fill = kBitInterleave[fill & 0xF] | kBitInterleave[fill >> 4] << 2;
+    }
+    B >>= recombine;
+    N_B <<= recombine;
+    while ((N_B & 1) == 0 && tf_change < 0) {
+        if (lowband) haar1(lowband, N_B, B);
+        fill |= fill << B;
+        B <<= 1;
+        N_B >>= 1;
+        time_divide++;
+        tf_change++;
+    }
+    B0 = B;
+    int N_B0 = N_B;
+    if (B0 > 1 && lowband) deinterleave_hadamard(lowband, N_B >> recombine, B0 << recombine, longBlocks, tmp);
+    unsigned cm = quant_partition(ctx, X, N, b, B, lowband, LM, gain, fill);
+    if (B0 > 1) interleave_hadamard(X, N_B >> recombine, B0 << recombine, longBlocks, tmp);
+    N_B = N_B0;
+    B = B0;
+    for (int k = 0; k < time_divide; k++) {
+        B >>= 1;
+        N_B <<= 1;
+        cm |= cm >> B;
+        haar1(X, N_B, B);
+    }
+    for (int k = 0; k < recombine; k++) {
+        cm = kBitDeinterleave[cm];
+        haar1(X, N0 >> k, 1 << k);
+    }
+    B <<= recombine;
+    if (lowband_out) {
+        int16_t n = (int16_t)celt_sqrt(shl32(N0, 22));
+        for (int j = 0; j < N0; j++) lowband_out[j] = (int16_t)mul16_16_q15(n, X[j]);
+    }
+    cm &= (1u << B) - 1;
+    return cm;
+}
+
+// quant_all_bands, decoder, mono (bands.c:1398-1672)
+CUO_FN void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, const int32_t *pulses,
+                            int shortBlocks, int spread, const int32_t *tf_res, int32_t total_bits,
+                            int32_t balance, RangeDec *ec, int LM, int codedBands, uint32_t *seed,
+                            CeltScratch *sc)
+{
+    const int16_t *eB = tab::kEBands;
+    const int M = 1 << LM;
+    const int B = shortBlocks ? M : 1;
+    const int norm_offset = M * eB[start];
+    int16_t *norm = sc->norm;
+    int16_t *lowband_scratch = X_ + M * eB[kNbBands - 1];
+    int lowband_offset = 0, update_lowband = 1;
+    BandCtx ctx;
+    ctx.ec = ec; ctx.spread = spread; ctx.seed = *seed; ctx.iy = sc->iy;
+    for (int i = start; i < end; i++) {
+        int effective_lowband = -1;
+        ctx.band = i;
+        int last = i == end - 1;
+        int16_t *X = X_ + M * eB[i];
+        int N = M * eB[i + 1] - M * eB[i];
+        int32_t tell = (int32_t)ec_tell_frac(ec);
+        if (i != start) balance -= tell;
+        int32_t remaining_bits = total_bits - tell - 1;
+        ctx.remaining_bits = remaining_bits;
+        int b;
+        if (i <= codedBands - 1) {
+            int32_t curr_balance = balance / imin(3, codedBands - i);
+            b = imax(0, imin(16383, imin(remaining_bits + 1, pulses[i] + curr_balance)));
+        } else {
+            b = 0;
+        }
+        if ((M * eB[i] - N >= M * eB[start] || i == start + 1) && (update_lowband || lowband_offset == 0))
+            lowband_offset = i;
+        if (i == start + 1) {
+            int n1 = M * (eB[start + 1] - eB[start]);
+            int n2 = M * (eB[start + 2] - eB[start + 1]);
+            for (int j = 0; j < n2 - n1; j++) norm[n1 + j] = norm[2 * n1 - n2 + j];
+        }
+        int tf_change = tf_res[i];
+        ctx.tf_change = tf_change;
+        if (last) lowband_scratch = nullptr;
+        unsigned x_cm;
+        if (lowband_offset != 0 && (spread != SPREAD_AGGRESSIVE || B > 1 || tf_change < 0)) {
+            effective_lowband = imax(0, M * eB[lowband_offset] - norm_offset - N);
+            int fold_start = lowband_offset;
+            while (M * eB[--fold_start] > effective_lowband + norm_offset) {}
+            int fold_end = lowband_offset - 1;
+            while (++fold_end < i && M * eB[fold_end] < effective_lowband + norm_offset + N) {}
+            x_cm = 0;
+            int fold_i = fold_start;
+            do { x_cm |= collapse[fold_i]; } while (++fold_i < fold_end);
+        } else {
+            x_cm = (1u << B) - 1;
+        }
+        x_cm = quant_band(&ctx, X, N, b, B, effective_lowband != -1 ? norm + effective_lowband : nullptr, LM,
+                          last ? nullptr : norm + M * eB[i] - norm_offset, kQ15One, lowband_scratch, (int)x_cm,
+                          sc->tmp);
+        collapse[i] = (uint8_t)x_cm;
+        balance += pulses[i] + tell;
+        update_lowband = b > (N << kBitRes);
+    }
+    *seed = ctx.seed;
+}
+
+// anti_collapse, mono (bands.c:268-370)
+CUO_FN void anti_collapse(int16_t *X_, const uint8_t *collapse, int LM, int start, int end, const int16_t *logE,
+                          const int16_t *prev1logE, const int16_t *prev2logE, const int32_t *pulses, uint32_t seed)
+{
+    const int16_t *eB = tab::kEBands;
+    for (int i = start; i < end; i++) {
+        int N0 = eB[i + 1] - eB[i];
+        int depth = ((1 + pulses[i]) / N0) >> LM;
+        int32_t thresh32 = celt_exp2((int16_t)-shl16(depth, 10 - kBitRes)) >> 1;
+        int16_t thresh = (int16_t)mul16_32_q15(16384, min32(32767, thresh32));
+        int32_t t = N0 << LM;
+        int shift = celt_ilog2(t) >> 1;
+        t = shl32(t, (7 - shift) << 1);
+        int16_t sqrt_1 = celt_rsqrt_norm(t);
+        int16_t prev1 = max16(prev1logE[i], prev1logE[kNbBands + i]);
+        int16_t prev2 = max16(prev2logE[i], prev2logE[kNbBands + i]);
+        int32_t Ediff = (int32_t)logE[i] - (int32_t)min16(prev1, prev2);
+        Ediff = max32(0, Ediff);
+        int16_t r;
+        if (Ediff < 16384) {
+            int32_t r32 = celt_exp2((int16_t)-(int16_t)Ediff) >> 1;
+            r = (int16_t)(2 * min32(16383, r32));
+        } else {
+            r = 0;
+        }
+        if (LM == 3) r = (int16_t)mul16_16_q14(23170, min32(23169, r));
+        r = (int16_t)(min16(thresh, r) >> 1);
+        r = (int16_t)(mul16_16_q15(sqrt_1, r) >> shift);
+        int16_t *X = X_ + (eB[i] << LM);
+        int renormalize = 0;
+        for (int k = 0; k < 1 << LM; k++) {
+            if (!(collapse[i] & 1 << k)) {
+                for (int j = 0; j < N0; j++) {
+                    seed = lcg_rand(seed);
+                    X[(j << LM) + k] = (seed & 0x8000) ? r : (int16_t)-r;
+                }
+                renormalize = 1;
+            }
+        }
+        if (renormalize) renormalise_vector(X, N0 << LM, kQ15One);
+    }
+}
+
+// ------------------------------------------------------------------ synthesis
+CUO_FN void denormalise_bands(const int16_t *X, int32_t *freq, const int16_t *bandLogE, int start, int end,
+                              int M, int silence)
+{
+    const int16_t *eB = tab::kEBands;
+    int N = M * 120;
+    int bound = M * eB[end];
+    if (silence) { bound = 0; start = end = 0; }
+    int32_t *f = freq;
+    const int16_t *x = X + M * eB[start];
+    for (int i = 0; i < M * eB[start]; i++) *f++ = 0;
+    for (int i = start; i < end; i++) {
+        int j = M * eB[i], band_end = M * eB[i + 1];
+        int16_t lg = sat16(bandLogE[i] + shl32((int32_t)tab::kEMeans[i], 6));
+        int shift = 16 - (lg >> kDbShift);
+        int16_t g;
+        if (shift > 31) { shift = 0; g = 0; }
+        else g = (int16_t)celt_exp2_frac((int16_t)(lg & ((1 << kDbShift) - 1)));
+        if (shift < 0) {
+            if (shift <= -2) { g = 16384; shift = -2; }
+            do { *f++ = shl32(mul16_16(*x++, g), -shift); } while (++j < band_end);
+        } else {
+            do { *f++ = shr32(mul16_16(*x++, g), shift); } while (++j < band_end);
+        }
+    }
+    for (int i = bound; i < N; i++) freq[i] = 0;
+}
+
+// FFT butterflies on interleaved int32 complex (kiss_fft.c:48-317); twiddles
+// are int16 pairs of the 480-point table, strided for smaller sizes.
+struct Cpx { int32_t r, i; };
+CUO_FN int32_t smul(int32_t a, int16_t b) { return mul16_32_q15(b, a); }
+
+CUO_FN void bfly2(Cpx *F, int N)
+{
+    const int16_t tw = 23170;
+    for (int i = 0; i < N; i++) {
+        Cpx *F2 = F + 4;
+        Cpx t = F2[0];
+        F2[0].r = sub32_ovf(F[0].r, t.r); F2[0].i = sub32_ovf(F[0].i, t.i);
+        F[0].r = add32_ovf(F[0].r, t.r); F[0].i = add32_ovf(F[0].i, t.i);
+        t.r = smul(add32_ovf(F2[1].r, F2[1].i), tw);
+        t.i = smul(sub32_ovf(F2[1].i, F2[1].r), tw);
+        F2[1].r = sub32_ovf(F[1].r, t.r); F2[1].i = sub32_ovf(F[1].i, t.i);
+        F[1].r = add32_ovf(F[1].r, t.r); F[1].i = add32_ovf(F[1].i, t.i);
+        t.r = F2[2].i; t.i = neg32_ovf(F2[2].r);
+        F2[2].r = sub32_ovf(F[2].r, t.r); F2[2].i = sub32_ovf(F[2].i, t.i);
+        F[2].r = add32_ovf(F[2].r, t.r); F[2].i = add32_ovf(F[2].i, t.i);
+        t.r = smul(sub32_ovf(F2[3].i, F2[3].r), tw);
+        t.i = smul(neg32_ovf(add32_ovf(F2[3].i, F2[3].r)), tw);
+        F2[3].r = sub32_ovf(F[3].r, t.r); F2[3].i = sub32_ovf(F[3].i, t.i);
+        F[3].r = add32_ovf(F[3].r, t.r); F[3].i = add32_ovf(F[3].i, t.i);
+        F += 8;
+    }
+}
+
+CUO_FN Cpx cmul(Cpx a, int16_t br, int16_t bi)
+{
+    Cpx m;
+    m.r = sub32_ovf(smul(a.r, br), smul(a.i, bi));
+    m.i = add32_ovf(smul(a.r, bi), smul(a.i, br));
+    return m;
+}
+
+CUO_FN void bfly4(Cpx *Fout, int fstride, int m, int N, int mm)
+{
+    if (m == 1) {
+        for (int i = 0; i < N; i++) {
+            Cpx s0, s1;
+            s0.r = sub32_ovf(Fout[0].r, Fout[2].r); s0.i = sub32_ovf(Fout[0].i, Fout[2].i);
+            Fout[0].r = add32_ovf(Fout[0].r, Fout[2].r); Fout[0].i = add32_ovf(Fout[0].i, Fout[2].i);
+            s1.r = add32_ovf(Fout[1].r, Fout[3].r); s1.i = add32_ovf(Fout[1].i, Fout[3].i);
+            Fout[2].r = sub32_ovf(Fout[0].r, s1.r); Fout[2].i = sub32_ovf(Fout[0].i, s1.i);
+            Fout[0].r = add32_ovf(Fout[0].r, s1.r); Fout[0].i = add32_ovf(Fout[0].i, s1.i);
+            s1.r = sub32_ovf(Fout[1].r, Fout[3].r); s1.i = sub32_ovf(Fout[1].i, Fout[3].i);
+            Fout[1].r = add32_ovf(s0.r, s1.i); Fout[1].i = sub32_ovf(s0.i, s1.r);
+            Fout[3].r = sub32_ovf(s0.r, s1.i); Fout[3].i = add32_ovf(s0.i, s1.r);
+            Fout += 4;
+        }
+        return;
+    }
+    const int16_t *tw = tab::kFftTwiddles;
+    Cpx *Fbeg = Fout;
+    for (int i = 0; i < N; i++) {
+        Fout = Fbeg + i * mm;
+        int t1 = 0, t2 = 0, t3 = 0;
+        for (int j = 0; j < m; j++) {
+            Cpx s0 = cmul(Fout[m], tw[2 * t1], tw[2 * t1 + 1]);
+            Cpx s1 = cmul(Fout[2 * m], tw[2 * t2], tw[2 * t2 + 1]);
+            Cpx s2 = cmul(Fout[3 * m], tw[2 * t3], tw[2 * t3 + 1]);
+            Cpx s5, s3, s4;
+            s5.r = sub32_ovf(Fout[0].r, s1.r); s5.i = sub32_ovf(Fout[0].i, s1.i);
+            Fout[0].r = add32_ovf(Fout[0].r, s1.r); Fout[0].i = add32_ovf(Fout[0].i, s1.i);
+            s3.r = add32_ovf(s0.r, s2.r); s3.i = add32_ovf(s0.i, s2.i);
+            s4.r = sub32_ovf(s0.r, s2.r); s4.i = sub32_ovf(s0.i, s2.i);
+            Fout[2 * m].r = sub32_ovf(Fout[0].r, s3.r); Fout[2 * m].i = sub32_ovf(Fout[0].i, s3.i);
+            t1 += fstride; t2 += fstride * 2; t3 += fstride * 3;
+            Fout[0].r = add32_ovf(Fout[0].r, s3.r); Fout[0].i = add32_ovf(Fout[0].i, s3.i);
+            Fout[m].r = add32_ovf(s5.r, s4.i); Fout[m].i = sub32_ovf(s5.i, s4.r);
+            Fout[3 * m].r = sub32_ovf(s5.r, s4.i); Fout[3 * m].i = add32_ovf(s5.i, s4.r);
+            ++Fout;
+        }
+    }
+}
+
+CUO_FN void bfly3(Cpx *Fout, int fstride, int m, int N, int mm)
+{
+    const int16_t *tw = tab::kFftTwiddles;
+    const int16_t epi3i = -28378;
+    Cpx *Fbeg = Fout;
+    for (int i = 0; i < N; i++) {
+        Fout = Fbeg + i * mm;
+        int t1 = 0, t2 = 0;
+        int k = m;
+        do {
+            Cpx s1 = cmul(Fout[m], tw[2 * t1], tw[2 * t1 + 1]);
+            Cpx s2 = cmul(Fout[2 * m], tw[2 * t2], tw[2 * t2 + 1]);
+            Cpx s3, s0;
+            s3.r = add32_ovf(s1.r, s2.r); s3.i = add32_ovf(s1.i, s2.i);
+            s0.r = sub32_ovf(s1.r, s2.r); s0.i = sub32_ovf(s1.i, s2.i);
+            t1 += fstride; t2 += fstride * 2;
+            Fout[m].r = sub32_ovf(Fout[0].r, s3.r >> 1);
+            Fout[m].i = sub32_ovf(Fout[0].i, s3.i >> 1);
+            s0.r = smul(s0.r, epi3i); s0.i = smul(s0.i, epi3i);
+            Fout[0].r = add32_ovf(Fout[0].r, s3.r); Fout[0].i = add32_ovf(Fout[0].i, s3.i);
+            Fout[2 * m].r = add32_ovf(Fout[m].r, s0.i);
+            Fout[2 * m].i = sub32_ovf(Fout[m].i, s0.r);
+            Fout[m].r = sub32_ovf(Fout[m].r, s0.i);
+            Fout[m].i = add32_ovf(Fout[m].i, s0.r);
+            ++Fout;
+        } while (--k);
+    }
+}
+
+CUO_FN void bfly5(Cpx *Fout, int fstride, int m, int N, int mm)
+{
+    const int16_t *tw = tab::kFftTwiddles;
+    const int16_t yar = 10126, yai = -31164, ybr = -26510, ybi = -19261;
+    Cpx *Fbeg = Fout;
+    for (int i = 0; i < N; i++) {
+        Fout = Fbeg + i * mm;
+        Cpx *F0 = Fout, *F1 = Fout + m, *F2 = Fout + 2 * m, *F3 = Fout + 3 * m, *F4 = Fout + 4 * m;
+        for (int u = 0; u < m; ++u) {
+            Cpx s0 = *F0;
+            int a1 = u * fstride, a2 = 2 * u * fstride, a3 = 3 * u * fstride, a4 = 4 * u * fstride;
+            Cpx s1 = cmul(*F1, tw[2 * a1], tw[2 * a1 + 1]);
+            Cpx s2 = cmul(*F2, tw[2 * a2], tw[2 * a2 + 1]);
+            Cpx s3 = cmul(*F3, tw[2 * a3], tw[2 * a3 + 1]);
+            Cpx s4 = cmul(*F4, tw[2 * a4], tw[2 * a4 + 1]);
+            Cpx s7, s10, s8, s9, s5, s6, s11, s12;
+            s7.r = add32_ovf(s1.r, s4.r); s7.i = add32_ovf(s1.i, s4.i);
+            s10.r = sub32_ovf(s1.r, s4.r); s10.i = sub32_ovf(s1.i, s4.i);
+            s8.r = add32_ovf(s2.r, s3.r); s8.i = add32_ovf(s2.i, s3.i);
+            s9.r = sub32_ovf(s2.r, s3.r); s9.i = sub32_ovf(s2.i, s3.i);
+            F0->r = add32_ovf(F0->r, add32_ovf(s7.r, s8.r));
+            F0->i = add32_ovf(F0->i, add32_ovf(s7.i, s8.i));
+            s5.r = add32_ovf(s0.r, add32_ovf(smul(s7.r, yar), smul(s8.r, ybr)));
+            s5.i = add32_ovf(s0.i, add32_ovf(smul(s7.i, yar), smul(s8.i, ybr)));
+            s6.r = add32_ovf(smul(s10.i, yai), smul(s9.i, ybi));
+            s6.i = neg32_ovf(add32_ovf(smul(s10.r, yai), smul(s9.r, ybi)));
+            F1->r = sub32_ovf(s5.r, s6.r); F1->i = sub32_ovf(s5.i, s6.i);
+            F4->r = add32_ovf(s5.r, s6.r); F4->i = add32_ovf(s5.i, s6.i);
+            s11.r = add32_ovf(s0.r, add32_ovf(smul(s7.r, ybr), smul(s8.r, yar)));
+            s11.i = add32_ovf(s0.i, add32_ovf(smul(s7.i, ybr), smul(s8.i, yar)));
+            s12.r = sub32_ovf(smul(s9.i, yai), smul(s10.i, ybi));
+            s12.i = sub32_ovf(smul(s10.r, ybi), smul(s9.r, yai));
+            F2->r = add32_ovf(s11.r, s12.r); F2->i = add32_ovf(s11.i, s12.i);
+            F3->r = sub32_ovf(s11.r, s12.r); F3->i = sub32_ovf(s11.i, s12.i);
+            ++F0; ++F1; ++F2; ++F3; ++F4;
+        }
+    }
+}
+
+// opus_fft_impl for FFT index k (0:480, 1:240, 2:120, 3:60) (kiss_fft.c:521-567)
+CUO_FN void fft_impl(int k, Cpx *fout)
+{
+    const int16_t *fac = tab::kFftFactors + 16 * k;
+    int shift = tab::kFftShift[k] > 0 ? tab::kFftShift[k] : 0;
+    int fstride[8];
+    fstride[0] = 1;
+    int L = 0, m;
+    do {
+        int p = fac[2 * L];
+        m = fac[2 * L + 1];
+        fstride[L + 1] = fstride[L] * p;
+        L++;
+    } while (m != 1);
+    m = fac[2 * L - 1];
+    for (int i = L - 1; i >= 0; i--) {
+        int m2 = i != 0 ? fac[2 * i - 1] : 1;
+        switch (fac[2 * i]) {
+        case 2: bfly2(fout, fstride[i]); break;
+        case 4: bfly4(fout, fstride[i] << shift, m, fstride[i], m2); break;
+        case 3: bfly3(fout, fstride[i] << shift, m, fstride[i], m2); break;
+        case 5: bfly5(fout, fstride[i] << shift, m, fstride[i], m2); break;
+        }
+        m = m2;
+    }
+}
+
+// clt_mdct_backward (mdct.c:242-342): in has stride `stride`, out gets the
+// overlap-added time signal.
+CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride)
+{
+    int N = tab::kMdctN;
+    const int16_t *trig = tab::kMdctTrig;
+    for (int i = 0; i < shift; i++) { N >>= 1; trig += N; }
+    const int N2 = N >> 1, N4 = N >> 2;
+    {
+        const int32_t *xp1 = in, *xp2 = in + stride * (N2 - 1);
+        int32_t *yp = out + (kOverlap >> 1);
+        const int16_t *bitrev = tab::kFftBitrev + tab::kFftBitrevOff[shift];
+        for (int i = 0; i < N4; i++) {
+            int rev = bitrev[i];
+            int32_t yr = add32_ovf(smul(*xp2, trig[i]), smul(*xp1, trig[N4 + i]));
+            int32_t yi = sub32_ovf(smul(*xp1, trig[i]), smul(*xp2, trig[N4 + i]));
+            yp[2 * rev + 1] = yr;
+            yp[2 * rev] = yi;
+            xp1 += 2 * stride;
+            xp2 -= 2 * stride;
+        }
+    }
+    fft_impl(shift, (Cpx *)(out + (kOverlap >> 1)));
+    {
+        int32_t *yp0 = out + (kOverlap >> 1), *yp1 = out + (kOverlap >> 1) + N2 - 2;
+        for (int i = 0; i < (N4 + 1) >> 1; i++) {
+            int32_t re = yp0[1], im = yp0[0];
+            int16_t t0 = trig[i], t1 = trig[N4 + i];
+            int32_t yr = add32_ovf(smul(re, t0), smul(im, t1));
+            int32_t yi = sub32_ovf(smul(re, t1), smul(im, t0));
+            re = yp1[1];
+            im = yp1[0];
+            yp0[0] = yr;
+            yp1[1] = yi;
+            t0 = trig[N4 - i - 1];
+            t1 = trig[N2 - i - 1];
+            yr = add32_ovf(smul(re, t0), smul(im, t1));
+            yi = sub32_ovf(smul(re, t1), smul(im, t0));
+            yp1[0] = yr;
+            yp0[1] = yi;
+            yp0 += 2;
+            yp1 -= 2;
+        }
+    }
+    {
+        int32_t *xp1 = out + kOverlap - 1, *yp1 = out;
+        const int16_t *wp1 = tab::kWindow, *wp2 = tab::kWindow + kOverlap - 1;
+        for (int i = 0; i < kOverlap / 2; i++) {
+            int32_t x1 = *xp1, x2 = *yp1;
+            *yp1++ = sub32_ovf(mul16_32_q15(*wp2, x2), mul16_32_q15(*wp1, x1));
+            *xp1-- = add32_ovf(mul16_32_q15(*wp1, x2), mul16_32_q15(*wp2, x1));
+            wp1++;
+            wp2--;
+        }
+    }
+}
+
+CUO_FN void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *oldBandE, int start, int effEnd,
+                           int isTransient, int LM, int silence, int32_t *freq)
+{
+    const int N = 120 << LM, M = 1 << LM;
+    int B, NB, shift;
+    if (isTransient) { B = M; NB = 120; shift = 3; }
+    else { B = 1; NB = 120 << LM; shift = 3 - LM; }
+    denormalise_bands(X, freq, oldBandE, start, effEnd, M, silence);
+    for (int b = 0; b < B; b++) mdct_backward(&freq[b], out_syn + NB * b, shift, B);
+    for (int i = 0; i < N; i++) out_syn[i] = saturate(out_syn[i], kSigSat);
+}
+
+// ------------------------------------------------------------------ post-filter
+static constexpr int16_t kCombGains[3][3] = {{10048, 7112, 4248}, {15200, 8784, 0}, {26208, 3280, 0}};
+
+CUO_FN void comb_filter_const(int32_t *y, const int32_t *x, int T, int N, int16_t g10, int16_t g11, int16_t g12)
+{
+    int32_t x4 = x[-T - 2], x3 = x[-T - 1], x2 = x[-T], x1 = x[-T + 1];
+    for (int i = 0; i < N; i++) {
+        int32_t x0 = x[i - T + 2];
+        int32_t v = x[i] + mul16_32_q15(g10, x2) + mul16_32_q15(g11, x1 + x3) + mul16_32_q15(g12, x0 + x4);
+        y[i] = saturate(v, kSigSat);
+        x4 = x3; x3 = x2; x2 = x1; x1 = x0;
+    }
+}
+
+// comb_filter (celt.c:190-256); x may equal y (in-place); window may be null
+CUO_FN void comb_filter(int32_t *y, int32_t *x, int T0, int T1, int N, int16_t g0, int16_t g1, int tapset0,
+                        int tapset1, const int16_t *window, int overlap)
+{
+    if (g0 == 0 && g1 == 0) {
+        if (x != y) for (int i = 0; i < N; i++) y[i] = x[i];
+        return;
+    }
+    T0 = imax(T0, kCombMinPeriod);
+    T1 = imax(T1, kCombMinPeriod);
+    int16_t g00 = (int16_t)mul16_16_p15(g0, kCombGains[tapset0][0]);
+    int16_t g01 = (int16_t)mul16_16_p15(g0, kCombGains[tapset0][1]);
+    int16_t g02 = (int16_t)mul16_16_p15(g0, kCombGains[tapset0][2]);
+    int16_t g10 = (int16_t)mul16_16_p15(g1, kCombGains[tapset1][0]);
+    int16_t g11 = (int16_t)mul16_16_p15(g1, kCombGains[tapset1][1]);
+    int16_t g12 = (int16_t)mul16_16_p15(g1, kCombGains[tapset1][2]);
+    int32_t x1 = x[-T1 + 1], x2 = x[-T1], x3 = x[-T1 - 1], x4 = x[-T1 - 2];
+    if (g0 == g1 && T0 == T1 && tapset0 == tapset1) overlap = 0;
+    int i;
+    for (i = 0; i < overlap; i++) {
+        int32_t x0 = x[i - T1 + 2];
+        int16_t f = (int16_t)mul16_16_q15(window[i], window[i]);
+        int32_t v = x[i]
+            + mul16_32_q15(mul16_16_q15((kQ15One - f), g00), x[i - T0])
+            + mul16_32_q15(mul16_16_q15((kQ15One - f), g01), x[i - T0 + 1] + x[i - T0 - 1])
+            + mul16_32_q15(mul16_16_q15((kQ15One - f), g02), x[i - T0 + 2] + x[i - T0 - 2])
+            + mul16_32_q15(mul16_16_q15(f, g10), x2)
+            + mul16_32_q15(mul16_16_q15(f, g11), x1 + x3)
+            + mul16_32_q15(mul16_16_q15(f, g12), x0 + x4);
+        y[i] = saturate(v, kSigSat);
+        x4 = x3; x3 = x2; x2 = x1; x1 = x0;
+    }
+    if (g1 == 0) {
+        if (x != y) for (int j = overlap; j < N; j++) y[j] = x[j];
+        return;
+    }
+    comb_filter_const(y + i, x + i, T1, N - i, g10, g11, g12);
+}
+
+// deemphasis, mono, no downsampling (celt_decoder.c:279-377)
+CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int accum)
+{
+    const int16_t coef0 = tab::kPreemph[0];
+    int32_t m = *mem;
+    if (accum) {
+        for (int j = 0; j < N; j++) {
+            int32_t tmp = x[j] + m;
+            m = mul16_32_q15(coef0, tmp);
+            pcm[j] = sat16(pcm[j] + sig2word16(tmp));
+        }
+    } else {
+        for (int j = 0; j < N; j++) {
+            int32_t tmp = x[j] + m;
+            m = mul16_32_q15(coef0, tmp);
+            pcm[j] = sig2word16(tmp);
+        }
+    }
+    *mem = m;
+}
+
+// ------------------------------------------------------------------ loss concealment
+CUO_FN void prefilter_and_fold(CeltState *st, int N, int32_t *etmp)
+{
+    int32_t *mem = st->mem;
+    comb_filter(etmp, mem + kDecBuf - N, st->pf_period_old, st->pf_period, kOverlap, (int16_t)-st->pf_gain_old,
+                (int16_t)-st->pf_gain, st->pf_tapset_old, st->pf_tapset, nullptr, 0);
+    for (int i = 0; i < kOverlap / 2; i++)
+        mem[kDecBuf - N + i] = mul16_32_q15(tab::kWindow[i], etmp[kOverlap - 1 - i])
+                             + mul16_32_q15(tab::kWindow[kOverlap - i - 1], etmp[i]);
+}
+
+// Pitch-based concealment (CELT-only frames after a loss); defined in celt_plc.h.
+CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc);
+
+// celt_decode_lost (celt_decoder.c:604-969). Returns 0, or -1 if the
+// concealment path is not available.
+CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
+{
+    int32_t *mem = st->mem;
+    int32_t *out_syn = mem + kDecBuf - N;
+    int loss_duration = st->loss_duration;
+    int start = st->start;
+    int noise_based = loss_duration >= 40 || start != 0 || st->skip_plc;
+    if (noise_based) {
+        int end = st->end;
+        int effEnd = imax(start, imin(end, kNbBands));
+        for (int i = 0; i < kDecBuf - N + kOverlap; i++) mem[i] = mem[i + N];
+        if (st->prefilter_and_fold) prefilter_and_fold(st, N, sc->etmp);
+        int16_t decay = loss_duration == 0 ? 1536 : 512;
+        for (int i = start; i < end; i++)
+            st->oldBandE[i] = max16(st->backgroundLogE[i], (int16_t)(st->oldBandE[i] - decay));
+        uint32_t seed = st->rng;
+        int16_t *X = sc->X;
+        for (int i = start; i < effEnd; i++) {
+            int boffs = tab::kEBands[i] << LM;
+            int blen = (tab::kEBands[i + 1] - tab::kEBands[i]) << LM;
+            for (int j = 0; j < blen; j++) {
+                seed = lcg_rand(seed);
+                X[boffs + j] = (int16_t)((int32_t)seed >> 20);
+            }
+            renormalise_vector(X + boffs, blen, kQ15One);
+        }
+        st->rng = seed;
+        celt_synthesis(X, out_syn, st->oldBandE, start, effEnd, 0, LM, 0, sc->freq);
+        st->prefilter_and_fold = 0;
+        st->skip_plc = 1;
+    } else {
+        if (celt_plc_periodic(st, N, sc) < 0) return -1;
+        st->prefilter_and_fold = 1;
+    }
+    st->loss_duration = imin(10000, loss_duration + (1 << LM));
+    return 0;
+}
+
+// ------------------------------------------------------------------ frame decode
+// celt_decode_with_ec (celt_decoder.c:970-1369).  data == nullptr or len <= 1
+// conceals.  ec may be nullptr (then a decoder is opened on data).  Returns the
+// number of samples (frame_size) or a negative error.
+CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size, RangeDec *dec,
+                       int accum, CeltScratch *sc)
+{
+    int LM;
+    for (LM = 0; LM <= 3; LM++)
+        if (120 << LM == frame_size) break;
+    if (LM > 3) return -1;  // OPUS_BAD_ARG
+    const int M = 1 << LM;
+    if (len < 0 || len > 1275 || pcm == nullptr) return -1;
+    const int N = M * 120;
+    int32_t *mem = st->mem;
+    int32_t *out_syn = mem + kDecBuf - N;
+    int start = st->start, end = st->end;
+    int effEnd = end > kNbBands ? kNbBands : end;
+    int16_t *oldBandE = st->oldBandE, *oldLogE = st->oldLogE, *oldLogE2 = st->oldLogE2;
+
+    if (data == nullptr || len <= 1) {
+        if (celt_decode_lost(st, N, LM, sc) < 0) return -3;
+        deemphasis(out_syn, pcm, N, &st->preemph_mem, accum);
+        return frame_size;
+    }
+    if (st->loss_duration == 0) st->skip_plc = 0;
+    RangeDec _dec;
+    if (dec == nullptr) { ec_init(&_dec, data, (uint32_t)len); dec = &_dec; }
+    for (int i = 0; i < kNbBands; i++) oldBandE[i] = max16(oldBandE[i], oldBandE[kNbBands + i]);
+
+    int32_t total_bits = len * 8;
+    int32_t tell = ec_tell(dec);
+    int silence;
+    if (tell >= total_bits) silence = 1;
+    else if (tell == 1) silence = ec_bit_logp(dec, 15);
+    else silence = 0;
+    if (silence) {
+        tell = len * 8;
+        dec->nbits_total += tell - ec_tell(dec);
+    }
+    int16_t postfilter_gain = 0;
+    int postfilter_pitch = 0, postfilter_tapset = 0;
+    if (start == 0 && tell + 16 <= total_bits) {
+        if (ec_bit_logp(dec, 1)) {
+            int octave = (int)ec_uint(dec, 6);
+            postfilter_pitch = (16 << octave) + (int)ec_bits(dec, 4 + octave) - 1;
+            int qg = (int)ec_bits(dec, 3);
+            if (ec_tell(dec) + 2 <= total_bits) postfilter_tapset = ec_icdf(dec, tab::kTapsetIcdf, 2);
+            postfilter_gain = (int16_t)(3072 * (qg + 1));
+        }
+        tell = ec_tell(dec);
+    }
+    int isTransient;
+    if (LM > 0 && tell + 3 <= total_bits) {
+        isTransient = ec_bit_logp(dec, 3);
+        tell = ec_tell(dec);
+    } else {
+        isTransient = 0;
+    }
+    int shortBlocks = isTransient ? M : 0;
+    int intra_ener = tell + 3 <= total_bits ? ec_bit_logp(dec, 3) : 0;
+    if (!intra_ener && st->loss_duration != 0) {
+        for (int c = 0; c < 2; c++) {
+            int16_t safety = 0;
+            int missing = imin(10, st->loss_duration >> LM);
+            if (LM == 0) safety = 1536;
+            else if (LM == 1) safety = 512;
+            for (int i = start; i < end; i++) {
+                int k = c * kNbBands + i;
+                if (oldBandE[k] < max16(oldLogE[k], oldLogE2[k])) {
+                    int32_t E0 = oldBandE[k], E1 = oldLogE[k], E2 = oldLogE2[k];
+                    int32_t slope = max32(E1 - E0, (E2 - E0) >> 1);
+                    E0 -= max32(0, (1 + missing) * slope);
+                    oldBandE[k] = (int16_t)max32(-(20 << kDbShift), E0);
+                } else {
+                    oldBandE[k] = min16(min16(oldBandE[k], oldLogE[k]), oldLogE2[k]);
+                }
+                oldBandE[k] = (int16_t)(oldBandE[k] - safety);
+            }
+        }
+    }
+    unquant_coarse_energy(start, end, oldBandE, intra_ener, dec, LM);
+    int32_t *tf_res = sc->tf_res;
+    tf_decode(start, end, isTransient, tf_res, LM, dec);
+    tell = ec_tell(dec);
+    int spread_decision = SPREAD_NORMAL;
+    if (tell + 4 <= total_bits) spread_decision = ec_icdf(dec, tab::kSpreadIcdf, 5);
+    int32_t *cap = sc->cap, *offsets = sc->offsets;
+    for (int i = 0; i < kNbBands; i++) {
+        int Nb = (tab::kEBands[i + 1] - tab::kEBands[i]) << LM;
+        cap[i] = (tab::kCacheCaps[kNbBands * (2 * LM + 1 - 1) + i] + 64) * 1 * Nb >> 2;
+    }
+    int dynalloc_logp = 6;
+    total_bits <<= kBitRes;
+    tell = (int32_t)ec_tell_frac(dec);
+    for (int i = start; i < end; i++) {
+        int width = (tab::kEBands[i + 1] - tab::kEBands[i]) << LM;
+        int quanta = imin(width << kBitRes, imax(6 << kBitRes, width));
+        int loop_logp = dynalloc_logp;
+        int boost = 0;
+        while (tell + (loop_logp << kBitRes) < total_bits && boost < cap[i]) {
+            int flag = ec_bit_logp(dec, loop_logp);
+            tell = (int32_t)ec_tell_frac(dec);
+            if (!flag) break;
+            boost += quanta;
+            total_bits -= quanta;
+            loop_logp = 1;
+        }
+        offsets[i] = boost;
+        if (boost > 0) dynalloc_logp = imax(2, dynalloc_logp - 1);
+    }
+    int alloc_trim = tell + (6 << kBitRes) <= total_bits ? ec_icdf(dec, tab::kTrimIcdf, 7) : 5;
+    int32_t bits = (((int32_t)len * 8) << kBitRes) - (int32_t)ec_tell_frac(dec) - 1;
+    int anti_collapse_rsv = isTransient && LM >= 2 && bits >= ((LM + 2) << kBitRes) ? (1 << kBitRes) : 0;
+    bits -= anti_collapse_rsv;
+    int32_t balance;
+    int32_t *pulses = sc->pulses, *fine_quant = sc->fine_quant, *fine_priority = sc->fine_priority;
+    int codedBands = compute_allocation(start, end, offsets, cap, alloc_trim, bits, &balance, pulses, fine_quant,
+                                        fine_priority, LM, dec, sc);
+    unquant_fine_energy(start, end, oldBandE, fine_quant, dec);
+    for (int i = 0; i < kDecBuf - N + kOverlap; i++) mem[i] = mem[i + N];
+    int16_t *X = sc->X;
+    quant_all_bands(start, end, X, sc->collapse, pulses, shortBlocks, spread_decision, tf_res,
+                    len * (8 << kBitRes) - anti_collapse_rsv, balance, dec, LM, codedBands, &st->rng, sc);
+    int anti_collapse_on = 0;
+    if (anti_collapse_rsv > 0) anti_collapse_on = (int)ec_bits(dec, 1);
+    unquant_energy_finalise(start, end, oldBandE, fine_quant, fine_priority, len * 8 - ec_tell(dec), dec);
+    if (anti_collapse_on)
+        anti_collapse(X, sc->collapse, LM, start, end, oldBandE, oldLogE, oldLogE2, pulses, st->rng);
+    if (silence)
+        for (int i = 0; i < kNbBands; i++) oldBandE[i] = -(28 << kDbShift);
+    if (st->prefilter_and_fold) prefilter_and_fold(st, N, sc->etmp);
+    celt_synthesis(X, out_syn, oldBandE, start, effEnd, isTransient, LM, silence, sc->freq);
+
+    st->pf_period = imax(st->pf_period, kCombMinPeriod);
+    st->pf_period_old = imax(st->pf_period_old, kCombMinPeriod);
+    comb_filter(out_syn, out_syn, st->pf_period_old, st->pf_period, 120, (int16_t)st->pf_gain_old,
+                (int16_t)st->pf_gain, st->pf_tapset_old, st->pf_tapset, tab::kWindow, kOverlap);
+    if (LM != 0)
+        comb_filter(out_syn + 120, out_syn + 120, st->pf_period, postfilter_pitch, N - 120, (int16_t)st->pf_gain,
+                    postfilter_gain, st->pf_tapset, postfilter_tapset, tab::kWindow, kOverlap);
+    st->pf_period_old = st->pf_period;
+    st->pf_gain_old = st->pf_gain;
+    st->pf_tapset_old = st->pf_tapset;
+    st->pf_period = postfilter_pitch;
+    st->pf_gain = postfilter_gain;
+    st->pf_tapset = postfilter_tapset;
+    if (LM != 0) {
+        st->pf_period_old = st->pf_period;
+        st->pf_gain_old = st->pf_gain;
+        st->pf_tapset_old = st->pf_tapset;
+    }
+    for (int i = 0; i < kNbBands; i++) oldBandE[kNbBands + i] = oldBandE[i];
+    if (!isTransient) {
+        for (int i = 0; i < 2 * kNbBands; i++) { oldLogE2[i] = oldLogE[i]; oldLogE[i] = oldBandE[i]; }
+    } else {
+        for (int i = 0; i < 2 * kNbBands; i++) oldLogE[i] = min16(oldLogE[i], oldBandE[i]);
+    }
+    int16_t max_bg_inc = (int16_t)(imin(160, st->loss_duration + M) * 1);  // QCONST16(0.001, 10) == 1
+    for (int i = 0; i < 2 * kNbBands; i++)
+        st->backgroundLogE[i] = min16((int16_t)(st->backgroundLogE[i] + max_bg_inc), oldBandE[i]);
+    for (int c = 0; c < 2; c++) {
+        for (int i = 0; i < start; i++) {
+            oldBandE[c * kNbBands + i] = 0;
+            oldLogE[c * kNbBands + i] = oldLogE2[c * kNbBands + i] = -(28 << kDbShift);
+        }
+        for (int i = end; i < kNbBands; i++) {
+            oldBandE[c * kNbBands + i] = 0;
+            oldLogE[c * kNbBands + i] = oldLogE2[c * kNbBands + i] = -(28 << kDbShift);
+        }
+    }
+    st->rng = dec->rng;
+    deemphasis(out_syn, pcm, N, &st->preemph_mem, accum);
+    st->loss_duration = 0;
+    st->prefilter_and_fold = 0;
+    if (ec_tell(dec) > 8 * len) return -3;  // OPUS_INTERNAL_ERROR
+    if (dec->error) st->error = 1;
+    return frame_size;
+}
+
+}  // namespace cuo

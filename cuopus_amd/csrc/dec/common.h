@@ -1,0 +1,297 @@
+// common.h -- qualifiers and the exact fixed-point arithmetic of the decoder.
+//
+// Every helper reproduces the integer semantics of libopus 1.5.2 built
+// FIXED_POINT on a 64-bit host (OPUS_FAST_INT64 = 1): the CELT macro layer
+// (vendor/celt/fixed_generic.h:40-190, arch.h:130-150), the SILK macro layer
+// (vendor/silk/macros.h:40-110, SigProc_FIX.h) and the shared math helpers
+// (vendor/celt/mathops.{c,h}).  Products that libopus forms in 64 bits are
+// formed in 64 bits here; sums that libopus lets wrap (the *_ovflw macros) wrap
+// here through unsigned arithmetic.  Results are returned as int32; callers
+// assign to int16 exactly where libopus stores into an opus_val16/opus_int16.
+//
+// All functions are __host__ __device__ so the same source runs on gfx950 and,
+// for the same-source CPU gate in tests/, on the host.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CUO_FN __host__ __device__ inline
+#define CUO_NOINLINE __host__ __device__ __attribute__((noinline))
+
+namespace cuo {
+
+// ---------------------------------------------------------------- basics
+CUO_FN int imin(int a, int b) { return a < b ? a : b; }
+CUO_FN int imax(int a, int b) { return a > b ? a : b; }
+CUO_FN int32_t min32(int32_t a, int32_t b) { return a < b ? a : b; }
+CUO_FN int32_t max32(int32_t a, int32_t b) { return a > b ? a : b; }
+CUO_FN int16_t min16(int16_t a, int16_t b) { return a < b ? a : b; }
+CUO_FN int16_t max16(int16_t a, int16_t b) { return a > b ? a : b; }
+CUO_FN int32_t abs32(int32_t a) { return a < 0 ? -a : a; }
+
+// number of significant bits of v (0 for v == 0): EC_ILOG
+CUO_FN int ilog(uint32_t v) { return v ? 32 - __builtin_clz(v) : 0; }
+
+// ---------------------------------------------------------------- CELT layer
+CUO_FN int32_t shl32(int32_t a, int s) { return (int32_t)((uint32_t)a << s); }
+CUO_FN int32_t shr32(int32_t a, int s) { return a >> s; }
+CUO_FN int16_t shl16(int32_t a, int s) { return (int16_t)((uint16_t)a << s); }
+CUO_FN int32_t pshr32(int32_t a, int s) { return (a + ((1 << s) >> 1)) >> s; }
+CUO_FN int32_t vshr32(int32_t a, int s) { return s > 0 ? shr32(a, s) : shl32(a, -s); }
+CUO_FN int32_t add32_ovf(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+CUO_FN int32_t sub32_ovf(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+CUO_FN int32_t neg32_ovf(int32_t a) { return (int32_t)(0u - (uint32_t)a); }
+CUO_FN int32_t saturate(int32_t x, int32_t a) { return x > a ? a : (x < -a ? -a : x); }
+CUO_FN int16_t sat16(int32_t x) { return (int16_t)(x > 32767 ? 32767 : (x < -32768 ? -32768 : x)); }
+
+CUO_FN int32_t mul16_16(int32_t a, int32_t b) { return (int32_t)(int16_t)a * (int32_t)(int16_t)b; }
+CUO_FN int32_t mul16_16su(int32_t a, int32_t b) { return (int32_t)(int16_t)a * (int32_t)(uint16_t)b; }
+CUO_FN int32_t mac16_16(int32_t c, int32_t a, int32_t b) { return c + mul16_16(a, b); }
+CUO_FN int32_t mul16_16_q15(int32_t a, int32_t b) { return mul16_16(a, b) >> 15; }
+CUO_FN int32_t mul16_16_q14(int32_t a, int32_t b) { return mul16_16(a, b) >> 14; }
+CUO_FN int32_t mul16_16_q13(int32_t a, int32_t b) { return mul16_16(a, b) >> 13; }
+CUO_FN int32_t mul16_16_p15(int32_t a, int32_t b) { return (16384 + mul16_16(a, b)) >> 15; }
+CUO_FN int32_t mul16_16_p14(int32_t a, int32_t b) { return (8192 + mul16_16(a, b)) >> 14; }
+CUO_FN int32_t mul16_32_q15(int32_t a, int32_t b) { return (int32_t)(((int64_t)(int16_t)a * (int64_t)b) >> 15); }
+CUO_FN int32_t mul16_32_q16(int32_t a, int32_t b) { return (int32_t)(((int64_t)(int16_t)a * (int64_t)b) >> 16); }
+CUO_FN int32_t mul16_32_p16(int32_t a, int32_t b) { return (int32_t)(((int64_t)(int16_t)a * (int64_t)b + 32768) >> 16); }
+CUO_FN int32_t mul32_32_q31(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)b) >> 31); }
+CUO_FN int32_t mul32_32_q16(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)b) >> 16); }
+CUO_FN int32_t mac16_32_q16(int32_t c, int32_t a, int32_t b) { return c + mul16_32_q16(a, b); }
+// FRAC_MUL16 (vendor/celt/mathops.h)
+CUO_FN int32_t frac_mul16(int32_t a, int32_t b) { return (16384 + (int32_t)(int16_t)a * (int16_t)b) >> 15; }
+CUO_FN int16_t round16(int32_t x, int a) { return (int16_t)pshr32(x, a); }
+CUO_FN int16_t sround16(int32_t x, int a) { return (int16_t)saturate(pshr32(x, a), 32767); }
+
+constexpr int kQ15One = 32767;
+constexpr int kSigShift = 12;
+constexpr int32_t kSigSat = 300000000;
+constexpr int kDbShift = 10;
+constexpr int kBitRes = 3;
+
+// celt_ilog2 / celt_zlog2
+CUO_FN int celt_ilog2(int32_t x) { return ilog((uint32_t)x) - 1; }
+CUO_FN int celt_zlog2(int32_t x) { return x <= 0 ? 0 : celt_ilog2(x); }
+
+// SIG2WORD16 (vendor/celt/fixed_generic.h:182-188)
+CUO_FN int16_t sig2word16(int32_t x)
+{
+    x = pshr32(x, kSigShift);
+    x = max32(x, -32768);
+    x = min32(x, 32767);
+    return (int16_t)x;
+}
+
+// isqrt32: floor(sqrt(v)), v > 0 (vendor/celt/mathops.c:43-63)
+CUO_FN uint32_t isqrt32(uint32_t v)
+{
+    uint32_t g = 0;
+    int bshift = (ilog(v) - 1) >> 1;
+    uint32_t b = 1u << bshift;
+    do {
+        uint32_t t = ((g << 1) + b) << bshift;
+        if (t <= v) { g += b; v -= t; }
+        b >>= 1;
+        bshift--;
+    } while (bshift >= 0);
+    return g;
+}
+
+// ADD16 / SUB16 exactly as the macros: operands narrowed to int16 first;
+// ADD16 narrows the sum, SUB16 does not.
+CUO_FN int16_t add16(int32_t a, int32_t b) { return (int16_t)((int16_t)a + (int16_t)b); }
+CUO_FN int32_t sub16(int32_t a, int32_t b) { return (int32_t)(int16_t)a - (int32_t)(int16_t)b; }
+
+// celt_rcp: Q15 reciprocal approximation (vendor/celt/mathops.c:181-205)
+CUO_FN int32_t celt_rcp(int32_t x)
+{
+    int i = celt_ilog2(x);
+    int16_t n = (int16_t)(vshr32(x, i - 15) - 32768);
+    int16_t r = add16(30840, mul16_16_q15(-15420, n));
+    r = (int16_t)sub16(r, mul16_16_q15(r, add16(mul16_16_q15(r, n), add16(r, -32768))));
+    r = (int16_t)sub16(r, add16(1, mul16_16_q15(r, add16(mul16_16_q15(r, n), add16(r, -32768)))));
+    return vshr32((int32_t)r, i - 16);
+}
+
+CUO_FN int32_t celt_div(int32_t a, int32_t b) { return mul32_32_q31(a, celt_rcp(b)); }
+
+// frac_div32 (vendor/celt/mathops.c:70-89)
+CUO_FN int32_t frac_div32(int32_t a, int32_t b)
+{
+    int shift = celt_ilog2(b) - 29;
+    a = vshr32(a, shift);
+    b = vshr32(b, shift);
+    int16_t rcp = round16(celt_rcp(round16(b, 16)), 3);
+    int32_t result = mul16_32_q15(rcp, a);
+    int32_t rem = pshr32(a, 2) - mul32_32_q31(result, b);
+    result = result + shl32(mul16_32_q15(rcp, rem), 2);
+    if (result >= 536870912) return 2147483647;
+    if (result <= -536870912) return -2147483647;
+    return shl32(result, 2);
+}
+
+// celt_rsqrt_norm: Q14 1/sqrt(x) for Q16 x in [0.25,1) (vendor/celt/mathops.c:91-117)
+CUO_FN int16_t celt_rsqrt_norm(int32_t x)
+{
+    int16_t n = (int16_t)(x - 32768);
+    int16_t r = add16(23557, mul16_16_q15(n, add16(-13490, mul16_16_q15(n, 6713))));
+    int16_t r2 = (int16_t)mul16_16_q15(r, r);
+    int16_t y = shl16(sub16(add16(mul16_16_q15(r2, n), r2), 16384), 1);
+    return add16(r, mul16_16_q15(r, mul16_16_q15(y, sub16(mul16_16_q15(y, 12288), 16384))));
+}
+
+// celt_sqrt (vendor/celt/mathops.c:119-140)
+CUO_FN int32_t celt_sqrt(int32_t x)
+{
+    if (x == 0) return 0;
+    if (x >= 1073741824) return 32767;
+    int k = (celt_ilog2(x) >> 1) - 7;
+    x = vshr32(x, 2 * k);
+    int16_t n = (int16_t)(x - 32768);
+    int32_t rt = add16(23175, mul16_16_q15(n, add16(11561, mul16_16_q15(n, add16(-3011,
+                 mul16_16_q15(n, add16(1699, mul16_16_q15(n, -664))))))));
+    return vshr32(rt, 7 - k);
+}
+
+// _celt_cos_pi_2 / celt_cos_norm (vendor/celt/mathops.c:142-179)
+CUO_FN int16_t cos_pi_2(int16_t x)
+{
+    int16_t x2 = (int16_t)mul16_16_p15(x, x);
+    int32_t v = sub16(32767, x2) + mul16_16_p15(x2, -7651 + mul16_16_p15(x2, 8277 + mul16_16_p15(-626, x2)));
+    return add16(1, min32(32766, v));
+}
+CUO_FN int16_t celt_cos_norm(int32_t x)
+{
+    x = x & 0x0001ffff;
+    if (x > (1 << 16)) x = (1 << 17) - x;
+    if (x & 0x00007fff) {
+        if (x < (1 << 15)) return cos_pi_2((int16_t)x);
+        return (int16_t)-cos_pi_2((int16_t)(65536 - x));
+    }
+    if (x & 0x0000ffff) return 0;
+    if (x & 0x0001ffff) return -32767;
+    return 32767;
+}
+
+// celt_exp2_frac / celt_exp2 / celt_log2 (vendor/celt/mathops.h, FIXED_POINT)
+CUO_FN int32_t celt_exp2_frac(int16_t x)
+{
+    int16_t f = shl16(x, 4);
+    return add16(16383, mul16_16_q15(f, add16(22804, mul16_16_q15(f, add16(14819, mul16_16_q15(10204, f))))));
+}
+CUO_FN int32_t celt_exp2(int16_t x)
+{
+    int integer = x >> 10;
+    if (integer > 14) return 0x7f000000;
+    if (integer < -15) return 0;
+    int16_t frac = (int16_t)celt_exp2_frac((int16_t)(x - shl16(integer, 10)));
+    return vshr32((int32_t)frac, -integer - 2);
+}
+CUO_FN int16_t celt_log2(int32_t x)
+{
+    if (x == 0) return -32767;
+    int i = celt_ilog2(x);
+    int16_t n = (int16_t)(vshr32(x, i - 15) - 32768 - 16384);
+    int16_t frac = add16(-6793, mul16_16_q15(n, add16(15746, mul16_16_q15(n, add16(-5217,
+                   mul16_16_q15(n, add16(2545, mul16_16_q15(n, -1401))))))));
+    return (int16_t)(shl16(i - 13, kDbShift) + (frac >> (14 - kDbShift)));
+}
+
+CUO_FN uint32_t lcg_rand(uint32_t seed) { return 1664525u * seed + 1013904223u; }
+
+// ---------------------------------------------------------------- SILK layer
+CUO_FN int32_t silk_smulwb(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)(int16_t)b) >> 16); }
+CUO_FN int32_t silk_smlawb(int32_t a, int32_t b, int32_t c) { return (int32_t)(a + (((int64_t)b * (int64_t)(int16_t)c) >> 16)); }
+CUO_FN int32_t silk_smulwt(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)(b >> 16)) >> 16); }
+CUO_FN int32_t silk_smlawt(int32_t a, int32_t b, int32_t c) { return (int32_t)(a + (((int64_t)b * (int64_t)(c >> 16)) >> 16)); }
+CUO_FN int32_t silk_smulbb(int32_t a, int32_t b) { return (int32_t)(int16_t)a * (int32_t)(int16_t)b; }
+CUO_FN int32_t silk_smlabb(int32_t a, int32_t b, int32_t c) { return a + (int32_t)(int16_t)b * (int32_t)(int16_t)c; }
+CUO_FN int32_t silk_smulbt(int32_t a, int32_t b) { return (int32_t)(int16_t)a * (b >> 16); }
+CUO_FN int32_t silk_smulww(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * b) >> 16); }
+CUO_FN int32_t silk_smlaww(int32_t a, int32_t b, int32_t c) { return (int32_t)(a + (((int64_t)b * c) >> 16)); }
+CUO_FN int64_t silk_smull(int32_t a, int32_t b) { return (int64_t)a * (int64_t)b; }
+CUO_FN int32_t silk_mul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+CUO_FN int32_t silk_mla(int32_t a, int32_t b, int32_t c) { return (int32_t)((uint32_t)a + (uint32_t)b * (uint32_t)c); }
+CUO_FN int32_t silk_rshift_round(int32_t a, int s) { return s == 1 ? (a >> 1) + (a & 1) : ((a >> (s - 1)) + 1) >> 1; }
+CUO_FN int64_t silk_rshift_round64(int64_t a, int s) { return s == 1 ? (a >> 1) + (a & 1) : ((a >> (s - 1)) + 1) >> 1; }
+CUO_FN int32_t silk_lshift(int32_t a, int s) { return (int32_t)((uint32_t)a << s); }
+CUO_FN int32_t silk_add_sat32(int32_t a, int32_t b)
+{
+    uint32_t s = (uint32_t)a + (uint32_t)b;
+    if ((s & 0x80000000u) == 0) return ((uint32_t)a & (uint32_t)b & 0x80000000u) ? INT32_MIN : (int32_t)s;
+    return (((uint32_t)a | (uint32_t)b) & 0x80000000u) == 0 ? INT32_MAX : (int32_t)s;
+}
+CUO_FN int32_t silk_sub_sat32(int32_t a, int32_t b)
+{
+    uint32_t s = (uint32_t)a - (uint32_t)b;
+    if ((s & 0x80000000u) == 0) return ((uint32_t)a & ((uint32_t)b ^ 0x80000000u) & 0x80000000u) ? INT32_MIN : (int32_t)s;
+    return (((uint32_t)a ^ 0x80000000u) & (uint32_t)b & 0x80000000u) ? INT32_MAX : (int32_t)s;
+}
+CUO_FN int32_t silk_sat16(int32_t a) { return a > 32767 ? 32767 : (a < -32768 ? -32768 : a); }
+CUO_FN int32_t silk_limit(int32_t a, int32_t l1, int32_t l2)
+{
+    return l1 > l2 ? (a > l1 ? l1 : (a < l2 ? l2 : a)) : (a > l2 ? l2 : (a < l1 ? l1 : a));
+}
+CUO_FN int32_t silk_lshift_sat32(int32_t a, int s)
+{
+    return silk_lshift(silk_limit(a, (int32_t)(INT32_MIN >> s), (int32_t)(INT32_MAX >> s)), s);
+}
+CUO_FN int32_t silk_clz32(int32_t a) { return a == 0 ? 32 : __builtin_clz((uint32_t)a); }
+CUO_FN int32_t silk_clz16(int16_t a) { return silk_clz32((int32_t)(uint16_t)a) - 16; }
+CUO_FN int32_t silk_ror32(int32_t a, int rot)
+{
+    uint32_t x = (uint32_t)a;
+    uint32_t r = (uint32_t)rot, m = (uint32_t)-rot;
+    if (rot == 0) return a;
+    if (rot < 0) return (int32_t)((x << m) | (x >> (32 - m)));
+    return (int32_t)((x << (32 - r)) | (x >> r));
+}
+CUO_FN void silk_clz_frac(int32_t in, int32_t *lz, int32_t *frac_Q7)
+{
+    int32_t l = silk_clz32(in);
+    *lz = l;
+    *frac_Q7 = silk_ror32(in, 24 - l) & 0x7f;
+}
+// silk_SQRT_APPROX (vendor/silk/Inlines.h)
+CUO_FN int32_t silk_sqrt_approx(int32_t x)
+{
+    if (x <= 0) return 0;
+    int32_t lz, frac_Q7;
+    silk_clz_frac(x, &lz, &frac_Q7);
+    int32_t y = (lz & 1) ? 32768 : 46214;
+    y >>= (lz >> 1);
+    y = silk_smlawb(y, y, silk_smulbb(213, frac_Q7));
+    return y;
+}
+// silk_DIV32_varQ (vendor/silk/Inlines.h)
+CUO_FN int32_t silk_div32_varq(int32_t a32, int32_t b32, int Qres)
+{
+    int a_headrm = silk_clz32(a32 < 0 ? -a32 : a32) - 1;
+    int32_t a32_nrm = silk_lshift(a32, a_headrm);
+    int b_headrm = silk_clz32(b32 < 0 ? -b32 : b32) - 1;
+    int32_t b32_nrm = silk_lshift(b32, b_headrm);
+    int32_t b32_inv = (INT32_MAX >> 2) / (b32_nrm >> 16);
+    int32_t result = silk_smulwb(a32_nrm, b32_inv);
+    a32_nrm = sub32_ovf(a32_nrm, (int32_t)((uint32_t)(int32_t)(silk_smull(b32_nrm, result) >> 32) << 3));
+    result = silk_smlawb(result, a32_nrm, b32_inv);
+    int lshift = 29 + a_headrm - b_headrm - Qres;
+    if (lshift < 0) return silk_lshift_sat32(result, -lshift);
+    if (lshift < 32) return result >> lshift;
+    return 0;
+}
+// silk_INVERSE32_varQ (vendor/silk/Inlines.h)
+CUO_FN int32_t silk_inverse32_varq(int32_t b32, int Qres)
+{
+    int b_headrm = silk_clz32(b32 < 0 ? -b32 : b32) - 1;
+    int32_t b32_nrm = silk_lshift(b32, b_headrm);
+    int32_t b32_inv = (INT32_MAX >> 2) / (b32_nrm >> 16);
+    int32_t result = silk_lshift(b32_inv, 16);
+    int32_t err_Q32 = silk_lshift((1 << 29) - silk_smulwb(b32_nrm, b32_inv), 3);
+    result = silk_smlaww(result, err_Q32, b32_inv);
+    int lshift = 61 - b_headrm - Qres;
+    if (lshift <= 0) return silk_lshift_sat32(result, -lshift);
+    if (lshift < 32) return result >> lshift;
+    return 0;
+}
+
+}  // namespace cuo

@@ -1,0 +1,472 @@
+// opus_dec.h -- Opus packet / frame layer over the SILK and CELT decoders.
+//
+// Restates libopus 1.5.2 (vendor/src/opus_decoder.c) for a mono 48 kHz
+// fixed-point decoder: opus_decoder_init (:129-174), smooth_fade (:205-220),
+// opus_decode_frame (:237-669: mode dispatch, celt_accum on the SILK PCM,
+// SILK/hybrid/CELT transitions with the 5 ms PLC crossfade, CELT redundancy in
+// both directions, rangeFinal = rng ^ redundant_rng), opus_decode_native
+// (:670-811: PLC chunking, the in-band FEC branch, per-frame loop) and the
+// packet parser (vendor/src/opus.c:153-354).
+//
+// libopus recurses into opus_decode_frame for concealment and transitions;
+// here those calls go to plc_frame(), a separate instantiation of the frame
+// decoder that has no data and therefore can never recurse again, so the
+// device call graph is acyclic and its stack is statically sized.
+#pragma once
+#include "celt.h"
+#include "silk.h"
+#include "celt_plc.h"
+
+namespace cuo {
+
+enum { MODE_SILK_ONLY = 1000, MODE_HYBRID = 1001, MODE_CELT_ONLY = 1002 };
+enum { BW_NB = 1101, BW_MB = 1102, BW_WB = 1103, BW_SWB = 1104, BW_FB = 1105 };
+enum { OPUS_OK = 0, OPUS_BAD_ARG = -1, OPUS_BUFFER_TOO_SMALL = -2, OPUS_INTERNAL_ERROR = -3,
+       OPUS_INVALID_PACKET = -4, OPUS_UNIMPLEMENTED = -5 };
+
+struct OpusDecState {
+    SilkDecoder silk;
+    CeltState celt;
+    SilkControl silk_ctl;
+    int32_t stream_channels;
+    int32_t bandwidth;
+    int32_t mode;
+    int32_t prev_mode;
+    int32_t frame_size;
+    int32_t prev_redundancy;
+    int32_t last_packet_duration;
+    uint32_t rangeFinal;
+};
+
+struct DecScratch {
+    CeltScratch celt;
+    SilkScratch silk;
+    int16_t pcm_silk[480];
+    int16_t pcm_transition[240];
+    int16_t redundant_audio[240];
+};
+
+CUO_FN void opus_dec_init(OpusDecState *st)
+{
+    silk_init_decoder(&st->silk);
+    celt_init(&st->celt);
+    st->silk_ctl.internalSampleRate = 0;
+    st->silk_ctl.payloadSize_ms = 0;
+    st->stream_channels = 1;
+    st->bandwidth = 0;
+    st->mode = 0;
+    st->prev_mode = 0;
+    st->frame_size = 48000 / 400;
+    st->prev_redundancy = 0;
+    st->last_packet_duration = 0;
+    st->rangeFinal = 0;
+}
+
+// ------------------------------------------------------------------ TOC helpers
+CUO_FN int toc_mode(uint8_t toc)
+{
+    if (toc & 0x80) return MODE_CELT_ONLY;
+    if ((toc & 0x60) == 0x60) return MODE_HYBRID;
+    return MODE_SILK_ONLY;
+}
+CUO_FN int toc_bandwidth(uint8_t toc)
+{
+    if (toc & 0x80) {
+        int bw = BW_MB + ((toc >> 5) & 0x3);
+        return bw == BW_MB ? BW_NB : bw;
+    }
+    if ((toc & 0x60) == 0x60) return (toc & 0x10) ? BW_FB : BW_SWB;
+    return BW_NB + ((toc >> 5) & 0x3);
+}
+CUO_FN int toc_samples_per_frame(uint8_t toc)
+{
+    int a;
+    if (toc & 0x80) { a = (toc >> 3) & 0x3; return (48000 << a) / 400; }
+    if ((toc & 0x60) == 0x60) return (toc & 0x08) ? 48000 / 50 : 48000 / 100;
+    a = (toc >> 3) & 0x3;
+    return a == 3 ? 48000 * 60 / 1000 : (48000 << a) / 100;
+}
+// opus_packet_get_nb_samples (opus_decoder.c:1121-1137); <= 0 on error
+CUO_FN int packet_nb_samples(const uint8_t *p, int len)
+{
+    if (len < 1) return OPUS_BAD_ARG;
+    int count = p[0] & 0x3;
+    if (count == 0) count = 1;
+    else if (count != 3) count = 2;
+    else if (len < 2) return OPUS_INVALID_PACKET;
+    else count = p[1] & 0x3F;
+    int samples = count * toc_samples_per_frame(p[0]);
+    if (samples * 25 > 48000 * 3) return OPUS_INVALID_PACKET;
+    return samples;
+}
+
+CUO_FN int parse_size(const uint8_t *data, int len, int16_t *size)
+{
+    if (len < 1) { *size = -1; return -1; }
+    if (data[0] < 252) { *size = data[0]; return 1; }
+    if (len < 2) { *size = -1; return -1; }
+    *size = (int16_t)(4 * data[1] + data[0]);
+    return 2;
+}
+
+// opus_packet_parse_impl, not self-delimited (opus.c:194-354).  Returns the
+// frame count and fills size[] and *payload_offset, or a negative error.
+CUO_FN int packet_parse(const uint8_t *data, int len, int16_t *size, int *payload_offset)
+{
+    if (len < 0) return OPUS_BAD_ARG;
+    if (len == 0) return OPUS_INVALID_PACKET;
+    const uint8_t *data0 = data;
+    int framesize = toc_samples_per_frame(data[0]);
+    uint8_t toc = *data++;
+    len--;
+    int32_t last_size = len;
+    int count, bytes;
+    switch (toc & 0x3) {
+    case 0:
+        count = 1;
+        break;
+    case 1:
+        count = 2;
+        if (len & 0x1) return OPUS_INVALID_PACKET;
+        last_size = len / 2;
+        size[0] = (int16_t)last_size;
+        break;
+    case 2:
+        count = 2;
+        bytes = parse_size(data, len, size);
+        len -= bytes;
+        if (size[0] < 0 || size[0] > len) return OPUS_INVALID_PACKET;
+        data += bytes;
+        last_size = len - size[0];
+        break;
+    default: {
+        if (len < 1) return OPUS_INVALID_PACKET;
+        uint8_t ch = *data++;
+        count = ch & 0x3F;
+        if (count <= 0 || framesize * count > 5760) return OPUS_INVALID_PACKET;
+        len--;
+        if (ch & 0x40) {
+            int p;
+            do {
+                if (len <= 0) return OPUS_INVALID_PACKET;
+                p = *data++;
+                len--;
+                int tmp = p == 255 ? 254 : p;
+                len -= tmp;
+            } while (p == 255);
+        }
+        if (len < 0) return OPUS_INVALID_PACKET;
+        int cbr = !(ch & 0x80);
+        if (!cbr) {
+            last_size = len;
+            for (int i = 0; i < count - 1; i++) {
+                bytes = parse_size(data, len, size + i);
+                len -= bytes;
+                if (size[i] < 0 || size[i] > len) return OPUS_INVALID_PACKET;
+                data += bytes;
+                last_size -= bytes + size[i];
+            }
+            if (last_size < 0) return OPUS_INVALID_PACKET;
+        } else {
+            last_size = len / count;
+            if (last_size * count != len) return OPUS_INVALID_PACKET;
+            for (int i = 0; i < count - 1; i++) size[i] = (int16_t)last_size;
+        }
+        break;
+    }
+    }
+    if (last_size > 1275) return OPUS_INVALID_PACKET;
+    size[count - 1] = (int16_t)last_size;
+    *payload_offset = (int)(data - data0);
+    return count;
+}
+
+// opus_packet_has_lbrr (opus_decoder.c:1139-1162), mono
+CUO_FN int packet_has_lbrr(const uint8_t *p, int len)
+{
+    if (toc_mode(p[0]) == MODE_CELT_ONLY) return 0;
+    int fs = toc_samples_per_frame(p[0]);
+    int nb_frames = fs > 960 ? fs / 960 : 1;
+    int16_t size[48];
+    int off;
+    int ret = packet_parse(p, len, size, &off);
+    if (ret <= 0) return ret;
+    int lbrr = (p[off] >> (7 - nb_frames)) & 0x1;
+    if (p[0] & 0x4) lbrr = lbrr || ((p[off] >> (6 - 2 * nb_frames)) & 0x1);
+    return lbrr;
+}
+
+CUO_FN void smooth_fade(const int16_t *in1, const int16_t *in2, int16_t *out, int overlap)
+{
+    for (int i = 0; i < overlap; i++) {
+        int16_t w = (int16_t)mul16_16_q15(tab::kWindow[i], tab::kWindow[i]);
+        out[i] = (int16_t)((mul16_16(w, in2[i]) + mul16_16(kQ15One - w, in1[i])) >> 15);
+    }
+}
+
+// ------------------------------------------------------------------ frame decode
+// Three instantiations: kFrameData decodes a frame (libopus' opus_decode_frame
+// with data), kFrameConceal conceals and may chunk a long gap into 20 ms
+// pieces, kFrameConcealLeaf conceals at most 20 ms.  Data frames reach
+// concealment only through the leaf (transition PLC is <= 5 ms), so the call
+// graph has no cycle.
+enum { kFrameData = 0, kFrameConceal = 1, kFrameConcealLeaf = 2 };
+template <int Kind>
+CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size,
+                        int decode_fec, DecScratch *sc);
+
+CUO_FN int plc_frame(OpusDecState *st, int16_t *pcm, int frame_size, DecScratch *sc)
+{
+    return frame_decode<kFrameConcealLeaf>(st, nullptr, 0, pcm, frame_size, 0, sc);
+}
+
+template <int Kind>
+CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size,
+                        int decode_fec, DecScratch *sc)
+{
+    constexpr bool HasData = Kind == kFrameData;
+    const int F20 = 960, F10 = 480, F5 = 240, F2_5 = 120;
+    RangeDec dec;
+    if (frame_size < F2_5) return OPUS_BUFFER_TOO_SMALL;
+    frame_size = imin(frame_size, 48000 / 25 * 3);
+    if constexpr (HasData) {
+        if (len <= 1)  // a payload of <= 1 byte is concealed (opus_decoder.c:285-290)
+            return frame_decode<kFrameConceal>(st, nullptr, 0, pcm, frame_size, 0, sc);
+    }
+    if (!HasData) {
+        data = nullptr;
+        frame_size = imin(frame_size, st->frame_size);
+    }
+    int audiosize, mode, bandwidth;
+    if (data != nullptr) {
+        audiosize = st->frame_size;
+        mode = st->mode;
+        bandwidth = st->bandwidth;
+        ec_init(&dec, data, (uint32_t)len);
+    } else {
+        audiosize = frame_size;
+        mode = st->prev_redundancy ? MODE_CELT_ONLY : st->prev_mode;
+        bandwidth = 0;
+        if (mode == 0) {
+            for (int i = 0; i < audiosize; i++) pcm[i] = 0;
+            return audiosize;
+        }
+        if constexpr (Kind == kFrameConceal) {
+            if (audiosize > F20) {
+                // chunk long concealment into <= 20 ms calls (opus_decoder.c:321-333)
+                do {
+                    int ret = frame_decode<kFrameConcealLeaf>(st, nullptr, 0, pcm, imin(audiosize, F20), 0, sc);
+                    if (ret < 0) return ret;
+                    pcm += ret;
+                    audiosize -= ret;
+                } while (audiosize > 0);
+                return frame_size;
+            }
+        }
+        if (audiosize < F20) {
+            if (audiosize > F10) audiosize = F10;
+            else if (mode != MODE_SILK_ONLY && audiosize > F5 && audiosize < F10) audiosize = F5;
+        }
+    }
+    const int celt_accum = (mode != MODE_CELT_ONLY) && (frame_size >= F10);
+    int transition = 0;
+    int16_t *pcm_transition = nullptr;
+    if (HasData && data != nullptr && st->prev_mode > 0 &&
+        ((mode == MODE_CELT_ONLY && st->prev_mode != MODE_CELT_ONLY && !st->prev_redundancy) ||
+         (mode != MODE_CELT_ONLY && st->prev_mode == MODE_CELT_ONLY)))
+        transition = 1;
+    if constexpr (HasData) {
+        if (transition && mode == MODE_CELT_ONLY) {
+            pcm_transition = sc->pcm_transition;
+            plc_frame(st, pcm_transition, imin(F5, audiosize), sc);
+        }
+    }
+    if (audiosize > frame_size) return OPUS_BAD_ARG;
+    frame_size = audiosize;
+
+    int16_t *pcm_silk = sc->pcm_silk;
+    if (mode != MODE_CELT_ONLY) {
+        int16_t *pcm_ptr = celt_accum ? pcm : pcm_silk;
+        if (st->prev_mode == MODE_CELT_ONLY) silk_reset_decoder(&st->silk);
+        st->silk_ctl.payloadSize_ms = imax(10, 1000 * audiosize / 48000);
+        if (data != nullptr) {
+            if (mode == MODE_SILK_ONLY) {
+                if (bandwidth == BW_NB) st->silk_ctl.internalSampleRate = 8000;
+                else if (bandwidth == BW_MB) st->silk_ctl.internalSampleRate = 12000;
+                else st->silk_ctl.internalSampleRate = 16000;
+            } else {
+                st->silk_ctl.internalSampleRate = 16000;
+            }
+        }
+        int lost_flag = data == nullptr ? 1 : 2 * !!decode_fec;
+        int decoded = 0;
+        do {
+            int32_t silk_frame_size;
+            int first_frame = decoded == 0;
+            int silk_ret = silk_decode(&st->silk, &st->silk_ctl, lost_flag, first_frame, &dec, pcm_ptr,
+                                       &silk_frame_size, &sc->silk);
+            if (silk_ret) {
+                if (lost_flag) {
+                    silk_frame_size = frame_size;
+                    for (int i = 0; i < frame_size; i++) pcm_ptr[i] = 0;
+                } else {
+                    return OPUS_INTERNAL_ERROR;
+                }
+            }
+            pcm_ptr += silk_frame_size;
+            decoded += silk_frame_size;
+        } while (decoded < frame_size);
+    }
+
+    int start_band = 0, redundancy = 0, redundancy_bytes = 0, celt_to_silk = 0;
+    if (HasData && !decode_fec && mode != MODE_CELT_ONLY && data != nullptr &&
+        ec_tell(&dec) + 17 + 20 * (mode == MODE_HYBRID) <= 8 * len) {
+        redundancy = mode == MODE_HYBRID ? ec_bit_logp(&dec, 12) : 1;
+        if (redundancy) {
+            celt_to_silk = ec_bit_logp(&dec, 1);
+            redundancy_bytes = mode == MODE_HYBRID ? (int)ec_uint(&dec, 256) + 2 : len - ((ec_tell(&dec) + 7) >> 3);
+            len -= redundancy_bytes;
+            if (len * 8 < ec_tell(&dec)) {
+                len = 0;
+                redundancy_bytes = 0;
+                redundancy = 0;
+            }
+            dec.storage -= (uint32_t)redundancy_bytes;
+        }
+    }
+    if (mode != MODE_CELT_ONLY) start_band = 17;
+    if (redundancy) transition = 0;
+    if constexpr (HasData) {
+        if (transition && mode != MODE_CELT_ONLY) {
+            pcm_transition = sc->pcm_transition;
+            plc_frame(st, pcm_transition, imin(F5, audiosize), sc);
+        }
+    }
+    if (bandwidth) {
+        int endband = 21;
+        switch (bandwidth) {
+        case BW_NB: endband = 13; break;
+        case BW_MB: case BW_WB: endband = 17; break;
+        case BW_SWB: endband = 19; break;
+        default: endband = 21; break;
+        }
+        st->celt.end = endband;
+    }
+    int16_t *redundant_audio = sc->redundant_audio;
+    uint32_t redundant_rng = 0;
+    if (HasData && redundancy && celt_to_silk) {
+        st->celt.start = 0;
+        celt_decode(&st->celt, data + len, redundancy_bytes, redundant_audio, F5, nullptr, 0, &sc->celt);
+        redundant_rng = st->celt.rng;
+    }
+    st->celt.start = start_band;
+    int celt_ret = 0;
+    if (mode != MODE_SILK_ONLY) {
+        int celt_frame_size = imin(F20, frame_size);
+        if (mode != st->prev_mode && st->prev_mode > 0 && !st->prev_redundancy) celt_reset(&st->celt);
+        celt_ret = celt_decode(&st->celt, decode_fec ? nullptr : data, len, pcm, celt_frame_size,
+                               data != nullptr ? &dec : nullptr, celt_accum, &sc->celt);
+    } else {
+        static constexpr uint8_t silence[2] = {0xFF, 0xFF};
+        if (!celt_accum)
+            for (int i = 0; i < frame_size; i++) pcm[i] = 0;
+        if (st->prev_mode == MODE_HYBRID && !(redundancy && celt_to_silk && st->prev_redundancy)) {
+            st->celt.start = 0;
+            celt_decode(&st->celt, silence, 2, pcm, F2_5, nullptr, celt_accum, &sc->celt);
+        }
+    }
+    if (mode != MODE_CELT_ONLY && !celt_accum)
+        for (int i = 0; i < frame_size; i++) pcm[i] = sat16(pcm[i] + pcm_silk[i]);
+    if (HasData && redundancy && !celt_to_silk) {
+        celt_reset(&st->celt);
+        st->celt.start = 0;
+        celt_decode(&st->celt, data + len, redundancy_bytes, redundant_audio, F5, nullptr, 0, &sc->celt);
+        redundant_rng = st->celt.rng;
+        smooth_fade(pcm + frame_size - F2_5, redundant_audio + F2_5, pcm + frame_size - F2_5, F2_5);
+    }
+    if (HasData && redundancy && celt_to_silk && (st->prev_mode != MODE_SILK_ONLY || st->prev_redundancy)) {
+        for (int i = 0; i < F2_5; i++) pcm[i] = redundant_audio[i];
+        smooth_fade(redundant_audio + F2_5, pcm + F2_5, pcm + F2_5, F2_5);
+    }
+    if (HasData && transition) {
+        if (audiosize >= F5) {
+            for (int i = 0; i < F2_5; i++) pcm[i] = pcm_transition[i];
+            smooth_fade(pcm_transition + F2_5, pcm + F2_5, pcm + F2_5, F2_5);
+        } else {
+            smooth_fade(pcm_transition, pcm, pcm, F2_5);
+        }
+    }
+    st->rangeFinal = len <= 1 ? 0 : (HasData ? dec.rng ^ redundant_rng : 0);
+    st->prev_mode = mode;
+    st->prev_redundancy = redundancy && !celt_to_silk;
+    return celt_ret < 0 ? celt_ret : audiosize;
+}
+
+// opus_decode_native's concealment branch (opus_decoder.c:713-729)
+CUO_FN int decode_plc(OpusDecState *st, int16_t *pcm, int frame_size, DecScratch *sc)
+{
+    if (frame_size % 120 != 0) return OPUS_BAD_ARG;
+    int count = 0;
+    do {
+        int ret = frame_decode<kFrameConceal>(st, nullptr, 0, pcm + count, frame_size - count, 0, sc);
+        if (ret < 0) return ret;
+        count += ret;
+    } while (count < frame_size);
+    st->last_packet_duration = count;
+    return count;
+}
+
+// opus_decode (opus_decoder.c:670-811) for int16 output.  data == nullptr or
+// len == 0 conceals frame_size samples.  Returns samples or a negative error.
+CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size,
+                              int decode_fec, DecScratch *sc)
+{
+    if (frame_size <= 0) return OPUS_BAD_ARG;
+    if (decode_fec < 0 || decode_fec > 1) return OPUS_BAD_ARG;
+    if ((decode_fec || len == 0 || data == nullptr) && frame_size % 120 != 0) return OPUS_BAD_ARG;
+    if (len == 0 || data == nullptr) return decode_plc(st, pcm, frame_size, sc);
+    if (len < 0) return OPUS_BAD_ARG;
+    int packet_mode = toc_mode(data[0]);
+    int packet_bandwidth = toc_bandwidth(data[0]);
+    int packet_frame_size = toc_samples_per_frame(data[0]);
+    int packet_stream_channels = (data[0] & 0x4) ? 2 : 1;
+    int16_t size[48];
+    int offset;
+    int count = packet_parse(data, len, size, &offset);
+    if (count < 0) return count;
+    if (packet_stream_channels != 1) return OPUS_UNIMPLEMENTED;  // mono streams only
+    data += offset;
+    if (decode_fec) {
+        if (frame_size < packet_frame_size || packet_mode == MODE_CELT_ONLY || st->mode == MODE_CELT_ONLY)
+            return decode_plc(st, pcm, frame_size, sc);
+        int duration_copy = st->last_packet_duration;
+        if (frame_size - packet_frame_size != 0) {
+            int ret = decode_plc(st, pcm, frame_size - packet_frame_size, sc);
+            if (ret < 0) { st->last_packet_duration = duration_copy; return ret; }
+        }
+        st->mode = packet_mode;
+        st->bandwidth = packet_bandwidth;
+        st->frame_size = packet_frame_size;
+        st->stream_channels = packet_stream_channels;
+        int ret = frame_decode<kFrameData>(st, data, size[0], pcm + (frame_size - packet_frame_size), packet_frame_size, 1, sc);
+        if (ret < 0) return ret;
+        st->last_packet_duration = frame_size;
+        return frame_size;
+    }
+    if (count * packet_frame_size > frame_size) return OPUS_BUFFER_TOO_SMALL;
+    st->mode = packet_mode;
+    st->bandwidth = packet_bandwidth;
+    st->frame_size = packet_frame_size;
+    st->stream_channels = packet_stream_channels;
+    int nb_samples = 0;
+    for (int i = 0; i < count; i++) {
+        int ret = frame_decode<kFrameData>(st, data, size[i], pcm + nb_samples, frame_size - nb_samples, 0, sc);
+        if (ret < 0) return ret;
+        data += size[i];
+        nb_samples += ret;
+    }
+    st->last_packet_duration = nb_samples;
+    return nb_samples;
+}
+
+}  // namespace cuo

@@ -1,0 +1,70 @@
+// hostgate.cpp -- the device decoder source compiled for the host CPU.
+//
+// Test infrastructure (the reference's "same-source host gate",
+// harness_host.cpp): the __host__ __device__ decoder in cuopus_amd/csrc/dec
+// runs here on the CPU so tests can localise a mismatch against the oracle to
+// the decoder logic rather than to GPU execution.  Never used by the product
+// path, which only runs the decoder on the GPU.
+#include <stdlib.h>
+#include <string.h>
+#include "../../cuopus_amd/csrc/dec/opus_dec.h"
+
+using namespace cuo;
+
+extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const uint32_t *len,
+                                const uint8_t *lost, int npk, int16_t *pcm, long cap,
+                                int32_t *nsamp, uint32_t *rng)
+{
+    OpusDecState *st = (OpusDecState *)malloc(sizeof(OpusDecState));
+    DecScratch *sc = (DecScratch *)calloc(1, sizeof(DecScratch));
+    opus_dec_init(st);
+    long total = 0;
+    long ret = 0;
+    for (int k = 0; k < npk; k++) {
+        const uint8_t *p = data + off[k];
+        int r;
+        if (cap - total < 5760) { ret = OPUS_BUFFER_TOO_SMALL; break; }
+        if (lost && lost[k]) {
+            int dur = packet_nb_samples(p, (int)len[k]);
+            if (dur <= 0) { ret = dur ? dur : OPUS_INVALID_PACKET; break; }
+            const uint8_t *q = k + 1 < npk ? data + off[k + 1] : nullptr;
+            if (q && !lost[k + 1] && packet_has_lbrr(q, (int)len[k + 1]) > 0)
+                r = opus_decode_packet(st, q, (int)len[k + 1], pcm + total, dur, 1, sc);
+            else
+                r = opus_decode_packet(st, nullptr, 0, pcm + total, dur, 0, sc);
+        } else {
+            r = opus_decode_packet(st, p, (int)len[k], pcm + total, 5760, 0, sc);
+        }
+        if (r < 0) { ret = r; break; }
+        if (nsamp) nsamp[k] = r;
+        if (rng) rng[k] = st->rangeFinal;
+        total += r;
+    }
+    free(st);
+    free(sc);
+    return ret < 0 ? ret : total;
+}
+
+extern "C" void hg_ec_schedule(const uint8_t *buf, uint32_t n, uint32_t *out)
+{
+    static const uint8_t icdf_tab[8] = {120, 112, 96, 64, 32, 16, 8, 0};
+    RangeDec d;
+    int k = 0;
+    ec_init(&d, buf, n);
+    for (int i = 0; i < 64; i++) out[k++] = (uint32_t)ec_bit_logp(&d, 1);
+    for (int i = 0; i < 32; i++) out[k++] = (uint32_t)ec_icdf(&d, icdf_tab, 7);
+    for (int i = 0; i < 32; i++) out[k++] = ec_uint(&d, 2000);
+    for (int i = 0; i < 16; i++) out[k++] = ec_bits(&d, 11);
+    out[k++] = (uint32_t)ec_tell(&d);
+    out[k++] = d.rng;
+    out[k++] = (uint32_t)d.error;
+}
+
+extern "C" int hg_sizes(int *out)
+{
+    out[0] = (int)sizeof(OpusDecState);
+    out[1] = (int)sizeof(DecScratch);
+    out[2] = (int)sizeof(CeltState);
+    out[3] = (int)sizeof(SilkDecoder);
+    return 4;
+}
