@@ -1,0 +1,67 @@
+"""Build steps: the HIP library (gfx950), the same-source host gate and the
+CPU oracle.
+
+* ``build_lib()``      hipcc -> cuopus_amd/libcuopus.so (the product).
+* ``build_hostgate()`` the device decoder source compiled for the host CPU
+                       (tests only) -> tests/hostgate/libhostgate.so.
+* ``build_oracle()``   make -C oracle (libopus 1.5.2 from the reference tree,
+                       tests / CPU baseline only) -> oracle/_ref/.
+"""
+import os
+import shutil
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "cuopus_amd")
+CSRC = os.path.join(PKG, "csrc")
+REFERENCE = "/root/reference"
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-Wno-unused-function",
+             # packed fp32 instructions stay off (hardware notes)
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("command failed (%d): %s\n%s" % (r.returncode, " ".join(cmd), r.stdout[-6000:]))
+    return r.stdout
+
+
+def build_lib(arch="gfx950"):
+    out = os.path.join(PKG, "libcuopus.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, "--offload-arch=" + arch] + HIP_FLAGS + [
+        "-shared", "-I" + CSRC,
+        os.path.join(CSRC, "libcuopus.hip"), os.path.join(CSRC, "host", "demux.cpp"),
+        "-o", tmp]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_hostgate():
+    src = os.path.join(ROOT, "tests", "hostgate", "hostgate.cpp")
+    out = os.path.join(ROOT, "tests", "hostgate", "libhostgate.so")
+    cmd = [HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-fPIC", "-shared",
+           "-Wno-unused-function", src, "-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_oracle(jobs=8):
+    """Compile the oracle when the reference tree is present; otherwise keep
+    whatever prebuilt oracle/_ref is there (the GPU box has no reference)."""
+    opus = os.path.join(REFERENCE, "vendor", "opus-1.5.2")
+    if not os.path.isdir(opus):
+        return None
+    _run(["make", "-C", os.path.join(ROOT, "oracle"), "-j%d" % jobs, "OPUS=" + opus])
+    return os.path.join(ROOT, "oracle", "_ref")
+
+
+def build_all():
+    build_oracle()
+    build_lib()
+    build_hostgate()

@@ -1,0 +1,179 @@
+// kernels.h -- device kernels of the batched decoder.
+//
+// k_decode_chains: one wavefront decodes one chain.  The 64 lanes run the
+//   serial Opus decoder in lockstep on identical data (wave-uniform state, so
+//   scalar values live in SGPRs), then split the fused epilogue: each decoded
+//   packet's samples that fall inside the chain's keep window are copied by
+//   all 64 lanes in parallel straight into the compacted 48 kHz output, so
+//   there is no separate compaction pass (cuopus_kernels.cuh:386-433 fused).
+//   Each lane only reads back PCM it wrote itself, so no barrier is needed.
+//   Chains are taken from a device work queue (an atomic counter) in the
+//   persistent variant, or one per workgroup in the plain variant.
+// k_resample_48to16: 49-tap Kaiser polyphase decimator, f32 out
+//   (cuopus_kernels.cuh:441-456; gpu_resample.py:20-49).  A workgroup stages
+//   its int16 input tile plus the 24-sample halo on each side in LDS once and
+//   every output reads its taps from LDS.
+#pragma once
+#include "dec/opus_dec.h"
+
+namespace cuo {
+
+constexpr int kMaxPktSamples = 5760;
+constexpr int kRsTaps = 49;
+constexpr int kRsPad = 24;
+
+struct ChainDesc {
+    uint32_t first;   // first packet (global index)
+    uint32_t npkts;
+    uint32_t trim;    // samples of the chain's output dropped before the keep window
+    uint32_t keep;    // samples kept
+    uint64_t dst;     // offset of the keep window in the compacted output
+};
+
+struct ChainResult {
+    int32_t stop;      // 0 or the first negative decoder return
+    uint32_t samples;  // samples produced by the chain's packets
+};
+
+// Per-slot decoder memory: persistent state, scratch and one packet of PCM.
+struct alignas(256) ChainSlot {
+    OpusDecState st;
+    DecScratch sc;
+    int16_t pcm[kMaxPktSamples];
+};
+
+struct DecodeArgs {
+    const uint8_t *data;
+    const uint32_t *off;
+    const uint32_t *len;
+    const uint8_t *lost;        // nullable: 1 = packet lost
+    const ChainDesc *chains;
+    const uint32_t *order;      // nullable: work order (chain indices)
+    uint32_t n_chains;
+    ChainSlot *slots;
+    uint32_t n_slots;
+    int16_t *out;               // compacted 48 kHz PCM
+    ChainResult *res;
+    int32_t *pkt_nsamp;         // nullable, per packet
+    uint32_t *pkt_rng;          // nullable, per packet
+    uint32_t *queue;            // work-queue counter (persistent variant)
+};
+
+// Decode one chain on the calling wavefront.
+__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, ChainSlot *slot, int lane)
+{
+    const ChainDesc ch = a.chains[c];
+    OpusDecState *st = &slot->st;
+    DecScratch *sc = &slot->sc;
+    int16_t *pcm = slot->pcm;
+    opus_dec_init(st);
+    uint64_t pos = 0;
+    int32_t stop = 0;
+    const uint64_t win_lo = ch.trim, win_hi = (uint64_t)ch.trim + ch.keep;
+    for (uint32_t k = 0; k < ch.npkts; k++) {
+        const uint32_t p = ch.first + k;
+        const uint8_t *pk = a.data + a.off[p];
+        int r;
+        if (a.lost && a.lost[p]) {
+            // loss rule: FEC from the next packet of the chain when it carries
+            // LBRR, else concealment of this packet's duration
+            int dur = packet_nb_samples(pk, (int)a.len[p]);
+            if (dur <= 0) { stop = dur ? dur : OPUS_INVALID_PACKET; break; }
+            bool fec = false;
+            if (k + 1 < ch.npkts && !a.lost[p + 1])
+                fec = packet_has_lbrr(a.data + a.off[p + 1], (int)a.len[p + 1]) > 0;
+            if (fec) r = opus_decode_packet(st, a.data + a.off[p + 1], (int)a.len[p + 1], pcm, dur, 1, sc);
+            else r = opus_decode_packet(st, nullptr, 0, pcm, dur, 0, sc);
+        } else {
+            r = opus_decode_packet(st, pk, (int)a.len[p], pcm, kMaxPktSamples, 0, sc);
+        }
+        if (r < 0) { stop = r; break; }
+        if (a.pkt_nsamp && lane == 0) {
+            a.pkt_nsamp[p] = r;
+            a.pkt_rng[p] = st->rangeFinal;
+        }
+        uint64_t lo = pos > win_lo ? pos : win_lo;
+        uint64_t hi = pos + (uint64_t)r < win_hi ? pos + (uint64_t)r : win_hi;
+        for (uint64_t i = lo + (uint64_t)lane; i < hi; i += 64) a.out[ch.dst + (i - win_lo)] = pcm[i - pos];
+        pos += (uint64_t)r;
+    }
+    if (lane == 0) {
+        a.res[c].stop = stop;
+        a.res[c].samples = (uint32_t)pos;
+    }
+}
+
+// Plain variant: workgroup g decodes chain order[g] in slot g.
+__global__ __launch_bounds__(64) void k_decode_chains(DecodeArgs a)
+{
+    const uint32_t g = blockIdx.x;
+    if (g >= a.n_chains) return;
+    const uint32_t c = a.order ? a.order[g] : g;
+    decode_one_chain(a, c, &a.slots[g], (int)threadIdx.x);
+}
+
+// Persistent variant: n_slots wavefronts pull chains from an atomic queue until
+// it is drained (every wavefront exits once the counter passes n_chains).
+__global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
+{
+    const uint32_t g = blockIdx.x;
+    ChainSlot *slot = &a.slots[g];
+    const int lane = (int)threadIdx.x;
+    while (true) {
+        uint32_t w = 0;
+        if (lane == 0) w = atomicAdd(a.queue, 1u);
+        w = __shfl(w, 0, 64);
+        if (w >= a.n_chains) break;
+        const uint32_t c = a.order ? a.order[w] : w;
+        decode_one_chain(a, c, slot, lane);
+    }
+}
+
+struct RsTaps {
+    float t[kRsTaps];
+};
+
+struct RsFile {
+    uint64_t in_off;   // offset of the file's compacted 48 kHz PCM
+    uint64_t in_len;   // playable samples of the file
+    uint64_t out_len;  // outputs of the file
+    float *out;        // output pointer (device)
+    uint64_t tile0;    // first global tile index of this file
+};
+
+constexpr int kRsTile = 256;  // outputs per workgroup
+
+// Batched resampler: workgroup b handles tile b of the concatenated per-file
+// tile list; out[i] = sum_t taps[t] * x[3 i - 24 + t] / 32768 with zeros
+// outside [0, in_len), accumulated in tap order.
+__global__ __launch_bounds__(kRsTile) void k_resample_48to16(const int16_t *__restrict__ in, const RsFile *__restrict__ files,
+                                                               int n_files, RsTaps taps)
+{
+    __shared__ float xs[3 * kRsTile + 2 * kRsPad + 4];
+    const uint64_t b = blockIdx.x;
+    // find the file of this tile (files are few per launch; binary search)
+    int lo = 0, hi = n_files - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (files[mid].tile0 <= b) lo = mid; else hi = mid - 1;
+    }
+    const RsFile f = files[lo];
+    const uint64_t o0 = (b - f.tile0) * kRsTile;
+    const int64_t base = (int64_t)(3 * o0) - kRsPad;
+    const int nin = 3 * kRsTile + 2 * kRsPad;
+    const int16_t *x = in + f.in_off;
+    for (int i = threadIdx.x; i < nin; i += kRsTile) {
+        int64_t gi = base + i;
+        xs[i] = (gi >= 0 && gi < (int64_t)f.in_len) ? (float)x[gi] * (1.0f / 32768.0f) : 0.f;
+    }
+    __syncthreads();
+    const uint64_t o = o0 + threadIdx.x;
+    if (o >= f.out_len) return;
+    const float *w = &xs[3 * threadIdx.x];
+    float acc = 0.f;
+#pragma unroll
+    for (int t = 0; t < kRsTaps; t++) acc += taps.t[t] * w[t];
+    f.out[o] = acc;
+}
+
+}  // namespace cuo

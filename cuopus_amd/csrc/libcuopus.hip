@@ -1,0 +1,635 @@
+// libcuopus.hip -- host runtime + C ABI of the MI355X batch Opus decoder.
+//
+// Drop-in for the reference's libcuopus.cu (same entry points and contract,
+// see include/cuopus.h).  Per call: demux every file on the host
+// (host/demux.cpp), merge all chains into one table, upload packets + tables,
+// run the decode kernel (one wavefront per chain, keep-window compaction fused
+// into its epilogue), run one batched resampler launch for every file, copy
+// the per-chain results back, synchronise, and validate every chain (stop
+// code 0 and decoded samples == the chain's TOC capacity) before reporting
+// success (libcuopus.cu:436-452).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/cuopus.h"
+#include "host/demux.h"
+#include "kernels.h"
+
+using namespace cuo;
+using cuo_host::Chain;
+using cuo_host::Demuxed;
+
+namespace {
+
+struct Err {
+    char *buf;
+    size_t cap;
+    char local[8];
+    Err(char *b, size_t c) : buf(b), cap(c)
+    {
+        if (!buf || !cap) { buf = local; cap = sizeof(local); }
+        buf[0] = 0;
+    }
+    int set(int rc, const char *fmt, ...) __attribute__((format(printf, 3, 4)))
+    {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, cap, fmt, ap);
+        va_end(ap);
+        return rc;
+    }
+};
+
+#define HIPCHK(e, call)                                                                                 \
+    do {                                                                                                \
+        hipError_t _s = (call);                                                                         \
+        if (_s != hipSuccess)                                                                           \
+            return (e).set(CUOPUS_RC_GPU, "HIP error %s at %s:%d: %s", #call, __FILE__, __LINE__,       \
+                           hipGetErrorString(_s));                                                      \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+long long g_dev_bytes = 0;
+
+int grow(DevBuf &b, size_t need, Err &e)
+{
+    if (need == 0) need = 1;
+    if (need <= b.cap) return 0;
+    if (b.p) {
+        hipFree(b.p);
+        g_dev_bytes -= (long long)b.cap;
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    hipError_t s = hipMalloc(&b.p, need);
+    if (s != hipSuccess) {
+        b.p = nullptr;
+        return e.set(CUOPUS_RC_GPU, "hipMalloc %zu bytes: %s", need, hipGetErrorString(s));
+    }
+    b.cap = need;
+    g_dev_bytes += (long long)need;
+    return 0;
+}
+
+struct Workspaces {
+    DevBuf data, off, len, lost, chains, order, slots, cat, res, nsamp, rng, files, queue;
+} W;
+
+double env_f(const char *name, double dflt)
+{
+    const char *v = getenv(name);
+    return (v && *v) ? atof(v) : dflt;
+}
+unsigned env_u(const char *name, unsigned dflt)
+{
+    const char *v = getenv(name);
+    return (v && atoi(v) > 0) ? (unsigned)atoi(v) : dflt;
+}
+
+// Kaiser I0 by its power series, in double.
+double bessel_i0(double x)
+{
+    double s = 1.0, t = 1.0, q = x * x / 4.0;
+    for (int k = 1; k < 64; k++) {
+        t *= q / ((double)k * k);
+        s += t;
+        if (t < 1e-18 * s) break;
+    }
+    return s;
+}
+
+// 49 taps: sinc at cutoff 1/3 x Kaiser(8.6), unity DC, double -> float
+// (cuopus_kernels.cuh:481-494, gpu_resample.py:23-31).
+RsTaps make_taps()
+{
+    RsTaps t;
+    double h[kRsTaps], sum = 0.0;
+    const double fc = 1.0 / 3.0;
+    for (int i = 0; i < kRsTaps; i++) {
+        double n = i - (kRsTaps - 1) / 2.0;
+        double s = (n == 0.0) ? fc : sin(M_PI * fc * n) / (M_PI * n);
+        double r = 2.0 * i / (kRsTaps - 1) - 1.0;
+        double w = bessel_i0(8.6 * sqrt(1.0 - r * r)) / bessel_i0(8.6);
+        h[i] = s * w;
+        sum += h[i];
+    }
+    for (int i = 0; i < kRsTaps; i++) t.t[i] = (float)(h[i] / sum);
+    return t;
+}
+
+// ---------------------------------------------------------------- planning
+struct FilePlan {
+    Demuxed d;
+    std::vector<Chain> chains;
+    uint64_t chain0 = 0;     // first merged chain index
+    uint64_t keep = 0;       // playable samples
+    uint64_t cat_off = 0;    // offset of the file in the compacted output
+    int64_t out16 = 0;
+};
+
+struct Plan {
+    std::vector<FilePlan> files;
+    std::vector<uint8_t> data;
+    std::vector<uint32_t> off, len;
+    std::vector<uint8_t> lost;
+    std::vector<ChainDesc> chains;
+    std::vector<uint64_t> cap;   // per chain: sum of TOC durations
+    uint64_t total_keep = 0;
+};
+
+int demux_file(const uint8_t *ogg, size_t n, FilePlan *fp, bool with_chains, Err &e)
+{
+    std::string msg;
+    if (!ogg || !n) return e.set(CUOPUS_RC_ARG, "empty input");
+    if (!cuo_host::demux(ogg, n, &fp->d, &msg)) return e.set(CUOPUS_RC_SOFT, "%s", msg.c_str());
+    if (fp->d.channels != 1) return e.set(CUOPUS_RC_SOFT, "only mono supported (channels=%d)", fp->d.channels);
+    if (fp->d.granule_match != fp->d.granule_checked)
+        return e.set(CUOPUS_RC_SOFT, "granule cross-check failed (%u/%u)", fp->d.granule_match, fp->d.granule_checked);
+    if (with_chains) {
+        double seg_s = env_f("CUOPUS_SEG_S", 1.0);
+        double preroll_ms = env_f("CUOPUS_PREROLL_MS", 250.0);
+        if (!cuo_host::build_chains(fp->d, seg_s, preroll_ms, &fp->chains, &msg))
+            return e.set(CUOPUS_RC_SOFT, "%s", msg.c_str());
+    }
+    fp->keep = fp->d.pcm_samples_48k > 0 ? (uint64_t)fp->d.pcm_samples_48k : 0;
+    fp->out16 = cuo_host::out16_len(fp->d.pcm_samples_48k);
+    return 0;
+}
+
+// Merge the demuxed files into one packet table and one chain table.
+int merge(Plan &pl, Err &e)
+{
+    uint64_t nbytes = 0, npk = 0, nch = 0;
+    for (auto &f : pl.files) {
+        nbytes += f.d.data.size();
+        npk += f.d.off.size();
+        nch += f.chains.size();
+    }
+    if (nbytes >= (1ull << 32)) return e.set(CUOPUS_RC_ARG, "merged packet data exceeds 4 GiB; split the call");
+    pl.data.resize(nbytes ? nbytes : 1);
+    pl.off.resize(npk);
+    pl.len.resize(npk);
+    pl.chains.resize(nch);
+    pl.cap.resize(nch);
+    uint64_t db = 0, pb = 0, cb = 0, cat = 0;
+    for (auto &f : pl.files) {
+        memcpy(pl.data.data() + db, f.d.data.data(), f.d.data.size());
+        for (size_t k = 0; k < f.d.off.size(); k++) {
+            pl.off[pb + k] = f.d.off[k] + (uint32_t)db;
+            pl.len[pb + k] = f.d.len[k];
+        }
+        f.chain0 = cb;
+        f.cat_off = cat;
+        uint64_t fk = 0;
+        for (size_t i = 0; i < f.chains.size(); i++) {
+            const Chain &c = f.chains[i];
+            if ((uint64_t)c.first + c.npkts > f.d.off.size())
+                return e.set(CUOPUS_RC_SOFT, "chain %zu out of packet range", i);
+            uint64_t s = 0;
+            for (uint32_t k = 0; k < c.npkts; k++) {
+                uint32_t p = c.first + k;
+                s += (uint64_t)cuo_host::pkt_samples48(f.d.data.data() + f.d.off[p], f.d.len[p]);
+            }
+            if (s < (uint64_t)c.trim + c.keep)
+                return e.set(CUOPUS_RC_SOFT, "chain %zu capacity %llu < trim %u + keep %u", i,
+                             (unsigned long long)s, c.trim, c.keep);
+            ChainDesc &cd = pl.chains[cb + i];
+            cd.first = c.first + (uint32_t)pb;
+            cd.npkts = c.npkts;
+            cd.trim = c.trim;
+            cd.keep = c.keep;
+            cd.dst = cat + fk;
+            pl.cap[cb + i] = s;
+            fk += c.keep;
+        }
+        if (fk != f.keep) return e.set(CUOPUS_RC_SOFT, "internal: keep mismatch");
+        cat += fk;
+        db += f.d.data.size();
+        pb += f.d.off.size();
+        cb += f.chains.size();
+    }
+    pl.total_keep = cat;
+    return 0;
+}
+
+int refuse_capture(hipStream_t st, Err &e)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipError_t s = hipStreamIsCapturing(st, &cs);
+    if (s == hipSuccess && cs != hipStreamCaptureStatusNone)
+        return e.set(CUOPUS_RC_ARG, "refusing to decode during HIP stream capture");
+    (void)hipGetLastError();
+    return 0;
+}
+
+// Longest-first work order by estimated cost (packets x frame cost).
+void lpt_order(const Plan &pl, std::vector<uint32_t> *order)
+{
+    const size_t n = pl.chains.size();
+    order->resize(n);
+    std::vector<std::pair<uint64_t, uint32_t>> key(n);
+    for (size_t i = 0; i < n; i++) key[i] = {pl.cap[i] + 960ull * pl.chains[i].npkts, (uint32_t)i};
+    std::stable_sort(key.begin(), key.end(), [](const std::pair<uint64_t, uint32_t> &a,
+                                                const std::pair<uint64_t, uint32_t> &b) { return a.first > b.first; });
+    for (size_t i = 0; i < n; i++) (*order)[i] = key[i].second;
+}
+
+struct RunOpts {
+    int16_t *s16_out = nullptr;          // decode straight into this (else workspace)
+    std::vector<float *> f16_out;        // per file resample targets (empty: no resample)
+    int32_t *pkt_nsamp = nullptr;        // device, per packet (gate)
+    uint32_t *pkt_rng = nullptr;
+    bool persistent = false;
+};
+
+// Upload, decode, resample, read back and validate.  results receives the
+// per-chain results (host).
+int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &results, Err &e)
+{
+    const uint32_t nch = (uint32_t)pl.chains.size();
+    const uint64_t npk = pl.off.size();
+    int rc;
+    if ((rc = grow(W.data, pl.data.size(), e))) return rc;
+    if ((rc = grow(W.off, 4 * npk, e))) return rc;
+    if ((rc = grow(W.len, 4 * npk, e))) return rc;
+    if ((rc = grow(W.chains, sizeof(ChainDesc) * nch, e))) return rc;
+    if ((rc = grow(W.order, 4ull * nch, e))) return rc;
+    if ((rc = grow(W.res, sizeof(ChainResult) * nch, e))) return rc;
+    if (!pl.lost.empty() && (rc = grow(W.lost, pl.lost.size(), e))) return rc;
+    int16_t *out = o.s16_out;
+    if (!out) {
+        if ((rc = grow(W.cat, 2 * (pl.total_keep ? pl.total_keep : 1), e))) return rc;
+        out = (int16_t *)W.cat.p;
+    }
+    unsigned max_batch = env_u("CUOPUS_MAX_BATCH", 16384);
+    uint32_t n_slots = o.persistent ? env_u("CUOPUS_SLOTS", 4096) : (nch < max_batch ? nch : max_batch);
+    if (n_slots > nch) n_slots = nch;
+    if (n_slots == 0) n_slots = 1;
+    if ((rc = grow(W.slots, sizeof(ChainSlot) * (size_t)n_slots, e))) return rc;
+    if ((rc = grow(W.queue, 256, e))) return rc;
+
+    std::vector<uint32_t> order;
+    lpt_order(pl, &order);
+    HIPCHK(e, hipMemcpyAsync(W.data.p, pl.data.data(), pl.data.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(e, hipMemcpyAsync(W.off.p, pl.off.data(), 4 * npk, hipMemcpyHostToDevice, st));
+    HIPCHK(e, hipMemcpyAsync(W.len.p, pl.len.data(), 4 * npk, hipMemcpyHostToDevice, st));
+    HIPCHK(e, hipMemcpyAsync(W.chains.p, pl.chains.data(), sizeof(ChainDesc) * nch, hipMemcpyHostToDevice, st));
+    HIPCHK(e, hipMemcpyAsync(W.order.p, order.data(), 4ull * nch, hipMemcpyHostToDevice, st));
+    if (!pl.lost.empty())
+        HIPCHK(e, hipMemcpyAsync(W.lost.p, pl.lost.data(), pl.lost.size(), hipMemcpyHostToDevice, st));
+
+    DecodeArgs a;
+    a.data = (const uint8_t *)W.data.p;
+    a.off = (const uint32_t *)W.off.p;
+    a.len = (const uint32_t *)W.len.p;
+    a.lost = pl.lost.empty() ? nullptr : (const uint8_t *)W.lost.p;
+    a.chains = (const ChainDesc *)W.chains.p;
+    a.slots = (ChainSlot *)W.slots.p;
+    a.n_slots = n_slots;
+    a.out = out;
+    a.res = (ChainResult *)W.res.p;
+    a.pkt_nsamp = o.pkt_nsamp;
+    a.pkt_rng = o.pkt_rng;
+    a.queue = (uint32_t *)W.queue.p;
+    if (o.persistent) {
+        HIPCHK(e, hipMemsetAsync(W.queue.p, 0, 4, st));
+        a.order = (const uint32_t *)W.order.p;
+        a.n_chains = nch;
+        hipLaunchKernelGGL(k_decode_persistent, dim3(n_slots), dim3(64), 0, st, a);
+        HIPCHK(e, hipGetLastError());
+    } else {
+        for (uint32_t c0 = 0; c0 < nch; c0 += n_slots) {
+            uint32_t bn = nch - c0 < n_slots ? nch - c0 : n_slots;
+            a.order = (const uint32_t *)W.order.p + c0;
+            a.n_chains = bn;
+            hipLaunchKernelGGL(k_decode_chains, dim3(bn), dim3(64), 0, st, a);
+            HIPCHK(e, hipGetLastError());
+        }
+    }
+    // one resampler launch over every file's tiles
+    if (!o.f16_out.empty()) {
+        std::vector<RsFile> rf;
+        uint64_t tiles = 0;
+        for (size_t f = 0; f < pl.files.size(); f++) {
+            if (pl.files[f].out16 <= 0) continue;
+            RsFile r;
+            r.in_off = pl.files[f].cat_off;
+            r.in_len = pl.files[f].keep;
+            r.out_len = (uint64_t)pl.files[f].out16;
+            r.out = o.f16_out[f];
+            r.tile0 = tiles;
+            tiles += (r.out_len + kRsTile - 1) / kRsTile;
+            rf.push_back(r);
+        }
+        if (!rf.empty()) {
+            if ((rc = grow(W.files, sizeof(RsFile) * rf.size(), e))) return rc;
+            HIPCHK(e, hipMemcpyAsync(W.files.p, rf.data(), sizeof(RsFile) * rf.size(), hipMemcpyHostToDevice, st));
+            static RsTaps taps = make_taps();
+            hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)tiles), dim3(kRsTile), 0, st, out,
+                               (const RsFile *)W.files.p, (int)rf.size(), taps);
+            HIPCHK(e, hipGetLastError());
+        }
+    }
+    results.resize(nch);
+    HIPCHK(e, hipMemcpyAsync(results.data(), W.res.p, sizeof(ChainResult) * nch, hipMemcpyDeviceToHost, st));
+    HIPCHK(e, hipStreamSynchronize(st));
+    for (uint32_t c = 0; c < nch; c++) {
+        if (results[c].stop != 0) return e.set(CUOPUS_RC_SOFT, "chain %u: device decode error %d", c, results[c].stop);
+        if (results[c].samples != pl.cap[c])
+            return e.set(CUOPUS_RC_SOFT, "chain %u: decoded %u samples, expected %llu", c, results[c].samples,
+                         (unsigned long long)pl.cap[c]);
+    }
+    return 0;
+}
+
+int decode_files(const uint8_t *const *oggs, const size_t *ns, int nfiles, const uint8_t *lost, size_t mask_n,
+                 RunOpts &o, hipStream_t st, Plan &pl, Err &e)
+{
+    int rc = refuse_capture(st, e);
+    if (rc) return rc;
+    pl.files.resize((size_t)nfiles);
+    for (int f = 0; f < nfiles; f++) {
+        char sub[256];
+        Err fe(sub, sizeof(sub));
+        if ((rc = demux_file(oggs[f], ns[f], &pl.files[f], true, fe)))
+            return nfiles > 1 ? e.set(rc, "file %d: %s", f, sub) : e.set(rc, "%s", sub);
+    }
+    if ((rc = merge(pl, e))) return rc;
+    if (lost) {
+        if (mask_n != pl.off.size())
+            return e.set(CUOPUS_RC_ARG, "loss mask has %zu entries, stream has %zu packets", mask_n, pl.off.size());
+        pl.lost.assign(lost, lost + mask_n);
+    }
+    if (pl.chains.empty()) return 0;
+    std::vector<ChainResult> res;
+    return run_plan(pl, st, o, res, e);
+}
+
+int last_ovf = 0;
+
+}  // namespace
+
+// =====================================================================================
+extern "C" int cuopus_last_arena_overflow(void) { return last_ovf; }
+
+extern "C" long long cuopus_workspace_bytes(void) { return g_dev_bytes; }
+
+extern "C" void cuopus_release_workspaces(void)
+{
+    DevBuf *all[] = {&W.data, &W.off, &W.len, &W.lost, &W.chains, &W.order, &W.slots,
+                     &W.cat, &W.res, &W.nsamp, &W.rng, &W.files, &W.queue};
+    for (DevBuf *b : all) {
+        if (b->p) hipFree(b->p);
+        b->p = nullptr;
+        b->cap = 0;
+    }
+    g_dev_bytes = 0;
+}
+
+extern "C" int cuopus_probe(const uint8_t *ogg, size_t n, int64_t *samples16k, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!samples16k) return e.set(CUOPUS_RC_ARG, "samples16k is NULL");
+    FilePlan fp;
+    int rc = demux_file(ogg, n, &fp, false, e);
+    if (rc) return rc;
+    *samples16k = fp.out16;
+    return 0;
+}
+
+extern "C" int cuopus_probe_packets(const uint8_t *ogg, size_t n, int64_t *n_packets, int64_t *n_chains, char *errbuf,
+                                    size_t errcap)
+{
+    Err e(errbuf, errcap);
+    FilePlan fp;
+    int rc = demux_file(ogg, n, &fp, true, e);
+    if (rc) return rc;
+    if (n_packets) *n_packets = (int64_t)fp.d.off.size();
+    if (n_chains) *n_chains = (int64_t)fp.chains.size();
+    return 0;
+}
+
+static int f16k_common(const uint8_t *ogg, size_t n, const uint8_t *lost, size_t mask_n, float *out_dev,
+                       size_t out_cap, int64_t *out_len, void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    last_ovf = 0;
+    if (out_len) *out_len = 0;
+    hipStream_t st = (hipStream_t)stream;
+    // size check before any GPU work
+    FilePlan probe;
+    int rc = demux_file(ogg, n, &probe, false, e);
+    if (rc) return rc;
+    if ((uint64_t)probe.out16 > out_cap)
+        return e.set(CUOPUS_RC_ARG, "output capacity %zu < required %lld samples", out_cap, (long long)probe.out16);
+    if (probe.out16 > 0 && !out_dev) return e.set(CUOPUS_RC_ARG, "out_dev is NULL");
+    if (probe.out16 == 0) {
+        if ((rc = refuse_capture(st, e))) return rc;
+        return 0;
+    }
+    Plan pl;
+    RunOpts o;
+    o.f16_out.push_back(out_dev);
+    o.persistent = env_u("CUOPUS_PERSISTENT", 0) != 0;
+    rc = decode_files(&ogg, &n, 1, lost, mask_n, o, st, pl, e);
+    if (rc) return rc;
+    if (out_len) *out_len = probe.out16;
+    return 0;
+}
+
+extern "C" int cuopus_decode_f16k_on(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap, int64_t *out_len,
+                                     void *stream, char *errbuf, size_t errcap)
+{
+    return f16k_common(ogg, n, nullptr, 0, out_dev, out_cap, out_len, stream, errbuf, errcap);
+}
+
+extern "C" int cuopus_decode_f16k(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap, int64_t *out_len,
+                                  char *errbuf, size_t errcap)
+{
+    return cuopus_decode_f16k_on(ogg, n, out_dev, out_cap, out_len, nullptr, errbuf, errcap);
+}
+
+extern "C" int cuopus_decode_lossy_f16k_on(const uint8_t *ogg, size_t n, const uint8_t *lost, size_t mask_n,
+                                           float *out_dev, size_t out_cap, int64_t *out_len, void *stream,
+                                           char *errbuf, size_t errcap)
+{
+    if (!lost) {
+        Err e(errbuf, errcap);
+        return e.set(CUOPUS_RC_ARG, "lost mask is NULL");
+    }
+    return f16k_common(ogg, n, lost, mask_n, out_dev, out_cap, out_len, stream, errbuf, errcap);
+}
+
+extern "C" int cuopus_decode_s16_on(const uint8_t *ogg, size_t n, int16_t *out_dev, size_t out_cap, int64_t *out_len,
+                                    void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    last_ovf = 0;
+    if (out_len) *out_len = 0;
+    hipStream_t st = (hipStream_t)stream;
+    FilePlan probe;
+    int rc = demux_file(ogg, n, &probe, false, e);
+    if (rc) return rc;
+    if (probe.keep > out_cap)
+        return e.set(CUOPUS_RC_ARG, "output capacity %zu < required %llu samples", out_cap,
+                     (unsigned long long)probe.keep);
+    if (probe.keep > 0 && !out_dev) return e.set(CUOPUS_RC_ARG, "out_dev is NULL");
+    Plan pl;
+    RunOpts o;
+    o.s16_out = out_dev;
+    o.persistent = env_u("CUOPUS_PERSISTENT", 0) != 0;
+    rc = decode_files(&ogg, &n, 1, nullptr, 0, o, st, pl, e);
+    if (rc) return rc;
+    if (out_len) *out_len = (int64_t)probe.keep;
+    return 0;
+}
+
+extern "C" int cuopus_decode_f16k_multi(const uint8_t *const *oggs, const size_t *ns, int nfiles,
+                                        float *const *out_devs, const size_t *out_caps, int64_t *out_lens,
+                                        void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    last_ovf = 0;
+    if (out_lens)
+        for (int f = 0; f < nfiles; f++) out_lens[f] = 0;
+    if (nfiles <= 0) return 0;
+    if (!oggs || !ns || !out_devs || !out_caps) return e.set(CUOPUS_RC_ARG, "NULL argument array");
+    hipStream_t st = (hipStream_t)stream;
+    Plan pl;
+    RunOpts o;
+    o.persistent = env_u("CUOPUS_PERSISTENT", 0) != 0;
+    // validate capacities first (host only)
+    std::vector<int64_t> need((size_t)nfiles);
+    for (int f = 0; f < nfiles; f++) {
+        FilePlan fp;
+        char sub[256];
+        Err fe(sub, sizeof(sub));
+        int rc = demux_file(oggs[f], ns[f], &fp, false, fe);
+        if (rc) return e.set(rc, "file %d: %s", f, sub);
+        need[f] = fp.out16;
+        if ((uint64_t)fp.out16 > out_caps[f])
+            return e.set(CUOPUS_RC_ARG, "file %d: output capacity %zu < required %lld", f, out_caps[f],
+                         (long long)fp.out16);
+        if (fp.out16 > 0 && !out_devs[f]) return e.set(CUOPUS_RC_ARG, "file %d: out_dev is NULL", f);
+        o.f16_out.push_back(out_devs[f]);
+    }
+    int rc = decode_files(oggs, ns, nfiles, nullptr, 0, o, st, pl, e);
+    if (rc) return rc;
+    if (out_lens)
+        for (int f = 0; f < nfiles; f++) out_lens[f] = need[f];
+    return 0;
+}
+
+extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, const uint32_t *len,
+                                  uint32_t n_packets, const uint8_t *lost, const uint32_t *first,
+                                  const uint32_t *npkts, const uint32_t *trim, const uint32_t *keep,
+                                  uint32_t n_chains, int16_t *pcm_out, int32_t *pkt_nsamp, uint32_t *pkt_rng,
+                                  int32_t *chain_stop, void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = refuse_capture(st, e);
+    if (rc) return rc;
+    if (!data || !off || !len || !first || !npkts || !trim || !keep || !pcm_out || !pkt_nsamp || !pkt_rng ||
+        !chain_stop)
+        return e.set(CUOPUS_RC_ARG, "NULL argument");
+    if (nbytes >= (1ull << 32)) return e.set(CUOPUS_RC_ARG, "packet data exceeds 4 GiB");
+    Plan pl;
+    pl.data.assign(data, data + (nbytes ? nbytes : 0));
+    if (pl.data.empty()) pl.data.push_back(0);
+    pl.off.assign(off, off + n_packets);
+    pl.len.assign(len, len + n_packets);
+    for (uint32_t p = 0; p < n_packets; p++)
+        if ((uint64_t)off[p] + len[p] > nbytes) return e.set(CUOPUS_RC_ARG, "packet %u out of range", p);
+    if (lost) pl.lost.assign(lost, lost + n_packets);
+    pl.chains.resize(n_chains);
+    pl.cap.resize(n_chains);
+    uint64_t cat = 0;
+    for (uint32_t c = 0; c < n_chains; c++) {
+        if ((uint64_t)first[c] + npkts[c] > n_packets) return e.set(CUOPUS_RC_ARG, "chain %u out of range", c);
+        pl.chains[c] = {first[c], npkts[c], trim[c], keep[c], cat};
+        uint64_t s = 0;
+        for (uint32_t k = 0; k < npkts[c]; k++)
+            s += (uint64_t)cuo_host::pkt_samples48(data + off[first[c] + k], len[first[c] + k]);
+        pl.cap[c] = s;
+        cat += keep[c];
+    }
+    pl.total_keep = cat;
+    if (n_chains == 0) return 0;
+    if ((rc = grow(W.nsamp, 4ull * n_packets, e))) return rc;
+    if ((rc = grow(W.rng, 4ull * n_packets, e))) return rc;
+    HIPCHK(e, hipMemsetAsync(W.nsamp.p, 0, 4ull * n_packets, st));
+    HIPCHK(e, hipMemsetAsync(W.rng.p, 0, 4ull * n_packets, st));
+    RunOpts o;
+    o.pkt_nsamp = (int32_t *)W.nsamp.p;
+    o.pkt_rng = (uint32_t *)W.rng.p;
+    o.persistent = env_u("CUOPUS_PERSISTENT", 0) != 0;
+    std::vector<ChainResult> res;
+    rc = run_plan(pl, st, o, res, e);
+    // results are copied back even when a chain failed validation
+    if (rc && rc != CUOPUS_RC_SOFT) return rc;
+    int rc2 = rc;
+    for (uint32_t c = 0; c < n_chains && c < res.size(); c++) chain_stop[c] = res[c].stop;
+    HIPCHK(e, hipMemcpyAsync(pkt_nsamp, W.nsamp.p, 4ull * n_packets, hipMemcpyDeviceToHost, st));
+    HIPCHK(e, hipMemcpyAsync(pkt_rng, W.rng.p, 4ull * n_packets, hipMemcpyDeviceToHost, st));
+    HIPCHK(e, hipMemcpyAsync(pcm_out, W.cat.p, 2 * (cat ? cat : 0), hipMemcpyDeviceToHost, st));
+    HIPCHK(e, hipStreamSynchronize(st));
+    return rc2;
+}
+
+extern "C" int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, float *out_dev, size_t out_cap,
+                                           int64_t *out_len, void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    hipStream_t st = (hipStream_t)stream;
+    int64_t n16 = cuo_host::out16_len(n48);
+    if (out_len) *out_len = 0;
+    if ((uint64_t)n16 > out_cap) return e.set(CUOPUS_RC_ARG, "output capacity %zu < required %lld", out_cap, (long long)n16);
+    if (n16 == 0) return 0;
+    if (!in_dev || !out_dev) return e.set(CUOPUS_RC_ARG, "NULL pointer");
+    RsFile r;
+    r.in_off = 0;
+    r.in_len = (uint64_t)n48;
+    r.out_len = (uint64_t)n16;
+    r.out = out_dev;
+    r.tile0 = 0;
+    int rc;
+    if ((rc = grow(W.files, sizeof(RsFile), e))) return rc;
+    HIPCHK(e, hipMemcpyAsync(W.files.p, &r, sizeof(RsFile), hipMemcpyHostToDevice, st));
+    static RsTaps taps = make_taps();
+    unsigned tiles = (unsigned)((n16 + kRsTile - 1) / kRsTile);
+    hipLaunchKernelGGL(k_resample_48to16, dim3(tiles), dim3(kRsTile), 0, st, in_dev, (const RsFile *)W.files.p, 1, taps);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipStreamSynchronize(st));
+    if (out_len) *out_len = n16;
+    return 0;
+}
+
+extern "C" int cuopus_warmup(char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    int rc;
+    if ((rc = grow(W.slots, sizeof(ChainSlot), e))) return rc;
+    if ((rc = grow(W.res, sizeof(ChainResult), e))) return rc;
+    DecodeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n_chains = 0;
+    a.slots = (ChainSlot *)W.slots.p;
+    a.res = (ChainResult *)W.res.p;
+    hipLaunchKernelGGL(k_decode_chains, dim3(1), dim3(64), 0, 0, a);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipDeviceSynchronize());
+    return 0;
+}

@@ -1,0 +1,242 @@
+"""cuopus_native -- ctypes binding of libcuopus.so (MI355X / ROCm).
+
+Same API as the reference's python/cuopus_native.py (:30-180):
+``probe``, ``decode_ogg_to_cuda_tensor``, ``decode_ogg_multi``, ``warmup``,
+``workspace_bytes``, ``last_arena_overflow``, ``release_workspaces`` and
+``CuopusError(rc)``.  Tensors are torch ROCm tensors (device "cuda" is the HIP
+device); every call runs on torch's current stream and the library
+synchronises that stream before returning.
+
+Additions: ``decode_ogg_to_s16_tensor`` (playable int16 48 kHz),
+``decode_ogg_lossy`` (per-packet loss mask with in-band FEC),
+``packet_count`` and ``gate_chains`` (per-packet sample counts and final
+ranges, the reference harness2.cu protocol).
+
+The library is the only decode path: if it is missing or no GPU is present,
+calls raise instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_ERRCAP = 512
+_RC_NAMES = {1: "soft", 2: "gpu", 3: "caller"}
+
+
+class CuopusError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__("cuopus[%s]: %s" % (_RC_NAMES.get(rc, rc), msg))
+        self.rc = rc
+
+
+def _find_lib():
+    env = os.environ.get("CUOPUS_LIB")
+    if env:
+        return env
+    here = os.path.dirname(os.path.abspath(__file__))
+    cand = os.path.join(here, "libcuopus.so")
+    if os.path.exists(cand):
+        return cand
+    raise OSError("libcuopus.so not found (build it with __graft_entry__.build() or set CUOPUS_LIB)")
+
+
+_lib = None
+P = ctypes.c_void_p
+SZ = ctypes.c_size_t
+I64P = ctypes.POINTER(ctypes.c_int64)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    lib = ctypes.CDLL(_find_lib())
+    sig = {
+        "cuopus_probe": (ctypes.c_int, [ctypes.c_char_p, SZ, I64P, ctypes.c_char_p, SZ]),
+        "cuopus_probe_packets": (ctypes.c_int, [ctypes.c_char_p, SZ, I64P, I64P, ctypes.c_char_p, SZ]),
+        "cuopus_decode_f16k": (ctypes.c_int, [ctypes.c_char_p, SZ, P, SZ, I64P, ctypes.c_char_p, SZ]),
+        "cuopus_decode_f16k_on": (ctypes.c_int, [ctypes.c_char_p, SZ, P, SZ, I64P, P, ctypes.c_char_p, SZ]),
+        "cuopus_decode_s16_on": (ctypes.c_int, [ctypes.c_char_p, SZ, P, SZ, I64P, P, ctypes.c_char_p, SZ]),
+        "cuopus_decode_lossy_f16k_on": (ctypes.c_int, [ctypes.c_char_p, SZ, P, SZ, P, SZ, I64P, P,
+                                                       ctypes.c_char_p, SZ]),
+        "cuopus_decode_f16k_multi": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(SZ),
+                                                    ctypes.c_int, ctypes.POINTER(P), ctypes.POINTER(SZ),
+                                                    I64P, P, ctypes.c_char_p, SZ]),
+        "cuopus_gate_chains": (ctypes.c_int, [P, SZ, P, P, ctypes.c_uint32, P, P, P, P, P, ctypes.c_uint32,
+                                              P, P, P, P, P, ctypes.c_char_p, SZ]),
+        "cuopus_resample_s16_to_f16k": (ctypes.c_int, [P, ctypes.c_int64, P, SZ, I64P, P, ctypes.c_char_p, SZ]),
+        "cuopus_warmup": (ctypes.c_int, [ctypes.c_char_p, SZ]),
+        "cuopus_last_arena_overflow": (ctypes.c_int, []),
+        "cuopus_workspace_bytes": (ctypes.c_longlong, []),
+        "cuopus_release_workspaces": (None, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _as_bytes(path_or_bytes):
+    if isinstance(path_or_bytes, (bytes, bytearray, memoryview)):
+        return bytes(path_or_bytes)
+    with open(path_or_bytes, "rb") as f:
+        return f.read()
+
+
+def _check(rc, err):
+    if rc:
+        raise CuopusError(rc, err.value.decode(errors="replace"))
+
+
+def probe(path_or_bytes) -> int:
+    """Exact output length in float32 16 kHz samples (host only)."""
+    lib = _load()
+    data = _as_bytes(path_or_bytes)
+    n16 = ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_probe(data, len(data), ctypes.byref(n16), err, _ERRCAP), err)
+    return n16.value
+
+
+def packet_count(path_or_bytes):
+    """(audio packets, chains) of a stream (host only)."""
+    lib = _load()
+    data = _as_bytes(path_or_bytes)
+    npk, nch = ctypes.c_int64(0), ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_probe_packets(data, len(data), ctypes.byref(npk), ctypes.byref(nch), err, _ERRCAP), err)
+    return npk.value, nch.value
+
+
+def _stream():
+    import torch
+    return torch.cuda.current_stream().cuda_stream
+
+
+def decode_ogg_to_cuda_tensor(path_or_bytes):
+    """Decode an Ogg/Opus file or bytes into a new float32 16 kHz device tensor."""
+    import torch
+    lib = _load()
+    data = _as_bytes(path_or_bytes)
+    n16 = probe(data)
+    out = torch.empty(n16, dtype=torch.float32, device="cuda")
+    if n16 == 0:
+        return out
+    out_len = ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_decode_f16k_on(data, len(data), out.data_ptr(), n16, ctypes.byref(out_len),
+                                     _stream(), err, _ERRCAP), err)
+    if out_len.value != n16:
+        raise CuopusError(1, "decode wrote %d samples, probe said %d" % (out_len.value, n16))
+    return out
+
+
+def decode_ogg_lossy(path_or_bytes, lost_mask):
+    """Decode with a per-packet loss mask (in-band FEC when available)."""
+    import torch
+    lib = _load()
+    data = _as_bytes(path_or_bytes)
+    mask = np.ascontiguousarray(np.asarray(lost_mask, dtype=np.uint8))
+    n16 = probe(data)
+    out = torch.empty(n16, dtype=torch.float32, device="cuda")
+    if n16 == 0:
+        return out
+    out_len = ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_decode_lossy_f16k_on(data, len(data), mask.ctypes.data, mask.size, out.data_ptr(), n16,
+                                           ctypes.byref(out_len), _stream(), err, _ERRCAP), err)
+    return out
+
+
+def decode_ogg_to_s16_tensor(path_or_bytes):
+    """Playable int16 48 kHz PCM (preroll and pre-skip removed) as a device tensor."""
+    import torch
+    lib = _load()
+    data = _as_bytes(path_or_bytes)
+    npk, nch = packet_count(data)
+    del npk, nch
+    n16 = probe(data)
+    # playable = 3*n16 + 48 when n16 > 0; size generously and trim
+    cap = 3 * n16 + 48 + 3 if n16 > 0 else 48 + 3
+    out = torch.empty(cap, dtype=torch.int16, device="cuda")
+    out_len = ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_decode_s16_on(data, len(data), out.data_ptr(), cap, ctypes.byref(out_len), _stream(),
+                                    err, _ERRCAP), err)
+    return out[:out_len.value]
+
+
+def decode_ogg_multi(items):
+    """Decode N files in one decode launch; returns a list of float32 16 kHz device tensors."""
+    import torch
+    lib = _load()
+    blobs = [_as_bytes(x) for x in items]
+    n = len(blobs)
+    if n == 0:
+        return []
+    n16 = [probe(b) for b in blobs]
+    outs = [torch.empty(k, dtype=torch.float32, device="cuda") for k in n16]
+    ba = (ctypes.c_char_p * n)(*blobs)
+    la = (SZ * n)(*[len(b) for b in blobs])
+    oa = (P * n)(*[o.data_ptr() for o in outs])
+    ca = (SZ * n)(*n16)
+    ol = (ctypes.c_int64 * n)()
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_decode_f16k_multi(ba, la, n, oa, ca, ol, _stream(), err, _ERRCAP), err)
+    for i in range(n):
+        if ol[i] != n16[i]:
+            raise CuopusError(1, "file %d: decode wrote %d samples, probe said %d" % (i, ol[i], n16[i]))
+    return outs
+
+
+def gate_chains(data, off, length, chains, lost=None):
+    """Per-packet gate on the GPU.
+
+    data: uint8 packet bytes; off/length: uint32 per packet; chains: int array
+    (n, 4) of (first, npkts, trim, keep).  Returns (pcm int16 sum(keep),
+    nsamp int32 per packet, rng uint32 per packet, stop int32 per chain, rc).
+    """
+    lib = _load()
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    off = np.ascontiguousarray(off, dtype=np.uint32)
+    length = np.ascontiguousarray(length, dtype=np.uint32)
+    ch = np.ascontiguousarray(np.asarray(chains, dtype=np.uint32).reshape(-1, 4))
+    first, npk, trim, keep = [np.ascontiguousarray(ch[:, i]) for i in range(4)]
+    n_pk, n_ch = off.size, first.size
+    pcm = np.zeros(int(keep.sum(dtype=np.uint64)), np.int16)
+    nsamp = np.zeros(n_pk, np.int32)
+    rng = np.zeros(n_pk, np.uint32)
+    stop = np.zeros(n_ch, np.int32)
+    lp = None
+    if lost is not None:
+        lost = np.ascontiguousarray(lost, dtype=np.uint8)
+        lp = lost.ctypes.data
+    err = ctypes.create_string_buffer(_ERRCAP)
+    rc = lib.cuopus_gate_chains(data.ctypes.data, data.size, off.ctypes.data, length.ctypes.data, n_pk, lp,
+                                first.ctypes.data, npk.ctypes.data, trim.ctypes.data, keep.ctypes.data, n_ch,
+                                pcm.ctypes.data, nsamp.ctypes.data, rng.ctypes.data, stop.ctypes.data,
+                                _stream(), err, _ERRCAP)
+    if rc not in (0, 1):
+        _check(rc, err)
+    return pcm, nsamp, rng, stop, rc
+
+
+def warmup():
+    lib = _load()
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_warmup(err, _ERRCAP), err)
+
+
+def workspace_bytes() -> int:
+    return int(_load().cuopus_workspace_bytes())
+
+
+def last_arena_overflow() -> bool:
+    return bool(_load().cuopus_last_arena_overflow())
+
+
+def release_workspaces():
+    _load().cuopus_release_workspaces()

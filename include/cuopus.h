@@ -1,0 +1,116 @@
+/* cuopus.h -- C ABI of the MI355X batch Opus decoder (libcuopus.so, ROCm/HIP).
+ *
+ * Drop-in for the reference's libcuopus (libcuopus.cu:1-74 contract, :201-776
+ * definitions): same symbol names, signatures, return codes and threading
+ * contract.  `void *stream` is a hipStream_t (torch.cuda.current_stream().
+ * cuda_stream on PyTorch-ROCm); device pointers are HIP device pointers.
+ *
+ * Return codes (libcuopus.cu:48-51):
+ *   0 ok; 1 soft failure (bad stream / decode validation, GPU healthy);
+ *   2 GPU (HIP) error; 3 caller error (capacity, NULL pointer, capture).
+ * errbuf, when non-NULL, is always NUL-terminated.
+ *
+ * Threading / capture contract (libcuopus.cu:53-68): no context creation and
+ * no hipSetDevice (runs on the caller's current device); no internal threads;
+ * the single-file and multi-file calls issue all work on the caller's stream,
+ * synchronise it before returning and refuse to run while that stream is
+ * capturing.  Workspaces are library-global and grow-only: not thread-safe.
+ */
+#ifndef CUOPUS_H
+#define CUOPUS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUOPUS_RC_OK 0
+#define CUOPUS_RC_SOFT 1
+#define CUOPUS_RC_GPU 2
+#define CUOPUS_RC_ARG 3
+
+/* ---------------------------------------------------------------------------
+ * The reference's eight entry points.
+ * ------------------------------------------------------------------------- */
+
+/* Demux + validate (mono, granule cross-check); *samples16k = exact f32 16 kHz
+ * output length, max(0, (playable48 - 48) / 3).  Host only, no GPU work.
+ * Replaces libcuopus.cu:223-236. */
+int cuopus_probe(const uint8_t *ogg, size_t n, int64_t *samples16k, char *errbuf, size_t errcap);
+
+/* Decode + resample into out_dev (capacity out_cap floats) on the default
+ * stream.  Replaces libcuopus.cu:462-468. */
+int cuopus_decode_f16k(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap, int64_t *out_len,
+                       char *errbuf, size_t errcap);
+
+/* Same on an explicit stream (hipStream_t).  Replaces libcuopus.cu:238-460. */
+int cuopus_decode_f16k_on(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap, int64_t *out_len,
+                          void *stream, char *errbuf, size_t errcap);
+
+/* N files in one decode launch + one batched resample launch; per-file output
+ * bit-identical to cuopus_decode_f16k_on.  Replaces libcuopus.cu:482-741
+ * (without the reference's max-batch limit: the chain count is bounded only
+ * by device memory). */
+int cuopus_decode_f16k_multi(const uint8_t *const *oggs, const size_t *ns, int nfiles, float *const *out_devs,
+                             const size_t *out_caps, int64_t *out_lens, void *stream, char *errbuf, size_t errcap);
+
+/* Optional: load the kernels and size the workspaces once at start-up.
+ * Replaces libcuopus.cu:746-776. */
+int cuopus_warmup(char *errbuf, size_t errcap);
+
+/* 1 if the last decode failed for lack of per-chain decoder memory.  The
+ * decoder state here is statically sized, so this is always 0 (kept for ABI
+ * compatibility).  Replaces libcuopus.cu:201. */
+int cuopus_last_arena_overflow(void);
+
+/* Bytes held in grow-only device workspaces.  Replaces libcuopus.cu:203. */
+long long cuopus_workspace_bytes(void);
+
+/* Free every workspace.  Replaces libcuopus.cu:205-221. */
+void cuopus_release_workspaces(void);
+
+/* ---------------------------------------------------------------------------
+ * Additions on the same path (no counterpart symbol in the reference).
+ * ------------------------------------------------------------------------- */
+
+/* Playable int16 48 kHz PCM (preroll and pre-skip removed) into out_dev;
+ * *out_len = playable samples.  The reference produces this buffer internally
+ * (its cuopus_compact output, cuopus_kernels.cuh:420-433). */
+int cuopus_decode_s16_on(const uint8_t *ogg, size_t n, int16_t *out_dev, size_t out_cap, int64_t *out_len,
+                         void *stream, char *errbuf, size_t errcap);
+
+/* As cuopus_decode_f16k_on with a per-packet loss mask (lost[i] != 0: audio
+ * packet i is treated as lost): it is decoded from the next packet's in-band
+ * FEC (LBRR) when that packet is in the same chain, not lost and carries LBRR,
+ * otherwise concealed for its TOC duration.  mask_n must equal the number of
+ * audio packets (see cuopus_probe_packets). */
+int cuopus_decode_lossy_f16k_on(const uint8_t *ogg, size_t n, const uint8_t *lost, size_t mask_n, float *out_dev,
+                                size_t out_cap, int64_t *out_len, void *stream, char *errbuf, size_t errcap);
+
+/* Number of audio packets and chains of a stream (host only). */
+int cuopus_probe_packets(const uint8_t *ogg, size_t n, int64_t *n_packets, int64_t *n_chains, char *errbuf,
+                         size_t errcap);
+
+/* Per-packet gate (the reference's harness2.cu:51-91 protocol, host
+ * buffers): decode the chains of an already-demuxed packet table, each chain
+ * from a fresh decoder.  Outputs per packet the decoded sample count and the
+ * final range, per chain the stop code, and the compacted keep-window PCM
+ * (pcm_out, sum(keep) int16).  lost may be NULL. */
+int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, const uint32_t *len,
+                       uint32_t n_packets, const uint8_t *lost, const uint32_t *first, const uint32_t *npkts,
+                       const uint32_t *trim, const uint32_t *keep, uint32_t n_chains, int16_t *pcm_out,
+                       int32_t *pkt_nsamp, uint32_t *pkt_rng, int32_t *chain_stop, void *stream, char *errbuf,
+                       size_t errcap);
+
+/* Resample playable int16 48 kHz device PCM to f32 16 kHz (length
+ * max(0,(n48-48)/3)) with the library's resampler kernel. */
+int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, float *out_dev, size_t out_cap,
+                                int64_t *out_len, void *stream, char *errbuf, size_t errcap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUOPUS_H */

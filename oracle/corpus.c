@@ -13,9 +13,11 @@
  *           scaled to -20 dBFS RMS.
  *   music   pink noise + log sine sweep 50 Hz..20 kHz (10 s period) + 8 tones
  *           with U(-30,-10) dBFS amplitudes redrawn every minute.
+ *   mixed   each second is speech or music (coin flip) and the encoder's
+ *           signal hint follows it, so the encoder switches modes.
  *
  * usage: mkcorpus --out F --seed S --dur SEC [--signal speech|music]
- *        [--app voip|audio|lowdelay] [--bw nb|mb|wb|swb|fb]
+ *        [--signal speech|music|mixed] [--app voip|audio|lowdelay] [--bw nb|mb|wb|swb|fb]
  *        [--force auto|silk|hybrid|celt] [--vbr 0|1] [--bitrate B]
  *        [--brange LO HI] [--frame N] [--frameset a,b,c] [--fec 0|1]
  *        [--loss P] [--complexity C] [--pcm F]
@@ -268,7 +270,18 @@ int main(int argc, char **argv)
 
     long n = (long)(dur * FS + 0.5);
     double *x = (double *)calloc((size_t)n + 1, sizeof(double));
-    if (!strcmp(signal, "music")) gen_music(x, n); else gen_speech(x, n);
+    long nsec = n / FS + 2;
+    unsigned char *is_music = (unsigned char *)calloc((size_t)nsec, 1);
+    int mixed = !strcmp(signal, "mixed");
+    if (!strcmp(signal, "music")) gen_music(x, n);
+    else if (mixed) {
+        double *y = (double *)calloc((size_t)n + 1, sizeof(double));
+        gen_speech(x, n);
+        gen_music(y, n);
+        for (long s = 0; s < nsec; s++) is_music[s] = runif() < 0.5;
+        for (long i = 0; i < n; i++) if (is_music[i / FS]) x[i] = y[i];
+        free(y);
+    } else gen_speech(x, n);
 
     int application = !strcmp(app, "voip") ? OPUS_APPLICATION_VOIP
                     : !strcmp(app, "lowdelay") ? OPUS_APPLICATION_RESTRICTED_LOWDELAY
@@ -346,6 +359,8 @@ int main(int argc, char **argv)
                 opus_encoder_ctl(enc, OPUS_SET_BITRATE(br));
             }
             if (nfs > 0) fsz = fs_list[(int)(runif() * nfs) % nfs];
+            if (mixed)
+                opus_encoder_ctl(enc, OPUS_SET_SIGNAL(is_music[s] ? OPUS_SIGNAL_MUSIC : OPUS_SIGNAL_VOICE));
             /* keep frames inside the second so the draw applies per segment */
         }
         int this_fs = fsz;

@@ -16,7 +16,9 @@ extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const 
                                 int32_t *nsamp, uint32_t *rng)
 {
     OpusDecState *st = (OpusDecState *)malloc(sizeof(OpusDecState));
-    DecScratch *sc = (DecScratch *)calloc(1, sizeof(DecScratch));
+    DecScratch *sc = (DecScratch *)malloc(sizeof(DecScratch));
+    memset(sc, 0xA5, sizeof(DecScratch));  // device slots start uninitialised
+    memset(st, 0x5A, sizeof(OpusDecState));
     opus_dec_init(st);
     long total = 0;
     long ret = 0;

@@ -1,0 +1,143 @@
+"""Test-side access to the CPU oracle (oracle/): corpus generation, per-chain
+fixed-point libopus decode, the resampler restatement and the Python demuxer.
+
+Only tests, the smoke check and bench.py's CPU-baseline leg use this module.
+"""
+import ctypes
+import os
+import shlex
+import subprocess
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from oracle import demux as odemux  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+MKCORPUS = os.path.join(REF, "mkcorpus")
+CACHE = os.path.join(ROOT, "corpus", "cache")
+
+_lib = None
+P = ctypes.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(REF, "liboracle.so")
+        if not os.path.exists(path):
+            raise OSError("oracle not built: %s (run __graft_entry__.build())" % path)
+        L = ctypes.CDLL(path)
+        L.orc_decode_chain.restype = ctypes.c_long
+        L.orc_decode_chain.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_long, P, P]
+        L.orc_decode_chains.restype = ctypes.c_int
+        L.orc_decode_chains.argtypes = [P, P, P, P, P, P, ctypes.c_int, P, P, P, P, P, P, ctypes.c_int]
+        L.orc_resample.restype = None
+        L.orc_resample.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64]
+        L.orc_taps.restype = None
+        L.orc_taps.argtypes = [P]
+        L.orc_ec_schedule.restype = None
+        L.orc_ec_schedule.argtypes = [P, ctypes.c_uint32, P]
+        L.orc_packet_has_lbrr.restype = ctypes.c_int
+        L.orc_packet_has_lbrr.argtypes = [P, ctypes.c_int]
+        L.orc_packet_nb_samples.restype = ctypes.c_int
+        L.orc_packet_nb_samples.argtypes = [P, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def small_corpus(name, args):
+    """Generate (once) a corpus file with oracle/_ref/mkcorpus and return its path."""
+    os.makedirs(CACHE, exist_ok=True)
+    path = os.path.join(CACHE, name + ".opus")
+    if not os.path.exists(path):
+        tmp = path + ".tmp%d" % os.getpid()
+        subprocess.run([MKCORPUS, "--out", tmp] + shlex.split(args), check=True)
+        os.replace(tmp, path)
+    return path
+
+
+def demux_chains(blob, seconds=1.0, preroll_ms=250.0):
+    d = odemux.demux(blob)
+    return d, odemux.build_chains(d, seconds, preroll_ms)
+
+
+def decode_chains(data, off, ln, chains, lost=None, threads=1):
+    """Decode every chain with a fresh fixed-point libopus decoder.
+
+    Returns dict: pcm (compacted keep windows, int16), nsamp/rng per packet,
+    ret per chain, full (list of per-chain full PCM arrays)."""
+    L = lib()
+    ch = np.asarray(chains, dtype=np.int64).reshape(-1, 4)
+    n = ch.shape[0]
+    first = np.ascontiguousarray(ch[:, 0], dtype=np.uint32)
+    npk = np.ascontiguousarray(ch[:, 1], dtype=np.uint32)
+    caps = (npk.astype(np.uint64) + 1) * 5760
+    pcm_off = np.zeros(n, np.uint64)
+    if n > 1:
+        pcm_off[1:] = np.cumsum(caps[:-1])
+    pcm = np.zeros(int(caps.sum()), np.int16)
+    nsamp = np.zeros(len(off), np.int32)
+    rng = np.zeros(len(off), np.uint32)
+    ret = np.zeros(n, np.int64)
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    off = np.ascontiguousarray(off, dtype=np.uint32)
+    ln = np.ascontiguousarray(ln, dtype=np.uint32)
+    lp = None
+    if lost is not None:
+        lost = np.ascontiguousarray(lost, dtype=np.uint8)
+        lp = lost.ctypes.data
+    L.orc_decode_chains(data.ctypes.data, off.ctypes.data, ln.ctypes.data, lp, first.ctypes.data, npk.ctypes.data,
+                        n, pcm_off.ctypes.data, caps.ctypes.data, pcm.ctypes.data, nsamp.ctypes.data,
+                        rng.ctypes.data, ret.ctypes.data, threads)
+    kept = []
+    full = []
+    for i in range(n):
+        r = int(ret[i])
+        seg = pcm[int(pcm_off[i]): int(pcm_off[i]) + max(r, 0)]
+        full.append(seg)
+        trim, keep = int(ch[i, 2]), int(ch[i, 3])
+        kept.append(seg[trim: trim + keep])
+    return {"pcm": np.concatenate(kept) if kept else np.zeros(0, np.int16), "nsamp": nsamp, "rng": rng,
+            "ret": ret, "full": full}
+
+
+def resample(pcm48):
+    """The reference resampler restated in C (float32, tap order)."""
+    pcm48 = np.ascontiguousarray(pcm48, dtype=np.int16)
+    n16 = odemux.out16_len(len(pcm48))
+    out = np.zeros(n16, np.float32)
+    if n16:
+        lib().orc_resample(pcm48.ctypes.data, len(pcm48), out.ctypes.data, n16)
+    return out
+
+
+def taps():
+    t = np.zeros(49, np.float32)
+    lib().orc_taps(t.ctypes.data)
+    return t
+
+
+def loss_mask(n, p, seed):
+    """Seeded Bernoulli(p) loss mask over n packets."""
+    rs = np.random.default_rng(seed)
+    return (rs.random(n) < p).astype(np.uint8)
+
+
+# Corpus recipes shared by the tests (small, seconds to generate)
+RECIPES = {
+    "silk_wb": "--seed 11 --dur 12 --signal speech --app voip --bw wb --force silk --vbr 0 --bitrate 16000",
+    "silk_nb": "--seed 12 --dur 12 --signal speech --app voip --bw nb --force silk --bitrate 12000",
+    "silk_mb": "--seed 13 --dur 12 --signal speech --app voip --bw mb --force silk --bitrate 14000",
+    "celt_fb": "--seed 14 --dur 12 --signal music --app lowdelay --bw fb --bitrate 64000 --frameset 120,240,480,960",
+    "hybrid_fb": "--seed 15 --dur 12 --signal speech --app voip --bw fb --force hybrid --bitrate 32000",
+    "hybrid_swb": "--seed 16 --dur 12 --signal speech --app voip --bw swb --force hybrid --bitrate 24000",
+    "hybrid_fec": "--seed 17 --dur 12 --signal speech --app voip --bw fb --force hybrid --bitrate 32000 "
+                  "--fec 1 --loss 2",
+    "mixed": "--seed 18 --dur 20 --signal mixed --app audio --bw fb --brange 12000 64000",
+    "silk_long": "--seed 19 --dur 12 --signal speech --app voip --bw wb --force silk --bitrate 20000 --frame 2880",
+}
