@@ -64,6 +64,8 @@ struct CeltScratch {
     int16_t fir_tmp[kMaxPeriod];
     int16_t lp_pitch[kDecBuf >> 1];
     int32_t xcorr[kMaxPeriod];
+    int16_t plc_a[kMaxPeriod + 8];                 // autocorrelation copy / pitch x4
+    int16_t plc_b[960 + kOverlap + kLpcOrder];     // IIR history / pitch y4
     int32_t etmp[kOverlap];
 };
 

@@ -421,6 +421,26 @@ extern "C" int cuopus_probe_packets(const uint8_t *ogg, size_t n, int64_t *n_pac
     return 0;
 }
 
+extern "C" int cuopus_dump_chains(const uint8_t *ogg, size_t n, uint32_t *out4, size_t cap_chains, int64_t *n_chains,
+                                  char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    FilePlan fp;
+    int rc = demux_file(ogg, n, &fp, true, e);
+    if (rc) return rc;
+    if (n_chains) *n_chains = (int64_t)fp.chains.size();
+    if (!out4) return 0;
+    if (cap_chains < fp.chains.size())
+        return e.set(CUOPUS_RC_ARG, "chain capacity %zu < %zu", cap_chains, fp.chains.size());
+    for (size_t i = 0; i < fp.chains.size(); i++) {
+        out4[4 * i + 0] = fp.chains[i].first;
+        out4[4 * i + 1] = fp.chains[i].npkts;
+        out4[4 * i + 2] = fp.chains[i].trim;
+        out4[4 * i + 3] = fp.chains[i].keep;
+    }
+    return 0;
+}
+
 static int f16k_common(const uint8_t *ogg, size_t n, const uint8_t *lost, size_t mask_n, float *out_dev,
                        size_t out_cap, int64_t *out_len, void *stream, char *errbuf, size_t errcap)
 {

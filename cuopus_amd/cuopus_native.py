@@ -66,6 +66,7 @@ def _load():
         "cuopus_gate_chains": (ctypes.c_int, [P, SZ, P, P, ctypes.c_uint32, P, P, P, P, P, ctypes.c_uint32,
                                               P, P, P, P, P, ctypes.c_char_p, SZ]),
         "cuopus_resample_s16_to_f16k": (ctypes.c_int, [P, ctypes.c_int64, P, SZ, I64P, P, ctypes.c_char_p, SZ]),
+        "cuopus_dump_chains": (ctypes.c_int, [ctypes.c_char_p, SZ, P, SZ, I64P, ctypes.c_char_p, SZ]),
         "cuopus_warmup": (ctypes.c_int, [ctypes.c_char_p, SZ]),
         "cuopus_last_arena_overflow": (ctypes.c_int, []),
         "cuopus_workspace_bytes": (ctypes.c_longlong, []),
@@ -109,6 +110,18 @@ def packet_count(path_or_bytes):
     err = ctypes.create_string_buffer(_ERRCAP)
     _check(lib.cuopus_probe_packets(data, len(data), ctypes.byref(npk), ctypes.byref(nch), err, _ERRCAP), err)
     return npk.value, nch.value
+
+
+def chain_table(path_or_bytes):
+    """(n, 4) uint32 array of (first, npkts, trim, keep) per chain (host only)."""
+    lib = _load()
+    data = _as_bytes(path_or_bytes)
+    n = ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_dump_chains(data, len(data), None, 0, ctypes.byref(n), err, _ERRCAP), err)
+    out = np.zeros((n.value, 4), np.uint32)
+    _check(lib.cuopus_dump_chains(data, len(data), out.ctypes.data, n.value, ctypes.byref(n), err, _ERRCAP), err)
+    return out
 
 
 def _stream():

@@ -93,6 +93,13 @@ int cuopus_decode_lossy_f16k_on(const uint8_t *ogg, size_t n, const uint8_t *los
 int cuopus_probe_packets(const uint8_t *ogg, size_t n, int64_t *n_packets, int64_t *n_chains, char *errbuf,
                          size_t errcap);
 
+/* Chain table of a stream (host only): out4[4*i..4*i+3] = first packet,
+ * packet count, trim, keep of chain i (the reference CLI's --dump-chains CUC2
+ * records, decode_cli.cu:693-701 / cuopus_demux.h:396-411).  With out4 NULL
+ * only *n_chains is returned. */
+int cuopus_dump_chains(const uint8_t *ogg, size_t n, uint32_t *out4, size_t cap_chains, int64_t *n_chains,
+                       char *errbuf, size_t errcap);
+
 /* Per-packet gate (the reference's harness2.cu:51-91 protocol, host
  * buffers): decode the chains of an already-demuxed packet table, each chain
  * from a fresh decoder.  Outputs per packet the decoded sample count and the
