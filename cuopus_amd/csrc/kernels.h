@@ -129,6 +129,10 @@ __global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
     }
 }
 
+// Zero a work-queue counter (a kernel rather than a memset so the sequence can
+// be captured into a HIP graph and replayed).
+__global__ void k_reset_counter(uint32_t *q) { if (threadIdx.x == 0) q[0] = 0; }
+
 struct RsTaps {
     float t[kRsTaps];
 };

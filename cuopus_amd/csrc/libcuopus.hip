@@ -637,6 +637,327 @@ extern "C" int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, f
     return 0;
 }
 
+
+// =====================================================================================
+// Resident batches
+// =====================================================================================
+struct cuopus_batch {
+    Plan pl;
+    cuopus_batch_opts opts;
+    DevBuf data, off, len, lost, chains, order, slots, cat, res, files, queue;
+    std::vector<RsFile> rsf;          // one entry per file with output
+    std::vector<uint64_t> file_tiles; // per rsf entry: tile count
+    std::vector<uint32_t> file_chain0, file_nchain;
+    std::vector<int> file_rs;         // per file: its rsf entry or -1
+    uint64_t tiles = 0;
+    uint32_t n_slots = 0;
+    RsTaps taps;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ran = false;
+};
+
+namespace {
+
+int batch_grow(cuopus_batch *b, DevBuf &d, size_t need, Err &e) { return grow(d, need, e); }
+
+void batch_free(cuopus_batch *b)
+{
+    DevBuf *all[] = {&b->data, &b->off, &b->len, &b->lost, &b->chains, &b->order, &b->slots,
+                     &b->cat, &b->res, &b->files, &b->queue};
+    for (DevBuf *d : all) {
+        if (d->p) { hipFree(d->p); g_dev_bytes -= (long long)d->cap; }
+        d->p = nullptr;
+        d->cap = 0;
+    }
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (auto &ev : b->ev)
+        if (ev) hipEventDestroy(ev);
+}
+
+// Enqueue the launch sequence of a batch (no host work, capturable).
+int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
+{
+    Plan &pl = b->pl;
+    DecodeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.data = (const uint8_t *)b->data.p;
+    a.off = (const uint32_t *)b->off.p;
+    a.len = (const uint32_t *)b->len.p;
+    a.lost = pl.lost.empty() ? nullptr : (const uint8_t *)b->lost.p;
+    a.chains = (const ChainDesc *)b->chains.p;
+    a.slots = (ChainSlot *)b->slots.p;
+    a.n_slots = b->n_slots;
+    a.out = (int16_t *)b->cat.p;
+    a.res = (ChainResult *)b->res.p;
+    a.queue = (uint32_t *)b->queue.p;
+    const bool prof = b->opts.profile && !b->opts.use_graph;
+    const int mode = b->opts.mode;
+    if (prof) HIPCHK(e, hipEventRecord(b->ev[0], st));
+    if (mode == 0) {
+        // plain per-file path: one decode launch and one resample launch per file
+        for (size_t f = 0; f < pl.files.size(); f++) {
+            uint32_t c0 = b->file_chain0[f], nc = b->file_nchain[f];
+            for (uint32_t k0 = 0; k0 < nc; k0 += b->n_slots) {
+                uint32_t bn = nc - k0 < b->n_slots ? nc - k0 : b->n_slots;
+                a.order = (const uint32_t *)b->order.p + c0 + k0;
+                a.n_chains = bn;
+                hipLaunchKernelGGL(k_decode_chains, dim3(bn), dim3(64), 0, st, a);
+                HIPCHK(e, hipGetLastError());
+            }
+            const int r = b->file_rs[f];
+            if (r >= 0) {
+                hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)b->file_tiles[r]), dim3(kRsTile), 0, st,
+                                   (const int16_t *)b->cat.p, (const RsFile *)b->files.p + r, 1, b->taps);
+                HIPCHK(e, hipGetLastError());
+            }
+        }
+        if (prof) {
+            HIPCHK(e, hipEventRecord(b->ev[1], st));
+            HIPCHK(e, hipEventRecord(b->ev[2], st));
+            HIPCHK(e, hipEventRecord(b->ev[3], st));
+        }
+        return 0;
+    }
+    const uint32_t nch = (uint32_t)pl.chains.size();
+    a.order = (const uint32_t *)b->order.p;
+    if (mode == 2) {
+        hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, (uint32_t *)b->queue.p);
+        HIPCHK(e, hipGetLastError());
+        a.n_chains = nch;
+        hipLaunchKernelGGL(k_decode_persistent, dim3(b->n_slots), dim3(64), 0, st, a);
+        HIPCHK(e, hipGetLastError());
+    } else {
+        for (uint32_t c0 = 0; c0 < nch; c0 += b->n_slots) {
+            uint32_t bn = nch - c0 < b->n_slots ? nch - c0 : b->n_slots;
+            a.order = (const uint32_t *)b->order.p + c0;
+            a.n_chains = bn;
+            hipLaunchKernelGGL(k_decode_chains, dim3(bn), dim3(64), 0, st, a);
+            HIPCHK(e, hipGetLastError());
+        }
+    }
+    if (prof) HIPCHK(e, hipEventRecord(b->ev[1], st));
+    if (b->tiles) {
+        if (prof) HIPCHK(e, hipEventRecord(b->ev[2], st));
+        hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)b->tiles), dim3(kRsTile), 0, st,
+                           (const int16_t *)b->cat.p, (const RsFile *)b->files.p, (int)b->rsf.size(), b->taps);
+        HIPCHK(e, hipGetLastError());
+        if (prof) HIPCHK(e, hipEventRecord(b->ev[3], st));
+    } else if (prof) {
+        HIPCHK(e, hipEventRecord(b->ev[2], st));
+        HIPCHK(e, hipEventRecord(b->ev[3], st));
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns, int nfiles,
+                                   const uint8_t *const *lost_masks, const size_t *mask_ns, float *const *out_devs,
+                                   const size_t *out_caps, const cuopus_batch_opts *opts, cuopus_batch **out,
+                                   char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!out) return e.set(CUOPUS_RC_ARG, "out is NULL");
+    *out = nullptr;
+    if (nfiles <= 0 || !oggs || !ns) return e.set(CUOPUS_RC_ARG, "no files");
+    cuopus_batch *b = new cuopus_batch();
+    if (opts) b->opts = *opts;
+    else memset(&b->opts, 0, sizeof(b->opts));
+    Plan &pl = b->pl;
+    pl.files.resize((size_t)nfiles);
+    int rc = 0;
+    for (int f = 0; f < nfiles && !rc; f++) {
+        char sub[256];
+        Err fe(sub, sizeof(sub));
+        if ((rc = demux_file(oggs[f], ns[f], &pl.files[f], true, fe))) rc = e.set(rc, "file %d: %s", f, sub);
+        else if (out_devs && out_caps && (uint64_t)pl.files[f].out16 > out_caps[f])
+            rc = e.set(CUOPUS_RC_ARG, "file %d: output capacity %zu < %lld", f, out_caps[f],
+                       (long long)pl.files[f].out16);
+        else if (out_devs && pl.files[f].out16 > 0 && !out_devs[f])
+            rc = e.set(CUOPUS_RC_ARG, "file %d: out_dev is NULL", f);
+    }
+    if (!rc) rc = merge(pl, e);
+    if (!rc && lost_masks) {
+        pl.lost.assign(pl.off.size(), 0);
+        uint64_t pb = 0;
+        for (int f = 0; f < nfiles && !rc; f++) {
+            size_t np = pl.files[f].d.off.size();
+            if (lost_masks[f]) {
+                if (!mask_ns || mask_ns[f] != np)
+                    rc = e.set(CUOPUS_RC_ARG, "file %d: loss mask size != %zu packets", f, np);
+                else memcpy(pl.lost.data() + pb, lost_masks[f], np);
+            }
+            pb += np;
+        }
+    }
+    if (!rc && pl.chains.empty()) rc = e.set(CUOPUS_RC_SOFT, "no chains");
+    if (rc) { delete b; return rc; }
+    // per-file chain ranges and the work order
+    std::vector<uint32_t> order;
+    b->file_chain0.resize((size_t)nfiles);
+    b->file_nchain.resize((size_t)nfiles);
+    for (int f = 0; f < nfiles; f++) {
+        b->file_chain0[f] = (uint32_t)pl.files[f].chain0;
+        b->file_nchain[f] = (uint32_t)pl.files[f].chains.size();
+    }
+    if (b->opts.lpt_order && b->opts.mode != 0) {
+        lpt_order(pl, &order);
+    } else {
+        order.resize(pl.chains.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = (uint32_t)i;
+    }
+    const uint32_t nch = (uint32_t)pl.chains.size();
+    unsigned max_batch = env_u("CUOPUS_MAX_BATCH", 16384);
+    if (b->opts.mode == 2) b->n_slots = b->opts.n_slots > 0 ? (uint32_t)b->opts.n_slots : 2048;
+    else b->n_slots = max_batch;
+    if (b->opts.mode != 0 && b->n_slots > nch) b->n_slots = nch;
+    if (b->opts.mode == 0) {
+        uint32_t mx = 1;
+        for (int f = 0; f < nfiles; f++) mx = std::max(mx, b->file_nchain[f]);
+        b->n_slots = std::min<uint32_t>(b->n_slots, mx);
+    }
+    b->taps = make_taps();
+    b->file_rs.assign(pl.files.size(), -1);
+    for (size_t f = 0; f < pl.files.size(); f++) {
+        if (!out_devs || pl.files[f].out16 <= 0) continue;
+        b->file_rs[f] = (int)b->rsf.size();
+        RsFile r;
+        r.in_off = pl.files[f].cat_off;
+        r.in_len = pl.files[f].keep;
+        r.out_len = (uint64_t)pl.files[f].out16;
+        r.out = out_devs[f];
+        r.tile0 = b->tiles;
+        uint64_t t = (r.out_len + kRsTile - 1) / kRsTile;
+        b->file_tiles.push_back(t);
+        b->tiles += t;
+        b->rsf.push_back(r);
+    }
+    // in mode 0 each file's resample launch indexes its own entry, so tile0 = 0
+    std::vector<RsFile> rsf_upload = b->rsf;
+    if (b->opts.mode == 0)
+        for (auto &r : rsf_upload) r.tile0 = 0;
+    const uint64_t npk = pl.off.size();
+    if ((rc = grow(b->data, pl.data.size(), e)) || (rc = grow(b->off, 4 * npk, e)) ||
+        (rc = grow(b->len, 4 * npk, e)) || (rc = grow(b->chains, sizeof(ChainDesc) * nch, e)) ||
+        (rc = grow(b->order, 4ull * nch, e)) || (rc = grow(b->res, sizeof(ChainResult) * nch, e)) ||
+        (rc = grow(b->slots, sizeof(ChainSlot) * (size_t)b->n_slots, e)) ||
+        (rc = grow(b->cat, 2 * (pl.total_keep ? pl.total_keep : 1), e)) || (rc = grow(b->queue, 256, e)) ||
+        (rc = grow(b->files, sizeof(RsFile) * (rsf_upload.size() ? rsf_upload.size() : 1), e)) ||
+        (!pl.lost.empty() && (rc = grow(b->lost, pl.lost.size(), e)))) {
+        batch_free(b);
+        delete b;
+        return rc;
+    }
+    hipError_t hs = hipSuccess;
+    if (hs == hipSuccess) hs = hipMemcpy(b->data.p, pl.data.data(), pl.data.size(), hipMemcpyHostToDevice);
+    if (hs == hipSuccess) hs = hipMemcpy(b->off.p, pl.off.data(), 4 * npk, hipMemcpyHostToDevice);
+    if (hs == hipSuccess) hs = hipMemcpy(b->len.p, pl.len.data(), 4 * npk, hipMemcpyHostToDevice);
+    if (hs == hipSuccess) hs = hipMemcpy(b->chains.p, pl.chains.data(), sizeof(ChainDesc) * nch, hipMemcpyHostToDevice);
+    if (hs == hipSuccess) hs = hipMemcpy(b->order.p, order.data(), 4ull * nch, hipMemcpyHostToDevice);
+    if (hs == hipSuccess && !rsf_upload.empty())
+        hs = hipMemcpy(b->files.p, rsf_upload.data(), sizeof(RsFile) * rsf_upload.size(), hipMemcpyHostToDevice);
+    if (hs == hipSuccess && !pl.lost.empty())
+        hs = hipMemcpy(b->lost.p, pl.lost.data(), pl.lost.size(), hipMemcpyHostToDevice);
+    for (int i = 0; i < 4 && hs == hipSuccess; i++) hs = hipEventCreate(&b->ev[i]);
+    if (hs != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return e.set(CUOPUS_RC_GPU, "batch upload: %s", hipGetErrorString(hs));
+    }
+    // packet bytes are kept on the device; release the host copies
+    std::vector<uint8_t>().swap(pl.data);
+    *out = b;
+    return 0;
+}
+
+extern "C" int cuopus_batch_run(cuopus_batch *b, void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!b) return e.set(CUOPUS_RC_ARG, "batch is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if (!b->opts.use_graph) {
+        if ((rc = refuse_capture(st, e))) return rc;
+        rc = batch_enqueue(b, st, e);
+        b->ran = rc == 0;
+        return rc;
+    }
+    if (!b->gexec) {
+        // capture on a private stream so the caller's stream is never in
+        // capture mode, then replay on the caller's stream
+        hipStream_t cs;
+        HIPCHK(e, hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipError_t s1 = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+        if (s1 != hipSuccess) { hipStreamDestroy(cs); return e.set(CUOPUS_RC_GPU, "begin capture: %s", hipGetErrorString(s1)); }
+        rc = batch_enqueue(b, cs, e);
+        hipError_t s2 = hipStreamEndCapture(cs, &b->graph);
+        hipStreamDestroy(cs);
+        if (rc) return rc;
+        if (s2 != hipSuccess) return e.set(CUOPUS_RC_GPU, "end capture: %s", hipGetErrorString(s2));
+        HIPCHK(e, hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(e, hipGraphLaunch(b->gexec, st));
+    b->ran = true;
+    return 0;
+}
+
+extern "C" int cuopus_batch_check(cuopus_batch *b, void *stream, int64_t *out_lens, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!b) return e.set(CUOPUS_RC_ARG, "batch is NULL");
+    if (!b->ran) return e.set(CUOPUS_RC_ARG, "batch has not run");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t nch = b->pl.chains.size();
+    std::vector<ChainResult> res(nch);
+    HIPCHK(e, hipMemcpyAsync(res.data(), b->res.p, sizeof(ChainResult) * nch, hipMemcpyDeviceToHost, st));
+    HIPCHK(e, hipStreamSynchronize(st));
+    for (size_t c = 0; c < nch; c++) {
+        if (res[c].stop != 0) return e.set(CUOPUS_RC_SOFT, "chain %zu: device decode error %d", c, res[c].stop);
+        if (res[c].samples != b->pl.cap[c])
+            return e.set(CUOPUS_RC_SOFT, "chain %zu: decoded %u samples, expected %llu", c, res[c].samples,
+                         (unsigned long long)b->pl.cap[c]);
+    }
+    if (out_lens)
+        for (size_t f = 0; f < b->pl.files.size(); f++) out_lens[f] = b->pl.files[f].out16;
+    return 0;
+}
+
+extern "C" int cuopus_batch_info(const cuopus_batch *b, int64_t *c)
+{
+    if (!b || !c) return CUOPUS_RC_ARG;
+    int64_t dec = 0, out16 = 0, bytes = 0;
+    for (uint64_t v : b->pl.cap) dec += (int64_t)v;
+    for (auto &f : b->pl.files) { out16 += f.out16; bytes += (int64_t)f.d.data.size(); }
+    c[0] = (int64_t)b->pl.chains.size();
+    c[1] = (int64_t)b->pl.off.size();
+    c[2] = (int64_t)b->pl.total_keep;
+    c[3] = dec;
+    c[4] = out16;
+    c[5] = bytes;
+    return 0;
+}
+
+extern "C" int cuopus_batch_kernel_ms(const cuopus_batch *b, float *decode_ms, float *resample_ms)
+{
+    if (!b || !b->opts.profile || b->opts.use_graph) return CUOPUS_RC_ARG;
+    float d = 0, r = 0;
+    if (hipEventElapsedTime(&d, b->ev[0], b->ev[1]) != hipSuccess) return CUOPUS_RC_GPU;
+    if (hipEventElapsedTime(&r, b->ev[2], b->ev[3]) != hipSuccess) return CUOPUS_RC_GPU;
+    if (decode_ms) *decode_ms = d;
+    if (resample_ms) *resample_ms = r;
+    return 0;
+}
+
+extern "C" void cuopus_batch_destroy(cuopus_batch *b)
+{
+    if (!b) return;
+    batch_free(b);
+    delete b;
+}
+
 extern "C" int cuopus_warmup(char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);

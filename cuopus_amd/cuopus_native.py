@@ -43,6 +43,12 @@ def _find_lib():
 
 _lib = None
 P = ctypes.c_void_p
+
+
+class BatchOpts(ctypes.Structure):
+    """Mirror of cuopus_batch_opts (include/cuopus.h)."""
+    _fields_ = [("mode", ctypes.c_int), ("n_slots", ctypes.c_int), ("lpt_order", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("profile", ctypes.c_int)]
 SZ = ctypes.c_size_t
 I64P = ctypes.POINTER(ctypes.c_int64)
 
@@ -71,6 +77,16 @@ def _load():
         "cuopus_last_arena_overflow": (ctypes.c_int, []),
         "cuopus_workspace_bytes": (ctypes.c_longlong, []),
         "cuopus_release_workspaces": (None, []),
+        "cuopus_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(SZ), ctypes.c_int,
+                                               ctypes.POINTER(P), ctypes.POINTER(SZ), ctypes.POINTER(P),
+                                               ctypes.POINTER(SZ), ctypes.POINTER(BatchOpts),
+                                               ctypes.POINTER(P), ctypes.c_char_p, SZ]),
+        "cuopus_batch_run": (ctypes.c_int, [P, P, ctypes.c_char_p, SZ]),
+        "cuopus_batch_check": (ctypes.c_int, [P, P, I64P, ctypes.c_char_p, SZ]),
+        "cuopus_batch_info": (ctypes.c_int, [P, I64P]),
+        "cuopus_batch_kernel_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float),
+                                                  ctypes.POINTER(ctypes.c_float)]),
+        "cuopus_batch_destroy": (None, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -253,3 +269,80 @@ def last_arena_overflow() -> bool:
 
 def release_workspaces():
     _load().cuopus_release_workspaces()
+
+
+MODE_PLAIN, MODE_ONE_LAUNCH, MODE_PERSISTENT = 0, 1, 2
+
+
+class Batch:
+    """A resident batch: packets demuxed and uploaded once, then decoded on the
+    device as many times as asked (run), without host work per run.
+
+    mode: MODE_PLAIN (one decode and one resample launch per file),
+    MODE_ONE_LAUNCH (every chain in one decode launch) or MODE_PERSISTENT
+    (n_slots wavefronts pulling chains from a device work queue).
+    Outputs are float32 16 kHz device tensors, one per file (``outputs``).
+    """
+
+    def __init__(self, items, lost_masks=None, mode=MODE_PERSISTENT, n_slots=0, lpt_order=True,
+                 use_graph=False, profile=False, outputs=True):
+        import torch
+        lib = _load()
+        self._lib = lib
+        self._h = P(0)
+        blobs = [_as_bytes(x) for x in items]
+        n = len(blobs)
+        if n == 0:
+            raise CuopusError(3, "no files")
+        self.n16 = [probe(b) for b in blobs]
+        self.outputs = [torch.empty(k, dtype=torch.float32, device="cuda") for k in self.n16] if outputs else None
+        ba = (ctypes.c_char_p * n)(*blobs)
+        la = (SZ * n)(*[len(b) for b in blobs])
+        oa = (P * n)(*[o.data_ptr() for o in self.outputs]) if outputs else None
+        ca = (SZ * n)(*self.n16) if outputs else None
+        ma = mn = None
+        self._masks = None
+        if lost_masks is not None:
+            self._masks = [None if m is None else np.ascontiguousarray(np.asarray(m, dtype=np.uint8))
+                           for m in lost_masks]
+            ma = (P * n)(*[None if m is None else m.ctypes.data for m in self._masks])
+            mn = (SZ * n)(*[0 if m is None else m.size for m in self._masks])
+        opts = BatchOpts(mode, n_slots, 1 if lpt_order else 0, 1 if use_graph else 0, 1 if profile else 0)
+        err = ctypes.create_string_buffer(_ERRCAP)
+        _check(lib.cuopus_batch_create(ba, la, n, oa, ca, ma, mn, ctypes.byref(opts), ctypes.byref(self._h),
+                                       err, _ERRCAP), err)
+        c = (ctypes.c_int64 * 6)()
+        lib.cuopus_batch_info(self._h, c)
+        self.chains, self.packets, self.playable, self.decoded, self.out16, self.packet_bytes = list(c)
+
+    def run(self, stream=None):
+        err = ctypes.create_string_buffer(_ERRCAP)
+        _check(self._lib.cuopus_batch_run(self._h, stream if stream is not None else _stream(), err, _ERRCAP), err)
+
+    def check(self, stream=None):
+        """Wait for the last run and verify every chain decoded fully; returns output lengths."""
+        n = len(self.n16)
+        ol = (ctypes.c_int64 * n)()
+        err = ctypes.create_string_buffer(_ERRCAP)
+        _check(self._lib.cuopus_batch_check(self._h, stream if stream is not None else _stream(), ol, err,
+                                            _ERRCAP), err)
+        return list(ol)
+
+    def kernel_ms(self):
+        """(decode, resample) milliseconds of the last profiled run (events on the launch stream)."""
+        d, r = ctypes.c_float(0), ctypes.c_float(0)
+        rc = self._lib.cuopus_batch_kernel_ms(self._h, ctypes.byref(d), ctypes.byref(r))
+        if rc:
+            raise CuopusError(rc, "batch was not created with profile=True (or uses a graph)")
+        return d.value, r.value
+
+    def close(self):
+        if self._h:
+            self._lib.cuopus_batch_destroy(self._h)
+            self._h = P(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -116,6 +116,41 @@ int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, 
 int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, float *out_dev, size_t out_cap,
                                 int64_t *out_len, void *stream, char *errbuf, size_t errcap);
 
+/* ---------------------------------------------------------------------------
+ * Resident batches: demux + upload once, then decode from HBM-resident
+ * inputs any number of times.  A batch owns its device memory; decoded f32
+ * 16 kHz output goes to caller-owned device buffers given at creation.
+ * Runs are asynchronous on the given stream; cuopus_batch_check reads back
+ * and validates the last run (synchronising the stream).
+ * ------------------------------------------------------------------------- */
+typedef struct cuopus_batch cuopus_batch;
+
+typedef struct {
+    int mode;        /* 0: one decode + one resample launch per file (the plain
+                        per-file path); 1: every chain in one launch, one
+                        workgroup per chain; 2: persistent kernel, n_slots
+                        wavefronts pull chains from a device work queue */
+    int n_slots;     /* mode 2: resident wavefronts (0 = library default) */
+    int lpt_order;   /* 1: longest chains first (modes 1-2) */
+    int use_graph;   /* 1: capture the launch sequence into a HIP graph on
+                        the first run and replay it afterwards */
+    int profile;     /* 1: time the decode and resample kernels with events */
+} cuopus_batch_opts;
+
+/* lost_masks may be NULL (no loss) or hold one mask per file (NULL entries
+ * allowed); mask_ns[i] must equal file i's audio packet count. */
+int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns, int nfiles, const uint8_t *const *lost_masks,
+                        const size_t *mask_ns, float *const *out_devs, const size_t *out_caps,
+                        const cuopus_batch_opts *opts, cuopus_batch **out, char *errbuf, size_t errcap);
+int cuopus_batch_run(cuopus_batch *b, void *stream, char *errbuf, size_t errcap);
+int cuopus_batch_check(cuopus_batch *b, void *stream, int64_t *out_lens, char *errbuf, size_t errcap);
+/* counts: [0] chains, [1] packets, [2] playable 48 kHz samples, [3] samples
+ * decoded including preroll, [4] f32 16 kHz outputs, [5] packet bytes */
+int cuopus_batch_info(const cuopus_batch *b, int64_t *counts6);
+/* Kernel times of the last profiled run in milliseconds (decode, resample). */
+int cuopus_batch_kernel_ms(const cuopus_batch *b, float *decode_ms, float *resample_ms);
+void cuopus_batch_destroy(cuopus_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
