@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the batched Opus decode path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                    [--mode persistent|one|plain] [--slots N] [--no-cpu]
+
+A step is one pass of the decode path over the whole workload with its packet
+bytes already resident in HBM: every chain decoded (fixed-point Opus, int16
+48 kHz, preroll dropped) and resampled to float32 16 kHz into per-file device
+buffers.  The metric is playable audio-seconds decoded per second (sum of
+keep / 48000 per step over the step's wall time, hipDeviceSynchronize on both
+sides).  Prints one JSON line.
+
+Default workload: one 2 h CELT-only fullband mono stream, 64 kbps VBR, frame
+size drawn per 1 s segment from {2.5, 5, 10, 20} ms; 7200 chains of 1 s with
+250 ms preroll, all in one launch.
+
+roofline: the decode kernel is the dominant kernel.  Its algorithmic bytes
+per launch are the packet bytes each chain reads (preroll packets once per
+chain), 8 B of offset/length per packet read, 2 B per playable int16 sample
+written, 40 B of chain descriptor + result per chain; achieved = those bytes
+over the kernel's average duration measured with HIP events on its launch
+stream.  The resampler's own roofline (10 B per 16 kHz output) is reported
+beside it.  traffic comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+in profiles/ for the same workload when present.
+
+cpu_baseline: fixed-point libopus (built from the reference's vendored 1.5.2
+sources) decoding the identical chain list with a fresh decoder per chain on
+the host's cores, plus the float32 resampler restated in C, timed on the
+whole workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BOX_CPU_SHARE = 16
+
+WORKLOADS = {
+    "celt_fb_64k_2h": {
+        "files": ["corpus/bench/celt_fb_64k_2h.opus"],
+        "desc": "CELT-only FB mono 64 kbps VBR, frame size per 1 s from {2.5,5,10,20} ms, one 2 h file, "
+                "1 s chains + 250 ms preroll, one launch, f32 16 kHz out",
+    },
+}
+
+
+def _args():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="celt_fb_64k_2h", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="persistent", choices=["persistent", "one", "plain"])
+    ap.add_argument("--slots", type=int, default=0, help="persistent wavefronts (0: tuned default)")
+    ap.add_argument("--graph", action="store_true", help="replay the launch sequence as a HIP graph")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-file comparison")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def _read_traffic(workload):
+    """HBM bytes per decode launch from the committed rocprofv3 counter summary
+    (a line 'traffic_bytes_per_launch: N'), or None."""
+    path = os.path.join(ROOT, "profiles", "decode_%s_pmc.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return int(float(line.split(":", 1)[1]))
+            except ValueError:
+                return None
+    return None
+
+
+def _time_batch(b, steps, warmup, torch):
+    for _ in range(warmup):
+        b.run()
+        b.check()
+    dec_ms, rs_ms = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+        torch.cuda.synchronize()
+        if b.profiled:
+            d, r = b.kernel_ms()
+            dec_ms.append(d)
+            rs_ms.append(r)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    b.check()
+    return wall, dec_ms, rs_ms
+
+
+def _cpu_baseline(blobs, threads):
+    """libopus on the host's cores over the identical chain list."""
+    import numpy as np
+    from tests import oracle_lib as ol
+    play = 0
+    t_dec = t_rs = 0.0
+    nch = 0
+    for blob in blobs:
+        d, chains = ol.demux_chains(blob)
+        data, off, ln = d.blob()
+        t0 = time.perf_counter()
+        ref = ol.decode_chains(data, off, ln, chains, threads=threads)
+        t1 = time.perf_counter()
+        out = ol.resample(ref["pcm"])
+        t2 = time.perf_counter()
+        assert (ref["ret"] >= 0).all()
+        play += int(ref["pcm"].size)
+        nch += len(chains)
+        t_dec += t1 - t0
+        t_rs += t2 - t1
+        del out
+    secs = play / 48000.0
+    return {
+        "value": round(secs / (t_dec + t_rs), 1),
+        "unit": "audio-s/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": "whole workload (%d chains, %.0f audio-s): fixed-point libopus 1.5.2 decode on %d threads "
+                  "(%.2f s) + float32 resampler on 1 thread (%.2f s)" % (nch, secs, threads, t_dec, t_rs),
+    }
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only (--gpus 1): there is no multi-GPU decode path; chains would shard "
+              "by file or by contiguous chain range with a 24-sample resampler halo", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[a.workload]
+    paths = [os.path.join(ROOT, p) for p in w["files"]]
+    for p in paths:
+        if not os.path.exists(p):
+            sys.exit("workload input %s missing: generate it with __graft_entry__.build()" % p)
+    blobs = [open(p, "rb").read() for p in paths]
+
+    import torch
+    from cuopus_amd import cuopus_native as cn
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    torch.cuda.set_device(0)
+    cn.warmup()
+    modes = {"persistent": cn.MODE_PERSISTENT, "one": cn.MODE_ONE_LAUNCH, "plain": cn.MODE_PLAIN}
+    b = cn.Batch(blobs, mode=modes[a.mode], n_slots=a.slots, use_graph=a.graph, profile=not a.graph)
+    wall, dec_ms, rs_ms = _time_batch(b, a.steps, a.warmup, torch)
+    audio_s = b.playable / 48000.0
+    ms = 1000.0 * wall / a.steps
+    value = audio_s * a.steps / wall
+
+    # dominant kernel roofline (decode), resampler beside it
+    alg_dec = b.chain_packet_bytes + 8 * b.chain_packets + 2 * b.playable + 40 * b.chains
+    roof = None
+    rs_roof = None
+    if dec_ms:
+        d_avg = sum(dec_ms) / len(dec_ms)
+        r_avg = sum(rs_ms) / len(rs_ms)
+        ach = alg_dec / (d_avg * 1e-3) / 1e9
+        traffic = _read_traffic(a.workload)
+        roof = {"bound": "hbm", "achieved": round(ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": "k_decode_persistent"
+                if a.mode == "persistent" else "k_decode_chains", "kernel_ms": round(d_avg, 3),
+                "algorithmic_bytes": int(alg_dec)}
+        rs_bytes = 10 * b.out16
+        rs_ach = rs_bytes / (r_avg * 1e-3) / 1e9
+        rs_roof = {"bound": "hbm", "achieved": round(rs_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(rs_ach / HBM_PEAK_GBS, 4), "kernel": "k_resample_48to16",
+                   "kernel_ms": round(r_avg, 3), "algorithmic_bytes": int(rs_bytes)}
+    b.close()
+    del b
+
+    plain = None
+    if not a.no_plain and a.mode != "plain":
+        bp = cn.Batch(blobs, mode=cn.MODE_PLAIN, profile=True)
+        pw, _, _ = _time_batch(bp, max(1, min(a.steps, 5)), 1, torch)
+        plain = round(bp.playable / 48000.0 * max(1, min(a.steps, 5)) / pw, 1)
+        bp.close()
+        del bp
+
+    cpu = None
+    if not a.no_cpu:
+        th = a.cpu_threads or min(BOX_CPU_SHARE, os.cpu_count() or 1)
+        cpu = _cpu_baseline(blobs, th)
+
+    line = {
+        "metric": "audio-sec decoded/s on 1 MI355X, bit-exact: batched vs plain; serve p99 ms",
+        "value": round(value, 1),
+        "unit": "audio-s/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "desc": w["desc"], "mode": a.mode, "graph": a.graph,
+                   "files": len(blobs), "audio_s_per_step": round(audio_s, 3)},
+        "plain_value": plain,
+        "roofline": roof,
+        "resampler_roofline": rs_roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,6 +28,8 @@ struct ChainDesc {
     uint32_t trim;    // samples of the chain's output dropped before the keep window
     uint32_t keep;    // samples kept
     uint64_t dst;     // offset of the keep window in the compacted output
+    uint32_t rec_end; // per-packet records (gate): only packets below this index
+    uint32_t pad;
 };
 
 struct ChainResult {
@@ -88,7 +90,10 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, C
             r = opus_decode_packet(st, pk, (int)a.len[p], pcm, kMaxPktSamples, 0, sc);
         }
         if (r < 0) { stop = r; break; }
-        if (a.pkt_nsamp && lane == 0) {
+        // a packet held by several chains (preroll overlap) is recorded by the
+        // last chain that holds it, which owns packets below the next chain's
+        // first packet
+        if (a.pkt_nsamp && lane == 0 && p < ch.rec_end) {
             a.pkt_nsamp[p] = r;
             a.pkt_rng[p] = st->rangeFinal;
         }

@@ -211,6 +211,8 @@ int merge(Plan &pl, Err &e)
             cd.trim = c.trim;
             cd.keep = c.keep;
             cd.dst = cat + fk;
+            cd.rec_end = 0;  // no per-packet records outside the gate
+            cd.pad = 0;
             pl.cap[cb + i] = s;
             fk += c.keep;
         }
@@ -579,7 +581,7 @@ extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint
     uint64_t cat = 0;
     for (uint32_t c = 0; c < n_chains; c++) {
         if ((uint64_t)first[c] + npkts[c] > n_packets) return e.set(CUOPUS_RC_ARG, "chain %u out of range", c);
-        pl.chains[c] = {first[c], npkts[c], trim[c], keep[c], cat};
+        pl.chains[c] = {first[c], npkts[c], trim[c], keep[c], cat, c + 1 < n_chains ? first[c + 1] : 0xFFFFFFFFu, 0};
         uint64_t s = 0;
         for (uint32_t k = 0; k < npkts[c]; k++)
             s += (uint64_t)cuo_host::pkt_samples48(data + off[first[c] + k], len[first[c] + k]);
@@ -651,6 +653,7 @@ struct cuopus_batch {
     std::vector<int> file_rs;         // per file: its rsf entry or -1
     uint64_t tiles = 0;
     uint32_t n_slots = 0;
+    int64_t chain_pkts = 0, chain_bytes = 0;
     RsTaps taps;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -840,6 +843,10 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
     if (b->opts.mode == 0)
         for (auto &r : rsf_upload) r.tile0 = 0;
     const uint64_t npk = pl.off.size();
+    for (const ChainDesc &c : pl.chains) {
+        b->chain_pkts += c.npkts;
+        for (uint32_t k = 0; k < c.npkts; k++) b->chain_bytes += pl.len[c.first + k];
+    }
     if ((rc = grow(b->data, pl.data.size(), e)) || (rc = grow(b->off, 4 * npk, e)) ||
         (rc = grow(b->len, 4 * npk, e)) || (rc = grow(b->chains, sizeof(ChainDesc) * nch, e)) ||
         (rc = grow(b->order, 4ull * nch, e)) || (rc = grow(b->res, sizeof(ChainResult) * nch, e)) ||
@@ -937,6 +944,8 @@ extern "C" int cuopus_batch_info(const cuopus_batch *b, int64_t *c)
     c[3] = dec;
     c[4] = out16;
     c[5] = bytes;
+    c[6] = b->chain_pkts;
+    c[7] = b->chain_bytes;
     return 0;
 }
 

@@ -290,6 +290,7 @@ class Batch:
         lib = _load()
         self._lib = lib
         self._h = P(0)
+        self.profiled = bool(profile) and not use_graph
         blobs = [_as_bytes(x) for x in items]
         n = len(blobs)
         if n == 0:
@@ -311,9 +312,10 @@ class Batch:
         err = ctypes.create_string_buffer(_ERRCAP)
         _check(lib.cuopus_batch_create(ba, la, n, oa, ca, ma, mn, ctypes.byref(opts), ctypes.byref(self._h),
                                        err, _ERRCAP), err)
-        c = (ctypes.c_int64 * 6)()
+        c = (ctypes.c_int64 * 8)()
         lib.cuopus_batch_info(self._h, c)
-        self.chains, self.packets, self.playable, self.decoded, self.out16, self.packet_bytes = list(c)
+        (self.chains, self.packets, self.playable, self.decoded, self.out16, self.packet_bytes,
+         self.chain_packets, self.chain_packet_bytes) = list(c)
 
     def run(self, stream=None):
         err = ctypes.create_string_buffer(_ERRCAP)

@@ -145,8 +145,10 @@ int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns, int nfiles
 int cuopus_batch_run(cuopus_batch *b, void *stream, char *errbuf, size_t errcap);
 int cuopus_batch_check(cuopus_batch *b, void *stream, int64_t *out_lens, char *errbuf, size_t errcap);
 /* counts: [0] chains, [1] packets, [2] playable 48 kHz samples, [3] samples
- * decoded including preroll, [4] f32 16 kHz outputs, [5] packet bytes */
-int cuopus_batch_info(const cuopus_batch *b, int64_t *counts6);
+ * decoded including preroll, [4] f32 16 kHz outputs, [5] packet bytes,
+ * [6] packets read by the chains (preroll packets counted once per chain),
+ * [7] packet bytes read by the chains */
+int cuopus_batch_info(const cuopus_batch *b, int64_t *counts8);
 /* Kernel times of the last profiled run in milliseconds (decode, resample). */
 int cuopus_batch_kernel_ms(const cuopus_batch *b, float *decode_ms, float *resample_ms);
 void cuopus_batch_destroy(cuopus_batch *b);

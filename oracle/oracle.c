@@ -110,11 +110,22 @@ static void *orc_worker(void *arg)
         pthread_mutex_unlock(&j->mu);
         if (c0 >= j->n_chains) break;
         for (int c = c0; c < c0 + 4 && c < j->n_chains; c++) {
-            uint32_t f = j->first[c];
+            uint32_t f = j->first[c], np = j->npkts[c];
+            int32_t *ns = (int32_t *)calloc(np ? np : 1, sizeof(int32_t));
+            uint32_t *rg = (uint32_t *)calloc(np ? np : 1, sizeof(uint32_t));
             j->chain_ret[c] = orc_decode_chain(
                 j->data, j->off + f, j->len + f, j->lost ? j->lost + f : NULL,
-                (int)j->npkts[c], j->pcm + j->pcm_off[c], (long)j->pcm_cap[c],
-                j->nsamp ? j->nsamp + f : NULL, j->rng ? j->rng + f : NULL);
+                (int)np, j->pcm + j->pcm_off[c], (long)j->pcm_cap[c], ns, rg);
+            /* A packet inside several chains (preroll overlap) is recorded by
+             * the last chain that holds it: chain c owns packets below the
+             * first packet of chain c + 1.  Deterministic on any thread count. */
+            uint32_t own_end = c + 1 < j->n_chains ? j->first[c + 1] : 0xFFFFFFFFu;
+            for (uint32_t k = 0; k < np && f + k < own_end; k++) {
+                if (j->nsamp) j->nsamp[f + k] = ns[k];
+                if (j->rng) j->rng[f + k] = rg[k];
+            }
+            free(ns);
+            free(rg);
         }
     }
     return NULL;

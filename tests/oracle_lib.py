@@ -141,3 +141,22 @@ RECIPES = {
     "mixed": "--seed 18 --dur 20 --signal mixed --app audio --bw fb --brange 12000 64000",
     "silk_long": "--seed 19 --dur 12 --signal speech --app voip --bw wb --force silk --bitrate 20000 --frame 2880",
 }
+
+
+# Benchmark inputs (minutes to generate; written once under corpus/bench)
+BENCH = os.path.join(ROOT, "corpus", "bench")
+BENCH_RECIPES = {
+    "celt_fb_64k_2h": "--seed 0x0C0FFEE2 --dur 7200 --signal music --app lowdelay --bw fb --vbr 1 --bitrate 64000 "
+                      "--frameset 120,240,480,960",
+}
+
+
+def bench_corpus(name):
+    """Generate (once) a benchmark input with oracle/_ref/mkcorpus and return its path."""
+    os.makedirs(BENCH, exist_ok=True)
+    path = os.path.join(BENCH, name + ".opus")
+    if not os.path.exists(path):
+        tmp = path + ".tmp%d" % os.getpid()
+        subprocess.run([MKCORPUS, "--out", tmp] + shlex.split(BENCH_RECIPES[name]), check=True)
+        os.replace(tmp, path)
+    return path
