@@ -63,6 +63,8 @@ def _args():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-file comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -152,6 +154,14 @@ def main():
         sys.exit("bench.py needs a GPU")
     torch.cuda.set_device(0)
     cn.warmup()
+    if a.copy_calib_mib:
+        n = a.copy_calib_mib << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     modes = {"persistent": cn.MODE_PERSISTENT, "one": cn.MODE_ONE_LAUNCH, "plain": cn.MODE_PLAIN}
     b = cn.Batch(blobs, mode=modes[a.mode], n_slots=a.slots, use_graph=a.graph, profile=not a.graph)
     wall, dec_ms, rs_ms = _time_batch(b, a.steps, a.warmup, torch)

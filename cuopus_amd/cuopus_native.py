@@ -310,7 +310,7 @@ class Batch:
             mn = (SZ * n)(*[0 if m is None else m.size for m in self._masks])
         opts = BatchOpts(mode, n_slots, 1 if lpt_order else 0, 1 if use_graph else 0, 1 if profile else 0)
         err = ctypes.create_string_buffer(_ERRCAP)
-        _check(lib.cuopus_batch_create(ba, la, n, oa, ca, ma, mn, ctypes.byref(opts), ctypes.byref(self._h),
+        _check(lib.cuopus_batch_create(ba, la, n, ma, mn, oa, ca, ctypes.byref(opts), ctypes.byref(self._h),
                                        err, _ERRCAP), err)
         c = (ctypes.c_int64 * 8)()
         lib.cuopus_batch_info(self._h, c)
