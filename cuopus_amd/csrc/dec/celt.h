@@ -50,23 +50,32 @@ struct CeltState {
 };
 
 struct CeltScratch {
-    int16_t X[960];
-    int32_t freq[960];
-    int16_t norm[8 * 78];
-    int16_t tmp[176];
-    int32_t iy[176];
+    // The frame-decode buffers and the pitch-based concealment buffers are
+    // never live at the same time, so they share storage (the scratch block
+    // is sized to sit in LDS beside the decoder state).
+    union {
+        struct {  // frame decode, noise-based concealment
+            int16_t X[960];
+            int32_t freq[960];
+            int16_t norm[8 * 78];
+            int16_t tmp[176];
+            int32_t iy[176];
+        };
+        struct {  // pitch-based concealment
+            int16_t exc[kMaxPeriod + kLpcOrder];
+            int16_t fir_tmp[kMaxPeriod];
+            int16_t lp_pitch[kDecBuf >> 1];
+            int32_t xcorr[kMaxPeriod];
+            int16_t plc_a[kMaxPeriod + 8];                 // autocorrelation copy / pitch x4
+            int16_t plc_b[960 + kOverlap + kLpcOrder];     // IIR history / pitch y4
+        };
+    };
     int32_t pulses[kNbBands], fine_quant[kNbBands], fine_priority[kNbBands];
     int32_t cap[kNbBands], offsets[kNbBands], tf_res[kNbBands];
     int32_t bits1[kNbBands], bits2[kNbBands], thresh[kNbBands], trim_offset[kNbBands];
     uint8_t collapse[kNbBands];
-    // loss concealment
-    int16_t exc[kMaxPeriod + kLpcOrder];
-    int16_t fir_tmp[kMaxPeriod];
-    int16_t lp_pitch[kDecBuf >> 1];
-    int32_t xcorr[kMaxPeriod];
-    int16_t plc_a[kMaxPeriod + 8];                 // autocorrelation copy / pitch x4
-    int16_t plc_b[960 + kOverlap + kLpcOrder];     // IIR history / pitch y4
     int32_t etmp[kOverlap];
+    RangeDec dec;  // decoder of a frame opened here (redundancy frames)
 };
 
 // ------------------------------------------------------------------ reset
@@ -1375,8 +1384,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
         return frame_size;
     }
     if (st->loss_duration == 0) st->skip_plc = 0;
-    RangeDec _dec;
-    if (dec == nullptr) { ec_init(&_dec, data, (uint32_t)len); dec = &_dec; }
+    if (dec == nullptr) { ec_init(&sc->dec, data, (uint32_t)len); dec = &sc->dec; }
     for (int i = 0; i < kNbBands; i++) oldBandE[i] = max16(oldBandE[i], oldBandE[kNbBands + i]);
 
     int32_t total_bits = len * 8;

@@ -38,12 +38,17 @@ struct OpusDecState {
     uint32_t rangeFinal;
 };
 
+// Per-call temporaries.  SILK and CELT decode one after the other inside a
+// frame, so their scratch shares storage.
 struct DecScratch {
-    CeltScratch celt;
-    SilkScratch silk;
+    union {
+        CeltScratch celt;
+        SilkScratch silk;
+    };
     int16_t pcm_silk[480];
     int16_t pcm_transition[240];
     int16_t redundant_audio[240];
+    RangeDec dec;  // the frame's range decoder (shared by SILK and CELT)
 };
 
 CUO_FN void opus_dec_init(OpusDecState *st)
@@ -226,7 +231,7 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
 {
     constexpr bool HasData = Kind == kFrameData;
     const int F20 = 960, F10 = 480, F5 = 240, F2_5 = 120;
-    RangeDec dec;
+    RangeDec &dec = sc->dec;
     if (frame_size < F2_5) return OPUS_BUFFER_TOO_SMALL;
     frame_size = imin(frame_size, 48000 / 25 * 3);
     if constexpr (HasData) {

@@ -37,12 +37,19 @@ struct ChainResult {
     uint32_t samples;  // samples produced by the chain's packets
 };
 
-// Per-slot decoder memory: persistent state, scratch and one packet of PCM.
-struct alignas(256) ChainSlot {
+// Decoder memory of one wavefront: the chain's persistent state and the
+// per-call scratch (about 31 KB).  Placed in LDS (kLdsState) or in the slot
+// arena in HBM.
+struct WaveState {
     OpusDecState st;
     DecScratch sc;
-    int16_t pcm[kMaxPktSamples];
 };
+
+// Per-slot HBM arena: one packet of PCM, then (HBM-state layout only) the
+// wavefront's decoder memory.
+constexpr size_t kPcmBytes = (kMaxPktSamples * 2 + 255) / 256 * 256;
+constexpr size_t kSlotBytesLds = kPcmBytes;
+constexpr size_t kSlotBytesHbm = kPcmBytes + (sizeof(WaveState) + 255) / 256 * 256;
 
 struct DecodeArgs {
     const uint8_t *data;
@@ -52,8 +59,9 @@ struct DecodeArgs {
     const ChainDesc *chains;
     const uint32_t *order;      // nullable: work order (chain indices)
     uint32_t n_chains;
-    ChainSlot *slots;
+    uint8_t *slots;             // slot arena, slot_bytes per slot
     uint32_t n_slots;
+    uint32_t slot_bytes;
     int16_t *out;               // compacted 48 kHz PCM
     ChainResult *res;
     int32_t *pkt_nsamp;         // nullable, per packet
@@ -62,12 +70,11 @@ struct DecodeArgs {
 };
 
 // Decode one chain on the calling wavefront.
-__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, ChainSlot *slot, int lane)
+__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int16_t *pcm, int lane)
 {
     const ChainDesc ch = a.chains[c];
-    OpusDecState *st = &slot->st;
-    DecScratch *sc = &slot->sc;
-    int16_t *pcm = slot->pcm;
+    OpusDecState *st = &ws->st;
+    DecScratch *sc = &ws->sc;
     opus_dec_init(st);
     uint64_t pos = 0;
     int32_t stop = 0;
@@ -109,20 +116,25 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, C
 }
 
 // Plain variant: workgroup g decodes chain order[g] in slot g.
+template <bool kLdsState>
 __global__ __launch_bounds__(64) void k_decode_chains(DecodeArgs a)
 {
     const uint32_t g = blockIdx.x;
     if (g >= a.n_chains) return;
     const uint32_t c = a.order ? a.order[g] : g;
-    decode_one_chain(a, c, &a.slots[g], (int)threadIdx.x);
+    uint8_t *base = a.slots + (size_t)g * a.slot_bytes;
+    if constexpr (kLdsState) {
+        __shared__ WaveState lds_ws;
+        decode_one_chain(a, c, &lds_ws, (int16_t *)base, (int)threadIdx.x);
+    } else {
+        decode_one_chain(a, c, (WaveState *)(base + kPcmBytes), (int16_t *)base, (int)threadIdx.x);
+    }
 }
 
 // Persistent variant: n_slots wavefronts pull chains from an atomic queue until
 // it is drained (every wavefront exits once the counter passes n_chains).
-__global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
+__device__ inline void persistent_loop(const DecodeArgs &a, WaveState *ws, int16_t *pcm)
 {
-    const uint32_t g = blockIdx.x;
-    ChainSlot *slot = &a.slots[g];
     const int lane = (int)threadIdx.x;
     while (true) {
         uint32_t w = 0;
@@ -130,7 +142,19 @@ __global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
         w = __shfl(w, 0, 64);
         if (w >= a.n_chains) break;
         const uint32_t c = a.order ? a.order[w] : w;
-        decode_one_chain(a, c, slot, lane);
+        decode_one_chain(a, c, ws, pcm, lane);
+    }
+}
+
+template <bool kLdsState>
+__global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
+{
+    uint8_t *base = a.slots + (size_t)blockIdx.x * a.slot_bytes;
+    if constexpr (kLdsState) {
+        __shared__ WaveState lds_ws;
+        persistent_loop(a, &lds_ws, (int16_t *)base);
+    } else {
+        persistent_loop(a, (WaveState *)(base + kPcmBytes), (int16_t *)base);
     }
 }
 

@@ -98,6 +98,23 @@ unsigned env_u(const char *name, unsigned dflt)
     return (v && atoi(v) > 0) ? (unsigned)atoi(v) : dflt;
 }
 
+// Decoder-state placement: LDS (default) or the HBM slot arena.
+bool state_in_lds_default() { return env_u("CUOPUS_STATE_HBM", 0) == 0; }
+size_t slot_bytes(bool lds) { return lds ? kSlotBytesLds : kSlotBytesHbm; }
+
+void launch_chains(bool lds, uint32_t grid, hipStream_t st, const DecodeArgs &a)
+{
+    if (lds) hipLaunchKernelGGL(k_decode_chains<true>, dim3(grid), dim3(64), 0, st, a);
+    else hipLaunchKernelGGL(k_decode_chains<false>, dim3(grid), dim3(64), 0, st, a);
+}
+
+void launch_persistent(bool lds, uint32_t grid, hipStream_t st, const DecodeArgs &a)
+{
+    if (lds) hipLaunchKernelGGL(k_decode_persistent<true>, dim3(grid), dim3(64), 0, st, a);
+    else hipLaunchKernelGGL(k_decode_persistent<false>, dim3(grid), dim3(64), 0, st, a);
+}
+
+
 // Kaiser I0 by its power series, in double.
 double bessel_i0(double x)
 {
@@ -279,7 +296,8 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     uint32_t n_slots = o.persistent ? env_u("CUOPUS_SLOTS", 4096) : (nch < max_batch ? nch : max_batch);
     if (n_slots > nch) n_slots = nch;
     if (n_slots == 0) n_slots = 1;
-    if ((rc = grow(W.slots, sizeof(ChainSlot) * (size_t)n_slots, e))) return rc;
+    const bool lds = state_in_lds_default();
+    if ((rc = grow(W.slots, slot_bytes(lds) * (size_t)n_slots, e))) return rc;
     if ((rc = grow(W.queue, 256, e))) return rc;
 
     std::vector<uint32_t> order;
@@ -298,8 +316,9 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     a.len = (const uint32_t *)W.len.p;
     a.lost = pl.lost.empty() ? nullptr : (const uint8_t *)W.lost.p;
     a.chains = (const ChainDesc *)W.chains.p;
-    a.slots = (ChainSlot *)W.slots.p;
+    a.slots = (uint8_t *)W.slots.p;
     a.n_slots = n_slots;
+    a.slot_bytes = (uint32_t)slot_bytes(lds);
     a.out = out;
     a.res = (ChainResult *)W.res.p;
     a.pkt_nsamp = o.pkt_nsamp;
@@ -309,14 +328,14 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
         HIPCHK(e, hipMemsetAsync(W.queue.p, 0, 4, st));
         a.order = (const uint32_t *)W.order.p;
         a.n_chains = nch;
-        hipLaunchKernelGGL(k_decode_persistent, dim3(n_slots), dim3(64), 0, st, a);
+        launch_persistent(lds, n_slots, st, a);
         HIPCHK(e, hipGetLastError());
     } else {
         for (uint32_t c0 = 0; c0 < nch; c0 += n_slots) {
             uint32_t bn = nch - c0 < n_slots ? nch - c0 : n_slots;
             a.order = (const uint32_t *)W.order.p + c0;
             a.n_chains = bn;
-            hipLaunchKernelGGL(k_decode_chains, dim3(bn), dim3(64), 0, st, a);
+            launch_chains(lds, bn, st, a);
             HIPCHK(e, hipGetLastError());
         }
     }
@@ -654,6 +673,7 @@ struct cuopus_batch {
     uint64_t tiles = 0;
     uint32_t n_slots = 0;
     int64_t chain_pkts = 0, chain_bytes = 0;
+    bool lds = true;  // decoder state in LDS
     RsTaps taps;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -691,8 +711,9 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
     a.len = (const uint32_t *)b->len.p;
     a.lost = pl.lost.empty() ? nullptr : (const uint8_t *)b->lost.p;
     a.chains = (const ChainDesc *)b->chains.p;
-    a.slots = (ChainSlot *)b->slots.p;
+    a.slots = (uint8_t *)b->slots.p;
     a.n_slots = b->n_slots;
+    a.slot_bytes = (uint32_t)slot_bytes(b->lds);
     a.out = (int16_t *)b->cat.p;
     a.res = (ChainResult *)b->res.p;
     a.queue = (uint32_t *)b->queue.p;
@@ -707,7 +728,7 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
                 uint32_t bn = nc - k0 < b->n_slots ? nc - k0 : b->n_slots;
                 a.order = (const uint32_t *)b->order.p + c0 + k0;
                 a.n_chains = bn;
-                hipLaunchKernelGGL(k_decode_chains, dim3(bn), dim3(64), 0, st, a);
+                launch_chains(b->lds, bn, st, a);
                 HIPCHK(e, hipGetLastError());
             }
             const int r = b->file_rs[f];
@@ -730,14 +751,14 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, (uint32_t *)b->queue.p);
         HIPCHK(e, hipGetLastError());
         a.n_chains = nch;
-        hipLaunchKernelGGL(k_decode_persistent, dim3(b->n_slots), dim3(64), 0, st, a);
+        launch_persistent(b->lds, b->n_slots, st, a);
         HIPCHK(e, hipGetLastError());
     } else {
         for (uint32_t c0 = 0; c0 < nch; c0 += b->n_slots) {
             uint32_t bn = nch - c0 < b->n_slots ? nch - c0 : b->n_slots;
             a.order = (const uint32_t *)b->order.p + c0;
             a.n_chains = bn;
-            hipLaunchKernelGGL(k_decode_chains, dim3(bn), dim3(64), 0, st, a);
+            launch_chains(b->lds, bn, st, a);
             HIPCHK(e, hipGetLastError());
         }
     }
@@ -769,6 +790,7 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
     cuopus_batch *b = new cuopus_batch();
     if (opts) b->opts = *opts;
     else memset(&b->opts, 0, sizeof(b->opts));
+    b->lds = b->opts.state_in_hbm ? false : state_in_lds_default();
     Plan &pl = b->pl;
     pl.files.resize((size_t)nfiles);
     int rc = 0;
@@ -850,7 +872,7 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
     if ((rc = grow(b->data, pl.data.size(), e)) || (rc = grow(b->off, 4 * npk, e)) ||
         (rc = grow(b->len, 4 * npk, e)) || (rc = grow(b->chains, sizeof(ChainDesc) * nch, e)) ||
         (rc = grow(b->order, 4ull * nch, e)) || (rc = grow(b->res, sizeof(ChainResult) * nch, e)) ||
-        (rc = grow(b->slots, sizeof(ChainSlot) * (size_t)b->n_slots, e)) ||
+        (rc = grow(b->slots, slot_bytes(b->lds) * (size_t)b->n_slots, e)) ||
         (rc = grow(b->cat, 2 * (pl.total_keep ? pl.total_keep : 1), e)) || (rc = grow(b->queue, 256, e)) ||
         (rc = grow(b->files, sizeof(RsFile) * (rsf_upload.size() ? rsf_upload.size() : 1), e)) ||
         (!pl.lost.empty() && (rc = grow(b->lost, pl.lost.size(), e)))) {
@@ -971,14 +993,16 @@ extern "C" int cuopus_warmup(char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);
     int rc;
-    if ((rc = grow(W.slots, sizeof(ChainSlot), e))) return rc;
+    const bool lds = state_in_lds_default();
+    if ((rc = grow(W.slots, slot_bytes(lds), e))) return rc;
     if ((rc = grow(W.res, sizeof(ChainResult), e))) return rc;
     DecodeArgs a;
     memset(&a, 0, sizeof(a));
-    a.n_chains = 0;
-    a.slots = (ChainSlot *)W.slots.p;
+    a.n_chains = 0;  // every workgroup returns at once: only loads the code objects
+    a.slots = (uint8_t *)W.slots.p;
+    a.slot_bytes = (uint32_t)slot_bytes(lds);
     a.res = (ChainResult *)W.res.p;
-    hipLaunchKernelGGL(k_decode_chains, dim3(1), dim3(64), 0, 0, a);
+    launch_chains(lds, 1, 0, a);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipDeviceSynchronize());
     return 0;
