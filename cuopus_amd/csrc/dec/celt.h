@@ -1221,6 +1221,10 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
 CUO_FN void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *oldBandE, int start, int effEnd,
                            int isTransient, int LM, int silence, int32_t *freq)
 {
+    X = lds_uniform(X);
+    out_syn = lds_uniform(out_syn);
+    oldBandE = lds_uniform(oldBandE);
+    freq = lds_uniform(freq);
     const int N = 120 << LM, M = 1 << LM;
     int B, NB, shift;
     if (isTransient) { B = M; NB = 120; shift = 3; }
@@ -1365,6 +1369,10 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
 CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size, RangeDec *dec,
                        int accum, CeltScratch *sc)
 {
+    st = lds_uniform(st);
+    sc = lds_uniform(sc);
+    if (pcm) pcm = lds_uniform(pcm);
+    if (dec) dec = lds_uniform(dec);
     int LM;
     for (LM = 0; LM <= 3; LM++)
         if (120 << LM == frame_size) break;

@@ -20,6 +20,33 @@
 
 namespace cuo {
 
+// Wave-uniform LDS pointer.  All decoder state, scratch and frame output of a
+// wavefront live in one LDS block (g_wave_lds, one 64-lane wavefront per
+// workgroup) and every lane runs the decoder on the same data.  On the
+// device, lds_uniform(p) rebuilds a pointer into that block from the block's
+// own address plus a scalar offset, so the compiler sees an LDS address that
+// is the same in every lane: accesses become LDS instructions and the values
+// loaded are uniform (scalar registers, scalar ALU).  The host build
+// (same-source CPU gate) uses the pointer as it is.
+constexpr size_t kWaveLdsBytes = 38 * 1024;
+__shared__ __attribute__((aligned(16))) char g_wave_lds[kWaveLdsBytes];
+#if defined(__HIP_DEVICE_COMPILE__)
+
+template <class T>
+__device__ __forceinline__ T *lds_uniform(T *p)
+{
+    uint32_t off = (uint32_t)((const char *)p - (const char *)g_wave_lds);
+    off = __builtin_amdgcn_readfirstlane(off);
+    return (T *)(g_wave_lds + off);
+}
+#else
+template <class T>
+__host__ __device__ inline T *lds_uniform(T *p)
+{
+    return p;
+}
+#endif
+
 // ---------------------------------------------------------------- basics
 CUO_FN int imin(int a, int b) { return a < b ? a : b; }
 CUO_FN int imax(int a, int b) { return a > b ? a : b; }

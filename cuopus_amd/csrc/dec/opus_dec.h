@@ -40,6 +40,8 @@ struct OpusDecState {
 
 // Per-call temporaries.  SILK and CELT decode one after the other inside a
 // frame, so their scratch shares storage.
+constexpr int kMaxFrameOut = 2880;
+
 struct DecScratch {
     union {
         CeltScratch celt;
@@ -49,6 +51,9 @@ struct DecScratch {
     int16_t pcm_transition[240];
     int16_t redundant_audio[240];
     RangeDec dec;  // the frame's range decoder (shared by SILK and CELT)
+    // One frame of output: a frame decode writes at most st->frame_size
+    // <= 2880 samples (60 ms), data or concealment.
+    int16_t frame_out[kMaxFrameOut];
 };
 
 CUO_FN void opus_dec_init(OpusDecState *st)
@@ -231,6 +236,9 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
 {
     constexpr bool HasData = Kind == kFrameData;
     const int F20 = 960, F10 = 480, F5 = 240, F2_5 = 120;
+    st = lds_uniform(st);
+    pcm = lds_uniform(pcm);
+    sc = lds_uniform(sc);
     RangeDec &dec = sc->dec;
     if (frame_size < F2_5) return OPUS_BUFFER_TOO_SMALL;
     frame_size = imin(frame_size, 48000 / 25 * 3);
@@ -407,14 +415,19 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
     return celt_ret < 0 ? celt_ret : audiosize;
 }
 
+// Output of the packet decoder: every frame is decoded into out.buf
+// (DecScratch::frame_out) and handed to out.emit(n) before the next frame.
+
 // opus_decode_native's concealment branch (opus_decoder.c:713-729)
-CUO_FN int decode_plc(OpusDecState *st, int16_t *pcm, int frame_size, DecScratch *sc)
+template <class Out>
+CUO_FN int decode_plc(OpusDecState *st, Out &out, int frame_size, DecScratch *sc)
 {
     if (frame_size % 120 != 0) return OPUS_BAD_ARG;
     int count = 0;
     do {
-        int ret = frame_decode<kFrameConceal>(st, nullptr, 0, pcm + count, frame_size - count, 0, sc);
+        int ret = frame_decode<kFrameConceal>(st, nullptr, 0, out.buf, frame_size - count, 0, sc);
         if (ret < 0) return ret;
+        out.emit(ret);
         count += ret;
     } while (count < frame_size);
     st->last_packet_duration = count;
@@ -423,13 +436,14 @@ CUO_FN int decode_plc(OpusDecState *st, int16_t *pcm, int frame_size, DecScratch
 
 // opus_decode (opus_decoder.c:670-811) for int16 output.  data == nullptr or
 // len == 0 conceals frame_size samples.  Returns samples or a negative error.
-CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size,
+template <class Out>
+CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, Out &out, int frame_size,
                               int decode_fec, DecScratch *sc)
 {
     if (frame_size <= 0) return OPUS_BAD_ARG;
     if (decode_fec < 0 || decode_fec > 1) return OPUS_BAD_ARG;
     if ((decode_fec || len == 0 || data == nullptr) && frame_size % 120 != 0) return OPUS_BAD_ARG;
-    if (len == 0 || data == nullptr) return decode_plc(st, pcm, frame_size, sc);
+    if (len == 0 || data == nullptr) return decode_plc(st, out, frame_size, sc);
     if (len < 0) return OPUS_BAD_ARG;
     int packet_mode = toc_mode(data[0]);
     int packet_bandwidth = toc_bandwidth(data[0]);
@@ -443,18 +457,19 @@ CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, in
     data += offset;
     if (decode_fec) {
         if (frame_size < packet_frame_size || packet_mode == MODE_CELT_ONLY || st->mode == MODE_CELT_ONLY)
-            return decode_plc(st, pcm, frame_size, sc);
+            return decode_plc(st, out, frame_size, sc);
         int duration_copy = st->last_packet_duration;
         if (frame_size - packet_frame_size != 0) {
-            int ret = decode_plc(st, pcm, frame_size - packet_frame_size, sc);
+            int ret = decode_plc(st, out, frame_size - packet_frame_size, sc);
             if (ret < 0) { st->last_packet_duration = duration_copy; return ret; }
         }
         st->mode = packet_mode;
         st->bandwidth = packet_bandwidth;
         st->frame_size = packet_frame_size;
         st->stream_channels = packet_stream_channels;
-        int ret = frame_decode<kFrameData>(st, data, size[0], pcm + (frame_size - packet_frame_size), packet_frame_size, 1, sc);
+        int ret = frame_decode<kFrameData>(st, data, size[0], out.buf, packet_frame_size, 1, sc);
         if (ret < 0) return ret;
+        out.emit(ret);
         st->last_packet_duration = frame_size;
         return frame_size;
     }
@@ -465,8 +480,9 @@ CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, in
     st->stream_channels = packet_stream_channels;
     int nb_samples = 0;
     for (int i = 0; i < count; i++) {
-        int ret = frame_decode<kFrameData>(st, data, size[i], pcm + nb_samples, frame_size - nb_samples, 0, sc);
+        int ret = frame_decode<kFrameData>(st, data, size[i], out.buf, frame_size - nb_samples, 0, sc);
         if (ret < 0) return ret;
+        out.emit(ret);
         data += size[i];
         nb_samples += ret;
     }

@@ -1202,6 +1202,11 @@ struct SilkControl {
 CUO_FN int silk_decode(SilkDecoder *dec, const SilkControl *ctl, int lostFlag, int newPacketFlag, RangeDec *d,
                        int16_t *samplesOut, int32_t *nSamplesOut, SilkScratch *sc)
 {
+    dec = lds_uniform(dec);
+    ctl = lds_uniform(ctl);
+    d = lds_uniform(d);
+    samplesOut = lds_uniform(samplesOut);
+    sc = lds_uniform(sc);
     SilkChannel *ch = &dec->ch;
     int ret = 0;
     if (newPacketFlag) ch->nFramesDecoded = 0;

@@ -19,6 +19,9 @@
 namespace cuo {
 
 constexpr int kMaxPktSamples = 5760;
+// Resident wavefronts of the persistent decoder at its LDS footprint: four
+// 37 KB wavefronts per CU (160 KB LDS) on 256 CUs.
+constexpr uint32_t kDefaultWaves = 1024;
 constexpr int kRsTaps = 49;
 constexpr int kRsPad = 24;
 
@@ -37,19 +40,15 @@ struct ChainResult {
     uint32_t samples;  // samples produced by the chain's packets
 };
 
-// Decoder memory of one wavefront: the chain's persistent state and the
-// per-call scratch (about 31 KB).  Placed in LDS (kLdsState) or in the slot
-// arena in HBM.
+// Decoder memory of one wavefront: the chain's persistent state, the per-call
+// scratch and one frame of output (about 37 KB), all in LDS.  Chains restart
+// from a fresh decoder, so nothing of it is ever saved to HBM: the only HBM
+// traffic of a chain is its packet bytes in and its kept PCM out.
 struct WaveState {
     OpusDecState st;
     DecScratch sc;
 };
-
-// Per-slot HBM arena: one packet of PCM, then (HBM-state layout only) the
-// wavefront's decoder memory.
-constexpr size_t kPcmBytes = (kMaxPktSamples * 2 + 255) / 256 * 256;
-constexpr size_t kSlotBytesLds = kPcmBytes;
-constexpr size_t kSlotBytesHbm = kPcmBytes + (sizeof(WaveState) + 255) / 256 * 256;
+static_assert(sizeof(WaveState) <= kWaveLdsBytes, "wavefront decoder memory exceeds its LDS block");
 
 struct DecodeArgs {
     const uint8_t *data;
@@ -59,9 +58,6 @@ struct DecodeArgs {
     const ChainDesc *chains;
     const uint32_t *order;      // nullable: work order (chain indices)
     uint32_t n_chains;
-    uint8_t *slots;             // slot arena, slot_bytes per slot
-    uint32_t n_slots;
-    uint32_t slot_bytes;
     int16_t *out;               // compacted 48 kHz PCM
     ChainResult *res;
     int32_t *pkt_nsamp;         // nullable, per packet
@@ -69,16 +65,39 @@ struct DecodeArgs {
     uint32_t *queue;            // work-queue counter (persistent variant)
 };
 
+// Frame sink of a chain: each decoded frame's samples that fall inside the
+// chain's keep window are copied by the 64 lanes in parallel from the LDS
+// frame buffer straight into the compacted output.
+struct ChainOut {
+    int16_t *buf;
+    int16_t *out;
+    uint64_t dst, lo, hi, pos;
+    int lane;
+    __device__ void emit(int n)
+    {
+        uint64_t a = pos > lo ? pos : lo;
+        uint64_t b = pos + (uint64_t)n < hi ? pos + (uint64_t)n : hi;
+        for (uint64_t i = a + (uint64_t)lane; i < b; i += 64) out[dst + (i - lo)] = buf[i - pos];
+        pos += (uint64_t)n;
+    }
+};
+
 // Decode one chain on the calling wavefront.
-__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int16_t *pcm, int lane)
+__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int lane)
 {
     const ChainDesc ch = a.chains[c];
-    OpusDecState *st = &ws->st;
-    DecScratch *sc = &ws->sc;
+    OpusDecState *st = lds_uniform(&ws->st);
+    DecScratch *sc = lds_uniform(&ws->sc);
     opus_dec_init(st);
-    uint64_t pos = 0;
+    ChainOut out;
+    out.buf = sc->frame_out;
+    out.out = a.out;
+    out.dst = ch.dst;
+    out.lo = ch.trim;
+    out.hi = (uint64_t)ch.trim + ch.keep;
+    out.pos = 0;
+    out.lane = lane;
     int32_t stop = 0;
-    const uint64_t win_lo = ch.trim, win_hi = (uint64_t)ch.trim + ch.keep;
     for (uint32_t k = 0; k < ch.npkts; k++) {
         const uint32_t p = ch.first + k;
         const uint8_t *pk = a.data + a.off[p];
@@ -91,10 +110,10 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
             bool fec = false;
             if (k + 1 < ch.npkts && !a.lost[p + 1])
                 fec = packet_has_lbrr(a.data + a.off[p + 1], (int)a.len[p + 1]) > 0;
-            if (fec) r = opus_decode_packet(st, a.data + a.off[p + 1], (int)a.len[p + 1], pcm, dur, 1, sc);
-            else r = opus_decode_packet(st, nullptr, 0, pcm, dur, 0, sc);
+            if (fec) r = opus_decode_packet(st, a.data + a.off[p + 1], (int)a.len[p + 1], out, dur, 1, sc);
+            else r = opus_decode_packet(st, nullptr, 0, out, dur, 0, sc);
         } else {
-            r = opus_decode_packet(st, pk, (int)a.len[p], pcm, kMaxPktSamples, 0, sc);
+            r = opus_decode_packet(st, pk, (int)a.len[p], out, kMaxPktSamples, 0, sc);
         }
         if (r < 0) { stop = r; break; }
         // a packet held by several chains (preroll overlap) is recorded by the
@@ -104,37 +123,27 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
             a.pkt_nsamp[p] = r;
             a.pkt_rng[p] = st->rangeFinal;
         }
-        uint64_t lo = pos > win_lo ? pos : win_lo;
-        uint64_t hi = pos + (uint64_t)r < win_hi ? pos + (uint64_t)r : win_hi;
-        for (uint64_t i = lo + (uint64_t)lane; i < hi; i += 64) a.out[ch.dst + (i - win_lo)] = pcm[i - pos];
-        pos += (uint64_t)r;
     }
     if (lane == 0) {
         a.res[c].stop = stop;
-        a.res[c].samples = (uint32_t)pos;
+        a.res[c].samples = (uint32_t)out.pos;
     }
 }
 
-// Plain variant: workgroup g decodes chain order[g] in slot g.
-template <bool kLdsState>
+// Plain variant: workgroup g decodes chain order[g].
 __global__ __launch_bounds__(64) void k_decode_chains(DecodeArgs a)
 {
     const uint32_t g = blockIdx.x;
     if (g >= a.n_chains) return;
     const uint32_t c = a.order ? a.order[g] : g;
-    uint8_t *base = a.slots + (size_t)g * a.slot_bytes;
-    if constexpr (kLdsState) {
-        __shared__ WaveState lds_ws;
-        decode_one_chain(a, c, &lds_ws, (int16_t *)base, (int)threadIdx.x);
-    } else {
-        decode_one_chain(a, c, (WaveState *)(base + kPcmBytes), (int16_t *)base, (int)threadIdx.x);
-    }
+    decode_one_chain(a, c, (WaveState *)g_wave_lds, (int)threadIdx.x);
 }
 
-// Persistent variant: n_slots wavefronts pull chains from an atomic queue until
-// it is drained (every wavefront exits once the counter passes n_chains).
-__device__ inline void persistent_loop(const DecodeArgs &a, WaveState *ws, int16_t *pcm)
+// Persistent variant: the grid's wavefronts pull chains from an atomic queue
+// until it is drained (every wavefront exits once the counter passes n_chains).
+__global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
 {
+    WaveState *ws = (WaveState *)g_wave_lds;
     const int lane = (int)threadIdx.x;
     while (true) {
         uint32_t w = 0;
@@ -142,19 +151,7 @@ __device__ inline void persistent_loop(const DecodeArgs &a, WaveState *ws, int16
         w = __shfl(w, 0, 64);
         if (w >= a.n_chains) break;
         const uint32_t c = a.order ? a.order[w] : w;
-        decode_one_chain(a, c, ws, pcm, lane);
-    }
-}
-
-template <bool kLdsState>
-__global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
-{
-    uint8_t *base = a.slots + (size_t)blockIdx.x * a.slot_bytes;
-    if constexpr (kLdsState) {
-        __shared__ WaveState lds_ws;
-        persistent_loop(a, &lds_ws, (int16_t *)base);
-    } else {
-        persistent_loop(a, (WaveState *)(base + kPcmBytes), (int16_t *)base);
+        decode_one_chain(a, c, ws, lane);
     }
 }
 

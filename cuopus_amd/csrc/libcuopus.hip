@@ -84,7 +84,7 @@ int grow(DevBuf &b, size_t need, Err &e)
 }
 
 struct Workspaces {
-    DevBuf data, off, len, lost, chains, order, slots, cat, res, nsamp, rng, files, queue;
+    DevBuf data, off, len, lost, chains, order, cat, res, nsamp, rng, files, queue;
 } W;
 
 double env_f(const char *name, double dflt)
@@ -98,22 +98,16 @@ unsigned env_u(const char *name, unsigned dflt)
     return (v && atoi(v) > 0) ? (unsigned)atoi(v) : dflt;
 }
 
-// Decoder-state placement: LDS (default) or the HBM slot arena.
-bool state_in_lds_default() { return env_u("CUOPUS_STATE_HBM", 0) == 0; }
-size_t slot_bytes(bool lds) { return lds ? kSlotBytesLds : kSlotBytesHbm; }
-
-void launch_chains(bool lds, uint32_t grid, hipStream_t st, const DecodeArgs &a)
+// Launch wrappers (64-lane workgroups, one wavefront per chain).
+void launch_chains(uint32_t grid, hipStream_t st, const DecodeArgs &a)
 {
-    if (lds) hipLaunchKernelGGL(k_decode_chains<true>, dim3(grid), dim3(64), 0, st, a);
-    else hipLaunchKernelGGL(k_decode_chains<false>, dim3(grid), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_decode_chains, dim3(grid), dim3(64), 0, st, a);
 }
 
-void launch_persistent(bool lds, uint32_t grid, hipStream_t st, const DecodeArgs &a)
+void launch_persistent(uint32_t grid, hipStream_t st, const DecodeArgs &a)
 {
-    if (lds) hipLaunchKernelGGL(k_decode_persistent<true>, dim3(grid), dim3(64), 0, st, a);
-    else hipLaunchKernelGGL(k_decode_persistent<false>, dim3(grid), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_decode_persistent, dim3(grid), dim3(64), 0, st, a);
 }
-
 
 // Kaiser I0 by its power series, in double.
 double bessel_i0(double x)
@@ -292,12 +286,7 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
         if ((rc = grow(W.cat, 2 * (pl.total_keep ? pl.total_keep : 1), e))) return rc;
         out = (int16_t *)W.cat.p;
     }
-    unsigned max_batch = env_u("CUOPUS_MAX_BATCH", 16384);
-    uint32_t n_slots = o.persistent ? env_u("CUOPUS_SLOTS", 4096) : (nch < max_batch ? nch : max_batch);
-    if (n_slots > nch) n_slots = nch;
-    if (n_slots == 0) n_slots = 1;
-    const bool lds = state_in_lds_default();
-    if ((rc = grow(W.slots, slot_bytes(lds) * (size_t)n_slots, e))) return rc;
+    const uint32_t n_waves = std::max<uint32_t>(1, std::min<uint32_t>(env_u("CUOPUS_SLOTS", kDefaultWaves), nch));
     if ((rc = grow(W.queue, 256, e))) return rc;
 
     std::vector<uint32_t> order;
@@ -316,9 +305,6 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     a.len = (const uint32_t *)W.len.p;
     a.lost = pl.lost.empty() ? nullptr : (const uint8_t *)W.lost.p;
     a.chains = (const ChainDesc *)W.chains.p;
-    a.slots = (uint8_t *)W.slots.p;
-    a.n_slots = n_slots;
-    a.slot_bytes = (uint32_t)slot_bytes(lds);
     a.out = out;
     a.res = (ChainResult *)W.res.p;
     a.pkt_nsamp = o.pkt_nsamp;
@@ -328,16 +314,13 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
         HIPCHK(e, hipMemsetAsync(W.queue.p, 0, 4, st));
         a.order = (const uint32_t *)W.order.p;
         a.n_chains = nch;
-        launch_persistent(lds, n_slots, st, a);
+        launch_persistent(n_waves, st, a);
         HIPCHK(e, hipGetLastError());
     } else {
-        for (uint32_t c0 = 0; c0 < nch; c0 += n_slots) {
-            uint32_t bn = nch - c0 < n_slots ? nch - c0 : n_slots;
-            a.order = (const uint32_t *)W.order.p + c0;
-            a.n_chains = bn;
-            launch_chains(lds, bn, st, a);
-            HIPCHK(e, hipGetLastError());
-        }
+        a.order = (const uint32_t *)W.order.p;
+        a.n_chains = nch;
+        launch_chains(nch, st, a);
+        HIPCHK(e, hipGetLastError());
     }
     // one resampler launch over every file's tiles
     if (!o.f16_out.empty()) {
@@ -409,7 +392,7 @@ extern "C" long long cuopus_workspace_bytes(void) { return g_dev_bytes; }
 
 extern "C" void cuopus_release_workspaces(void)
 {
-    DevBuf *all[] = {&W.data, &W.off, &W.len, &W.lost, &W.chains, &W.order, &W.slots,
+    DevBuf *all[] = {&W.data, &W.off, &W.len, &W.lost, &W.chains, &W.order,
                      &W.cat, &W.res, &W.nsamp, &W.rng, &W.files, &W.queue};
     for (DevBuf *b : all) {
         if (b->p) hipFree(b->p);
@@ -665,7 +648,7 @@ extern "C" int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, f
 struct cuopus_batch {
     Plan pl;
     cuopus_batch_opts opts;
-    DevBuf data, off, len, lost, chains, order, slots, cat, res, files, queue;
+    DevBuf data, off, len, lost, chains, order, cat, res, files, queue;
     std::vector<RsFile> rsf;          // one entry per file with output
     std::vector<uint64_t> file_tiles; // per rsf entry: tile count
     std::vector<uint32_t> file_chain0, file_nchain;
@@ -673,7 +656,6 @@ struct cuopus_batch {
     uint64_t tiles = 0;
     uint32_t n_slots = 0;
     int64_t chain_pkts = 0, chain_bytes = 0;
-    bool lds = true;  // decoder state in LDS
     RsTaps taps;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -687,7 +669,7 @@ int batch_grow(cuopus_batch *b, DevBuf &d, size_t need, Err &e) { return grow(d,
 
 void batch_free(cuopus_batch *b)
 {
-    DevBuf *all[] = {&b->data, &b->off, &b->len, &b->lost, &b->chains, &b->order, &b->slots,
+    DevBuf *all[] = {&b->data, &b->off, &b->len, &b->lost, &b->chains, &b->order,
                      &b->cat, &b->res, &b->files, &b->queue};
     for (DevBuf *d : all) {
         if (d->p) { hipFree(d->p); g_dev_bytes -= (long long)d->cap; }
@@ -711,9 +693,6 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
     a.len = (const uint32_t *)b->len.p;
     a.lost = pl.lost.empty() ? nullptr : (const uint8_t *)b->lost.p;
     a.chains = (const ChainDesc *)b->chains.p;
-    a.slots = (uint8_t *)b->slots.p;
-    a.n_slots = b->n_slots;
-    a.slot_bytes = (uint32_t)slot_bytes(b->lds);
     a.out = (int16_t *)b->cat.p;
     a.res = (ChainResult *)b->res.p;
     a.queue = (uint32_t *)b->queue.p;
@@ -724,11 +703,10 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
         // plain per-file path: one decode launch and one resample launch per file
         for (size_t f = 0; f < pl.files.size(); f++) {
             uint32_t c0 = b->file_chain0[f], nc = b->file_nchain[f];
-            for (uint32_t k0 = 0; k0 < nc; k0 += b->n_slots) {
-                uint32_t bn = nc - k0 < b->n_slots ? nc - k0 : b->n_slots;
-                a.order = (const uint32_t *)b->order.p + c0 + k0;
-                a.n_chains = bn;
-                launch_chains(b->lds, bn, st, a);
+            if (nc) {
+                a.order = (const uint32_t *)b->order.p + c0;
+                a.n_chains = nc;
+                launch_chains(nc, st, a);
                 HIPCHK(e, hipGetLastError());
             }
             const int r = b->file_rs[f];
@@ -751,16 +729,12 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, (uint32_t *)b->queue.p);
         HIPCHK(e, hipGetLastError());
         a.n_chains = nch;
-        launch_persistent(b->lds, b->n_slots, st, a);
+        launch_persistent(b->n_slots, st, a);
         HIPCHK(e, hipGetLastError());
     } else {
-        for (uint32_t c0 = 0; c0 < nch; c0 += b->n_slots) {
-            uint32_t bn = nch - c0 < b->n_slots ? nch - c0 : b->n_slots;
-            a.order = (const uint32_t *)b->order.p + c0;
-            a.n_chains = bn;
-            launch_chains(b->lds, bn, st, a);
-            HIPCHK(e, hipGetLastError());
-        }
+        a.n_chains = nch;
+        launch_chains(nch, st, a);
+        HIPCHK(e, hipGetLastError());
     }
     if (prof) HIPCHK(e, hipEventRecord(b->ev[1], st));
     if (b->tiles) {
@@ -790,7 +764,6 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
     cuopus_batch *b = new cuopus_batch();
     if (opts) b->opts = *opts;
     else memset(&b->opts, 0, sizeof(b->opts));
-    b->lds = b->opts.state_in_hbm ? false : state_in_lds_default();
     Plan &pl = b->pl;
     pl.files.resize((size_t)nfiles);
     int rc = 0;
@@ -835,15 +808,10 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
         for (size_t i = 0; i < order.size(); i++) order[i] = (uint32_t)i;
     }
     const uint32_t nch = (uint32_t)pl.chains.size();
-    unsigned max_batch = env_u("CUOPUS_MAX_BATCH", 16384);
-    if (b->opts.mode == 2) b->n_slots = b->opts.n_slots > 0 ? (uint32_t)b->opts.n_slots : 2048;
-    else b->n_slots = max_batch;
-    if (b->opts.mode != 0 && b->n_slots > nch) b->n_slots = nch;
-    if (b->opts.mode == 0) {
-        uint32_t mx = 1;
-        for (int f = 0; f < nfiles; f++) mx = std::max(mx, b->file_nchain[f]);
-        b->n_slots = std::min<uint32_t>(b->n_slots, mx);
-    }
+    // persistent grid: n_slots wavefronts (default: the chip's resident
+    // capacity at the kernel's LDS footprint), never more than the chains
+    b->n_slots = b->opts.n_slots > 0 ? (uint32_t)b->opts.n_slots : kDefaultWaves;
+    if (b->n_slots > nch) b->n_slots = nch;
     b->taps = make_taps();
     b->file_rs.assign(pl.files.size(), -1);
     for (size_t f = 0; f < pl.files.size(); f++) {
@@ -872,7 +840,6 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
     if ((rc = grow(b->data, pl.data.size(), e)) || (rc = grow(b->off, 4 * npk, e)) ||
         (rc = grow(b->len, 4 * npk, e)) || (rc = grow(b->chains, sizeof(ChainDesc) * nch, e)) ||
         (rc = grow(b->order, 4ull * nch, e)) || (rc = grow(b->res, sizeof(ChainResult) * nch, e)) ||
-        (rc = grow(b->slots, slot_bytes(b->lds) * (size_t)b->n_slots, e)) ||
         (rc = grow(b->cat, 2 * (pl.total_keep ? pl.total_keep : 1), e)) || (rc = grow(b->queue, 256, e)) ||
         (rc = grow(b->files, sizeof(RsFile) * (rsf_upload.size() ? rsf_upload.size() : 1), e)) ||
         (!pl.lost.empty() && (rc = grow(b->lost, pl.lost.size(), e)))) {
@@ -993,16 +960,12 @@ extern "C" int cuopus_warmup(char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);
     int rc;
-    const bool lds = state_in_lds_default();
-    if ((rc = grow(W.slots, slot_bytes(lds), e))) return rc;
     if ((rc = grow(W.res, sizeof(ChainResult), e))) return rc;
     DecodeArgs a;
     memset(&a, 0, sizeof(a));
     a.n_chains = 0;  // every workgroup returns at once: only loads the code objects
-    a.slots = (uint8_t *)W.slots.p;
-    a.slot_bytes = (uint32_t)slot_bytes(lds);
     a.res = (ChainResult *)W.res.p;
-    launch_chains(lds, 1, 0, a);
+    launch_chains(1, 0, a);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipDeviceSynchronize());
     return 0;

@@ -48,7 +48,7 @@ P = ctypes.c_void_p
 class BatchOpts(ctypes.Structure):
     """Mirror of cuopus_batch_opts (include/cuopus.h)."""
     _fields_ = [("mode", ctypes.c_int), ("n_slots", ctypes.c_int), ("lpt_order", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("profile", ctypes.c_int), ("state_in_hbm", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("profile", ctypes.c_int)]
 SZ = ctypes.c_size_t
 I64P = ctypes.POINTER(ctypes.c_int64)
 
@@ -285,7 +285,7 @@ class Batch:
     """
 
     def __init__(self, items, lost_masks=None, mode=MODE_PERSISTENT, n_slots=0, lpt_order=True,
-                 use_graph=False, profile=False, outputs=True, state_in_hbm=False):
+                 use_graph=False, profile=False, outputs=True):
         import torch
         lib = _load()
         self._lib = lib
@@ -308,8 +308,7 @@ class Batch:
                            for m in lost_masks]
             ma = (P * n)(*[None if m is None else m.ctypes.data for m in self._masks])
             mn = (SZ * n)(*[0 if m is None else m.size for m in self._masks])
-        opts = BatchOpts(mode, n_slots, 1 if lpt_order else 0, 1 if use_graph else 0, 1 if profile else 0,
-                         1 if state_in_hbm else 0)
+        opts = BatchOpts(mode, n_slots, 1 if lpt_order else 0, 1 if use_graph else 0, 1 if profile else 0)
         err = ctypes.create_string_buffer(_ERRCAP)
         _check(lib.cuopus_batch_create(ba, la, n, ma, mn, oa, ca, ctypes.byref(opts), ctypes.byref(self._h),
                                        err, _ERRCAP), err)

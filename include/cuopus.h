@@ -135,8 +135,7 @@ typedef struct {
     int use_graph;   /* 1: capture the launch sequence into a HIP graph on
                         the first run and replay it afterwards */
     int profile;     /* 1: time the decode and resample kernels with events */
-    int state_in_hbm; /* 1: keep each wavefront's decoder state and scratch in
-                         the HBM slot arena instead of LDS */
+
 } cuopus_batch_opts;
 
 /* lost_masks may be NULL (no loss) or hold one mask per file (NULL entries

@@ -11,6 +11,19 @@
 
 using namespace cuo;
 
+// Frame sink: appends each decoded frame (decoded into sc->frame_out) to the
+// caller's PCM buffer.
+struct HostOut {
+    int16_t *buf;
+    int16_t *dst;
+    long pos;
+    void emit(int n)
+    {
+        memcpy(dst + pos, buf, sizeof(int16_t) * (size_t)n);
+        pos += n;
+    }
+};
+
 extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const uint32_t *len,
                                 const uint8_t *lost, int npk, int16_t *pcm, long cap,
                                 int32_t *nsamp, uint32_t *rng)
@@ -20,6 +33,7 @@ extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const 
     memset(sc, 0xA5, sizeof(DecScratch));  // device slots start uninitialised
     memset(st, 0x5A, sizeof(OpusDecState));
     opus_dec_init(st);
+    HostOut out{sc->frame_out, pcm, 0};
     long total = 0;
     long ret = 0;
     for (int k = 0; k < npk; k++) {
@@ -31,16 +45,17 @@ extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const 
             if (dur <= 0) { ret = dur ? dur : OPUS_INVALID_PACKET; break; }
             const uint8_t *q = k + 1 < npk ? data + off[k + 1] : nullptr;
             if (q && !lost[k + 1] && packet_has_lbrr(q, (int)len[k + 1]) > 0)
-                r = opus_decode_packet(st, q, (int)len[k + 1], pcm + total, dur, 1, sc);
+                r = opus_decode_packet(st, q, (int)len[k + 1], out, dur, 1, sc);
             else
-                r = opus_decode_packet(st, nullptr, 0, pcm + total, dur, 0, sc);
+                r = opus_decode_packet(st, nullptr, 0, out, dur, 0, sc);
         } else {
-            r = opus_decode_packet(st, p, (int)len[k], pcm + total, 5760, 0, sc);
+            r = opus_decode_packet(st, p, (int)len[k], out, 5760, 0, sc);
         }
         if (r < 0) { ret = r; break; }
         if (nsamp) nsamp[k] = r;
         if (rng) rng[k] = st->rangeFinal;
         total += r;
+        if (out.pos != total) { ret = OPUS_INTERNAL_ERROR; break; }
     }
     free(st);
     free(sc);
