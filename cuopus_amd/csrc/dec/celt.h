@@ -102,6 +102,14 @@ CUO_FN void celt_init(CeltState *st)
 }
 
 // ------------------------------------------------------------------ small helpers
+// mem[0 .. kDecBuf - N + kOverlap) = mem[N ..): the decode history moves down
+// by one frame.  Lanes copy 64-element blocks in increasing order; with
+// N >= 120 no block reads what an earlier block wrote.
+CUO_FN void shift_history(int32_t *mem, int N)
+{
+    CUO_PFOR(i, 0, kDecBuf - N + kOverlap) mem[i] = mem[i + N];
+}
+
 CUO_FN int32_t inner_prod16(const int16_t *x, const int16_t *y, int n)
 {
     int32_t s = 0;
@@ -969,11 +977,9 @@ CUO_FN void denormalise_bands(const int16_t *X, int32_t *freq, const int16_t *ba
     int N = M * 120;
     int bound = M * eB[end];
     if (silence) { bound = 0; start = end = 0; }
-    int32_t *f = freq;
-    const int16_t *x = X + M * eB[start];
-    for (int i = 0; i < M * eB[start]; i++) *f++ = 0;
+    CUO_PFOR(j, 0, M * eB[start]) freq[j] = 0;
     for (int i = start; i < end; i++) {
-        int j = M * eB[i], band_end = M * eB[i + 1];
+        const int lo = M * eB[i], hi = M * eB[i + 1];
         int16_t lg = sat16(bandLogE[i] + shl32((int32_t)tab::kEMeans[i], 6));
         int shift = 16 - (lg >> kDbShift);
         int16_t g;
@@ -981,12 +987,12 @@ CUO_FN void denormalise_bands(const int16_t *X, int32_t *freq, const int16_t *ba
         else g = (int16_t)celt_exp2_frac((int16_t)(lg & ((1 << kDbShift) - 1)));
         if (shift < 0) {
             if (shift <= -2) { g = 16384; shift = -2; }
-            do { *f++ = shl32(mul16_16(*x++, g), -shift); } while (++j < band_end);
+            CUO_PFOR(j, lo, hi) freq[j] = shl32(mul16_16(X[j], g), -shift);
         } else {
-            do { *f++ = shr32(mul16_16(*x++, g), shift); } while (++j < band_end);
+            CUO_PFOR(j, lo, hi) freq[j] = shr32(mul16_16(X[j], g), shift);
         }
     }
-    for (int i = bound; i < N; i++) freq[i] = 0;
+    CUO_PFOR(j, bound, N) freq[j] = 0;
 }
 
 // FFT butterflies on interleaved int32 complex (kiss_fft.c:48-317); twiddles
@@ -994,10 +1000,13 @@ CUO_FN void denormalise_bands(const int16_t *X, int32_t *freq, const int16_t *ba
 struct Cpx { int32_t r, i; };
 CUO_FN int32_t smul(int32_t a, int16_t b) { return mul16_32_q15(b, a); }
 
-CUO_FN void bfly2(Cpx *F, int N)
+// Butterflies of one radix stage are independent: each lane takes whole
+// butterflies (index i over the N sub-transforms, j over the m positions).
+CUO_FN void bfly2(Cpx *F0, int N)
 {
     const int16_t tw = 23170;
-    for (int i = 0; i < N; i++) {
+    CUO_PFOR(i, 0, N) {
+        Cpx *F = F0 + 8 * i;
         Cpx *F2 = F + 4;
         Cpx t = F2[0];
         F2[0].r = sub32_ovf(F[0].r, t.r); F2[0].i = sub32_ovf(F[0].i, t.i);
@@ -1013,7 +1022,6 @@ CUO_FN void bfly2(Cpx *F, int N)
         t.i = smul(neg32_ovf(add32_ovf(F2[3].i, F2[3].r)), tw);
         F2[3].r = sub32_ovf(F[3].r, t.r); F2[3].i = sub32_ovf(F[3].i, t.i);
         F[3].r = add32_ovf(F[3].r, t.r); F[3].i = add32_ovf(F[3].i, t.i);
-        F += 8;
     }
 }
 
@@ -1025,10 +1033,11 @@ CUO_FN Cpx cmul(Cpx a, int16_t br, int16_t bi)
     return m;
 }
 
-CUO_FN void bfly4(Cpx *Fout, int fstride, int m, int N, int mm)
+CUO_FN void bfly4(Cpx *Fbeg, int fstride, int m, int N, int mm)
 {
     if (m == 1) {
-        for (int i = 0; i < N; i++) {
+        CUO_PFOR(i, 0, N) {
+            Cpx *Fout = Fbeg + 4 * i;
             Cpx s0, s1;
             s0.r = sub32_ovf(Fout[0].r, Fout[2].r); s0.i = sub32_ovf(Fout[0].i, Fout[2].i);
             Fout[0].r = add32_ovf(Fout[0].r, Fout[2].r); Fout[0].i = add32_ovf(Fout[0].i, Fout[2].i);
@@ -1038,99 +1047,86 @@ CUO_FN void bfly4(Cpx *Fout, int fstride, int m, int N, int mm)
             s1.r = sub32_ovf(Fout[1].r, Fout[3].r); s1.i = sub32_ovf(Fout[1].i, Fout[3].i);
             Fout[1].r = add32_ovf(s0.r, s1.i); Fout[1].i = sub32_ovf(s0.i, s1.r);
             Fout[3].r = sub32_ovf(s0.r, s1.i); Fout[3].i = add32_ovf(s0.i, s1.r);
-            Fout += 4;
         }
         return;
     }
     const int16_t *tw = tab::kFftTwiddles;
-    Cpx *Fbeg = Fout;
-    for (int i = 0; i < N; i++) {
-        Fout = Fbeg + i * mm;
-        int t1 = 0, t2 = 0, t3 = 0;
-        for (int j = 0; j < m; j++) {
-            Cpx s0 = cmul(Fout[m], tw[2 * t1], tw[2 * t1 + 1]);
-            Cpx s1 = cmul(Fout[2 * m], tw[2 * t2], tw[2 * t2 + 1]);
-            Cpx s2 = cmul(Fout[3 * m], tw[2 * t3], tw[2 * t3 + 1]);
-            Cpx s5, s3, s4;
-            s5.r = sub32_ovf(Fout[0].r, s1.r); s5.i = sub32_ovf(Fout[0].i, s1.i);
-            Fout[0].r = add32_ovf(Fout[0].r, s1.r); Fout[0].i = add32_ovf(Fout[0].i, s1.i);
-            s3.r = add32_ovf(s0.r, s2.r); s3.i = add32_ovf(s0.i, s2.i);
-            s4.r = sub32_ovf(s0.r, s2.r); s4.i = sub32_ovf(s0.i, s2.i);
-            Fout[2 * m].r = sub32_ovf(Fout[0].r, s3.r); Fout[2 * m].i = sub32_ovf(Fout[0].i, s3.i);
-            t1 += fstride; t2 += fstride * 2; t3 += fstride * 3;
-            Fout[0].r = add32_ovf(Fout[0].r, s3.r); Fout[0].i = add32_ovf(Fout[0].i, s3.i);
-            Fout[m].r = add32_ovf(s5.r, s4.i); Fout[m].i = sub32_ovf(s5.i, s4.r);
-            Fout[3 * m].r = sub32_ovf(s5.r, s4.i); Fout[3 * m].i = add32_ovf(s5.i, s4.r);
-            ++Fout;
-        }
+    CUO_PFOR(k, 0, N * m) {
+        const int i = k / m, j = k - i * m;
+        Cpx *Fout = Fbeg + i * mm + j;
+        const int t1 = j * fstride, t2 = 2 * j * fstride, t3 = 3 * j * fstride;
+        Cpx s0 = cmul(Fout[m], tw[2 * t1], tw[2 * t1 + 1]);
+        Cpx s1 = cmul(Fout[2 * m], tw[2 * t2], tw[2 * t2 + 1]);
+        Cpx s2 = cmul(Fout[3 * m], tw[2 * t3], tw[2 * t3 + 1]);
+        Cpx s5, s3, s4;
+        s5.r = sub32_ovf(Fout[0].r, s1.r); s5.i = sub32_ovf(Fout[0].i, s1.i);
+        Fout[0].r = add32_ovf(Fout[0].r, s1.r); Fout[0].i = add32_ovf(Fout[0].i, s1.i);
+        s3.r = add32_ovf(s0.r, s2.r); s3.i = add32_ovf(s0.i, s2.i);
+        s4.r = sub32_ovf(s0.r, s2.r); s4.i = sub32_ovf(s0.i, s2.i);
+        Fout[2 * m].r = sub32_ovf(Fout[0].r, s3.r); Fout[2 * m].i = sub32_ovf(Fout[0].i, s3.i);
+        Fout[0].r = add32_ovf(Fout[0].r, s3.r); Fout[0].i = add32_ovf(Fout[0].i, s3.i);
+        Fout[m].r = add32_ovf(s5.r, s4.i); Fout[m].i = sub32_ovf(s5.i, s4.r);
+        Fout[3 * m].r = sub32_ovf(s5.r, s4.i); Fout[3 * m].i = add32_ovf(s5.i, s4.r);
     }
 }
 
-CUO_FN void bfly3(Cpx *Fout, int fstride, int m, int N, int mm)
+CUO_FN void bfly3(Cpx *Fbeg, int fstride, int m, int N, int mm)
 {
     const int16_t *tw = tab::kFftTwiddles;
     const int16_t epi3i = -28378;
-    Cpx *Fbeg = Fout;
-    for (int i = 0; i < N; i++) {
-        Fout = Fbeg + i * mm;
-        int t1 = 0, t2 = 0;
-        int k = m;
-        do {
-            Cpx s1 = cmul(Fout[m], tw[2 * t1], tw[2 * t1 + 1]);
-            Cpx s2 = cmul(Fout[2 * m], tw[2 * t2], tw[2 * t2 + 1]);
-            Cpx s3, s0;
-            s3.r = add32_ovf(s1.r, s2.r); s3.i = add32_ovf(s1.i, s2.i);
-            s0.r = sub32_ovf(s1.r, s2.r); s0.i = sub32_ovf(s1.i, s2.i);
-            t1 += fstride; t2 += fstride * 2;
-            Fout[m].r = sub32_ovf(Fout[0].r, s3.r >> 1);
-            Fout[m].i = sub32_ovf(Fout[0].i, s3.i >> 1);
-            s0.r = smul(s0.r, epi3i); s0.i = smul(s0.i, epi3i);
-            Fout[0].r = add32_ovf(Fout[0].r, s3.r); Fout[0].i = add32_ovf(Fout[0].i, s3.i);
-            Fout[2 * m].r = add32_ovf(Fout[m].r, s0.i);
-            Fout[2 * m].i = sub32_ovf(Fout[m].i, s0.r);
-            Fout[m].r = sub32_ovf(Fout[m].r, s0.i);
-            Fout[m].i = add32_ovf(Fout[m].i, s0.r);
-            ++Fout;
-        } while (--k);
+    CUO_PFOR(k, 0, N * m) {
+        const int i = k / m, j = k - i * m;
+        Cpx *Fout = Fbeg + i * mm + j;
+        const int t1 = j * fstride, t2 = 2 * j * fstride;
+        Cpx s1 = cmul(Fout[m], tw[2 * t1], tw[2 * t1 + 1]);
+        Cpx s2 = cmul(Fout[2 * m], tw[2 * t2], tw[2 * t2 + 1]);
+        Cpx s3, s0;
+        s3.r = add32_ovf(s1.r, s2.r); s3.i = add32_ovf(s1.i, s2.i);
+        s0.r = sub32_ovf(s1.r, s2.r); s0.i = sub32_ovf(s1.i, s2.i);
+        Cpx fm;
+        fm.r = sub32_ovf(Fout[0].r, s3.r >> 1);
+        fm.i = sub32_ovf(Fout[0].i, s3.i >> 1);
+        s0.r = smul(s0.r, epi3i); s0.i = smul(s0.i, epi3i);
+        Fout[0].r = add32_ovf(Fout[0].r, s3.r); Fout[0].i = add32_ovf(Fout[0].i, s3.i);
+        Fout[2 * m].r = add32_ovf(fm.r, s0.i);
+        Fout[2 * m].i = sub32_ovf(fm.i, s0.r);
+        Fout[m].r = sub32_ovf(fm.r, s0.i);
+        Fout[m].i = add32_ovf(fm.i, s0.r);
     }
 }
 
-CUO_FN void bfly5(Cpx *Fout, int fstride, int m, int N, int mm)
+CUO_FN void bfly5(Cpx *Fbeg, int fstride, int m, int N, int mm)
 {
     const int16_t *tw = tab::kFftTwiddles;
     const int16_t yar = 10126, yai = -31164, ybr = -26510, ybi = -19261;
-    Cpx *Fbeg = Fout;
-    for (int i = 0; i < N; i++) {
-        Fout = Fbeg + i * mm;
-        Cpx *F0 = Fout, *F1 = Fout + m, *F2 = Fout + 2 * m, *F3 = Fout + 3 * m, *F4 = Fout + 4 * m;
-        for (int u = 0; u < m; ++u) {
-            Cpx s0 = *F0;
-            int a1 = u * fstride, a2 = 2 * u * fstride, a3 = 3 * u * fstride, a4 = 4 * u * fstride;
-            Cpx s1 = cmul(*F1, tw[2 * a1], tw[2 * a1 + 1]);
-            Cpx s2 = cmul(*F2, tw[2 * a2], tw[2 * a2 + 1]);
-            Cpx s3 = cmul(*F3, tw[2 * a3], tw[2 * a3 + 1]);
-            Cpx s4 = cmul(*F4, tw[2 * a4], tw[2 * a4 + 1]);
-            Cpx s7, s10, s8, s9, s5, s6, s11, s12;
-            s7.r = add32_ovf(s1.r, s4.r); s7.i = add32_ovf(s1.i, s4.i);
-            s10.r = sub32_ovf(s1.r, s4.r); s10.i = sub32_ovf(s1.i, s4.i);
-            s8.r = add32_ovf(s2.r, s3.r); s8.i = add32_ovf(s2.i, s3.i);
-            s9.r = sub32_ovf(s2.r, s3.r); s9.i = sub32_ovf(s2.i, s3.i);
-            F0->r = add32_ovf(F0->r, add32_ovf(s7.r, s8.r));
-            F0->i = add32_ovf(F0->i, add32_ovf(s7.i, s8.i));
-            s5.r = add32_ovf(s0.r, add32_ovf(smul(s7.r, yar), smul(s8.r, ybr)));
-            s5.i = add32_ovf(s0.i, add32_ovf(smul(s7.i, yar), smul(s8.i, ybr)));
-            s6.r = add32_ovf(smul(s10.i, yai), smul(s9.i, ybi));
-            s6.i = neg32_ovf(add32_ovf(smul(s10.r, yai), smul(s9.r, ybi)));
-            F1->r = sub32_ovf(s5.r, s6.r); F1->i = sub32_ovf(s5.i, s6.i);
-            F4->r = add32_ovf(s5.r, s6.r); F4->i = add32_ovf(s5.i, s6.i);
-            s11.r = add32_ovf(s0.r, add32_ovf(smul(s7.r, ybr), smul(s8.r, yar)));
-            s11.i = add32_ovf(s0.i, add32_ovf(smul(s7.i, ybr), smul(s8.i, yar)));
-            s12.r = sub32_ovf(smul(s9.i, yai), smul(s10.i, ybi));
-            s12.i = sub32_ovf(smul(s10.r, ybi), smul(s9.r, yai));
-            F2->r = add32_ovf(s11.r, s12.r); F2->i = add32_ovf(s11.i, s12.i);
-            F3->r = sub32_ovf(s11.r, s12.r); F3->i = sub32_ovf(s11.i, s12.i);
-            ++F0; ++F1; ++F2; ++F3; ++F4;
-        }
+    CUO_PFOR(k, 0, N * m) {
+        const int i = k / m, u = k - i * m;
+        Cpx *F0 = Fbeg + i * mm + u, *F1 = F0 + m, *F2 = F0 + 2 * m, *F3 = F0 + 3 * m, *F4 = F0 + 4 * m;
+        Cpx s0 = *F0;
+        int a1 = u * fstride, a2 = 2 * u * fstride, a3 = 3 * u * fstride, a4 = 4 * u * fstride;
+        Cpx s1 = cmul(*F1, tw[2 * a1], tw[2 * a1 + 1]);
+        Cpx s2 = cmul(*F2, tw[2 * a2], tw[2 * a2 + 1]);
+        Cpx s3 = cmul(*F3, tw[2 * a3], tw[2 * a3 + 1]);
+        Cpx s4 = cmul(*F4, tw[2 * a4], tw[2 * a4 + 1]);
+        Cpx s7, s10, s8, s9, s5, s6, s11, s12;
+        s7.r = add32_ovf(s1.r, s4.r); s7.i = add32_ovf(s1.i, s4.i);
+        s10.r = sub32_ovf(s1.r, s4.r); s10.i = sub32_ovf(s1.i, s4.i);
+        s8.r = add32_ovf(s2.r, s3.r); s8.i = add32_ovf(s2.i, s3.i);
+        s9.r = sub32_ovf(s2.r, s3.r); s9.i = sub32_ovf(s2.i, s3.i);
+        F0->r = add32_ovf(F0->r, add32_ovf(s7.r, s8.r));
+        F0->i = add32_ovf(F0->i, add32_ovf(s7.i, s8.i));
+        s5.r = add32_ovf(s0.r, add32_ovf(smul(s7.r, yar), smul(s8.r, ybr)));
+        s5.i = add32_ovf(s0.i, add32_ovf(smul(s7.i, yar), smul(s8.i, ybr)));
+        s6.r = add32_ovf(smul(s10.i, yai), smul(s9.i, ybi));
+        s6.i = neg32_ovf(add32_ovf(smul(s10.r, yai), smul(s9.r, ybi)));
+        F1->r = sub32_ovf(s5.r, s6.r); F1->i = sub32_ovf(s5.i, s6.i);
+        F4->r = add32_ovf(s5.r, s6.r); F4->i = add32_ovf(s5.i, s6.i);
+        s11.r = add32_ovf(s0.r, add32_ovf(smul(s7.r, ybr), smul(s8.r, yar)));
+        s11.i = add32_ovf(s0.i, add32_ovf(smul(s7.i, ybr), smul(s8.i, yar)));
+        s12.r = sub32_ovf(smul(s9.i, yai), smul(s10.i, ybi));
+        s12.i = sub32_ovf(smul(s10.r, ybi), smul(s9.r, yai));
+        F2->r = add32_ovf(s11.r, s12.r); F2->i = add32_ovf(s11.i, s12.i);
+        F3->r = sub32_ovf(s11.r, s12.r); F3->i = sub32_ovf(s11.i, s12.i);
     }
 }
 
@@ -1169,24 +1165,24 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
     const int16_t *trig = tab::kMdctTrig;
     for (int i = 0; i < shift; i++) { N >>= 1; trig += N; }
     const int N2 = N >> 1, N4 = N >> 2;
+    int32_t *yp = out + (kOverlap >> 1);
     {
-        const int32_t *xp1 = in, *xp2 = in + stride * (N2 - 1);
-        int32_t *yp = out + (kOverlap >> 1);
+        // pre-rotation into bit-reversed order (independent per i)
         const int16_t *bitrev = tab::kFftBitrev + tab::kFftBitrevOff[shift];
-        for (int i = 0; i < N4; i++) {
+        CUO_PFOR(i, 0, N4) {
             int rev = bitrev[i];
-            int32_t yr = add32_ovf(smul(*xp2, trig[i]), smul(*xp1, trig[N4 + i]));
-            int32_t yi = sub32_ovf(smul(*xp1, trig[i]), smul(*xp2, trig[N4 + i]));
+            int32_t x1 = in[2 * stride * i], x2 = in[stride * (N2 - 1) - 2 * stride * i];
+            int32_t yr = add32_ovf(smul(x2, trig[i]), smul(x1, trig[N4 + i]));
+            int32_t yi = sub32_ovf(smul(x1, trig[i]), smul(x2, trig[N4 + i]));
             yp[2 * rev + 1] = yr;
             yp[2 * rev] = yi;
-            xp1 += 2 * stride;
-            xp2 -= 2 * stride;
         }
     }
-    fft_impl(shift, (Cpx *)(out + (kOverlap >> 1)));
+    fft_impl(shift, (Cpx *)yp);
     {
-        int32_t *yp0 = out + (kOverlap >> 1), *yp1 = out + (kOverlap >> 1) + N2 - 2;
-        for (int i = 0; i < (N4 + 1) >> 1; i++) {
+        // post-rotation: iteration i owns the pairs at 2i and N2-2-2i
+        CUO_PFOR(i, 0, (N4 + 1) >> 1) {
+            int32_t *yp0 = yp + 2 * i, *yp1 = yp + N2 - 2 - 2 * i;
             int32_t re = yp0[1], im = yp0[0];
             int16_t t0 = trig[i], t1 = trig[N4 + i];
             int32_t yr = add32_ovf(smul(re, t0), smul(im, t1));
@@ -1201,19 +1197,16 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
             yi = sub32_ovf(smul(re, t1), smul(im, t0));
             yp1[0] = yr;
             yp0[1] = yi;
-            yp0 += 2;
-            yp1 -= 2;
         }
     }
     {
-        int32_t *xp1 = out + kOverlap - 1, *yp1 = out;
-        const int16_t *wp1 = tab::kWindow, *wp2 = tab::kWindow + kOverlap - 1;
-        for (int i = 0; i < kOverlap / 2; i++) {
-            int32_t x1 = *xp1, x2 = *yp1;
-            *yp1++ = sub32_ovf(mul16_32_q15(*wp2, x2), mul16_32_q15(*wp1, x1));
-            *xp1-- = add32_ovf(mul16_32_q15(*wp1, x2), mul16_32_q15(*wp2, x1));
-            wp1++;
-            wp2--;
+        // TDAC mirror over the overlap: iteration i owns out[i] and out[119-i]
+        const int16_t *win = tab::kWindow;
+        CUO_PFOR(i, 0, kOverlap / 2) {
+            int32_t x1 = out[kOverlap - 1 - i], x2 = out[i];
+            int16_t w1 = win[i], w2 = win[kOverlap - 1 - i];
+            out[i] = sub32_ovf(mul16_32_q15(w2, x2), mul16_32_q15(w1, x1));
+            out[kOverlap - 1 - i] = add32_ovf(mul16_32_q15(w1, x2), mul16_32_q15(w2, x1));
         }
     }
 }
@@ -1231,20 +1224,26 @@ CUO_FN void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *ol
     else { B = 1; NB = 120 << LM; shift = 3 - LM; }
     denormalise_bands(X, freq, oldBandE, start, effEnd, M, silence);
     for (int b = 0; b < B; b++) mdct_backward(&freq[b], out_syn + NB * b, shift, B);
-    for (int i = 0; i < N; i++) out_syn[i] = saturate(out_syn[i], kSigSat);
+    CUO_PFOR(i, 0, N) out_syn[i] = saturate(out_syn[i], kSigSat);
 }
 
 // ------------------------------------------------------------------ post-filter
 static constexpr int16_t kCombGains[3][3] = {{10048, 7112, 4248}, {15200, 8784, 0}, {26208, 3280, 0}};
 
-CUO_FN void comb_filter_const(int32_t *y, const int32_t *x, int T, int N, int16_t g10, int16_t g11, int16_t g12)
+// y[i] = x[i] + g10 x[i-T] + g11 (x[i-T+1] + x[i-T-1]) + g12 (x[i-T+2] + x[i-T-2]),
+// computed in blocks of `blk` consecutive outputs spread over the lanes.  When
+// y == x the filter is recursive with lag >= T - 2, so blk <= T - 2 keeps every
+// input of a block already final, exactly as in the serial loop.
+CUO_FN void comb_filter_const(int32_t *y, const int32_t *x, int T, int N, int16_t g10, int16_t g11, int16_t g12,
+                              int blk)
 {
-    int32_t x4 = x[-T - 2], x3 = x[-T - 1], x2 = x[-T], x1 = x[-T + 1];
-    for (int i = 0; i < N; i++) {
-        int32_t x0 = x[i - T + 2];
-        int32_t v = x[i] + mul16_32_q15(g10, x2) + mul16_32_q15(g11, x1 + x3) + mul16_32_q15(g12, x0 + x4);
-        y[i] = saturate(v, kSigSat);
-        x4 = x3; x3 = x2; x2 = x1; x1 = x0;
+    for (int i0 = 0; i0 < N; i0 += blk) {
+        const int i1 = imin(N, i0 + blk);
+        CUO_PFOR(i, i0, i1) {
+            int32_t v = x[i] + mul16_32_q15(g10, x[i - T]) + mul16_32_q15(g11, x[i - T + 1] + x[i - T - 1])
+                      + mul16_32_q15(g12, x[i - T + 2] + x[i - T - 2]);
+            y[i] = saturate(v, kSigSat);
+        }
     }
 }
 
@@ -1253,7 +1252,7 @@ CUO_FN void comb_filter(int32_t *y, int32_t *x, int T0, int T1, int N, int16_t g
                         int tapset1, const int16_t *window, int overlap)
 {
     if (g0 == 0 && g1 == 0) {
-        if (x != y) for (int i = 0; i < N; i++) y[i] = x[i];
+        if (x != y) CUO_PFOR(i, 0, N) y[i] = x[i];
         return;
     }
     T0 = imax(T0, kCombMinPeriod);
@@ -1264,27 +1263,30 @@ CUO_FN void comb_filter(int32_t *y, int32_t *x, int T0, int T1, int N, int16_t g
     int16_t g10 = (int16_t)mul16_16_p15(g1, kCombGains[tapset1][0]);
     int16_t g11 = (int16_t)mul16_16_p15(g1, kCombGains[tapset1][1]);
     int16_t g12 = (int16_t)mul16_16_p15(g1, kCombGains[tapset1][2]);
-    int32_t x1 = x[-T1 + 1], x2 = x[-T1], x3 = x[-T1 - 1], x4 = x[-T1 - 2];
     if (g0 == g1 && T0 == T1 && tapset0 == tapset1) overlap = 0;
-    int i;
-    for (i = 0; i < overlap; i++) {
-        int32_t x0 = x[i - T1 + 2];
-        int16_t f = (int16_t)mul16_16_q15(window[i], window[i]);
-        int32_t v = x[i]
-            + mul16_32_q15(mul16_16_q15((kQ15One - f), g00), x[i - T0])
-            + mul16_32_q15(mul16_16_q15((kQ15One - f), g01), x[i - T0 + 1] + x[i - T0 - 1])
-            + mul16_32_q15(mul16_16_q15((kQ15One - f), g02), x[i - T0 + 2] + x[i - T0 - 2])
-            + mul16_32_q15(mul16_16_q15(f, g10), x2)
-            + mul16_32_q15(mul16_16_q15(f, g11), x1 + x3)
-            + mul16_32_q15(mul16_16_q15(f, g12), x0 + x4);
-        y[i] = saturate(v, kSigSat);
-        x4 = x3; x3 = x2; x2 = x1; x1 = x0;
+    const bool inplace = x == y;
+    int blk = inplace ? imin(T0, T1) - 2 : kLanes;
+    if (blk > kLanes) blk = kLanes;
+    for (int i0 = 0; i0 < overlap; i0 += blk) {
+        const int i1 = imin(overlap, i0 + blk);
+        CUO_PFOR(i, i0, i1) {
+            int16_t f = (int16_t)mul16_16_q15(window[i], window[i]);
+            int32_t v = x[i]
+                + mul16_32_q15(mul16_16_q15((kQ15One - f), g00), x[i - T0])
+                + mul16_32_q15(mul16_16_q15((kQ15One - f), g01), x[i - T0 + 1] + x[i - T0 - 1])
+                + mul16_32_q15(mul16_16_q15((kQ15One - f), g02), x[i - T0 + 2] + x[i - T0 - 2])
+                + mul16_32_q15(mul16_16_q15(f, g10), x[i - T1])
+                + mul16_32_q15(mul16_16_q15(f, g11), x[i - T1 + 1] + x[i - T1 - 1])
+                + mul16_32_q15(mul16_16_q15(f, g12), x[i - T1 + 2] + x[i - T1 - 2]);
+            y[i] = saturate(v, kSigSat);
+        }
     }
     if (g1 == 0) {
-        if (x != y) for (int j = overlap; j < N; j++) y[j] = x[j];
+        if (x != y) CUO_PFOR(j, overlap, N) y[j] = x[j];
         return;
     }
-    comb_filter_const(y + i, x + i, T1, N - i, g10, g11, g12);
+    int blk1 = inplace ? imin(kLanes, T1 - 2) : kLanes;
+    comb_filter_const(y + overlap, x + overlap, T1, N - overlap, g10, g11, g12, blk1);
 }
 
 // deemphasis, mono, no downsampling (celt_decoder.c:279-377)
@@ -1314,7 +1316,7 @@ CUO_FN void prefilter_and_fold(CeltState *st, int N, int32_t *etmp)
     int32_t *mem = st->mem;
     comb_filter(etmp, mem + kDecBuf - N, st->pf_period_old, st->pf_period, kOverlap, (int16_t)-st->pf_gain_old,
                 (int16_t)-st->pf_gain, st->pf_tapset_old, st->pf_tapset, nullptr, 0);
-    for (int i = 0; i < kOverlap / 2; i++)
+    CUO_PFOR(i, 0, kOverlap / 2)
         mem[kDecBuf - N + i] = mul16_32_q15(tab::kWindow[i], etmp[kOverlap - 1 - i])
                              + mul16_32_q15(tab::kWindow[kOverlap - i - 1], etmp[i]);
 }
@@ -1334,7 +1336,7 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
     if (noise_based) {
         int end = st->end;
         int effEnd = imax(start, imin(end, kNbBands));
-        for (int i = 0; i < kDecBuf - N + kOverlap; i++) mem[i] = mem[i + N];
+        shift_history(mem, N);
         if (st->prefilter_and_fold) prefilter_and_fold(st, N, sc->etmp);
         int16_t decay = loss_duration == 0 ? 1536 : 512;
         for (int i = start; i < end; i++)
@@ -1485,7 +1487,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     int codedBands = compute_allocation(start, end, offsets, cap, alloc_trim, bits, &balance, pulses, fine_quant,
                                         fine_priority, LM, dec, sc);
     unquant_fine_energy(start, end, oldBandE, fine_quant, dec);
-    for (int i = 0; i < kDecBuf - N + kOverlap; i++) mem[i] = mem[i + N];
+    shift_history(mem, N);
     int16_t *X = sc->X;
     quant_all_bands(start, end, X, sc->collapse, pulses, shortBlocks, spread_decision, tf_res,
                     len * (8 << kBitRes) - anti_collapse_rsv, balance, dec, LM, codedBands, &st->rng, sc);

@@ -47,6 +47,21 @@ __host__ __device__ inline T *lds_uniform(T *p)
 }
 #endif
 
+// Lanes of the wavefront.  The decoder's control flow is wave-uniform; loops
+// whose iterations are independent spread their iterations over the 64 lanes
+// (CUO_PFOR).  Each iteration's arithmetic is the serial loop's, so results
+// are identical.  Writes made in such a loop are read afterwards by other
+// lanes through LDS, which executes a wavefront's accesses in program order.
+// On the host there is one lane.
+#if defined(__HIP_DEVICE_COMPILE__)
+constexpr int kLanes = 64;
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+#else
+constexpr int kLanes = 1;
+__host__ __device__ inline int lane_id() { return 0; }
+#endif
+#define CUO_PFOR(i, lo, hi) for (int i = (lo) + lane_id(); i < (hi); i += kLanes)
+
 // ---------------------------------------------------------------- basics
 CUO_FN int imin(int a, int b) { return a < b ? a : b; }
 CUO_FN int imax(int a, int b) { return a > b ? a : b; }
