@@ -18,7 +18,8 @@ REFERENCE = "/root/reference"
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-Wno-unused-function",
-             # packed fp32 instructions stay off (hardware notes)
+             # packed fp32 instructions stay off: some of their forms have given
+             # wrong results on this hardware without any error
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
@@ -29,10 +30,12 @@ def _run(cmd, cwd=None):
     return r.stdout
 
 
-def build_lib(arch="gfx950"):
-    out = os.path.join(PKG, "libcuopus.so")
+def build_lib(arch="gfx950", out=None, defines=()):
+    """defines: extra preprocessor flags, e.g. ("CUO_PROF",) for the
+    cycle-accounting build (written to `out`, never over the product)."""
+    out = out or os.path.join(PKG, "libcuopus.so")
     tmp = out + ".tmp"
-    cmd = [HIPCC, "--offload-arch=" + arch] + HIP_FLAGS + [
+    cmd = [HIPCC, "--offload-arch=" + arch] + HIP_FLAGS + ["-D" + d for d in defines] + [
         "-shared", "-I" + CSRC,
         os.path.join(CSRC, "libcuopus.hip"), os.path.join(CSRC, "host", "demux.cpp"),
         "-o", tmp]

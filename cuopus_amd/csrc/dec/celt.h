@@ -1214,6 +1214,7 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
 CUO_FN void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *oldBandE, int start, int effEnd,
                            int isTransient, int LM, int silence, int32_t *freq)
 {
+    CUO_PROF_SCOPE(kProfCeltSynth);
     X = lds_uniform(X);
     out_syn = lds_uniform(out_syn);
     oldBandE = lds_uniform(oldBandE);
@@ -1292,6 +1293,7 @@ CUO_FN void comb_filter(int32_t *y, int32_t *x, int T0, int T1, int N, int16_t g
 // deemphasis, mono, no downsampling (celt_decoder.c:279-377)
 CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int accum)
 {
+    CUO_PROF_SCOPE(kProfCeltDeemph);
     const int16_t coef0 = tab::kPreemph[0];
     int32_t m = *mem;
     if (accum) {
@@ -1328,6 +1330,7 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc);
 // concealment path is not available.
 CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
 {
+    CUO_PROF_SCOPE(kProfCeltLost);
     int32_t *mem = st->mem;
     int32_t *out_syn = mem + kDecBuf - N;
     int loss_duration = st->loss_duration;
@@ -1371,6 +1374,7 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
 CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size, RangeDec *dec,
                        int accum, CeltScratch *sc)
 {
+    CUO_PROF_SCOPE(kProfCelt);
     st = lds_uniform(st);
     sc = lds_uniform(sc);
     if (pcm) pcm = lds_uniform(pcm);
@@ -1448,7 +1452,10 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
             }
         }
     }
-    unquant_coarse_energy(start, end, oldBandE, intra_ener, dec, LM);
+    {
+        CUO_PROF_SCOPE(kProfCeltEnergy);
+        unquant_coarse_energy(start, end, oldBandE, intra_ener, dec, LM);
+    }
     int32_t *tf_res = sc->tf_res;
     tf_decode(start, end, isTransient, tf_res, LM, dec);
     tell = ec_tell(dec);
@@ -1484,13 +1491,20 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     bits -= anti_collapse_rsv;
     int32_t balance;
     int32_t *pulses = sc->pulses, *fine_quant = sc->fine_quant, *fine_priority = sc->fine_priority;
-    int codedBands = compute_allocation(start, end, offsets, cap, alloc_trim, bits, &balance, pulses, fine_quant,
+    int codedBands;
+    {
+        CUO_PROF_SCOPE(kProfCeltAlloc);
+        codedBands = compute_allocation(start, end, offsets, cap, alloc_trim, bits, &balance, pulses, fine_quant,
                                         fine_priority, LM, dec, sc);
+    }
     unquant_fine_energy(start, end, oldBandE, fine_quant, dec);
     shift_history(mem, N);
     int16_t *X = sc->X;
-    quant_all_bands(start, end, X, sc->collapse, pulses, shortBlocks, spread_decision, tf_res,
-                    len * (8 << kBitRes) - anti_collapse_rsv, balance, dec, LM, codedBands, &st->rng, sc);
+    {
+        CUO_PROF_SCOPE(kProfCeltBands);
+        quant_all_bands(start, end, X, sc->collapse, pulses, shortBlocks, spread_decision, tf_res,
+                        len * (8 << kBitRes) - anti_collapse_rsv, balance, dec, LM, codedBands, &st->rng, sc);
+    }
     int anti_collapse_on = 0;
     if (anti_collapse_rsv > 0) anti_collapse_on = (int)ec_bits(dec, 1);
     unquant_energy_finalise(start, end, oldBandE, fine_quant, fine_priority, len * 8 - ec_tell(dec), dec);
@@ -1503,11 +1517,15 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
 
     st->pf_period = imax(st->pf_period, kCombMinPeriod);
     st->pf_period_old = imax(st->pf_period_old, kCombMinPeriod);
-    comb_filter(out_syn, out_syn, st->pf_period_old, st->pf_period, 120, (int16_t)st->pf_gain_old,
-                (int16_t)st->pf_gain, st->pf_tapset_old, st->pf_tapset, tab::kWindow, kOverlap);
-    if (LM != 0)
-        comb_filter(out_syn + 120, out_syn + 120, st->pf_period, postfilter_pitch, N - 120, (int16_t)st->pf_gain,
-                    postfilter_gain, st->pf_tapset, postfilter_tapset, tab::kWindow, kOverlap);
+    {
+        CUO_PROF_SCOPE(kProfCeltComb);
+        comb_filter(out_syn, out_syn, st->pf_period_old, st->pf_period, 120, (int16_t)st->pf_gain_old,
+                    (int16_t)st->pf_gain, st->pf_tapset_old, st->pf_tapset, tab::kWindow, kOverlap);
+        if (LM != 0)
+            comb_filter(out_syn + 120, out_syn + 120, st->pf_period, postfilter_pitch, N - 120,
+                        (int16_t)st->pf_gain, postfilter_gain, st->pf_tapset, postfilter_tapset, tab::kWindow,
+                        kOverlap);
+    }
     st->pf_period_old = st->pf_period;
     st->pf_gain_old = st->pf_gain;
     st->pf_tapset_old = st->pf_tapset;

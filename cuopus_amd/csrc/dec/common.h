@@ -62,6 +62,34 @@ __host__ __device__ inline int lane_id() { return 0; }
 #endif
 #define CUO_PFOR(i, lo, hi) for (int i = (lo) + lane_id(); i < (hi); i += kLanes)
 
+// Optional cycle accounting (built with -DCUO_PROF only): a scope adds the
+// shader clock cycles it spans to g_prof[id], once per wavefront.
+#if defined(CUO_PROF)
+__device__ unsigned long long g_prof[32];
+#endif
+#if defined(CUO_PROF) && defined(__HIP_DEVICE_COMPILE__)
+struct ProfScope {
+    int id;
+    unsigned long long t0;
+    __device__ explicit ProfScope(int i) : id(i), t0(__builtin_readcyclecounter()) {}
+    __device__ ~ProfScope()
+    {
+        unsigned long long dt = __builtin_readcyclecounter() - t0;
+        if (lane_id() == 0) atomicAdd(&g_prof[id], dt);
+    }
+};
+#define CUO_PROF_CAT2(a, b) a##b
+#define CUO_PROF_CAT(a, b) CUO_PROF_CAT2(a, b)
+#define CUO_PROF_SCOPE(id) ProfScope CUO_PROF_CAT(prof_scope_, __LINE__)(id)
+#else
+#define CUO_PROF_SCOPE(id)
+#endif
+enum {
+    kProfChain = 0, kProfFrame, kProfCelt, kProfCeltHead, kProfCeltBands, kProfCeltSynth, kProfCeltComb,
+    kProfCeltDeemph, kProfSilk, kProfEmit, kProfCeltLost, kProfCeltEnergy, kProfCeltAlloc, kProfCeltTail,
+    kProfCount
+};
+
 // ---------------------------------------------------------------- basics
 CUO_FN int imin(int a, int b) { return a < b ? a : b; }
 CUO_FN int imax(int a, int b) { return a > b ? a : b; }

@@ -1202,6 +1202,7 @@ struct SilkControl {
 CUO_FN int silk_decode(SilkDecoder *dec, const SilkControl *ctl, int lostFlag, int newPacketFlag, RangeDec *d,
                        int16_t *samplesOut, int32_t *nSamplesOut, SilkScratch *sc)
 {
+    CUO_PROF_SCOPE(kProfSilk);
     dec = lds_uniform(dec);
     ctl = lds_uniform(ctl);
     d = lds_uniform(d);

@@ -75,6 +75,7 @@ struct ChainOut {
     int lane;
     __device__ void emit(int n)
     {
+        CUO_PROF_SCOPE(kProfEmit);
         uint64_t a = pos > lo ? pos : lo;
         uint64_t b = pos + (uint64_t)n < hi ? pos + (uint64_t)n : hi;
         for (uint64_t i = a + (uint64_t)lane; i < b; i += 64) out[dst + (i - lo)] = buf[i - pos];
@@ -85,6 +86,7 @@ struct ChainOut {
 // Decode one chain on the calling wavefront.
 __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int lane)
 {
+    CUO_PROF_SCOPE(kProfChain);
     const ChainDesc ch = a.chains[c];
     OpusDecState *st = lds_uniform(&ws->st);
     DecScratch *sc = lds_uniform(&ws->sc);

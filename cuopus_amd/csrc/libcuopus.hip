@@ -956,6 +956,24 @@ extern "C" void cuopus_batch_destroy(cuopus_batch *b)
     delete b;
 }
 
+// Cycle accounting of a -DCUO_PROF build: copies and clears the per-scope
+// cycle sums (returns the number of scopes, 0 in a normal build).
+extern "C" int cuopus_prof_read(unsigned long long *out, int cap)
+{
+#if defined(CUO_PROF)
+    unsigned long long h[32];
+    if (hipMemcpyFromSymbol(h, HIP_SYMBOL(g_prof), sizeof(h)) != hipSuccess) return -1;
+    unsigned long long z[32] = {0};
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_prof), z, sizeof(z)) != hipSuccess) return -1;
+    for (int i = 0; i < cap && i < kProfCount; i++) out[i] = h[i];
+    return kProfCount;
+#else
+    (void)out;
+    (void)cap;
+    return 0;
+#endif
+}
+
 extern "C" int cuopus_warmup(char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);

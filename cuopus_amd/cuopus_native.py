@@ -77,6 +77,7 @@ def _load():
         "cuopus_last_arena_overflow": (ctypes.c_int, []),
         "cuopus_workspace_bytes": (ctypes.c_longlong, []),
         "cuopus_release_workspaces": (None, []),
+        "cuopus_prof_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]),
         "cuopus_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(SZ), ctypes.c_int,
                                                ctypes.POINTER(P), ctypes.POINTER(SZ), ctypes.POINTER(P),
                                                ctypes.POINTER(SZ), ctypes.POINTER(BatchOpts),
@@ -269,6 +270,18 @@ def last_arena_overflow() -> bool:
 
 def release_workspaces():
     _load().cuopus_release_workspaces()
+
+
+PROF_SCOPES = ["chain", "frame", "celt", "celt_head", "celt_bands", "celt_synth", "celt_comb", "celt_deemph",
+               "silk", "emit", "celt_lost", "celt_energy", "celt_alloc", "celt_tail"]
+
+
+def prof_read():
+    """Per-scope shader-clock cycle sums of a -DCUO_PROF library (cleared on
+    read); empty dict for a normal build."""
+    buf = (ctypes.c_ulonglong * 32)()
+    n = _load().cuopus_prof_read(buf, 32)
+    return {PROF_SCOPES[i]: int(buf[i]) for i in range(max(n, 0)) if buf[i]}
 
 
 MODE_PLAIN, MODE_ONE_LAUNCH, MODE_PERSISTENT = 0, 1, 2

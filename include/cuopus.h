@@ -60,6 +60,12 @@ int cuopus_decode_f16k_multi(const uint8_t *const *oggs, const size_t *ns, int n
  * Replaces libcuopus.cu:746-776. */
 int cuopus_warmup(char *errbuf, size_t errcap);
 
+/* Development aid, no counterpart in the reference: in a library built with
+ * -DCUO_PROF, copies the decoder's per-scope shader-clock cycle sums into
+ * out (up to cap entries), clears them and returns the scope count; returns
+ * 0 in a normal build. */
+int cuopus_prof_read(unsigned long long *out, int cap);
+
 /* 1 if the last decode failed for lack of per-chain decoder memory.  The
  * decoder state here is statically sized, so this is always 0 (kept for ABI
  * compatibility).  Replaces libcuopus.cu:201. */
