@@ -146,25 +146,25 @@ CUO_FN int bitexact_log2tan(int isin, int icos)
 // ------------------------------------------------------------------ rate helpers (rate.h)
 CUO_FN const uint8_t *pulse_cache(int band, int LMp1)
 {
-    return tab::kCacheBits + tab::kCacheIndex[LMp1 * kNbBands + band];
+    return tab::kCacheBits + ld_s16(tab::kCacheIndex, LMp1 * kNbBands + band);
 }
 CUO_FN int get_pulses(int i) { return i < 8 ? i : (8 + (i & 7)) << ((i >> 3) - 1); }
 CUO_FN int bits2pulses(int band, int LM, int bits)
 {
     const uint8_t *cache = pulse_cache(band, LM + 1);
-    int lo = 0, hi = cache[0];
+    int lo = 0, hi = (int)ld_u8(cache, 0);
     bits--;
     for (int i = 0; i < 6; i++) {
         int mid = (lo + hi + 1) >> 1;
-        if ((int)cache[mid] >= bits) hi = mid; else lo = mid;
+        if ((int)ld_u8(cache, mid) >= bits) hi = mid; else lo = mid;
     }
-    if (bits - (lo == 0 ? -1 : (int)cache[lo]) <= (int)cache[hi] - bits) return lo;
+    if (bits - (lo == 0 ? -1 : (int)ld_u8(cache, lo)) <= (int)ld_u8(cache, hi) - bits) return lo;
     return hi;
 }
 CUO_FN int pulses2bits(int band, int LM, int pulses)
 {
     const uint8_t *cache = pulse_cache(band, LM + 1);
-    return pulses == 0 ? 0 : cache[pulses] + 1;
+    return pulses == 0 ? 0 : (int)ld_u8(cache, pulses) + 1;
 }
 
 // ------------------------------------------------------------------ coarse/fine energy
@@ -288,26 +288,41 @@ CUO_FN int compute_allocation(int start, int end, const int32_t *offsets, const 
     int skip_start = start;
     int skip_rsv = total >= 1 << kBitRes ? 1 << kBitRes : 0;
     total -= skip_rsv;
-    for (int j = start; j < end; j++) {
+    CUO_PFOR(j, start, end) {
         thresh[j] = imax(C << kBitRes, (3 * (eB[j + 1] - eB[j]) << LM << kBitRes) >> 4);
-        trim_offset[j] = C * (eB[j + 1] - eB[j]) * (alloc_trim - 5 - LM) * (end - j - 1) * (1 << (LM + kBitRes)) >> 6;
-        if ((eB[j + 1] - eB[j]) << LM == 1) trim_offset[j] -= C << kBitRes;
+        int t = C * (eB[j + 1] - eB[j]) * (alloc_trim - 5 - LM) * (end - j - 1) * (1 << (LM + kBitRes)) >> 6;
+        if ((eB[j + 1] - eB[j]) << LM == 1) t -= C << kBitRes;
+        trim_offset[j] = t;
     }
+    // The band loops below run from the top band down with a `done` flag that
+    // latches at the first band (from the top) whose bits reach its threshold:
+    // every band at or below the highest such band counts min(bits, cap), every
+    // band above it counts the floor or nothing.  The highest flagged band is a
+    // max over lanes and the sum a sum over lanes, so each loop is one pass.
+    auto vec_bits = [&](int v, int j) -> int {
+        int bj = C * (eB[j + 1] - eB[j]) * tab::kAllocVectors[v * kNbBands + j] << LM >> 2;
+        if (bj > 0) bj = imax(0, bj + trim_offset[j]);
+        return bj + offsets[j];
+    };
     int lo = 1, hi = tab::kNbAllocVectors - 1;
+    CUO_PROF_SCOPE(kProfAllocBisect);
     do {
-        int done = 0, psum = 0, mid = (lo + hi) >> 1;
-        for (int j = end; j-- > start;) {
-            int N = eB[j + 1] - eB[j];
-            int bitsj = C * N * tab::kAllocVectors[mid * kNbBands + j] << LM >> 2;
-            if (bitsj > 0) bitsj = imax(0, bitsj + trim_offset[j]);
-            bitsj += offsets[j];
-            if (bitsj >= thresh[j] || done) { done = 1; psum += imin(bitsj, cap[j]); }
-            else if (bitsj >= C << kBitRes) psum += C << kBitRes;
+        int mid = (lo + hi) >> 1;
+        int jtop = -1;
+        CUO_PFOR(j, start, end) if (vec_bits(mid, j) >= thresh[j]) jtop = imax(jtop, j);
+        jtop = wave_max(jtop);
+        int32_t part = 0;
+        CUO_PFOR(j, start, end) {
+            int bitsj = vec_bits(mid, j);
+            if (j <= jtop) part += imin(bitsj, cap[j]);
+            else if (bitsj >= C << kBitRes) part += C << kBitRes;
         }
+        int32_t psum = wave_sum(part);
         if (psum > total) hi = mid - 1; else lo = mid + 1;
     } while (lo <= hi);
     hi = lo--;
-    for (int j = start; j < end; j++) {
+    int skip_top = -1;
+    CUO_PFOR(j, start, end) {
         int N = eB[j + 1] - eB[j];
         int bits1j = C * N * tab::kAllocVectors[lo * kNbBands + j] << LM >> 2;
         int bits2j = hi >= tab::kNbAllocVectors ? cap[j] : C * N * tab::kAllocVectors[hi * kNbBands + j] << LM >> 2;
@@ -315,36 +330,45 @@ CUO_FN int compute_allocation(int start, int end, const int32_t *offsets, const 
         if (bits2j > 0) bits2j = imax(0, bits2j + trim_offset[j]);
         if (lo > 0) bits1j += offsets[j];
         bits2j += offsets[j];
-        if (offsets[j] > 0) skip_start = j;
+        if (offsets[j] > 0) skip_top = imax(skip_top, j);
         bits2j = imax(0, bits2j - bits1j);
         bits1[j] = bits1j;
         bits2[j] = bits2j;
     }
+    skip_top = wave_max(skip_top);
+    if (skip_top >= 0) skip_start = skip_top;
     // ---- interp_bits2pulses
     const int alloc_floor = C << kBitRes;
     const int logM = LM << kBitRes;
     lo = 0; hi = 1 << 6;
-    int32_t psum;
     for (int i = 0; i < 6; i++) {
         int mid = (lo + hi) >> 1;
-        psum = 0;
-        int done = 0;
-        for (int j = end; j-- > start;) {
+        int jtop = -1;
+        CUO_PFOR(j, start, end) if (bits1[j] + (mid * (int32_t)bits2[j] >> 6) >= thresh[j]) jtop = imax(jtop, j);
+        jtop = wave_max(jtop);
+        int32_t part = 0;
+        CUO_PFOR(j, start, end) {
             int tmp = bits1[j] + (mid * (int32_t)bits2[j] >> 6);
-            if (tmp >= thresh[j] || done) { done = 1; psum += imin(tmp, cap[j]); }
-            else if (tmp >= alloc_floor) psum += alloc_floor;
+            if (j <= jtop) part += imin(tmp, cap[j]);
+            else if (tmp >= alloc_floor) part += alloc_floor;
         }
+        int32_t psum = wave_sum(part);
         if (psum > total) hi = mid; else lo = mid;
     }
-    psum = 0;
-    int done = 0;
-    for (int j = end; j-- > start;) {
-        int tmp = bits1[j] + ((int32_t)lo * bits2[j] >> 6);
-        if (tmp < thresh[j] && !done) tmp = tmp >= alloc_floor ? alloc_floor : 0;
-        else done = 1;
-        tmp = imin(tmp, cap[j]);
-        bits[j] = tmp;
-        psum += tmp;
+    int32_t psum;
+    {
+        int jtop = -1;
+        CUO_PFOR(j, start, end) if (bits1[j] + ((int32_t)lo * bits2[j] >> 6) >= thresh[j]) jtop = imax(jtop, j);
+        jtop = wave_max(jtop);
+        int32_t part = 0;
+        CUO_PFOR(j, start, end) {
+            int tmp = bits1[j] + ((int32_t)lo * bits2[j] >> 6);
+            if (j > jtop) tmp = tmp >= alloc_floor ? alloc_floor : 0;
+            tmp = imin(tmp, cap[j]);
+            bits[j] = tmp;
+            part += tmp;
+        }
+        psum = wave_sum(part);
     }
     int codedBands;
     for (codedBands = end;; codedBands--) {
@@ -369,11 +393,13 @@ CUO_FN int compute_allocation(int start, int end, const int32_t *offsets, const 
     int32_t left = total - psum;
     int32_t percoeff = left / (eB[codedBands] - eB[start]);
     left -= (eB[codedBands] - eB[start]) * percoeff;
-    for (int j = start; j < codedBands; j++) bits[j] += (int)percoeff * (eB[j + 1] - eB[j]);
-    for (int j = start; j < codedBands; j++) {
-        int tmp = (int)imin(left, eB[j + 1] - eB[j]);
-        bits[j] += tmp;
-        left -= tmp;
+    // the serial remainder hand-out gives band j min(max(left - (eB[j] -
+    // eB[start]), 0), width_j) when left >= 0, and all of a negative left to
+    // the first band
+    CUO_PFOR(j, start, codedBands) {
+        int w = eB[j + 1] - eB[j];
+        int share = left >= 0 ? imin(imax(left - (eB[j] - eB[start]), 0), w) : (j == start ? left : 0);
+        bits[j] += (int)percoeff * w + share;
     }
     int32_t balance = 0;
     int j;
@@ -425,13 +451,16 @@ CUO_FN int compute_allocation(int start, int end, const int32_t *offsets, const 
 CUO_FN uint32_t pvq_u(int n, int k)
 {
     int r = imin(n, k), c = imax(n, k);
-    return tab::kPvqU[tab::kPvqURow[r] + c];
+    return tab::kPvqU[ld_s16(tab::kPvqURow, r) + c];
 }
 CUO_FN uint32_t pvq_v(int n, int k) { return pvq_u(n, k) + pvq_u(n, k + 1); }
 
 // cwrsi: codeword index -> pulse vector, returns sum of squares (cwrs.c:463-537)
 CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
 {
+    n = uniform(n);
+    k = uniform(k);
+    i = uniform(i);
     int32_t yy = 0;
     uint32_t p;
     int s, k0;
@@ -439,7 +468,7 @@ CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
     while (n > 2) {
         uint32_t q;
         if (k >= n) {
-            const uint32_t *row = tab::kPvqU + tab::kPvqURow[n];
+            const uint32_t *row = tab::kPvqU + ld_s16(tab::kPvqURow, n);
             p = row[k + 1];
             s = -(i >= p);
             i -= p & (uint32_t)s;
@@ -447,7 +476,7 @@ CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
             q = row[n];
             if (q > i) {
                 k = n;
-                do p = tab::kPvqU[tab::kPvqURow[--k] + n]; while (p > i);
+                do p = tab::kPvqU[ld_s16(tab::kPvqURow, --k) + n]; while (p > i);
             } else {
                 for (p = row[k]; p > i; p = row[k]) k--;
             }
@@ -456,8 +485,8 @@ CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
             *y++ = val;
             yy = mac16_16(yy, val, val);
         } else {
-            p = tab::kPvqU[tab::kPvqURow[k] + n];
-            q = tab::kPvqU[tab::kPvqURow[k + 1] + n];
+            p = tab::kPvqU[ld_s16(tab::kPvqURow, k) + n];
+            q = tab::kPvqU[ld_s16(tab::kPvqURow, k + 1) + n];
             if (p <= i && i < q) {
                 i -= p;
                 *y++ = 0;
@@ -465,7 +494,7 @@ CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
                 s = -(i >= q);
                 i -= q & (uint32_t)s;
                 k0 = k;
-                do p = tab::kPvqU[tab::kPvqURow[--k] + n]; while (p > i);
+                do p = tab::kPvqU[ld_s16(tab::kPvqURow, --k) + n]; while (p > i);
                 i -= p;
                 val = (int16_t)((k0 - k + s) ^ s);
                 *y++ = val;
@@ -534,13 +563,26 @@ CUO_FN void exp_rotation(int16_t *X, int len, int dir, int stride, int K, int sp
 
 CUO_FN unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDec *d, int16_t gain, int32_t *iy)
 {
-    int32_t Ryy = cwrsi(N, K, ec_uint(d, pvq_v(N, K)), iy);
+    CUO_PROF_SCOPE(kProfPvq);
+    uint32_t idx;
+    {
+        CUO_PROF_SCOPE(kProfEcUint);
+        idx = ec_uint(d, pvq_v(N, K));
+    }
+    int32_t Ryy;
+    {
+        CUO_PROF_SCOPE(kProfCwrsi);
+        Ryy = cwrsi(N, K, idx, iy);
+    }
     // normalise_residual
     int k = celt_ilog2(Ryy) >> 1;
     int32_t t = vshr32(Ryy, 2 * (k - 7));
     int16_t g = (int16_t)mul16_16_p15(celt_rsqrt_norm(t), gain);
     for (int i = 0; i < N; i++) X[i] = (int16_t)pshr32(mul16_16(g, iy[i]), k + 1);
-    exp_rotation(X, N, -1, B, K, spread);
+    {
+        CUO_PROF_SCOPE(kProfExpRot);
+        exp_rotation(X, N, -1, B, K, spread);
+    }
     // extract_collapse_mask
     if (B <= 1) return 1;
     int N0 = N / B;
@@ -697,7 +739,11 @@ CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B
                 f.LM -= 1;
                 if (f.B == 1) f.fill = (f.fill & 1) | (f.fill << 1);
                 f.B = (f.B + 1) >> 1;
-                SplitOut so = compute_theta_mono(ctx, f.N, &f.b, f.B, f.B0, f.LM, &f.fill);
+                SplitOut so;
+                {
+                    CUO_PROF_SCOPE(kProfTheta);
+                    so = compute_theta_mono(ctx, f.N, &f.b, f.B, f.B0, f.LM, &f.fill);
+                }
                 f.mid = (int16_t)so.imid;
                 f.side = (int16_t)so.iside;
                 int delta = so.delta;
@@ -738,6 +784,7 @@ CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B
             if (q != 0) {
                 cm = alg_unquant(f.X, f.N, get_pulses(q), ctx->spread, f.B, ctx->ec, f.gain, ctx->iy);
             } else {
+                CUO_PROF_SCOPE(kProfLeafFold);
                 unsigned cm_mask = (unsigned)(1UL << f.B) - 1;
                 int fill = f.fill & (int)cm_mask;
                 if (!fill) {
@@ -817,6 +864,10 @@ CUO_FN unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_
         return 1;
     }
     if (tf_change > 0) recombine = tf_change;
+    unsigned cm;
+    int N_B0;
+    {
+    CUO_PROF_SCOPE(kProfBandPre);
     if (lowband_scratch && lowband && (recombine || ((N_B & 1) == 0 && tf_change < 0) || B0 > 1)) {
         for (int j = 0; j < N; j++) lowband_scratch[j] = lowband[j];
         lowband = lowband_scratch;
@@ -836,9 +887,11 @@ CUO_FN unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_
         tf_change++;
     }
     B0 = B;
-    int N_B0 = N_B;
+    N_B0 = N_B;
     if (B0 > 1 && lowband) deinterleave_hadamard(lowband, N_B >> recombine, B0 << recombine, longBlocks, tmp);
-    unsigned cm = quant_partition(ctx, X, N, b, B, lowband, LM, gain, fill);
+    }
+    cm = quant_partition(ctx, X, N, b, B, lowband, LM, gain, fill);
+    CUO_PROF_SCOPE(kProfBandPost);
     if (B0 > 1) interleave_hadamard(X, N_B >> recombine, B0 << recombine, longBlocks, tmp);
     N_B = N_B0;
     B = B0;

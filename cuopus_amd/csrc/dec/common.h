@@ -47,6 +47,59 @@ __host__ __device__ inline T *lds_uniform(T *p)
 }
 #endif
 
+// Wave-uniform value: on the device, moves a value every lane holds alike
+// into a scalar register, so what is computed from it runs on the scalar ALU
+// and table lookups indexed by it become scalar loads.  Identity on the host.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+__device__ __forceinline__ T uniform(T v)
+{
+    static_assert(sizeof(T) <= 4, "32-bit values only");
+    return (T)__builtin_amdgcn_readfirstlane((int)v);
+}
+#else
+template <class T>
+__host__ __device__ inline T uniform(T v)
+{
+    return v;
+}
+#endif
+
+// Table and packet-byte reads with a wave-uniform address.  On the device the
+// aligned 32-bit word holding the element is read through the constant
+// address space, which the compiler turns into a scalar (K-cache) load; the
+// element is then extracted with a shift.  Only for memory that no kernel
+// writes while the decoder runs: the constant tables and the packet bytes
+// (whose device buffer is padded so the word of the last byte is inside it).
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v)
+{
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (uint64_t)hi << 32 | lo;
+}
+__device__ __forceinline__ uint32_t const_word(const void *p)
+{
+    uint64_t a = uniform_u64((uint64_t)(uintptr_t)p);
+    typedef __attribute__((address_space(4))) const uint32_t cw_t;
+    return *(cw_t *)(uintptr_t)(a & ~(uint64_t)3);
+}
+__device__ __forceinline__ uint32_t ld_u8(const uint8_t *t, int i)
+{
+    const uint8_t *p = t + i;
+    return (const_word(p) >> (8 * ((uint32_t)(uintptr_t)p & 3))) & 0xFFu;
+}
+__device__ __forceinline__ int32_t ld_s16(const int16_t *t, int i)
+{
+    const int16_t *p = t + i;
+    uint32_t w = const_word(p);
+    return ((uint32_t)(uintptr_t)p & 2) ? (int32_t)w >> 16 : (int32_t)(int16_t)w;
+}
+#else
+__host__ __device__ inline uint32_t ld_u8(const uint8_t *t, int i) { return t[i]; }
+__host__ __device__ inline int32_t ld_s16(const int16_t *t, int i) { return t[i]; }
+#endif
+
 // Lanes of the wavefront.  The decoder's control flow is wave-uniform; loops
 // whose iterations are independent spread their iterations over the 64 lanes
 // (CUO_PFOR).  Each iteration's arithmetic is the serial loop's, so results
@@ -61,6 +114,43 @@ constexpr int kLanes = 1;
 __host__ __device__ inline int lane_id() { return 0; }
 #endif
 #define CUO_PFOR(i, lo, hi) for (int i = (lo) + lane_id(); i < (hi); i += kLanes)
+
+// Reductions over the lanes of per-lane partial results (identity on the
+// host).  Integer sums wrap like the serial loop's, so any order is exact.
+// Call with every lane active (outside CUO_PFOR bodies).
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ int32_t wave_sum(int32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = (int32_t)((uint32_t)v + (uint32_t)__shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int32_t wave_max(int32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        int32_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ int32_t wave_min(int32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        int32_t w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_or(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
+}
+#else
+__host__ __device__ inline int32_t wave_sum(int32_t v) { return v; }
+__host__ __device__ inline int32_t wave_max(int32_t v) { return v; }
+__host__ __device__ inline int32_t wave_min(int32_t v) { return v; }
+__host__ __device__ inline uint32_t wave_or(uint32_t v) { return v; }
+#endif
 
 // Optional cycle accounting (built with -DCUO_PROF only): a scope adds the
 // shader clock cycles it spans to g_prof[id], once per wavefront.
@@ -87,7 +177,8 @@ struct ProfScope {
 enum {
     kProfChain = 0, kProfFrame, kProfCelt, kProfCeltHead, kProfCeltBands, kProfCeltSynth, kProfCeltComb,
     kProfCeltDeemph, kProfSilk, kProfEmit, kProfCeltLost, kProfCeltEnergy, kProfCeltAlloc, kProfCeltTail,
-    kProfCount
+    kProfPvq, kProfCwrsi, kProfExpRot, kProfTheta, kProfLeafFold, kProfBandPre, kProfBandPost, kProfEcUint,
+    kProfAllocBisect, kProfAllocTail, kProfCount
 };
 
 // ---------------------------------------------------------------- basics

@@ -32,23 +32,41 @@ constexpr uint32_t kEcCodeTop = 1u << 31;
 constexpr uint32_t kEcCodeBot = kEcCodeTop >> 8;
 constexpr int kEcCodeExtra = 7;  // (32-2)%8+1
 
-CUO_FN int ec_read_byte(RangeDec *d) { return d->offs < d->storage ? d->buf[d->offs++] : 0; }
-CUO_FN int ec_read_byte_end(RangeDec *d)
+// Every function loads the fields it needs as wave-uniform values (scalar
+// registers), updates them and stores them back.
+CUO_FN const uint8_t *ec_buf(const RangeDec *d)
 {
-    return d->end_offs < d->storage ? d->buf[d->storage - ++(d->end_offs)] : 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (const uint8_t *)(uintptr_t)uniform_u64((uint64_t)(uintptr_t)d->buf);
+#else
+    return d->buf;
+#endif
 }
 
-CUO_FN void ec_normalize(RangeDec *d)
+// Renormalise after a symbol: rng and val are the updated values.
+CUO_FN void ec_normalize_rv(RangeDec *d, uint32_t rng, uint32_t val)
 {
-    while (d->rng <= kEcCodeBot) {
-        d->nbits_total += 8;
-        d->rng <<= 8;
-        int sym = d->rem;
-        d->rem = ec_read_byte(d);
-        sym = (sym << 8 | d->rem) >> (8 - kEcCodeExtra);
-        d->val = ((d->val << 8) + (255 & ~(uint32_t)sym)) & (kEcCodeTop - 1);
+    if (rng <= kEcCodeBot) {
+        int nbits = uniform(d->nbits_total);
+        int rem = uniform(d->rem);
+        uint32_t offs = uniform(d->offs), storage = uniform(d->storage);
+        const uint8_t *buf = ec_buf(d);
+        do {
+            nbits += 8;
+            rng <<= 8;
+            int sym = rem;
+            rem = offs < storage ? (int)ld_u8(buf, (int)offs++) : 0;
+            sym = (sym << 8 | rem) >> (8 - kEcCodeExtra);
+            val = ((val << 8) + (255 & ~(uint32_t)sym)) & (kEcCodeTop - 1);
+        } while (rng <= kEcCodeBot);
+        d->nbits_total = nbits;
+        d->rem = rem;
+        d->offs = offs;
     }
+    d->rng = rng;
+    d->val = val;
 }
+CUO_FN void ec_normalize(RangeDec *d) { ec_normalize_rv(d, uniform(d->rng), uniform(d->val)); }
 
 CUO_FN void ec_init(RangeDec *d, const uint8_t *buf, uint32_t storage)
 {
@@ -58,84 +76,93 @@ CUO_FN void ec_init(RangeDec *d, const uint8_t *buf, uint32_t storage)
     d->end_window = 0;
     d->nend_bits = 0;
     d->nbits_total = 32 + 1 - ((32 - kEcCodeExtra) / 8) * 8;
-    d->offs = 0;
-    d->rng = 1u << kEcCodeExtra;
-    d->rem = ec_read_byte(d);
-    d->val = d->rng - 1 - (d->rem >> (8 - kEcCodeExtra));
+    d->offs = storage > 0 ? 1 : 0;
+    int rem = storage > 0 ? (int)ld_u8(buf, 0) : 0;
+    d->rem = rem;
+    uint32_t rng = 1u << kEcCodeExtra;
     d->error = 0;
     d->ext = 0;
-    ec_normalize(d);
+    ec_normalize_rv(d, rng, rng - 1 - (uint32_t)(rem >> (8 - kEcCodeExtra)));
 }
 
 CUO_FN uint32_t ec_decode(RangeDec *d, uint32_t ft)
 {
-    d->ext = d->rng / ft;
-    uint32_t s = d->val / d->ext;
+    uint32_t ext = uniform(d->rng) / ft;
+    d->ext = ext;
+    uint32_t s = uniform(d->val) / ext;
     uint32_t t = s + 1;
     return ft - (t < ft ? t : ft);
 }
 
 CUO_FN uint32_t ec_decode_bin(RangeDec *d, int bits)
 {
-    d->ext = d->rng >> bits;
-    uint32_t s = d->val / d->ext;
+    uint32_t ext = uniform(d->rng) >> bits;
+    d->ext = ext;
+    uint32_t s = uniform(d->val) / ext;
     uint32_t t = s + 1, ft = 1u << bits;
     return ft - (t < ft ? t : ft);
 }
 
 CUO_FN void ec_update(RangeDec *d, uint32_t fl, uint32_t fh, uint32_t ft)
 {
-    uint32_t s = d->ext * (ft - fh);
-    d->val -= s;
-    d->rng = fl > 0 ? d->ext * (fh - fl) : d->rng - s;
-    ec_normalize(d);
+    uint32_t ext = uniform(d->ext), rng = uniform(d->rng), val = uniform(d->val);
+    uint32_t s = ext * (ft - fh);
+    val -= s;
+    rng = fl > 0 ? ext * (fh - fl) : rng - s;
+    ec_normalize_rv(d, rng, val);
 }
 
 CUO_FN int ec_bit_logp(RangeDec *d, int logp)
 {
-    uint32_t r = d->rng, v = d->val;
+    uint32_t r = uniform(d->rng), v = uniform(d->val);
     uint32_t s = r >> logp;
     int ret = v < s;
-    if (!ret) d->val = v - s;
-    d->rng = ret ? s : r - s;
-    ec_normalize(d);
+    if (!ret) v -= s;
+    ec_normalize_rv(d, ret ? s : r - s, v);
     return ret;
 }
 
 template <typename T>
+CUO_FN uint32_t icdf_at(const T *icdf, int i);
+template <>
+CUO_FN uint32_t icdf_at<uint8_t>(const uint8_t *icdf, int i) { return ld_u8(icdf, i); }
+
+template <typename T>
 CUO_FN int ec_icdf_t(RangeDec *d, const T *icdf, int ftb)
 {
-    uint32_t s = d->rng, v = d->val;
+    uint32_t s = uniform(d->rng), v = uniform(d->val);
     uint32_t r = s >> ftb;
     int ret = -1;
     uint32_t t;
     do {
         t = s;
-        s = r * (uint32_t)icdf[++ret];
+        s = r * icdf_at<T>(icdf, ++ret);
     } while (v < s);
-    d->val = v - s;
-    d->rng = t - s;
-    ec_normalize(d);
+    ec_normalize_rv(d, t - s, v - s);
     return ret;
 }
 CUO_FN int ec_icdf(RangeDec *d, const uint8_t *icdf, int ftb) { return ec_icdf_t<uint8_t>(d, icdf, ftb); }
 
 CUO_FN uint32_t ec_bits(RangeDec *d, int bits)
 {
-    uint32_t window = d->end_window;
-    int available = d->nend_bits;
+    uint32_t window = uniform(d->end_window);
+    int available = uniform(d->nend_bits);
     if (available < bits) {
+        uint32_t end_offs = uniform(d->end_offs), storage = uniform(d->storage);
+        const uint8_t *buf = ec_buf(d);
         do {
-            window |= (uint32_t)ec_read_byte_end(d) << available;
+            uint32_t b = end_offs < storage ? ld_u8(buf, (int)(storage - ++end_offs)) : 0;
+            window |= b << available;
             available += 8;
         } while (available <= 32 - 8);
+        d->end_offs = end_offs;
     }
     uint32_t ret = window & ((1u << bits) - 1u);
     window >>= bits;
     available -= bits;
     d->end_window = window;
     d->nend_bits = available;
-    d->nbits_total += bits;
+    d->nbits_total = uniform(d->nbits_total) + bits;
     return ret;
 }
 
@@ -159,7 +186,7 @@ CUO_FN uint32_t ec_uint(RangeDec *d, uint32_t ft)
     return s;
 }
 
-CUO_FN int ec_tell(const RangeDec *d) { return d->nbits_total - ilog(d->rng); }
+CUO_FN int ec_tell(const RangeDec *d) { return uniform(d->nbits_total) - ilog(uniform(d->rng)); }
 
 static constexpr uint32_t kTellCorrection[8] = {35733, 38967, 42495, 46340, 50535, 55109, 60097, 65535};
 
@@ -167,9 +194,10 @@ CUO_FN uint32_t ec_tell_frac(const RangeDec *d)
 {
     // 1/8-bit resolution via a linear estimate plus threshold table
     // (vendor/celt/entcode.c:69-86)
-    uint32_t nbits = (uint32_t)d->nbits_total << kBitRes;
-    int l = ilog(d->rng);
-    uint32_t r = d->rng >> (l - 16);
+    uint32_t nbits = (uint32_t)uniform(d->nbits_total) << kBitRes;
+    uint32_t rng = uniform(d->rng);
+    int l = ilog(rng);
+    uint32_t r = rng >> (l - 16);
     uint32_t b = (r >> 12) - 8;
     b += r > kTellCorrection[b];
     l = (l << 3) + (int)b;

@@ -380,7 +380,7 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
         celt_ret = celt_decode(&st->celt, decode_fec ? nullptr : data, len, pcm, celt_frame_size,
                                data != nullptr ? &dec : nullptr, celt_accum, &sc->celt);
     } else {
-        static constexpr uint8_t silence[2] = {0xFF, 0xFF};
+        alignas(4) static constexpr uint8_t silence[4] = {0xFF, 0xFF, 0, 0};  // a 2-byte frame
         if (!celt_accum)
             for (int i = 0; i < frame_size; i++) pcm[i] = 0;
         if (st->prev_mode == MODE_HYBRID && !(redundancy && celt_to_silk && st->prev_redundancy)) {

@@ -31,7 +31,7 @@ static FILE *o;
 
 #define EMIT(ctype, name, arr, n)                                            \
     do {                                                                     \
-        fprintf(o, "static constexpr %s %s[%d] = {", ctype, name, (int)(n)); \
+        fprintf(o, "alignas(4) static constexpr %s %s[%d] = {", ctype, name, (int)(n)); \
         for (int i_ = 0; i_ < (int)(n); i_++)                                \
             fprintf(o, "%s%s%ld", i_ ? "," : "", (i_ % 16) ? "" : "\n  ",    \
                     (long)(arr)[i_]);                                        \
@@ -40,7 +40,7 @@ static FILE *o;
 
 #define EMITU(ctype, name, arr, n)                                           \
     do {                                                                     \
-        fprintf(o, "static constexpr %s %s[%d] = {", ctype, name, (int)(n)); \
+        fprintf(o, "alignas(4) static constexpr %s %s[%d] = {", ctype, name, (int)(n)); \
         for (int i_ = 0; i_ < (int)(n); i_++)                                \
             fprintf(o, "%s%s%luu", i_ ? "," : "", (i_ % 12) ? "" : "\n  ",   \
                     (unsigned long)(arr)[i_]);                               \

@@ -274,7 +274,8 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     const uint32_t nch = (uint32_t)pl.chains.size();
     const uint64_t npk = pl.off.size();
     int rc;
-    if ((rc = grow(W.data, pl.data.size(), e))) return rc;
+    // +4: the decoder reads packet bytes as aligned 32-bit words
+    if ((rc = grow(W.data, pl.data.size() + 4, e))) return rc;
     if ((rc = grow(W.off, 4 * npk, e))) return rc;
     if ((rc = grow(W.len, 4 * npk, e))) return rc;
     if ((rc = grow(W.chains, sizeof(ChainDesc) * nch, e))) return rc;
@@ -837,7 +838,7 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
         b->chain_pkts += c.npkts;
         for (uint32_t k = 0; k < c.npkts; k++) b->chain_bytes += pl.len[c.first + k];
     }
-    if ((rc = grow(b->data, pl.data.size(), e)) || (rc = grow(b->off, 4 * npk, e)) ||
+    if ((rc = grow(b->data, pl.data.size() + 4, e)) || (rc = grow(b->off, 4 * npk, e)) ||
         (rc = grow(b->len, 4 * npk, e)) || (rc = grow(b->chains, sizeof(ChainDesc) * nch, e)) ||
         (rc = grow(b->order, 4ull * nch, e)) || (rc = grow(b->res, sizeof(ChainResult) * nch, e)) ||
         (rc = grow(b->cat, 2 * (pl.total_keep ? pl.total_keep : 1), e)) || (rc = grow(b->queue, 256, e)) ||

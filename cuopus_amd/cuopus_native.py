@@ -273,7 +273,8 @@ def release_workspaces():
 
 
 PROF_SCOPES = ["chain", "frame", "celt", "celt_head", "celt_bands", "celt_synth", "celt_comb", "celt_deemph",
-               "silk", "emit", "celt_lost", "celt_energy", "celt_alloc", "celt_tail"]
+               "silk", "emit", "celt_lost", "celt_energy", "celt_alloc", "celt_tail", "pvq", "cwrsi", "exp_rot",
+               "theta", "leaf_fold", "band_pre", "band_post", "ec_uint", "alloc_bisect", "alloc_tail"]
 
 
 def prof_read():
