@@ -76,6 +76,8 @@ struct CeltScratch {
     uint8_t collapse[kNbBands];
     int32_t etmp[kOverlap];
     RangeDec dec;  // decoder of a frame opened here (redundancy frames)
+    // explicit stack of the band splitter (quant_partition), 6 frames
+    alignas(8) unsigned char qp_stack[6 * 96];
 };
 
 // ------------------------------------------------------------------ reset
@@ -119,11 +121,13 @@ CUO_FN int32_t inner_prod16(const int16_t *x, const int16_t *y, int n)
 
 CUO_FN void renormalise_vector(int16_t *X, int N, int16_t gain)
 {
-    int32_t E = 1 + inner_prod16(X, X, N);
+    int32_t part = 0;
+    CUO_PFOR(i, 0, N) part = mac16_16(part, X[i], X[i]);
+    int32_t E = 1 + wave_sum(part);
     int k = celt_ilog2(E) >> 1;
     int32_t t = vshr32(E, 2 * (k - 7));
     int16_t g = (int16_t)mul16_16_p15(celt_rsqrt_norm(t), gain);
-    for (int i = 0; i < N; i++) X[i] = (int16_t)pshr32(mul16_16(g, X[i]), k + 1);
+    CUO_PFOR(i, 0, N) X[i] = (int16_t)pshr32(mul16_16(g, X[i]), k + 1);
 }
 
 CUO_FN int16_t bitexact_cos(int16_t x)
@@ -636,6 +640,7 @@ CUO_FN void interleave_hadamard(int16_t *X, int N0, int stride, int hadamard, in
 }
 
 struct BandCtx {
+    unsigned char *qp_stack;
     RangeDec *ec;
     int band;
     int spread;
@@ -716,11 +721,12 @@ struct QpFrame {
     int stage, mfirst, mbits, sbits, itheta, rebalance;
     unsigned cm;
 };
+static_assert(sizeof(QpFrame) <= 96, "QpFrame exceeds its stack slot");
 
 CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B0, int16_t *lowband0,
                                 int LM0, int16_t gain0, int fill0)
 {
-    QpFrame stk[6];
+    QpFrame *stk = lds_uniform((QpFrame *)ctx->qp_stack);
     int sp = 0;
     unsigned ret = 0;
     {
@@ -781,30 +787,33 @@ CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B
                 ctx->remaining_bits -= curr_bits;
             }
             unsigned cm = 0;
+            int16_t *fX = lds_uniform(f.X);
+            int16_t *flow = f.lowband ? lds_uniform(f.lowband) : nullptr;
             if (q != 0) {
-                cm = alg_unquant(f.X, f.N, get_pulses(q), ctx->spread, f.B, ctx->ec, f.gain, ctx->iy);
+                cm = alg_unquant(fX, f.N, get_pulses(q), ctx->spread, f.B, ctx->ec, f.gain, ctx->iy);
             } else {
                 CUO_PROF_SCOPE(kProfLeafFold);
                 unsigned cm_mask = (unsigned)(1UL << f.B) - 1;
                 int fill = f.fill & (int)cm_mask;
+                const int n = f.N;
                 if (!fill) {
-                    for (int j = 0; j < f.N; j++) f.X[j] = 0;
+                    CUO_PFOR(j, 0, n) fX[j] = 0;
                 } else {
-                    if (f.lowband == nullptr) {
-                        for (int j = 0; j < f.N; j++) {
-                            ctx->seed = lcg_rand(ctx->seed);
-                            f.X[j] = (int16_t)((int32_t)ctx->seed >> 20);
-                        }
+                    // LCG noise: lane j draws seed_{j+1} = A^(j+1) seed + C_(j+1)
+                    // (jump-ahead of the serial recurrence, exact mod 2^32)
+                    const uint32_t seed0 = ctx->seed;
+                    if (flow == nullptr) {
+                        CUO_PFOR(j, 0, n) fX[j] = (int16_t)((int32_t)lcg_jump(seed0, j + 1) >> 20);
                         cm = cm_mask;
                     } else {
-                        for (int j = 0; j < f.N; j++) {
-                            ctx->seed = lcg_rand(ctx->seed);
-                            int16_t tmp = (ctx->seed & 0x8000) ? 4 : -4;  // QCONST16(1/256, 10)
-                            f.X[j] = (int16_t)(f.lowband[j] + tmp);
+                        CUO_PFOR(j, 0, n) {
+                            int16_t tmp = (lcg_jump(seed0, j + 1) & 0x8000) ? 4 : -4;  // QCONST16(1/256, 10)
+                            fX[j] = (int16_t)(flow[j] + tmp);
                         }
                         cm = (unsigned)fill;
                     }
-                    renormalise_vector(f.X, f.N, f.gain);
+                    ctx->seed = lcg_jump(seed0, n);
+                    renormalise_vector(fX, n, f.gain);
                 }
             }
             ret = cm;
@@ -928,7 +937,7 @@ CUO_FN void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, 
     int16_t *lowband_scratch = X_ + M * eB[kNbBands - 1];
     int lowband_offset = 0, update_lowband = 1;
     BandCtx ctx;
-    ctx.ec = ec; ctx.spread = spread; ctx.seed = *seed; ctx.iy = sc->iy;
+    ctx.ec = ec; ctx.spread = spread; ctx.seed = *seed; ctx.iy = sc->iy; ctx.qp_stack = sc->qp_stack;
     for (int i = start; i < end; i++) {
         int effective_lowband = -1;
         ctx.band = i;

@@ -360,6 +360,32 @@ CUO_FN int16_t celt_log2(int32_t x)
 
 CUO_FN uint32_t lcg_rand(uint32_t seed) { return 1664525u * seed + 1013904223u; }
 
+// n steps of lcg_rand at once: lcg_rand^n(s) = A_n s + C_n (mod 2^32) with
+// A_n = A_{n-1} a, C_n = C_{n-1} a + c.  Lets lanes draw a band's noise in
+// parallel with the exact serial sequence.
+constexpr int kLcgJumpMax = 256;
+struct LcgJumpTable {
+    uint32_t a[kLcgJumpMax + 1], c[kLcgJumpMax + 1];
+};
+constexpr LcgJumpTable make_lcg_jump_table()
+{
+    LcgJumpTable t{};
+    t.a[0] = 1u;
+    t.c[0] = 0u;
+    for (int n = 1; n <= kLcgJumpMax; n++) {
+        t.a[n] = t.a[n - 1] * 1664525u;
+        t.c[n] = t.c[n - 1] * 1664525u + 1013904223u;
+    }
+    return t;
+}
+alignas(16) static constexpr LcgJumpTable kLcgJump = make_lcg_jump_table();
+CUO_FN uint32_t lcg_jump(uint32_t seed, int n)
+{
+    if (n <= kLcgJumpMax) return kLcgJump.a[n] * seed + kLcgJump.c[n];
+    for (int i = 0; i < n; i++) seed = lcg_rand(seed);
+    return seed;
+}
+
 // ---------------------------------------------------------------- SILK layer
 CUO_FN int32_t silk_smulwb(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)(int16_t)b) >> 16); }
 CUO_FN int32_t silk_smlawb(int32_t a, int32_t b, int32_t c) { return (int32_t)(a + (((int64_t)b * (int64_t)(int16_t)c) >> 16)); }

@@ -127,6 +127,7 @@ CUO_FN uint32_t icdf_at(const T *icdf, int i);
 template <>
 CUO_FN uint32_t icdf_at<uint8_t>(const uint8_t *icdf, int i) { return ld_u8(icdf, i); }
 
+// icdf must point into a constant table (it is read with scalar loads).
 template <typename T>
 CUO_FN int ec_icdf_t(RangeDec *d, const T *icdf, int ftb)
 {
@@ -142,6 +143,20 @@ CUO_FN int ec_icdf_t(RangeDec *d, const T *icdf, int ftb)
     return ret;
 }
 CUO_FN int ec_icdf(RangeDec *d, const uint8_t *icdf, int ftb) { return ec_icdf_t<uint8_t>(d, icdf, ftb); }
+
+// ec_icdf with the two-entry table {icdf0, 0}: symbol 0 takes [r*icdf0, rng),
+// symbol 1 takes [0, r*icdf0).
+CUO_FN int ec_icdf2(RangeDec *d, uint32_t icdf0, int ftb)
+{
+    uint32_t rng = uniform(d->rng), v = uniform(d->val);
+    uint32_t s = (rng >> ftb) * icdf0;
+    if (v >= s) {
+        ec_normalize_rv(d, rng - s, v - s);
+        return 0;
+    }
+    ec_normalize_rv(d, s, v);
+    return 1;
+}
 
 CUO_FN uint32_t ec_bits(RangeDec *d, int bits)
 {

@@ -51,6 +51,7 @@ struct DecScratch {
     int16_t pcm_transition[240];
     int16_t redundant_audio[240];
     RangeDec dec;  // the frame's range decoder (shared by SILK and CELT)
+    int16_t frame_sizes[48];  // packet_parse output
     // One frame of output: a frame decode writes at most st->frame_size
     // <= 2880 samples (60 ms), data or concealment.
     int16_t frame_out[kMaxFrameOut];
@@ -449,7 +450,7 @@ CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, Ou
     int packet_bandwidth = toc_bandwidth(data[0]);
     int packet_frame_size = toc_samples_per_frame(data[0]);
     int packet_stream_channels = (data[0] & 0x4) ? 2 : 1;
-    int16_t size[48];
+    int16_t *size = lds_uniform(sc->frame_sizes);
     int offset;
     int count = packet_parse(data, len, size, &offset);
     if (count < 0) return count;

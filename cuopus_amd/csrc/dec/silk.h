@@ -759,17 +759,16 @@ CUO_FN void silk_decode_pulses(RangeDec *d, int16_t *pulses, int signalType, int
         }
     }
     // signs (code_signs.c:75-115)
-    uint8_t icdf[2];
-    icdf[1] = 0;
+    // each sign is one symbol of the two-entry icdf {icdf0, 0}
     int16_t *q = pulses;
     const uint8_t *icdf_ptr = &tab::kSignIcdf[silk_smulbb(7, quantOffsetType + (signalType << 1))];
     int length = (frame_length + 16 / 2) >> 4;
     for (int i = 0; i < length; i++) {
         int p = sum_pulses[i];
         if (p > 0) {
-            icdf[0] = icdf_ptr[imin(p & 0x1F, 6)];
+            uint32_t icdf0 = ld_u8(icdf_ptr, imin(p & 0x1F, 6));
             for (int j = 0; j < 16; j++)
-                if (q[j] > 0) q[j] = (int16_t)(q[j] * (silk_lshift(ec_icdf(d, icdf, 8), 1) - 1));
+                if (q[j] > 0) q[j] = (int16_t)(q[j] * (silk_lshift(ec_icdf2(d, icdf0, 8), 1) - 1));
         }
         q += 16;
     }

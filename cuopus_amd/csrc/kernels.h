@@ -84,7 +84,7 @@ struct ChainOut {
 };
 
 // Decode one chain on the calling wavefront.
-__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int lane)
+__device__ __forceinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int lane)
 {
     CUO_PROF_SCOPE(kProfChain);
     const ChainDesc ch = a.chains[c];
