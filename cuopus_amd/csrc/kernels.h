@@ -84,7 +84,9 @@ struct ChainOut {
 };
 
 // Decode one chain on the calling wavefront.
-__device__ __forceinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int lane)
+// Kept out of line: the persistent loop around it then stays a small loop
+// whose every iteration runs with all 64 lanes together.
+__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int lane)
 {
     CUO_PROF_SCOPE(kProfChain);
     const ChainDesc ch = a.chains[c];
@@ -148,9 +150,13 @@ __global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
     WaveState *ws = (WaveState *)g_wave_lds;
     const int lane = (int)threadIdx.x;
     while (true) {
+        // The fetched index is made wave-uniform with readfirstlane (not a
+        // lane shuffle) so the loop exit is a scalar branch: with a shuffle the
+        // compiler treats the exit as divergent and may let lanes run different
+        // iterations, which a wavefront-per-chain decoder cannot allow.
         uint32_t w = 0;
         if (lane == 0) w = atomicAdd(a.queue, 1u);
-        w = __shfl(w, 0, 64);
+        w = (uint32_t)__builtin_amdgcn_readfirstlane((int)w);
         if (w >= a.n_chains) break;
         const uint32_t c = a.order ? a.order[w] : w;
         decode_one_chain(a, c, ws, lane);

@@ -85,3 +85,44 @@ extern "C" int hg_sizes(int *out)
     out[3] = (int)sizeof(SilkDecoder);
     return 4;
 }
+
+// Several chains decoded one after another in the same state and scratch
+// memory (no re-initialisation of the scratch between chains), the way a
+// persistent wavefront reuses its LDS block.  pcm receives each chain's full
+// output back to back; ret[c] = samples or the negative error.
+extern "C" void hg_decode_chains_reuse(const uint8_t *data, const uint32_t *off, const uint32_t *len,
+                                       const uint8_t *lost, const uint32_t *first, const uint32_t *npkts,
+                                       int nch, int16_t *pcm, long cap, long *ret)
+{
+    OpusDecState *st = (OpusDecState *)malloc(sizeof(OpusDecState));
+    DecScratch *sc = (DecScratch *)malloc(sizeof(DecScratch));
+    memset(sc, 0xA5, sizeof(DecScratch));
+    memset(st, 0x5A, sizeof(OpusDecState));
+    long pos = 0;
+    for (int c = 0; c < nch; c++) {
+        opus_dec_init(st);
+        HostOut out{sc->frame_out, pcm + pos, 0};
+        long r = 0;
+        for (uint32_t k = 0; k < npkts[c]; k++) {
+            uint32_t p = first[c] + k;
+            const uint8_t *pk = data + off[p];
+            int n;
+            if (cap - pos - out.pos < 5760) { r = OPUS_BUFFER_TOO_SMALL; break; }
+            if (lost && lost[p]) {
+                int dur = packet_nb_samples(pk, (int)len[p]);
+                if (dur <= 0) { r = dur ? dur : OPUS_INVALID_PACKET; break; }
+                if (k + 1 < npkts[c] && !lost[p + 1] && packet_has_lbrr(data + off[p + 1], (int)len[p + 1]) > 0)
+                    n = opus_decode_packet(st, data + off[p + 1], (int)len[p + 1], out, dur, 1, sc);
+                else
+                    n = opus_decode_packet(st, nullptr, 0, out, dur, 0, sc);
+            } else {
+                n = opus_decode_packet(st, pk, (int)len[p], out, 5760, 0, sc);
+            }
+            if (n < 0) { r = n; break; }
+        }
+        ret[c] = r < 0 ? r : out.pos;
+        if (r >= 0) pos += out.pos;
+    }
+    free(st);
+    free(sc);
+}
