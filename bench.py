@@ -48,6 +48,13 @@ WORKLOADS = {
         "desc": "CELT-only FB mono 64 kbps VBR, frame size per 1 s from {2.5,5,10,20} ms, one 2 h file, "
                 "1 s chains + 250 ms preroll, one launch, f32 16 kHz out",
     },
+    "uneven_250": {
+        "glob": "corpus/bench/uneven_250/f*.opus",
+        "desc": "250 mono speech files, durations log-uniform 5 s-30 min (18.1 h), mode per file from "
+                "SILK-NB/WB, hybrid-SWB/FB, CELT-FB, bitrate redrawn every 1 s in 12-64 kbps (mode "
+                "switches, redundancy), 20 ms frames; all chains in one persistent kernel vs one launch "
+                "per file",
+    },
 }
 
 
@@ -63,6 +70,8 @@ def _args():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-file comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-audio-s", type=float, default=7200.0,
+                    help="playable audio-seconds of the workload the CPU baseline decodes")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -103,35 +112,52 @@ def _time_batch(b, steps, warmup, torch):
     return wall, dec_ms, rs_ms
 
 
-def _cpu_baseline(blobs, threads):
-    """libopus on the host's cores over the identical chain list."""
+def _cpu_baseline(blobs, threads, max_audio_s=7200.0):
+    """libopus on the host's cores over the identical chain list of the first
+    files of the workload (up to max_audio_s of playable audio): all their
+    chains in one thread pool (fresh decoder per chain), then the float32
+    resampler per file on the same threads."""
     import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
     from tests import oracle_lib as ol
-    play = 0
-    t_dec = t_rs = 0.0
-    nch = 0
+    datas, offs, lens, chains, keeps = [], [], [], [], []
+    base_b = base_p = 0
+    audio = 0.0
+    nfiles = 0
     for blob in blobs:
-        d, chains = ol.demux_chains(blob)
+        d, ch = ol.demux_chains(blob)
         data, off, ln = d.blob()
-        t0 = time.perf_counter()
-        ref = ol.decode_chains(data, off, ln, chains, threads=threads)
-        t1 = time.perf_counter()
-        out = ol.resample(ref["pcm"])
-        t2 = time.perf_counter()
-        assert (ref["ret"] >= 0).all()
-        play += int(ref["pcm"].size)
-        nch += len(chains)
-        t_dec += t1 - t0
-        t_rs += t2 - t1
-        del out
-    secs = play / 48000.0
+        datas.append(data)
+        offs.append(off.astype(np.uint64) + base_b)
+        lens.append(ln)
+        chains += [(f + base_p, n, t, k) for (f, n, t, k) in ch]
+        keeps.append(sum(c[3] for c in ch))
+        base_b += data.size
+        base_p += off.size
+        audio += keeps[-1] / 48000.0
+        nfiles += 1
+        if audio >= max_audio_s:
+            break
+    data = np.concatenate(datas)
+    off = np.concatenate(offs).astype(np.uint32)
+    ln = np.concatenate(lens)
+    t0 = time.perf_counter()
+    ref = ol.decode_chains(data, off, ln, chains, threads=threads)
+    t1 = time.perf_counter()
+    assert (ref["ret"] >= 0).all()
+    pcm = ref["pcm"]
+    bounds = np.cumsum([0] + keeps)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda i: ol.resample(pcm[bounds[i]:bounds[i + 1]]), range(nfiles)))
+    t2 = time.perf_counter()
+    secs = pcm.size / 48000.0
     return {
-        "value": round(secs / (t_dec + t_rs), 1),
+        "value": round(secs / (t2 - t0), 1),
         "unit": "audio-s/s",
         "cores": threads,
         "kind": "reference",
-        "sample": "whole workload (%d chains, %.0f audio-s): fixed-point libopus 1.5.2 decode on %d threads "
-                  "(%.2f s) + float32 resampler on 1 thread (%.2f s)" % (nch, secs, threads, t_dec, t_rs),
+        "sample": "%d files, %d chains, %.0f audio-s: fixed-point libopus 1.5.2 decode (%.2f s) + float32 "
+                  "resampler (%.2f s), %d threads" % (nfiles, len(chains), secs, t1 - t0, t2 - t1, threads),
     }
 
 
@@ -142,7 +168,13 @@ def main():
               "by file or by contiguous chain range with a 24-sample resampler halo", file=sys.stderr)
         sys.exit(2)
     w = WORKLOADS[a.workload]
-    paths = [os.path.join(ROOT, p) for p in w["files"]]
+    if "glob" in w:
+        import glob
+        paths = sorted(glob.glob(os.path.join(ROOT, w["glob"])))
+        if not paths:
+            sys.exit("workload input %s missing: generate it with tests.oracle_lib.bench_set()" % w["glob"])
+    else:
+        paths = [os.path.join(ROOT, p) for p in w["files"]]
     for p in paths:
         if not os.path.exists(p):
             sys.exit("workload input %s missing: generate it with __graft_entry__.build()" % p)
@@ -201,7 +233,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         th = a.cpu_threads or min(BOX_CPU_SHARE, os.cpu_count() or 1)
-        cpu = _cpu_baseline(blobs, th)
+        cpu = _cpu_baseline(blobs, th, a.cpu_audio_s)
 
     line = {
         "metric": "audio-sec decoded/s on 1 MI355X, bit-exact: batched vs plain; serve p99 ms",
@@ -219,6 +251,7 @@ def main():
         "config": {"workload": a.workload, "desc": w["desc"], "mode": a.mode, "graph": a.graph,
                    "files": len(blobs), "audio_s_per_step": round(audio_s, 3)},
         "plain_value": plain,
+        "batched_vs_plain": round(value / plain, 3) if plain else None,
         "roofline": roof,
         "resampler_roofline": rs_roof,
         "cpu_baseline": cpu,

@@ -735,6 +735,7 @@ CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B
         f.fill = fill0; f.stage = 0;
     }
     while (sp > 0) {
+        CUO_PROF_SCOPE(kProfQpStage);
         QpFrame &f = stk[sp - 1];
         if (f.stage == 0) {
             const uint8_t *cache = pulse_cache(ctx->band, f.LM + 1);
@@ -777,14 +778,18 @@ CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B
                 continue;
             }
             // leaf: PVQ or folding
-            int q = bits2pulses(ctx->band, f.LM, f.b);
-            int curr_bits = pulses2bits(ctx->band, f.LM, q);
-            ctx->remaining_bits -= curr_bits;
-            while (ctx->remaining_bits < 0 && q > 0) {
-                ctx->remaining_bits += curr_bits;
-                q--;
-                curr_bits = pulses2bits(ctx->band, f.LM, q);
+            int q;
+            {
+                CUO_PROF_SCOPE(kProfQpLeafBits);
+                q = bits2pulses(ctx->band, f.LM, f.b);
+                int curr_bits = pulses2bits(ctx->band, f.LM, q);
                 ctx->remaining_bits -= curr_bits;
+                while (ctx->remaining_bits < 0 && q > 0) {
+                    ctx->remaining_bits += curr_bits;
+                    q--;
+                    curr_bits = pulses2bits(ctx->band, f.LM, q);
+                    ctx->remaining_bits -= curr_bits;
+                }
             }
             unsigned cm = 0;
             int16_t *fX = lds_uniform(f.X);
@@ -916,8 +921,9 @@ CUO_FN unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_
     }
     B <<= recombine;
     if (lowband_out) {
+        CUO_PROF_SCOPE(kProfLowbandOut);
         int16_t n = (int16_t)celt_sqrt(shl32(N0, 22));
-        for (int j = 0; j < N0; j++) lowband_out[j] = (int16_t)mul16_16_q15(n, X[j]);
+        CUO_PFOR(j, 0, N0) lowband_out[j] = (int16_t)mul16_16_q15(n, X[j]);
     }
     cm &= (1u << B) - 1;
     return cm;
@@ -942,6 +948,7 @@ CUO_FN void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, 
         int effective_lowband = -1;
         ctx.band = i;
         int last = i == end - 1;
+        CUO_PROF_SCOPE(kProfBandsPrologue);
         int16_t *X = X_ + M * eB[i];
         int N = M * eB[i + 1] - M * eB[i];
         int32_t tell = (int32_t)ec_tell_frac(ec);

@@ -158,6 +158,9 @@ __host__ __device__ inline uint32_t wave_or(uint32_t v) { return v; }
 __device__ unsigned long long g_prof[32];
 #endif
 #if defined(CUO_PROF) && defined(__HIP_DEVICE_COMPILE__)
+// Per-wavefront sums in LDS (lane 0 adds; one wavefront per workgroup), added
+// to g_prof once per chain by prof_flush().
+__shared__ unsigned long long g_prof_lds[32];
 struct ProfScope {
     int id;
     unsigned long long t0;
@@ -165,20 +168,31 @@ struct ProfScope {
     __device__ ~ProfScope()
     {
         unsigned long long dt = __builtin_readcyclecounter() - t0;
-        if (lane_id() == 0) atomicAdd(&g_prof[id], dt);
+        if (lane_id() == 0) g_prof_lds[id] += dt;
     }
 };
+__device__ inline void prof_reset()
+{
+    if (lane_id() < 32) g_prof_lds[lane_id()] = 0;
+}
+__device__ inline void prof_flush()
+{
+    if (lane_id() < 32 && g_prof_lds[lane_id()]) atomicAdd(&g_prof[lane_id()], g_prof_lds[lane_id()]);
+}
 #define CUO_PROF_CAT2(a, b) a##b
 #define CUO_PROF_CAT(a, b) CUO_PROF_CAT2(a, b)
 #define CUO_PROF_SCOPE(id) ProfScope CUO_PROF_CAT(prof_scope_, __LINE__)(id)
 #else
 #define CUO_PROF_SCOPE(id)
+#define prof_reset()
+#define prof_flush()
 #endif
 enum {
     kProfChain = 0, kProfFrame, kProfCelt, kProfCeltHead, kProfCeltBands, kProfCeltSynth, kProfCeltComb,
     kProfCeltDeemph, kProfSilk, kProfEmit, kProfCeltLost, kProfCeltEnergy, kProfCeltAlloc, kProfCeltTail,
     kProfPvq, kProfCwrsi, kProfExpRot, kProfTheta, kProfLeafFold, kProfBandPre, kProfBandPost, kProfEcUint,
-    kProfAllocBisect, kProfAllocTail, kProfCount
+    kProfAllocBisect, kProfAllocTail, kProfQpLeafBits, kProfBandsPrologue, kProfLowbandOut, kProfQpStage,
+    kProfCount
 };
 
 // ---------------------------------------------------------------- basics

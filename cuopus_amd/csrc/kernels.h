@@ -88,6 +88,8 @@ struct ChainOut {
 // whose every iteration runs with all 64 lanes together.
 __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int lane)
 {
+    prof_reset();
+    {
     CUO_PROF_SCOPE(kProfChain);
     const ChainDesc ch = a.chains[c];
     OpusDecState *st = lds_uniform(&ws->st);
@@ -132,6 +134,8 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
         a.res[c].stop = stop;
         a.res[c].samples = (uint32_t)out.pos;
     }
+    }
+    prof_flush();
 }
 
 // Plain variant: workgroup g decodes chain order[g].

@@ -160,3 +160,61 @@ def bench_corpus(name):
         subprocess.run([MKCORPUS, "--out", tmp] + shlex.split(BENCH_RECIPES[name]), check=True)
         os.replace(tmp, path)
     return path
+
+
+# Multi-file benchmark sets: many streams of uneven length (log-uniform
+# duration), each with a mode drawn per file; bitrate redrawn every second in
+# [12, 64] kbps and the mode not forced, so the encoder switches SILK / hybrid /
+# CELT and emits redundancy at the switches.  20 ms frames, speech-like signal.
+BENCH_SET_MODES = {
+    "silk_nb": "--app voip --bw nb",
+    "silk_wb": "--app voip --bw wb",
+    "hybrid_swb": "--app voip --bw swb",
+    "hybrid_fb": "--app voip --bw fb",
+    "celt_fb": "--app lowdelay --bw fb",
+}
+BENCH_SETS = {
+    # 250 files, durations exp(U(ln 5, ln 1800)) s
+    "uneven_250": dict(n=250, seed=0x0C0FFEE3, dmin=5.0, dmax=1800.0),
+}
+
+
+def bench_set_plan(name):
+    cfg = BENCH_SETS[name]
+    plan = []
+    modes = sorted(BENCH_SET_MODES)
+    for i in range(cfg["n"]):
+        rs = np.random.default_rng(cfg["seed"] + i)
+        dur = float(np.exp(rs.uniform(np.log(cfg["dmin"]), np.log(cfg["dmax"]))))
+        mode = modes[int(rs.integers(0, len(modes)))]
+        args = "--seed %d --dur %.3f --signal speech %s --brange 12000 64000 --frame 960" % (
+            cfg["seed"] + i, dur, BENCH_SET_MODES[mode])
+        plan.append(("f%04d" % i, round(dur, 3), mode, args))
+    return plan
+
+
+def bench_set(name, jobs=8):
+    """Generate (once) a multi-file benchmark set under corpus/bench/<name>/
+    with a manifest.json; returns the list of file paths."""
+    import json
+    from concurrent.futures import ThreadPoolExecutor
+    d = os.path.join(BENCH, name)
+    os.makedirs(d, exist_ok=True)
+    plan = bench_set_plan(name)
+
+    def one(item):
+        fname, _, _, args = item
+        path = os.path.join(d, fname + ".opus")
+        if not os.path.exists(path):
+            tmp = path + ".tmp%d" % os.getpid()
+            subprocess.run([MKCORPUS, "--out", tmp] + shlex.split(args), check=True)
+            os.replace(tmp, path)
+        return path
+
+    # longest first so the pool finishes together
+    order = sorted(plan, key=lambda it: -it[1])
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(one, order))
+    with open(os.path.join(d, "manifest.json"), "w") as f:
+        json.dump([{"file": p[0] + ".opus", "dur": p[1], "mode": p[2], "args": p[3]} for p in plan], f, indent=0)
+    return [os.path.join(d, p[0] + ".opus") for p in plan]
