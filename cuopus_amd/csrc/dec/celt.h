@@ -103,6 +103,27 @@ CUO_FN void celt_init(CeltState *st)
     celt_reset(st);
 }
 
+// ------------------------------------------------------------------ 32-bit table copies
+// Small tables read with wave-uniform indices in serial code, widened to
+// 32-bit entries so each read is one scalar load.
+template <size_t N>
+struct I32Table {
+    int32_t v[N];
+};
+template <class T, size_t N>
+constexpr I32Table<N> widen(const T (&a)[N])
+{
+    I32Table<N> t{};
+    for (size_t i = 0; i < N; i++) t.v[i] = (int32_t)a[i];
+    return t;
+}
+alignas(16) static constexpr I32Table<22> kEB32 = widen(tab::kEBands);
+alignas(16) static constexpr I32Table<21> kLogN32 = widen(tab::kLogN);
+alignas(16) static constexpr I32Table<168> kCacheCaps32 = widen(tab::kCacheCaps);
+alignas(16) static constexpr I32Table<25> kEMeans32 = widen(tab::kEMeans);
+alignas(16) static constexpr I32Table<336> kEProb32 = widen(tab::kEProbModel);
+alignas(16) static constexpr I32Table<32> kTfSelect32 = widen(tab::kTfSelect);
+
 // ------------------------------------------------------------------ small helpers
 // mem[0 .. kDecBuf - N + kOverlap) = mem[N ..): the decode history moves down
 // by one frame.  Lanes copy 64-element blocks in increasing order; with
@@ -202,12 +223,12 @@ CUO_FN int laplace_decode(RangeDec *d, uint32_t fs, int decay)
 
 CUO_FN void unquant_coarse_energy(int start, int end, int16_t *oldE, int intra, RangeDec *d, int LM)
 {
-    const uint8_t *prob = tab::kEProbModel + (LM * 2 + intra) * 42;
+    const int32_t *prob = kEProb32.v + (LM * 2 + intra) * 42;
     int32_t prev = 0;
     int16_t coef, beta;
     if (intra) { coef = 0; beta = tab::kBetaIntra; }
-    else { beta = tab::kBetaCoef[LM]; coef = tab::kPredCoef[LM]; }
-    int32_t budget = (int32_t)d->storage * 8;
+    else { beta = (int16_t)ld_s16(tab::kBetaCoef, LM); coef = (int16_t)ld_s16(tab::kPredCoef, LM); }
+    int32_t budget = (int32_t)uniform(d->storage) * 8;
     for (int i = start; i < end; i++) {
         int qi;
         int32_t tell = ec_tell(d);
@@ -274,7 +295,7 @@ CUO_FN void tf_decode(int start, int end, int isTransient, int32_t *tf_res, int 
         logp = isTransient ? 4 : 5;
     }
     int tf_select = 0;
-    const int8_t *tsel = tab::kTfSelect + 8 * LM;
+    const int32_t *tsel = kTfSelect32.v + 8 * LM;
     if (tf_select_rsv && tsel[4 * isTransient + 0 + tf_changed] != tsel[4 * isTransient + 2 + tf_changed])
         tf_select = ec_bit_logp(d, 1);
     for (int i = start; i < end; i++) tf_res[i] = tsel[4 * isTransient + 2 * tf_select + tf_res[i]];
@@ -285,7 +306,7 @@ CUO_FN int compute_allocation(int start, int end, const int32_t *offsets, const 
                               int32_t total, int32_t *balance_out, int32_t *bits, int32_t *ebits,
                               int32_t *fine_priority, int LM, RangeDec *d, CeltScratch *sc)
 {
-    const int16_t *eB = tab::kEBands;
+    const int32_t *eB = kEB32.v;
     int32_t *bits1 = sc->bits1, *bits2 = sc->bits2, *thresh = sc->thresh, *trim_offset = sc->trim_offset;
     const int C = 1;
     total = max32(total, 0);
@@ -416,7 +437,7 @@ CUO_FN int compute_allocation(int start, int end, const int32_t *offsets, const 
             excess = max32(bit - cap[j], 0);
             bits[j] = bit - excess;
             int den = C * N;
-            int NClogN = den * (tab::kLogN[j] + logM);
+            int NClogN = den * (kLogN32.v[j] + logM);
             int offset = (NClogN >> 1) - den * 21;
             if (N == 2) offset += den << kBitRes >> 2;
             if (bits[j] + offset < den * 2 << kBitRes) offset += NClogN >> 2;
@@ -523,20 +544,68 @@ CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
     return yy;
 }
 
+// exp_rotation1 (vq.c:74-95): a forward then a backward pass of 2x2
+// rotations between X[i] and X[i+stride].  Residue classes mod stride are
+// independent chains; stride 1 is one serial chain, run register-blocked.
 CUO_FN void exp_rotation1(int16_t *X, int len, int stride, int16_t c, int16_t s)
 {
-    int16_t ms = (int16_t)-s;
-    int16_t *Xp = X;
-    for (int i = 0; i < len - stride; i++) {
-        int16_t x1 = Xp[0], x2 = Xp[stride];
-        Xp[stride] = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
-        *Xp++ = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
+    c = uniform(c);
+    s = uniform(s);
+    len = uniform(len);
+    const int16_t ms = (int16_t)-s;
+    if (stride == 1) {
+        // forward: carry = X[i] as updated by step i-1
+        if (len >= 2) {
+            int32_t carry = uniform((int32_t)X[0]);
+            for (int i0 = 0; i0 < len - 1; i0 += kLanes) {
+                const int nb = uniform(imin(kLanes, len - 1 - i0));
+                const int l = lane_id();
+                int32_t v = l < nb ? X[i0 + l + 1] : 0;
+                int32_t o = 0;
+                for (int k = 0; k < nb; k++) {
+                    int32_t x1 = carry, x2 = lane_get(v, k);
+                    o = lane_set(o, k, (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15));
+                    carry = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
+                }
+                if (l < nb) X[i0 + l] = (int16_t)o;
+            }
+            X[len - 1] = (int16_t)carry;
+        }
+        // backward: carry = X[i+1] as updated by step i+1
+        if (len >= 3) {
+            int32_t carry = uniform((int32_t)X[len - 2]);
+            for (int ih = len - 3; ih >= 0; ih -= kLanes) {
+                const int nb = uniform(imin(kLanes, ih + 1));
+                const int l = lane_id();
+                int32_t v = l < nb ? X[ih - l] : 0;
+                int32_t o = 0;
+                for (int k = 0; k < nb; k++) {
+                    int32_t x1 = lane_get(v, k), x2 = carry;
+                    o = lane_set(o, k, (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15));
+                    carry = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
+                }
+                if (l < nb) X[ih - l + 1] = (int16_t)o;
+            }
+            X[0] = (int16_t)carry;
+        }
+        return;
     }
-    Xp = &X[len - 2 * stride - 1];
-    for (int i = len - 2 * stride - 1; i >= 0; i--) {
-        int16_t x1 = Xp[0], x2 = Xp[stride];
-        Xp[stride] = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
-        *Xp-- = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
+    CUO_PFOR(r, 0, stride) {
+        for (int i = r; i < len - stride; i += stride) {
+            int16_t x1 = X[i], x2 = X[i + stride];
+            X[i + stride] = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
+            X[i] = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
+        }
+    }
+    const int top = len - 2 * stride - 1;
+    CUO_PFOR(r, 0, stride) {
+        if (top >= r) {
+            for (int i = top - (top - r) % stride; i >= 0; i -= stride) {
+                int16_t x1 = X[i], x2 = X[i + stride];
+                X[i + stride] = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
+                X[i] = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
+            }
+        }
     }
 }
 
@@ -582,7 +651,7 @@ CUO_FN unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDe
     int k = celt_ilog2(Ryy) >> 1;
     int32_t t = vshr32(Ryy, 2 * (k - 7));
     int16_t g = (int16_t)mul16_16_p15(celt_rsqrt_norm(t), gain);
-    for (int i = 0; i < N; i++) X[i] = (int16_t)pshr32(mul16_16(g, iy[i]), k + 1);
+    CUO_PFOR(i, 0, N) X[i] = (int16_t)pshr32(mul16_16(g, iy[i]), k + 1);
     {
         CUO_PROF_SCOPE(kProfExpRot);
         exp_rotation(X, N, -1, B, K, spread);
@@ -591,12 +660,8 @@ CUO_FN unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDe
     if (B <= 1) return 1;
     int N0 = N / B;
     unsigned cm = 0;
-    for (int i = 0; i < B; i++) {
-        unsigned tmp = 0;
-        for (int j = 0; j < N0; j++) tmp |= (unsigned)iy[i * N0 + j];
-        cm |= (unsigned)(tmp != 0) << i;
-    }
-    return cm;
+    CUO_PFOR(j, 0, N) if (iy[j]) cm |= 1u << (j / N0);
+    return wave_or(cm);
 }
 
 CUO_FN void haar1(int16_t *X, int N0, int stride)
@@ -670,7 +735,7 @@ CUO_FN SplitOut compute_theta_mono(BandCtx *ctx, int N, int *b, int B, int B0, i
 {
     SplitOut o;
     int i = ctx->band;
-    int pulse_cap = tab::kLogN[i] + LM * (1 << kBitRes);
+    int pulse_cap = kLogN32.v[i] + LM * (1 << kBitRes);
     int offset = (pulse_cap >> 1) - 4;
     int qn = compute_qn(N, *b, offset, pulse_cap);
     int32_t tell = (int32_t)ec_tell_frac(ctx->ec);
@@ -723,9 +788,67 @@ struct QpFrame {
 };
 static_assert(sizeof(QpFrame) <= 96, "QpFrame exceeds its stack slot");
 
+// Leaf of the band splitter: PVQ decode of the pulses the bits buy, or
+// folding / noise when there are none (bands.c:1046-1105).
+CUO_FN unsigned qp_leaf(BandCtx *ctx, int16_t *fX, int N, int b, int B, int16_t *flow, int LM, int16_t gain,
+                        int fill_in)
+{
+    int q;
+    {
+        CUO_PROF_SCOPE(kProfQpLeafBits);
+        q = bits2pulses(ctx->band, LM, b);
+        int curr_bits = pulses2bits(ctx->band, LM, q);
+        ctx->remaining_bits -= curr_bits;
+        while (ctx->remaining_bits < 0 && q > 0) {
+            ctx->remaining_bits += curr_bits;
+            q--;
+            curr_bits = pulses2bits(ctx->band, LM, q);
+            ctx->remaining_bits -= curr_bits;
+        }
+    }
+    unsigned cm = 0;
+    if (q != 0) {
+        cm = alg_unquant(fX, N, get_pulses(q), ctx->spread, B, ctx->ec, gain, ctx->iy);
+    } else {
+        CUO_PROF_SCOPE(kProfLeafFold);
+        unsigned cm_mask = (unsigned)(1UL << B) - 1;
+        int fill = fill_in & (int)cm_mask;
+        const int n = N;
+        if (!fill) {
+            CUO_PFOR(j, 0, n) fX[j] = 0;
+        } else {
+            // LCG noise: lane j draws seed_{j+1} = A^(j+1) seed + C_(j+1)
+            // (jump-ahead of the serial recurrence, exact mod 2^32)
+            const uint32_t seed0 = ctx->seed;
+            if (flow == nullptr) {
+                CUO_PFOR(j, 0, n) fX[j] = (int16_t)((int32_t)lcg_jump(seed0, j + 1) >> 20);
+                cm = cm_mask;
+            } else {
+                CUO_PFOR(j, 0, n) {
+                    int16_t tmp = (lcg_jump(seed0, j + 1) & 0x8000) ? 4 : -4;  // QCONST16(1/256, 10)
+                    fX[j] = (int16_t)(flow[j] + tmp);
+                }
+                cm = (unsigned)fill;
+            }
+            ctx->seed = lcg_jump(seed0, n);
+            renormalise_vector(fX, n, gain);
+        }
+    }
+    return cm;
+}
+
+// Split test of quant_partition (bands.c:970-975).
+CUO_FN bool qp_splits(const BandCtx *ctx, int N, int b, int LM)
+{
+    const uint8_t *cache = pulse_cache(ctx->band, LM + 1);
+    return LM != -1 && b > (int)ld_u8(cache, (int)ld_u8(cache, 0)) + 12 && N > 2;
+}
+
 CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B0, int16_t *lowband0,
                                 int LM0, int16_t gain0, int fill0)
 {
+    // most partitions are leaves: decode those without the explicit stack
+    if (!qp_splits(ctx, N0, b0, LM0)) return qp_leaf(ctx, X0, N0, b0, B0, lowband0, LM0, gain0, fill0);
     QpFrame *stk = lds_uniform((QpFrame *)ctx->qp_stack);
     int sp = 0;
     unsigned ret = 0;
@@ -738,8 +861,7 @@ CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B
         CUO_PROF_SCOPE(kProfQpStage);
         QpFrame &f = stk[sp - 1];
         if (f.stage == 0) {
-            const uint8_t *cache = pulse_cache(ctx->band, f.LM + 1);
-            if (f.LM != -1 && f.b > cache[cache[0]] + 12 && f.N > 2) {
+            if (qp_splits(ctx, f.N, f.b, f.LM)) {
                 f.B0 = f.B;
                 f.N >>= 1;
                 f.Y = f.X + f.N;
@@ -777,50 +899,8 @@ CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B
                 }
                 continue;
             }
-            // leaf: PVQ or folding
-            int q;
-            {
-                CUO_PROF_SCOPE(kProfQpLeafBits);
-                q = bits2pulses(ctx->band, f.LM, f.b);
-                int curr_bits = pulses2bits(ctx->band, f.LM, q);
-                ctx->remaining_bits -= curr_bits;
-                while (ctx->remaining_bits < 0 && q > 0) {
-                    ctx->remaining_bits += curr_bits;
-                    q--;
-                    curr_bits = pulses2bits(ctx->band, f.LM, q);
-                    ctx->remaining_bits -= curr_bits;
-                }
-            }
-            unsigned cm = 0;
-            int16_t *fX = lds_uniform(f.X);
-            int16_t *flow = f.lowband ? lds_uniform(f.lowband) : nullptr;
-            if (q != 0) {
-                cm = alg_unquant(fX, f.N, get_pulses(q), ctx->spread, f.B, ctx->ec, f.gain, ctx->iy);
-            } else {
-                CUO_PROF_SCOPE(kProfLeafFold);
-                unsigned cm_mask = (unsigned)(1UL << f.B) - 1;
-                int fill = f.fill & (int)cm_mask;
-                const int n = f.N;
-                if (!fill) {
-                    CUO_PFOR(j, 0, n) fX[j] = 0;
-                } else {
-                    // LCG noise: lane j draws seed_{j+1} = A^(j+1) seed + C_(j+1)
-                    // (jump-ahead of the serial recurrence, exact mod 2^32)
-                    const uint32_t seed0 = ctx->seed;
-                    if (flow == nullptr) {
-                        CUO_PFOR(j, 0, n) fX[j] = (int16_t)((int32_t)lcg_jump(seed0, j + 1) >> 20);
-                        cm = cm_mask;
-                    } else {
-                        CUO_PFOR(j, 0, n) {
-                            int16_t tmp = (lcg_jump(seed0, j + 1) & 0x8000) ? 4 : -4;  // QCONST16(1/256, 10)
-                            fX[j] = (int16_t)(flow[j] + tmp);
-                        }
-                        cm = (unsigned)fill;
-                    }
-                    ctx->seed = lcg_jump(seed0, n);
-                    renormalise_vector(fX, n, f.gain);
-                }
-            }
+            unsigned cm = qp_leaf(ctx, lds_uniform(f.X), f.N, f.b, f.B,
+                                  f.lowband ? lds_uniform(f.lowband) : nullptr, f.LM, f.gain, f.fill);
             ret = cm;
             sp--;
             continue;
@@ -935,7 +1015,7 @@ CUO_FN void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, 
                             int32_t balance, RangeDec *ec, int LM, int codedBands, uint32_t *seed,
                             CeltScratch *sc)
 {
-    const int16_t *eB = tab::kEBands;
+    const int32_t *eB = kEB32.v;
     const int M = 1 << LM;
     const int B = shortBlocks ? M : 1;
     const int norm_offset = M * eB[start];
@@ -999,7 +1079,7 @@ CUO_FN void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, 
 CUO_FN void anti_collapse(int16_t *X_, const uint8_t *collapse, int LM, int start, int end, const int16_t *logE,
                           const int16_t *prev1logE, const int16_t *prev2logE, const int32_t *pulses, uint32_t seed)
 {
-    const int16_t *eB = tab::kEBands;
+    const int32_t *eB = kEB32.v;
     for (int i = start; i < end; i++) {
         int N0 = eB[i + 1] - eB[i];
         int depth = ((1 + pulses[i]) / N0) >> LM;
@@ -1042,14 +1122,14 @@ CUO_FN void anti_collapse(int16_t *X_, const uint8_t *collapse, int LM, int star
 CUO_FN void denormalise_bands(const int16_t *X, int32_t *freq, const int16_t *bandLogE, int start, int end,
                               int M, int silence)
 {
-    const int16_t *eB = tab::kEBands;
+    const int32_t *eB = kEB32.v;
     int N = M * 120;
     int bound = M * eB[end];
     if (silence) { bound = 0; start = end = 0; }
     CUO_PFOR(j, 0, M * eB[start]) freq[j] = 0;
     for (int i = start; i < end; i++) {
         const int lo = M * eB[i], hi = M * eB[i + 1];
-        int16_t lg = sat16(bandLogE[i] + shl32((int32_t)tab::kEMeans[i], 6));
+        int16_t lg = sat16(bandLogE[i] + shl32(kEMeans32.v[i], 6));
         int shift = 16 - (lg >> kDbShift);
         int16_t g;
         if (shift > 31) { shift = 0; g = 0; }
@@ -1364,18 +1444,23 @@ CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int 
 {
     CUO_PROF_SCOPE(kProfCeltDeemph);
     const int16_t coef0 = tab::kPreemph[0];
-    int32_t m = *mem;
-    if (accum) {
-        for (int j = 0; j < N; j++) {
-            int32_t tmp = x[j] + m;
+    int32_t m = uniform(*mem);
+    N = uniform(N);
+    // serial one-pole recurrence, register-blocked: lanes load and store, the
+    // recurrence runs on scalar registers
+    for (int j0 = 0; j0 < N; j0 += kLanes) {
+        const int nb = uniform(imin(kLanes, N - j0));
+        const int l = lane_id();
+        int32_t v = l < nb ? x[j0 + l] : 0;
+        int32_t t = 0;
+        for (int k = 0; k < nb; k++) {
+            int32_t tmp = lane_get(v, k) + m;
             m = mul16_32_q15(coef0, tmp);
-            pcm[j] = sat16(pcm[j] + sig2word16(tmp));
+            t = lane_set(t, k, tmp);
         }
-    } else {
-        for (int j = 0; j < N; j++) {
-            int32_t tmp = x[j] + m;
-            m = mul16_32_q15(coef0, tmp);
-            pcm[j] = sig2word16(tmp);
+        if (l < nb) {
+            int16_t y = sig2word16(t);
+            pcm[j0 + l] = accum ? sat16(pcm[j0 + l] + y) : y;
         }
     }
     *mem = m;
@@ -1416,8 +1501,8 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
         uint32_t seed = st->rng;
         int16_t *X = sc->X;
         for (int i = start; i < effEnd; i++) {
-            int boffs = tab::kEBands[i] << LM;
-            int blen = (tab::kEBands[i + 1] - tab::kEBands[i]) << LM;
+            int boffs = kEB32.v[i] << LM;
+            int blen = (kEB32.v[i + 1] - kEB32.v[i]) << LM;
             for (int j = 0; j < blen; j++) {
                 seed = lcg_rand(seed);
                 X[boffs + j] = (int16_t)((int32_t)seed >> 20);
@@ -1468,7 +1553,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     }
     if (st->loss_duration == 0) st->skip_plc = 0;
     if (dec == nullptr) { ec_init(&sc->dec, data, (uint32_t)len); dec = &sc->dec; }
-    for (int i = 0; i < kNbBands; i++) oldBandE[i] = max16(oldBandE[i], oldBandE[kNbBands + i]);
+    CUO_PFOR(i, 0, kNbBands) oldBandE[i] = max16(oldBandE[i], oldBandE[kNbBands + i]);
 
     int32_t total_bits = len * 8;
     int32_t tell = ec_tell(dec);
@@ -1531,15 +1616,15 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     int spread_decision = SPREAD_NORMAL;
     if (tell + 4 <= total_bits) spread_decision = ec_icdf(dec, tab::kSpreadIcdf, 5);
     int32_t *cap = sc->cap, *offsets = sc->offsets;
-    for (int i = 0; i < kNbBands; i++) {
-        int Nb = (tab::kEBands[i + 1] - tab::kEBands[i]) << LM;
-        cap[i] = (tab::kCacheCaps[kNbBands * (2 * LM + 1 - 1) + i] + 64) * 1 * Nb >> 2;
+    CUO_PFOR(i, 0, kNbBands) {
+        int Nb = (kEB32.v[i + 1] - kEB32.v[i]) << LM;
+        cap[i] = (kCacheCaps32.v[kNbBands * (2 * LM + 1 - 1) + i] + 64) * 1 * Nb >> 2;
     }
     int dynalloc_logp = 6;
     total_bits <<= kBitRes;
     tell = (int32_t)ec_tell_frac(dec);
     for (int i = start; i < end; i++) {
-        int width = (tab::kEBands[i + 1] - tab::kEBands[i]) << LM;
+        int width = (kEB32.v[i + 1] - kEB32.v[i]) << LM;
         int quanta = imin(width << kBitRes, imax(6 << kBitRes, width));
         int loop_logp = dynalloc_logp;
         int boost = 0;
@@ -1606,23 +1691,20 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
         st->pf_gain_old = st->pf_gain;
         st->pf_tapset_old = st->pf_tapset;
     }
-    for (int i = 0; i < kNbBands; i++) oldBandE[kNbBands + i] = oldBandE[i];
+    CUO_PFOR(i, 0, kNbBands) oldBandE[kNbBands + i] = oldBandE[i];
     if (!isTransient) {
-        for (int i = 0; i < 2 * kNbBands; i++) { oldLogE2[i] = oldLogE[i]; oldLogE[i] = oldBandE[i]; }
+        CUO_PFOR(i, 0, 2 * kNbBands) { oldLogE2[i] = oldLogE[i]; oldLogE[i] = oldBandE[i]; }
     } else {
-        for (int i = 0; i < 2 * kNbBands; i++) oldLogE[i] = min16(oldLogE[i], oldBandE[i]);
+        CUO_PFOR(i, 0, 2 * kNbBands) oldLogE[i] = min16(oldLogE[i], oldBandE[i]);
     }
     int16_t max_bg_inc = (int16_t)(imin(160, st->loss_duration + M) * 1);  // QCONST16(0.001, 10) == 1
-    for (int i = 0; i < 2 * kNbBands; i++)
-        st->backgroundLogE[i] = min16((int16_t)(st->backgroundLogE[i] + max_bg_inc), oldBandE[i]);
-    for (int c = 0; c < 2; c++) {
-        for (int i = 0; i < start; i++) {
-            oldBandE[c * kNbBands + i] = 0;
-            oldLogE[c * kNbBands + i] = oldLogE2[c * kNbBands + i] = -(28 << kDbShift);
-        }
-        for (int i = end; i < kNbBands; i++) {
-            oldBandE[c * kNbBands + i] = 0;
-            oldLogE[c * kNbBands + i] = oldLogE2[c * kNbBands + i] = -(28 << kDbShift);
+    int16_t *bgE = st->backgroundLogE;
+    CUO_PFOR(i, 0, 2 * kNbBands) bgE[i] = min16((int16_t)(bgE[i] + max_bg_inc), oldBandE[i]);
+    CUO_PFOR(k, 0, 2 * kNbBands) {
+        int i = k < kNbBands ? k : k - kNbBands;
+        if (i < start || i >= end) {
+            oldBandE[k] = 0;
+            oldLogE[k] = oldLogE2[k] = -(28 << kDbShift);
         }
     }
     st->rng = dec->rng;

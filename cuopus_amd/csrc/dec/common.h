@@ -115,6 +115,19 @@ __host__ __device__ inline int lane_id() { return 0; }
 #endif
 #define CUO_PFOR(i, lo, hi) for (int i = (lo) + lane_id(); i < (hi); i += kLanes)
 
+// Register-blocked serial recurrences: a block of kLanes consecutive elements
+// is loaded one per lane, the recurrence walks the block reading element k
+// with lane_get(v, k) (a scalar register) and collecting results with
+// lane_set(v, k, x); the block is then stored one per lane.  The serial
+// order of the arithmetic is unchanged.  k must be wave-uniform.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ int32_t lane_get(int32_t v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ __forceinline__ int32_t lane_set(int32_t v, int k, int32_t x) { return lane_id() == k ? x : v; }
+#else
+__host__ __device__ inline int32_t lane_get(int32_t v, int) { return v; }
+__host__ __device__ inline int32_t lane_set(int32_t, int, int32_t x) { return x; }
+#endif
+
 // Reductions over the lanes of per-lane partial results (identity on the
 // host).  Integer sums wrap like the serial loop's, so any order is exact.
 // Call with every lane active (outside CUO_PFOR bodies).
