@@ -77,6 +77,10 @@ def _args():
     return ap.parse_args()
 
 
+def _log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def _read_traffic(workload):
     """HBM bytes per decode launch from the committed rocprofv3 counter summary
     (a line 'traffic_bytes_per_launch: N'), or None."""
@@ -196,7 +200,9 @@ def main():
         del src, dst
     modes = {"persistent": cn.MODE_PERSISTENT, "one": cn.MODE_ONE_LAUNCH, "plain": cn.MODE_PLAIN}
     b = cn.Batch(blobs, mode=modes[a.mode], n_slots=a.slots, use_graph=a.graph, profile=not a.graph)
+    _log("batch ready: %d files, %d chains, %.0f audio-s" % (len(blobs), b.chains, b.playable / 48000.0))
     wall, dec_ms, rs_ms = _time_batch(b, a.steps, a.warmup, torch)
+    _log("%s: %.1f ms/step" % (a.mode, 1000.0 * wall / a.steps))
     audio_s = b.playable / 48000.0
     ms = 1000.0 * wall / a.steps
     value = audio_s * a.steps / wall
@@ -227,6 +233,7 @@ def main():
         bp = cn.Batch(blobs, mode=cn.MODE_PLAIN, profile=True)
         pw, _, _ = _time_batch(bp, max(1, min(a.steps, 5)), 1, torch)
         plain = round(bp.playable / 48000.0 * max(1, min(a.steps, 5)) / pw, 1)
+        _log("plain: %.1f ms/step" % (1000.0 * pw / max(1, min(a.steps, 5))))
         bp.close()
         del bp
 
@@ -234,6 +241,7 @@ def main():
     if not a.no_cpu:
         th = a.cpu_threads or min(BOX_CPU_SHARE, os.cpu_count() or 1)
         cpu = _cpu_baseline(blobs, th, a.cpu_audio_s)
+        _log("cpu baseline: %s" % cpu["sample"])
 
     line = {
         "metric": "audio-sec decoded/s on 1 MI355X, bit-exact: batched vs plain; serve p99 ms",

@@ -1445,22 +1445,17 @@ CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int 
     CUO_PROF_SCOPE(kProfCeltDeemph);
     const int16_t coef0 = tab::kPreemph[0];
     int32_t m = uniform(*mem);
-    N = uniform(N);
-    // serial one-pole recurrence, register-blocked: lanes load and store, the
-    // recurrence runs on scalar registers
-    for (int j0 = 0; j0 < N; j0 += kLanes) {
-        const int nb = uniform(imin(kLanes, N - j0));
-        const int l = lane_id();
-        int32_t v = l < nb ? x[j0 + l] : 0;
-        int32_t t = 0;
-        for (int k = 0; k < nb; k++) {
-            int32_t tmp = lane_get(v, k) + m;
+    if (accum) {
+        for (int j = 0; j < N; j++) {
+            int32_t tmp = x[j] + m;
             m = mul16_32_q15(coef0, tmp);
-            t = lane_set(t, k, tmp);
+            pcm[j] = sat16(pcm[j] + sig2word16(tmp));
         }
-        if (l < nb) {
-            int16_t y = sig2word16(t);
-            pcm[j0 + l] = accum ? sat16(pcm[j0 + l] + y) : y;
+    } else {
+        for (int j = 0; j < N; j++) {
+            int32_t tmp = x[j] + m;
+            m = mul16_32_q15(coef0, tmp);
+            pcm[j] = sig2word16(tmp);
         }
     }
     *mem = m;
