@@ -29,6 +29,10 @@ constexpr int kCombMinPeriod = 15;
 enum { SPREAD_NONE = 0, SPREAD_LIGHT = 1, SPREAD_NORMAL = 2, SPREAD_AGGRESSIVE = 3 };
 
 struct CeltState {
+    // decode history, kDecBuf + kOverlap samples, owned by the caller (HBM on
+    // the device: it is only touched by lane-parallel loops and concealment)
+    int32_t *mem;
+    struct CeltPlcScratch *plc;  // pitch-concealment buffers, caller-owned
     int32_t start, end;
     // ---- cleared by reset (OPUS_RESET_STATE) from here on
     uint32_t rng;
@@ -41,7 +45,6 @@ struct CeltState {
     int32_t pf_tapset, pf_tapset_old;
     int32_t prefilter_and_fold;
     int32_t preemph_mem;
-    int32_t mem[kDecBuf + kOverlap];
     int16_t lpc[kLpcOrder];
     int16_t oldBandE[2 * kNbBands];
     int16_t oldLogE[2 * kNbBands];
@@ -49,27 +52,24 @@ struct CeltState {
     int16_t backgroundLogE[2 * kNbBands];
 };
 
+// Buffers of the pitch-based concealment (rare): caller-owned, HBM on the
+// device.
+struct CeltPlcScratch {
+    int16_t exc[kMaxPeriod + kLpcOrder];
+    int16_t fir_tmp[kMaxPeriod];
+    int16_t lp_pitch[kDecBuf >> 1];
+    int32_t xcorr[kMaxPeriod];
+    int16_t plc_a[kMaxPeriod + 8];                 // autocorrelation copy / pitch x4
+    int16_t plc_b[960 + kOverlap + kLpcOrder];     // IIR history / pitch y4
+};
+
 struct CeltScratch {
-    // The frame-decode buffers and the pitch-based concealment buffers are
-    // never live at the same time, so they share storage (the scratch block
-    // is sized to sit in LDS beside the decoder state).
-    union {
-        struct {  // frame decode, noise-based concealment
-            int16_t X[960];
-            int32_t freq[960];
-            int16_t norm[8 * 78];
-            int16_t tmp[176];
-            int32_t iy[176];
-        };
-        struct {  // pitch-based concealment
-            int16_t exc[kMaxPeriod + kLpcOrder];
-            int16_t fir_tmp[kMaxPeriod];
-            int16_t lp_pitch[kDecBuf >> 1];
-            int32_t xcorr[kMaxPeriod];
-            int16_t plc_a[kMaxPeriod + 8];                 // autocorrelation copy / pitch x4
-            int16_t plc_b[960 + kOverlap + kLpcOrder];     // IIR history / pitch y4
-        };
-    };
+    // frame decode and noise-based concealment
+    int16_t X[960];
+    int32_t freq[960];
+    int16_t norm[8 * 78];
+    int16_t tmp[176];
+    int32_t iy[176];
     int32_t pulses[kNbBands], fine_quant[kNbBands], fine_priority[kNbBands];
     int32_t cap[kNbBands], offsets[kNbBands], tf_res[kNbBands];
     int32_t bits1[kNbBands], bits2[kNbBands], thresh[kNbBands], trim_offset[kNbBands];
@@ -86,7 +86,8 @@ CUO_FN void celt_reset(CeltState *st)
     st->rng = 0; st->error = 0; st->last_pitch_index = 0; st->loss_duration = 0;
     st->pf_period = st->pf_period_old = 0; st->pf_gain = st->pf_gain_old = 0;
     st->pf_tapset = st->pf_tapset_old = 0; st->prefilter_and_fold = 0; st->preemph_mem = 0;
-    for (int i = 0; i < kDecBuf + kOverlap; i++) st->mem[i] = 0;
+    int32_t *mem = st->mem;
+    CUO_PFOR(i, 0, kDecBuf + kOverlap) mem[i] = 0;
     for (int i = 0; i < kLpcOrder; i++) st->lpc[i] = 0;
     for (int i = 0; i < 2 * kNbBands; i++) {
         st->oldBandE[i] = 0;
@@ -1365,7 +1366,6 @@ CUO_FN void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *ol
 {
     CUO_PROF_SCOPE(kProfCeltSynth);
     X = lds_uniform(X);
-    out_syn = lds_uniform(out_syn);
     oldBandE = lds_uniform(oldBandE);
     freq = lds_uniform(freq);
     const int N = 120 << LM, M = 1 << LM;
@@ -1440,22 +1440,27 @@ CUO_FN void comb_filter(int32_t *y, int32_t *x, int T0, int T1, int N, int16_t g
 }
 
 // deemphasis, mono, no downsampling (celt_decoder.c:279-377)
+// The history (x) and the frame output live in HBM: the one-pole recurrence
+// runs register-blocked, lanes loading and storing 64 samples at a time.
 CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int accum)
 {
     CUO_PROF_SCOPE(kProfCeltDeemph);
     const int16_t coef0 = tab::kPreemph[0];
     int32_t m = uniform(*mem);
-    if (accum) {
-        for (int j = 0; j < N; j++) {
-            int32_t tmp = x[j] + m;
+    N = uniform(N);
+    for (int j0 = 0; j0 < N; j0 += kLanes) {
+        const int nb = uniform(imin(kLanes, N - j0));
+        const int l = lane_id();
+        int32_t v = l < nb ? x[j0 + l] : 0;
+        int32_t t = 0;
+        for (int k = 0; k < nb; k++) {
+            int32_t tmp = lane_get(v, k) + m;
             m = mul16_32_q15(coef0, tmp);
-            pcm[j] = sat16(pcm[j] + sig2word16(tmp));
+            t = lane_set(t, k, tmp);
         }
-    } else {
-        for (int j = 0; j < N; j++) {
-            int32_t tmp = x[j] + m;
-            m = mul16_32_q15(coef0, tmp);
-            pcm[j] = sig2word16(tmp);
+        if (l < nb) {
+            int16_t y = sig2word16(t);
+            pcm[j0 + l] = accum ? sat16(pcm[j0 + l] + y) : y;
         }
     }
     *mem = m;
@@ -1526,7 +1531,6 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     CUO_PROF_SCOPE(kProfCelt);
     st = lds_uniform(st);
     sc = lds_uniform(sc);
-    if (pcm) pcm = lds_uniform(pcm);
     if (dec) dec = lds_uniform(dec);
     int LM;
     for (LM = 0; LM <= 3; LM++)

@@ -314,10 +314,10 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc)
     int16_t fade = kQ15One;
     int pitch_index;
     if (st->loss_duration == 0) {
-        int16_t *lp = sc->lp_pitch;
-        pitch_downsample(buf, lp, kDecBuf, sc->plc_a);
+        int16_t *lp = st->plc->lp_pitch;
+        pitch_downsample(buf, lp, kDecBuf, st->plc->plc_a);
         pitch_index = pitch_search(lp + (kPlcLagMax >> 1), lp, kDecBuf - kPlcLagMax, kPlcLagMax - kPlcLagMin,
-                                   sc->plc_a, sc->plc_b, sc->xcorr);
+                                   st->plc->plc_a, st->plc->plc_b, st->plc->xcorr);
         pitch_index = kPlcLagMax - pitch_index;
         st->last_pitch_index = pitch_index;
     } else {
@@ -325,12 +325,12 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc)
         fade = 26214;  // QCONST16(.8f, 15)
     }
     const int exc_length = imin(2 * pitch_index, kMaxPeriod);
-    int16_t *exc = sc->exc + kLpcOrder;
+    int16_t *exc = st->plc->exc + kLpcOrder;
     for (int i = 0; i < kMaxPeriod + kLpcOrder; i++)
         exc[i - kLpcOrder] = sround16(buf[kDecBuf - kMaxPeriod - kLpcOrder + i], kSigShift);
     if (st->loss_duration == 0) {
         int32_t ac[kLpcOrder + 1];
-        celt_autocorr(exc, ac, window, kOverlap, kLpcOrder, kMaxPeriod, sc->plc_a);
+        celt_autocorr(exc, ac, window, kOverlap, kLpcOrder, kMaxPeriod, st->plc->plc_a);
         ac[0] += ac[0] >> 13;
         for (int i = 1; i <= kLpcOrder; i++) ac[i] -= mul16_32_q15(2 * i * i, ac[i]);
         celt_lpc(st->lpc, ac, kLpcOrder);
@@ -345,8 +345,8 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc)
             }
         }
     }
-    celt_fir(exc + kMaxPeriod - exc_length, st->lpc, sc->fir_tmp, exc_length, kLpcOrder);
-    for (int i = 0; i < exc_length; i++) exc[kMaxPeriod - exc_length + i] = sc->fir_tmp[i];
+    celt_fir(exc + kMaxPeriod - exc_length, st->lpc, st->plc->fir_tmp, exc_length, kLpcOrder);
+    for (int i = 0; i < exc_length; i++) exc[kMaxPeriod - exc_length + i] = st->plc->fir_tmp[i];
     int16_t decay;
     {
         int32_t E1 = 1, E2 = 1;
@@ -379,7 +379,7 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc)
     {
         int16_t lpc_mem[kLpcOrder];
         for (int i = 0; i < kLpcOrder; i++) lpc_mem[i] = sround16(buf[kDecBuf - N - 1 - i], kSigShift);
-        celt_iir(buf + kDecBuf - N, st->lpc, buf + kDecBuf - N, extrapolation_len, kLpcOrder, lpc_mem, sc->plc_b);
+        celt_iir(buf + kDecBuf - N, st->lpc, buf + kDecBuf - N, extrapolation_len, kLpcOrder, lpc_mem, st->plc->plc_b);
         for (int i = 0; i < extrapolation_len; i++) buf[kDecBuf - N + i] = saturate(buf[kDecBuf - N + i], kSigSat);
     }
     {

@@ -52,10 +52,24 @@ struct DecScratch {
     int16_t redundant_audio[240];
     RangeDec dec;  // the frame's range decoder (shared by SILK and CELT)
     int16_t frame_sizes[48];  // packet_parse output
-    // One frame of output: a frame decode writes at most st->frame_size
-    // <= 2880 samples (60 ms), data or concealment.
-    int16_t frame_out[kMaxFrameOut];
+    // One frame of output, caller-owned (HBM on the device): a frame decode
+    // writes at most st->frame_size <= 2880 samples (60 ms).
+    int16_t *frame_out;
 };
+
+// Memory of a decoder that lives outside the state/scratch block.
+struct DecHeap {
+    int32_t celt_mem[2048 + 120];
+    int16_t frame_out[kMaxFrameOut];
+    CeltPlcScratch plc;
+};
+
+CUO_FN void dec_bind(OpusDecState *st, DecScratch *sc, DecHeap *h)
+{
+    st->celt.mem = h->celt_mem;
+    st->celt.plc = &h->plc;
+    sc->frame_out = h->frame_out;
+}
 
 CUO_FN void opus_dec_init(OpusDecState *st)
 {
@@ -209,7 +223,7 @@ CUO_FN int packet_has_lbrr(const uint8_t *p, int len)
 
 CUO_FN void smooth_fade(const int16_t *in1, const int16_t *in2, int16_t *out, int overlap)
 {
-    for (int i = 0; i < overlap; i++) {
+    CUO_PFOR(i, 0, overlap) {
         int16_t w = (int16_t)mul16_16_q15(tab::kWindow[i], tab::kWindow[i]);
         out[i] = (int16_t)((mul16_16(w, in2[i]) + mul16_16(kQ15One - w, in1[i])) >> 15);
     }
@@ -238,7 +252,6 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
     constexpr bool HasData = Kind == kFrameData;
     const int F20 = 960, F10 = 480, F5 = 240, F2_5 = 120;
     st = lds_uniform(st);
-    pcm = lds_uniform(pcm);
     sc = lds_uniform(sc);
     RangeDec &dec = sc->dec;
     if (frame_size < F2_5) return OPUS_BUFFER_TOO_SMALL;
@@ -262,7 +275,7 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
         mode = st->prev_redundancy ? MODE_CELT_ONLY : st->prev_mode;
         bandwidth = 0;
         if (mode == 0) {
-            for (int i = 0; i < audiosize; i++) pcm[i] = 0;
+            CUO_PFOR(i, 0, audiosize) pcm[i] = 0;
             return audiosize;
         }
         if constexpr (Kind == kFrameConceal) {
@@ -322,7 +335,7 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
             if (silk_ret) {
                 if (lost_flag) {
                     silk_frame_size = frame_size;
-                    for (int i = 0; i < frame_size; i++) pcm_ptr[i] = 0;
+                    CUO_PFOR(i, 0, frame_size) pcm_ptr[i] = 0;
                 } else {
                     return OPUS_INTERNAL_ERROR;
                 }
@@ -382,15 +395,13 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
                                data != nullptr ? &dec : nullptr, celt_accum, &sc->celt);
     } else {
         alignas(4) static constexpr uint8_t silence[4] = {0xFF, 0xFF, 0, 0};  // a 2-byte frame
-        if (!celt_accum)
-            for (int i = 0; i < frame_size; i++) pcm[i] = 0;
+        if (!celt_accum) CUO_PFOR(i, 0, frame_size) pcm[i] = 0;
         if (st->prev_mode == MODE_HYBRID && !(redundancy && celt_to_silk && st->prev_redundancy)) {
             st->celt.start = 0;
             celt_decode(&st->celt, silence, 2, pcm, F2_5, nullptr, celt_accum, &sc->celt);
         }
     }
-    if (mode != MODE_CELT_ONLY && !celt_accum)
-        for (int i = 0; i < frame_size; i++) pcm[i] = sat16(pcm[i] + pcm_silk[i]);
+    if (mode != MODE_CELT_ONLY && !celt_accum) CUO_PFOR(i, 0, frame_size) pcm[i] = sat16(pcm[i] + pcm_silk[i]);
     if (HasData && redundancy && !celt_to_silk) {
         celt_reset(&st->celt);
         st->celt.start = 0;
@@ -399,12 +410,12 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
         smooth_fade(pcm + frame_size - F2_5, redundant_audio + F2_5, pcm + frame_size - F2_5, F2_5);
     }
     if (HasData && redundancy && celt_to_silk && (st->prev_mode != MODE_SILK_ONLY || st->prev_redundancy)) {
-        for (int i = 0; i < F2_5; i++) pcm[i] = redundant_audio[i];
+        CUO_PFOR(i, 0, F2_5) pcm[i] = redundant_audio[i];
         smooth_fade(redundant_audio + F2_5, pcm + F2_5, pcm + F2_5, F2_5);
     }
     if (HasData && transition) {
         if (audiosize >= F5) {
-            for (int i = 0; i < F2_5; i++) pcm[i] = pcm_transition[i];
+            CUO_PFOR(i, 0, F2_5) pcm[i] = pcm_transition[i];
             smooth_fade(pcm_transition + F2_5, pcm + F2_5, pcm + F2_5, F2_5);
         } else {
             smooth_fade(pcm_transition, pcm, pcm, F2_5);

@@ -1205,7 +1205,6 @@ CUO_FN int silk_decode(SilkDecoder *dec, const SilkControl *ctl, int lostFlag, i
     dec = lds_uniform(dec);
     ctl = lds_uniform(ctl);
     d = lds_uniform(d);
-    samplesOut = lds_uniform(samplesOut);
     sc = lds_uniform(sc);
     SilkChannel *ch = &dec->ch;
     int ret = 0;

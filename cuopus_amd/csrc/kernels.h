@@ -19,9 +19,9 @@
 namespace cuo {
 
 constexpr int kMaxPktSamples = 5760;
-// Resident wavefronts of the persistent decoder at its LDS footprint: four
-// 37 KB wavefronts per CU (160 KB LDS) on 256 CUs.
-constexpr uint32_t kDefaultWaves = 1024;
+// Resident wavefronts of the persistent decoder: two per SIMD (LDS < 20 KB
+// and <= 256 VGPRs per wavefront), 8 per CU, 256 CUs.
+constexpr uint32_t kDefaultWaves = 2048;
 constexpr int kRsTaps = 49;
 constexpr int kRsPad = 24;
 
@@ -40,15 +40,18 @@ struct ChainResult {
     uint32_t samples;  // samples produced by the chain's packets
 };
 
-// Decoder memory of one wavefront: the chain's persistent state, the per-call
-// scratch and one frame of output (about 37 KB), all in LDS.  Chains restart
-// from a fresh decoder, so nothing of it is ever saved to HBM: the only HBM
-// traffic of a chain is its packet bytes in and its kept PCM out.
+// Decoder memory of one wavefront.  The chain's state and per-call scratch
+// (WaveState, under 19 KB) sit in LDS, so two wavefronts fit per SIMD; the
+// CELT history, one frame of output and the pitch-concealment buffers
+// (DecHeap, 29 KB) sit in a per-wavefront HBM slot, where only lane-parallel
+// loops and the rare concealment touch them.  Chains restart from a fresh
+// decoder, so nothing is carried between chains.
 struct WaveState {
     OpusDecState st;
     DecScratch sc;
 };
 static_assert(sizeof(WaveState) <= kWaveLdsBytes, "wavefront decoder memory exceeds its LDS block");
+static_assert(kWaveLdsBytes * 8 <= 160 * 1024, "two wavefronts per SIMD need <= 20 KB of LDS each");
 
 struct DecodeArgs {
     const uint8_t *data;
@@ -58,6 +61,7 @@ struct DecodeArgs {
     const ChainDesc *chains;
     const uint32_t *order;      // nullable: work order (chain indices)
     uint32_t n_chains;
+    DecHeap *heap;              // one per workgroup of the launch
     int16_t *out;               // compacted 48 kHz PCM
     ChainResult *res;
     int32_t *pkt_nsamp;         // nullable, per packet
@@ -86,7 +90,8 @@ struct ChainOut {
 // Decode one chain on the calling wavefront.
 // Kept out of line: the persistent loop around it then stays a small loop
 // whose every iteration runs with all 64 lanes together.
-__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, int lane)
+__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, DecHeap *heap,
+                                              int lane)
 {
     prof_reset();
     {
@@ -94,6 +99,7 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
     const ChainDesc ch = a.chains[c];
     OpusDecState *st = lds_uniform(&ws->st);
     DecScratch *sc = lds_uniform(&ws->sc);
+    dec_bind(st, sc, heap);
     opus_dec_init(st);
     ChainOut out;
     out.buf = sc->frame_out;
@@ -139,17 +145,17 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
 }
 
 // Plain variant: workgroup g decodes chain order[g].
-__global__ __launch_bounds__(64) void k_decode_chains(DecodeArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_decode_chains(DecodeArgs a)
 {
     const uint32_t g = blockIdx.x;
     if (g >= a.n_chains) return;
     const uint32_t c = a.order ? a.order[g] : g;
-    decode_one_chain(a, c, (WaveState *)g_wave_lds, (int)threadIdx.x);
+    decode_one_chain(a, c, (WaveState *)g_wave_lds, a.heap + g, (int)threadIdx.x);
 }
 
 // Persistent variant: the grid's wavefronts pull chains from an atomic queue
 // until it is drained (every wavefront exits once the counter passes n_chains).
-__global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_decode_persistent(DecodeArgs a)
 {
     WaveState *ws = (WaveState *)g_wave_lds;
     const int lane = (int)threadIdx.x;
@@ -163,7 +169,7 @@ __global__ __launch_bounds__(64) void k_decode_persistent(DecodeArgs a)
         w = (uint32_t)__builtin_amdgcn_readfirstlane((int)w);
         if (w >= a.n_chains) break;
         const uint32_t c = a.order ? a.order[w] : w;
-        decode_one_chain(a, c, ws, lane);
+        decode_one_chain(a, c, ws, a.heap + blockIdx.x, lane);
     }
 }
 

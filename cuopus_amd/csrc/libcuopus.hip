@@ -84,7 +84,7 @@ int grow(DevBuf &b, size_t need, Err &e)
 }
 
 struct Workspaces {
-    DevBuf data, off, len, lost, chains, order, cat, res, nsamp, rng, files, queue;
+    DevBuf data, off, len, lost, chains, order, cat, res, nsamp, rng, files, queue, heap;
 } W;
 
 double env_f(const char *name, double dflt)
@@ -289,6 +289,8 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     }
     const uint32_t n_waves = std::max<uint32_t>(1, std::min<uint32_t>(env_u("CUOPUS_SLOTS", kDefaultWaves), nch));
     if ((rc = grow(W.queue, 256, e))) return rc;
+    // one HBM heap per workgroup of the launch
+    if ((rc = grow(W.heap, sizeof(DecHeap) * (size_t)(o.persistent ? n_waves : nch), e))) return rc;
 
     std::vector<uint32_t> order;
     lpt_order(pl, &order);
@@ -308,6 +310,7 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     a.chains = (const ChainDesc *)W.chains.p;
     a.out = out;
     a.res = (ChainResult *)W.res.p;
+    a.heap = (DecHeap *)W.heap.p;
     a.pkt_nsamp = o.pkt_nsamp;
     a.pkt_rng = o.pkt_rng;
     a.queue = (uint32_t *)W.queue.p;
@@ -393,7 +396,7 @@ extern "C" long long cuopus_workspace_bytes(void) { return g_dev_bytes; }
 
 extern "C" void cuopus_release_workspaces(void)
 {
-    DevBuf *all[] = {&W.data, &W.off, &W.len, &W.lost, &W.chains, &W.order,
+    DevBuf *all[] = {&W.data, &W.off, &W.len, &W.lost, &W.chains, &W.order, &W.heap,
                      &W.cat, &W.res, &W.nsamp, &W.rng, &W.files, &W.queue};
     for (DevBuf *b : all) {
         if (b->p) hipFree(b->p);
@@ -649,13 +652,14 @@ extern "C" int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, f
 struct cuopus_batch {
     Plan pl;
     cuopus_batch_opts opts;
-    DevBuf data, off, len, lost, chains, order, cat, res, files, queue;
+    DevBuf data, off, len, lost, chains, order, cat, res, files, queue, heap;
     std::vector<RsFile> rsf;          // one entry per file with output
     std::vector<uint64_t> file_tiles; // per rsf entry: tile count
     std::vector<uint32_t> file_chain0, file_nchain;
     std::vector<int> file_rs;         // per file: its rsf entry or -1
     uint64_t tiles = 0;
     uint32_t n_slots = 0;
+    uint64_t n_heaps = 0;  // HBM decoder heaps: one per workgroup of the largest launch
     int64_t chain_pkts = 0, chain_bytes = 0;
     RsTaps taps;
     hipGraphExec_t gexec = nullptr;
@@ -670,7 +674,7 @@ int batch_grow(cuopus_batch *b, DevBuf &d, size_t need, Err &e) { return grow(d,
 
 void batch_free(cuopus_batch *b)
 {
-    DevBuf *all[] = {&b->data, &b->off, &b->len, &b->lost, &b->chains, &b->order,
+    DevBuf *all[] = {&b->data, &b->off, &b->len, &b->lost, &b->chains, &b->order, &b->heap,
                      &b->cat, &b->res, &b->files, &b->queue};
     for (DevBuf *d : all) {
         if (d->p) { hipFree(d->p); g_dev_bytes -= (long long)d->cap; }
@@ -697,6 +701,7 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
     a.out = (int16_t *)b->cat.p;
     a.res = (ChainResult *)b->res.p;
     a.queue = (uint32_t *)b->queue.p;
+    a.heap = (DecHeap *)b->heap.p;
     const bool prof = b->opts.profile && !b->opts.use_graph;
     const int mode = b->opts.mode;
     if (prof) HIPCHK(e, hipEventRecord(b->ev[0], st));
@@ -813,6 +818,14 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
     // capacity at the kernel's LDS footprint), never more than the chains
     b->n_slots = b->opts.n_slots > 0 ? (uint32_t)b->opts.n_slots : kDefaultWaves;
     if (b->n_slots > nch) b->n_slots = nch;
+    if (b->opts.mode == 2) {
+        b->n_heaps = b->n_slots;
+    } else if (b->opts.mode == 1) {
+        b->n_heaps = nch;
+    } else {
+        b->n_heaps = 1;
+        for (int f = 0; f < nfiles; f++) b->n_heaps = std::max<uint64_t>(b->n_heaps, b->file_nchain[f]);
+    }
     b->taps = make_taps();
     b->file_rs.assign(pl.files.size(), -1);
     for (size_t f = 0; f < pl.files.size(); f++) {
@@ -842,6 +855,7 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
         (rc = grow(b->len, 4 * npk, e)) || (rc = grow(b->chains, sizeof(ChainDesc) * nch, e)) ||
         (rc = grow(b->order, 4ull * nch, e)) || (rc = grow(b->res, sizeof(ChainResult) * nch, e)) ||
         (rc = grow(b->cat, 2 * (pl.total_keep ? pl.total_keep : 1), e)) || (rc = grow(b->queue, 256, e)) ||
+        (rc = grow(b->heap, sizeof(DecHeap) * (size_t)b->n_heaps, e)) ||
         (rc = grow(b->files, sizeof(RsFile) * (rsf_upload.size() ? rsf_upload.size() : 1), e)) ||
         (!pl.lost.empty() && (rc = grow(b->lost, pl.lost.size(), e)))) {
         batch_free(b);
@@ -984,6 +998,8 @@ extern "C" int cuopus_warmup(char *errbuf, size_t errcap)
     memset(&a, 0, sizeof(a));
     a.n_chains = 0;  // every workgroup returns at once: only loads the code objects
     a.res = (ChainResult *)W.res.p;
+    if ((rc = grow(W.heap, sizeof(DecHeap), e))) return rc;
+    a.heap = (DecHeap *)W.heap.p;
     launch_chains(1, 0, a);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipDeviceSynchronize());

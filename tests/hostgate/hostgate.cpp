@@ -30,8 +30,11 @@ extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const 
 {
     OpusDecState *st = (OpusDecState *)malloc(sizeof(OpusDecState));
     DecScratch *sc = (DecScratch *)malloc(sizeof(DecScratch));
+    DecHeap *heap = (DecHeap *)malloc(sizeof(DecHeap));
     memset(sc, 0xA5, sizeof(DecScratch));  // device slots start uninitialised
     memset(st, 0x5A, sizeof(OpusDecState));
+    memset(heap, 0x3C, sizeof(DecHeap));
+    dec_bind(st, sc, heap);
     opus_dec_init(st);
     HostOut out{sc->frame_out, pcm, 0};
     long total = 0;
@@ -59,6 +62,7 @@ extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const 
     }
     free(st);
     free(sc);
+    free(heap);
     return ret < 0 ? ret : total;
 }
 
@@ -83,7 +87,8 @@ extern "C" int hg_sizes(int *out)
     out[1] = (int)sizeof(DecScratch);
     out[2] = (int)sizeof(CeltState);
     out[3] = (int)sizeof(SilkDecoder);
-    return 4;
+    out[4] = (int)sizeof(DecHeap);
+    return 5;
 }
 
 // Several chains decoded one after another in the same state and scratch
@@ -96,10 +101,13 @@ extern "C" void hg_decode_chains_reuse(const uint8_t *data, const uint32_t *off,
 {
     OpusDecState *st = (OpusDecState *)malloc(sizeof(OpusDecState));
     DecScratch *sc = (DecScratch *)malloc(sizeof(DecScratch));
+    DecHeap *heap = (DecHeap *)malloc(sizeof(DecHeap));
     memset(sc, 0xA5, sizeof(DecScratch));
     memset(st, 0x5A, sizeof(OpusDecState));
+    memset(heap, 0x3C, sizeof(DecHeap));
     long pos = 0;
     for (int c = 0; c < nch; c++) {
+        dec_bind(st, sc, heap);
         opus_dec_init(st);
         HostOut out{sc->frame_out, pcm + pos, 0};
         long r = 0;
@@ -125,4 +133,5 @@ extern "C" void hg_decode_chains_reuse(const uint8_t *data, const uint32_t *off,
     }
     free(st);
     free(sc);
+    free(heap);
 }
