@@ -72,6 +72,9 @@ def _args():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-audio-s", type=float, default=7200.0,
                     help="playable audio-seconds of the workload the CPU baseline decodes")
+    ap.add_argument("--serve-qps", default="50",
+                    help="comma-separated request rates for the serving run ('' to skip)")
+    ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per rate")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -165,6 +168,55 @@ def _cpu_baseline(blobs, threads, max_audio_s=7200.0):
     }
 
 
+def _serve(cn, torch, qps_list, seconds, seed=0x5E7E):
+    """Poisson arrivals of requests drawn from the serving pool (durations
+    lognormal, median 45 s, sigma 1.0, clipped 2 s-2 h; mode/bitrate mix as the
+    uneven set) against the continuous-batching server.  Latency = arrival to
+    the poll that returns the finished tensor."""
+    import glob
+    import numpy as np
+    paths = sorted(glob.glob(os.path.join(ROOT, "corpus", "bench", "serve_pool", "f*.opus")))
+    if not paths:
+        return None
+    pool = [open(p, "rb").read() for p in paths]
+    secs = [cn.probe(b) / 16000.0 for b in pool]
+    out = []
+    rs = np.random.default_rng(seed)
+    for qps in qps_list:
+        srv = cn.Server(n_streams=4, mem_budget=16 << 30)
+        n = max(1, int(qps * seconds))
+        gaps = rs.exponential(1.0 / qps, size=n)
+        arrive = np.cumsum(gaps)
+        pick = rs.integers(0, len(pool), size=n)
+        lat = []
+        audio = 0.0
+        t0 = time.perf_counter()
+        k = 0
+        pending = {}
+        while k < n or pending:
+            now = time.perf_counter() - t0
+            while k < n and arrive[k] <= now:
+                rid = srv.submit(pool[pick[k]])
+                pending[rid] = (arrive[k], secs[pick[k]])
+                k += 1
+            for rid, _t, rc in srv.poll():
+                if rc:
+                    raise RuntimeError("request failed rc=%d" % rc)
+                ta, sa = pending.pop(rid)
+                lat.append(time.perf_counter() - t0 - ta)
+                audio += sa
+            time.sleep(0.0002)
+        wall = time.perf_counter() - t0
+        srv.close()
+        lat = np.array(lat) * 1000.0
+        out.append({"qps": qps, "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                    "p99_ms": round(float(np.percentile(lat, 99)), 2), "audio_s_per_s": round(audio / wall, 1),
+                    "offered_audio_s_per_s": round(float(np.mean(secs)) * qps, 1)})
+        _log("serve qps %g: p50 %.1f ms p99 %.1f ms, %.0f audio-s/s" % (qps, out[-1]["p50_ms"], out[-1]["p99_ms"],
+                                                                       out[-1]["audio_s_per_s"]))
+    return out
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -237,6 +289,10 @@ def main():
         bp.close()
         del bp
 
+    serve = None
+    if a.serve_qps:
+        serve = _serve(cn, torch, [float(q) for q in a.serve_qps.split(",")], a.serve_seconds)
+
     cpu = None
     if not a.no_cpu:
         th = a.cpu_threads or min(BOX_CPU_SHARE, os.cpu_count() or 1)
@@ -263,6 +319,7 @@ def main():
         "roofline": roof,
         "resampler_roofline": rs_roof,
         "cpu_baseline": cpu,
+        "serve": serve,
     }
     print(json.dumps(line), flush=True)
 

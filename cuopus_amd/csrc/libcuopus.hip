@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <string>
 #include <vector>
+#include <deque>
 
 #include "../../include/cuopus.h"
 #include "host/demux.h"
@@ -969,6 +970,356 @@ extern "C" void cuopus_batch_destroy(cuopus_batch *b)
     if (!b) return;
     batch_free(b);
     delete b;
+}
+
+
+// =====================================================================================
+// Serving: continuous batching of decode requests
+// =====================================================================================
+// Requests are demuxed on submit and queue in arrival order.  poll() retires
+// finished launches (event query, no blocking), then admits queued requests
+// into a new launch on a free stream while the bytes they need on the device
+// (packets, chain table, int16 PCM, f32 output) fit under the memory budget.
+// Several launches run at once on their own streams, so requests that arrive
+// while others decode start as soon as a stream and memory are free and their
+// work overlaps on the device.  Each launch is the batch path: one persistent
+// decode kernel over the admitted requests' chains (longest first) and one
+// resample launch over their files.
+struct SrvReq {
+    uint64_t id = 0;
+    FilePlan fp;
+    std::vector<uint8_t> lost;
+    float *out = nullptr;
+    int64_t out16 = 0;
+    uint64_t mem = 0;
+    int rc = 0;
+    char err[160] = {0};
+};
+
+struct SrvLaunch {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    uint64_t mem = 0;
+    std::vector<SrvReq *> reqs;
+    std::vector<uint64_t> cap;         // per chain expected samples
+    DevBuf data, off, len, lost, chains, order, cat, res, files, queue, heap;
+    uint8_t *h_stage = nullptr;        // pinned staging for uploads
+    size_t h_stage_cap = 0;
+    ChainResult *h_res = nullptr;      // pinned results
+    size_t h_res_cap = 0;
+};
+
+struct cuopus_server {
+    cuopus_server_opts opts;
+    std::deque<SrvReq *> pending;
+    std::vector<SrvLaunch> launches;
+    std::deque<SrvReq *> finished;
+    uint64_t mem_in_use = 0;
+    uint64_t next_id = 1;
+    RsTaps taps;
+};
+
+namespace {
+
+int pinned_grow(uint8_t **p, size_t *cap, size_t need, Err &e)
+{
+    if (*cap >= need) return 0;
+    if (*p) hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t n = need + need / 4 + 4096;
+    hipError_t s = hipHostMalloc((void **)p, n, hipHostMallocDefault);
+    if (s != hipSuccess) return e.set(CUOPUS_RC_GPU, "hipHostMalloc(%zu): %s", n, hipGetErrorString(s));
+    *cap = n;
+    return 0;
+}
+
+uint64_t req_device_bytes(const SrvReq *r)
+{
+    uint64_t nch = r->fp.chains.size(), npk = r->fp.d.off.size();
+    return r->fp.d.data.size() + 8 * npk + r->lost.size() + (sizeof(ChainDesc) + sizeof(ChainResult) + 4) * nch +
+           2 * r->fp.keep + 4 * (uint64_t)(r->out16 > 0 ? r->out16 : 0) + 64;
+}
+
+// Build and enqueue one launch for the requests in L.reqs (no host waits).
+int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
+{
+    Plan pl;
+    pl.files.reserve(L.reqs.size());
+    bool any_lost = false;
+    for (SrvReq *r : L.reqs) {
+        pl.files.push_back(r->fp);  // copy: the request keeps its demux for error reports
+        any_lost = any_lost || !r->lost.empty();
+    }
+    int rc = merge(pl, e);
+    if (rc) return rc;
+    const uint32_t nch = (uint32_t)pl.chains.size();
+    const uint64_t npk = pl.off.size();
+    if (any_lost) {
+        pl.lost.assign(npk, 0);
+        uint64_t pb = 0;
+        for (size_t f = 0; f < L.reqs.size(); f++) {
+            if (!L.reqs[f]->lost.empty()) memcpy(pl.lost.data() + pb, L.reqs[f]->lost.data(), L.reqs[f]->lost.size());
+            pb += pl.files[f].d.off.size();
+        }
+    }
+    std::vector<uint32_t> order;
+    lpt_order(pl, &order);
+    std::vector<RsFile> rsf;
+    uint64_t tiles = 0;
+    for (size_t f = 0; f < pl.files.size(); f++) {
+        if (L.reqs[f]->out16 <= 0) continue;
+        RsFile r;
+        r.in_off = pl.files[f].cat_off;
+        r.in_len = pl.files[f].keep;
+        r.out_len = (uint64_t)L.reqs[f]->out16;
+        r.out = L.reqs[f]->out;
+        r.tile0 = tiles;
+        tiles += (r.out_len + kRsTile - 1) / kRsTile;
+        rsf.push_back(r);
+    }
+    const uint32_t waves = std::max<uint32_t>(1, std::min<uint32_t>(
+        srv->opts.n_waves > 0 ? (uint32_t)srv->opts.n_waves : kDefaultWaves, nch));
+    // device buffers (grow-only per launch slot)
+    if ((rc = grow(L.data, pl.data.size() + 4, e)) || (rc = grow(L.off, 4 * npk, e)) ||
+        (rc = grow(L.len, 4 * npk, e)) || (rc = grow(L.chains, sizeof(ChainDesc) * nch, e)) ||
+        (rc = grow(L.order, 4ull * nch, e)) || (rc = grow(L.res, sizeof(ChainResult) * nch, e)) ||
+        (rc = grow(L.cat, 2 * (pl.total_keep ? pl.total_keep : 1), e)) || (rc = grow(L.queue, 256, e)) ||
+        (rc = grow(L.files, sizeof(RsFile) * (rsf.empty() ? 1 : rsf.size()), e)) ||
+        (rc = grow(L.heap, sizeof(DecHeap) * (size_t)waves, e)) ||
+        (any_lost && (rc = grow(L.lost, npk, e))))
+        return rc;
+    // stage everything in one pinned block, then async copies on the launch stream
+    const size_t o_data = 0, o_off = o_data + pl.data.size(), o_len = o_off + 4 * npk, o_ch = o_len + 4 * npk,
+                 o_ord = o_ch + sizeof(ChainDesc) * nch, o_rs = o_ord + 4ull * nch,
+                 o_lost = o_rs + sizeof(RsFile) * rsf.size(), total = o_lost + (any_lost ? npk : 0);
+    if ((rc = pinned_grow(&L.h_stage, &L.h_stage_cap, total, e))) return rc;
+    memcpy(L.h_stage + o_data, pl.data.data(), pl.data.size());
+    memcpy(L.h_stage + o_off, pl.off.data(), 4 * npk);
+    memcpy(L.h_stage + o_len, pl.len.data(), 4 * npk);
+    memcpy(L.h_stage + o_ch, pl.chains.data(), sizeof(ChainDesc) * nch);
+    memcpy(L.h_stage + o_ord, order.data(), 4ull * nch);
+    if (!rsf.empty()) memcpy(L.h_stage + o_rs, rsf.data(), sizeof(RsFile) * rsf.size());
+    if (any_lost) memcpy(L.h_stage + o_lost, pl.lost.data(), npk);
+    size_t rneed = sizeof(ChainResult) * nch;
+    {
+        uint8_t *hr = (uint8_t *)L.h_res;
+        if ((rc = pinned_grow(&hr, &L.h_res_cap, rneed, e))) return rc;
+        L.h_res = (ChainResult *)hr;
+    }
+    hipStream_t st = L.st;
+    HIPCHK(e, hipMemcpyAsync(L.data.p, L.h_stage + o_data, pl.data.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(e, hipMemcpyAsync(L.off.p, L.h_stage + o_off, 4 * npk, hipMemcpyHostToDevice, st));
+    HIPCHK(e, hipMemcpyAsync(L.len.p, L.h_stage + o_len, 4 * npk, hipMemcpyHostToDevice, st));
+    HIPCHK(e, hipMemcpyAsync(L.chains.p, L.h_stage + o_ch, sizeof(ChainDesc) * nch, hipMemcpyHostToDevice, st));
+    HIPCHK(e, hipMemcpyAsync(L.order.p, L.h_stage + o_ord, 4ull * nch, hipMemcpyHostToDevice, st));
+    if (!rsf.empty())
+        HIPCHK(e, hipMemcpyAsync(L.files.p, L.h_stage + o_rs, sizeof(RsFile) * rsf.size(), hipMemcpyHostToDevice, st));
+    if (any_lost) HIPCHK(e, hipMemcpyAsync(L.lost.p, L.h_stage + o_lost, npk, hipMemcpyHostToDevice, st));
+    DecodeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.data = (const uint8_t *)L.data.p;
+    a.off = (const uint32_t *)L.off.p;
+    a.len = (const uint32_t *)L.len.p;
+    a.lost = any_lost ? (const uint8_t *)L.lost.p : nullptr;
+    a.chains = (const ChainDesc *)L.chains.p;
+    a.order = (const uint32_t *)L.order.p;
+    a.n_chains = nch;
+    a.heap = (DecHeap *)L.heap.p;
+    a.out = (int16_t *)L.cat.p;
+    a.res = (ChainResult *)L.res.p;
+    a.queue = (uint32_t *)L.queue.p;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, a.queue);
+    HIPCHK(e, hipGetLastError());
+    launch_persistent(waves, st, a);
+    HIPCHK(e, hipGetLastError());
+    if (tiles) {
+        hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)tiles), dim3(kRsTile), 0, st, (const int16_t *)L.cat.p,
+                           (const RsFile *)L.files.p, (int)rsf.size(), srv->taps);
+        HIPCHK(e, hipGetLastError());
+    }
+    HIPCHK(e, hipMemcpyAsync(L.h_res, L.res.p, rneed, hipMemcpyDeviceToHost, st));
+    HIPCHK(e, hipEventRecord(L.done, st));
+    L.cap = pl.cap;
+    L.busy = true;
+    return 0;
+}
+
+// Retire a finished launch: validate each chain, report each request.
+void srv_retire(cuopus_server *srv, SrvLaunch &L)
+{
+    size_t c0 = 0;
+    for (SrvReq *r : L.reqs) {
+        size_t nc = r->fp.chains.size();
+        for (size_t c = c0; c < c0 + nc && !r->rc; c++) {
+            if (L.h_res[c].stop != 0) {
+                r->rc = CUOPUS_RC_SOFT;
+                snprintf(r->err, sizeof(r->err), "chain %zu: device decode error %d", c - c0, L.h_res[c].stop);
+            } else if (L.h_res[c].samples != L.cap[c]) {
+                r->rc = CUOPUS_RC_SOFT;
+                snprintf(r->err, sizeof(r->err), "chain %zu: decoded %u samples, expected %llu", c - c0,
+                         L.h_res[c].samples, (unsigned long long)L.cap[c]);
+            }
+        }
+        c0 += nc;
+        srv->finished.push_back(r);
+    }
+    srv->mem_in_use -= L.mem;
+    L.mem = 0;
+    L.reqs.clear();
+    L.busy = false;
+}
+
+}  // namespace
+
+extern "C" int cuopus_server_create(const cuopus_server_opts *opts, cuopus_server **out, char *errbuf,
+                                    size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!out) return e.set(CUOPUS_RC_ARG, "out is NULL");
+    *out = nullptr;
+    cuopus_server *s = new cuopus_server();
+    memset(&s->opts, 0, sizeof(s->opts));
+    if (opts) s->opts = *opts;
+    if (s->opts.n_streams <= 0) s->opts.n_streams = 4;
+    if (s->opts.mem_budget <= 0) s->opts.mem_budget = 8ll << 30;
+    if (s->opts.max_chains_per_launch <= 0) s->opts.max_chains_per_launch = 1 << 20;
+    s->taps = make_taps();
+    s->launches.resize((size_t)s->opts.n_streams);
+    for (auto &L : s->launches) {
+        hipError_t a = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking);
+        hipError_t b = a == hipSuccess ? hipEventCreateWithFlags(&L.done, hipEventDisableTiming) : a;
+        if (b != hipSuccess) {
+            cuopus_server_destroy(s);
+            return e.set(CUOPUS_RC_GPU, "stream/event: %s", hipGetErrorString(b));
+        }
+    }
+    *out = s;
+    return 0;
+}
+
+extern "C" int cuopus_server_submit(cuopus_server *srv, const uint8_t *ogg, size_t n, const uint8_t *lost,
+                                    size_t mask_n, float *out_dev, size_t out_cap, uint64_t *req_id, char *errbuf,
+                                    size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!srv || !req_id) return e.set(CUOPUS_RC_ARG, "NULL argument");
+    SrvReq *r = new SrvReq();
+    int rc = demux_file(ogg, n, &r->fp, true, e);
+    if (!rc && lost) {
+        if (mask_n != r->fp.d.off.size()) rc = e.set(CUOPUS_RC_ARG, "loss mask size != %zu packets", r->fp.d.off.size());
+        else r->lost.assign(lost, lost + mask_n);
+    }
+    if (!rc && r->fp.out16 > 0 && (!out_dev || (uint64_t)r->fp.out16 > out_cap))
+        rc = e.set(CUOPUS_RC_ARG, "output capacity %zu < %lld", out_cap, (long long)r->fp.out16);
+    if (!rc && r->fp.chains.empty()) rc = e.set(CUOPUS_RC_SOFT, "no chains");
+    if (rc) { delete r; return rc; }
+    r->out = out_dev;
+    r->out16 = r->fp.out16;
+    r->mem = req_device_bytes(r);
+    if (r->mem > (uint64_t)srv->opts.mem_budget) {
+        unsigned long long need = (unsigned long long)r->mem;
+        delete r;
+        return e.set(CUOPUS_RC_ARG, "request needs %llu device bytes, budget is %lld", need,
+                     (long long)srv->opts.mem_budget);
+    }
+    r->id = srv->next_id++;
+    *req_id = r->id;
+    srv->pending.push_back(r);
+    return 0;
+}
+
+extern "C" int cuopus_server_poll(cuopus_server *srv, uint64_t *ids, int *rcs, int cap, int *n_done, char *errbuf,
+                                  size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!srv || !n_done) return e.set(CUOPUS_RC_ARG, "NULL argument");
+    // 1. retire finished launches
+    for (auto &L : srv->launches) {
+        if (!L.busy) continue;
+        hipError_t q = hipEventQuery(L.done);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return e.set(CUOPUS_RC_GPU, "launch: %s", hipGetErrorString(q));
+        srv_retire(srv, L);
+    }
+    // 2. admit queued requests onto free streams while memory allows
+    for (auto &L : srv->launches) {
+        if (L.busy || srv->pending.empty()) continue;
+        uint64_t mem = 0, nch = 0;
+        while (!srv->pending.empty()) {
+            SrvReq *r = srv->pending.front();
+            if (srv->mem_in_use + mem + r->mem > (uint64_t)srv->opts.mem_budget) break;
+            if (!L.reqs.empty() && nch + r->fp.chains.size() > (uint64_t)srv->opts.max_chains_per_launch) break;
+            L.reqs.push_back(r);
+            mem += r->mem;
+            nch += r->fp.chains.size();
+            srv->pending.pop_front();
+        }
+        if (L.reqs.empty()) break;  // head request must wait for memory
+        L.mem = mem;
+        srv->mem_in_use += mem;
+        int rc = srv_launch(srv, L, e);
+        if (rc) {
+            for (SrvReq *r : L.reqs) {
+                r->rc = rc;
+                snprintf(r->err, sizeof(r->err), "%s", errbuf ? errbuf : "launch failed");
+                srv->finished.push_back(r);
+            }
+            srv->mem_in_use -= L.mem;
+            L.mem = 0;
+            L.reqs.clear();
+            if (rc == CUOPUS_RC_GPU) return rc;
+        }
+    }
+    // 3. report finished requests
+    int k = 0;
+    while (k < cap && !srv->finished.empty()) {
+        SrvReq *r = srv->finished.front();
+        srv->finished.pop_front();
+        if (ids) ids[k] = r->id;
+        if (rcs) rcs[k] = r->rc;
+        if (r->rc) e.set(r->rc, "request %llu: %s", (unsigned long long)r->id, r->err);
+        delete r;
+        k++;
+    }
+    *n_done = k;
+    return 0;
+}
+
+extern "C" int cuopus_server_stats(const cuopus_server *srv, int64_t *s4)
+{
+    if (!srv || !s4) return CUOPUS_RC_ARG;
+    int64_t busy = 0;
+    for (auto &L : srv->launches) busy += L.busy;
+    s4[0] = (int64_t)srv->pending.size();
+    s4[1] = busy;
+    s4[2] = (int64_t)srv->mem_in_use;
+    s4[3] = (int64_t)srv->finished.size();
+    return 0;
+}
+
+extern "C" void cuopus_server_destroy(cuopus_server *srv)
+{
+    if (!srv) return;
+    for (auto &L : srv->launches) {
+        if (L.st) hipStreamSynchronize(L.st);
+        DevBuf *all[] = {&L.data, &L.off, &L.len, &L.lost, &L.chains, &L.order, &L.cat, &L.res, &L.files,
+                         &L.queue, &L.heap};
+        for (DevBuf *d : all) {
+            if (d->p) { hipFree(d->p); g_dev_bytes -= (long long)d->cap; }
+            d->p = nullptr;
+            d->cap = 0;
+        }
+        if (L.h_stage) hipHostFree(L.h_stage);
+        if (L.h_res) hipHostFree(L.h_res);
+        for (SrvReq *r : L.reqs) delete r;
+        if (L.done) hipEventDestroy(L.done);
+        if (L.st) hipStreamDestroy(L.st);
+    }
+    for (SrvReq *r : srv->pending) delete r;
+    for (SrvReq *r : srv->finished) delete r;
+    delete srv;
 }
 
 // Cycle accounting of a -DCUO_PROF build: copies and clears the per-scope

@@ -45,6 +45,12 @@ _lib = None
 P = ctypes.c_void_p
 
 
+class ServerOpts(ctypes.Structure):
+    """Mirror of cuopus_server_opts (include/cuopus.h)."""
+    _fields_ = [("n_streams", ctypes.c_int), ("n_waves", ctypes.c_int), ("mem_budget", ctypes.c_longlong),
+                ("max_chains_per_launch", ctypes.c_int)]
+
+
 class BatchOpts(ctypes.Structure):
     """Mirror of cuopus_batch_opts (include/cuopus.h)."""
     _fields_ = [("mode", ctypes.c_int), ("n_slots", ctypes.c_int), ("lpt_order", ctypes.c_int),
@@ -78,6 +84,13 @@ def _load():
         "cuopus_workspace_bytes": (ctypes.c_longlong, []),
         "cuopus_release_workspaces": (None, []),
         "cuopus_prof_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]),
+        "cuopus_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOpts), ctypes.POINTER(P), ctypes.c_char_p, SZ]),
+        "cuopus_server_submit": (ctypes.c_int, [P, ctypes.c_char_p, SZ, P, SZ, P, SZ,
+                                                ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, SZ]),
+        "cuopus_server_poll": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int),
+                                              ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, SZ]),
+        "cuopus_server_stats": (ctypes.c_int, [P, I64P]),
+        "cuopus_server_destroy": (None, [P]),
         "cuopus_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(SZ), ctypes.c_int,
                                                ctypes.POINTER(P), ctypes.POINTER(SZ), ctypes.POINTER(P),
                                                ctypes.POINTER(SZ), ctypes.POINTER(BatchOpts),
@@ -356,6 +369,71 @@ class Batch:
     def close(self):
         if self._h:
             self._lib.cuopus_batch_destroy(self._h)
+            self._h = P(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching decode server (see cuopus_server_* in include/cuopus.h).
+
+    submit(ogg) demuxes on the host, allocates the float32 16 kHz output
+    tensor and queues the request; poll() admits queued requests as streams
+    and device memory free up, and returns [(request id, tensor, rc)] for the
+    requests that finished.  Never blocks.
+    """
+
+    def __init__(self, n_streams=4, n_waves=0, mem_budget=8 << 30, max_chains_per_launch=0):
+        self._lib = _load()
+        self._h = P(0)
+        opts = ServerOpts(n_streams, n_waves, mem_budget, max_chains_per_launch)
+        err = ctypes.create_string_buffer(_ERRCAP)
+        _check(self._lib.cuopus_server_create(ctypes.byref(opts), ctypes.byref(self._h), err, _ERRCAP), err)
+        self._out = {}
+        self._ids = (ctypes.c_uint64 * 256)()
+        self._rcs = (ctypes.c_int * 256)()
+
+    def submit(self, path_or_bytes, lost_mask=None):
+        import torch
+        data = _as_bytes(path_or_bytes)
+        n16 = probe(data)
+        out = torch.empty(n16, dtype=torch.float32, device="cuda")
+        mp, mn = None, 0
+        if lost_mask is not None:
+            m = np.ascontiguousarray(np.asarray(lost_mask, dtype=np.uint8))
+            mp, mn = m.ctypes.data, m.size
+        rid = ctypes.c_uint64(0)
+        err = ctypes.create_string_buffer(_ERRCAP)
+        _check(self._lib.cuopus_server_submit(self._h, data, len(data), mp, mn, out.data_ptr() if n16 else None,
+                                              n16, ctypes.byref(rid), err, _ERRCAP), err)
+        self._out[rid.value] = out
+        return rid.value
+
+    def poll(self):
+        done = []
+        while True:
+            n = ctypes.c_int(0)
+            err = ctypes.create_string_buffer(_ERRCAP)
+            _check(self._lib.cuopus_server_poll(self._h, self._ids, self._rcs, 256, ctypes.byref(n), err, _ERRCAP),
+                   err)
+            for i in range(n.value):
+                rid = int(self._ids[i])
+                done.append((rid, self._out.pop(rid), int(self._rcs[i])))
+            if n.value < 256:
+                return done
+
+    def stats(self):
+        s = (ctypes.c_int64 * 4)()
+        self._lib.cuopus_server_stats(self._h, s)
+        return {"queued": s[0], "in_flight": s[1], "admitted_bytes": s[2], "finished": s[3]}
+
+    def close(self):
+        if self._h:
+            self._lib.cuopus_server_destroy(self._h)
             self._h = P(0)
 
     def __del__(self):

@@ -160,6 +160,37 @@ int cuopus_batch_info(const cuopus_batch *b, int64_t *counts8);
 int cuopus_batch_kernel_ms(const cuopus_batch *b, float *decode_ms, float *resample_ms);
 void cuopus_batch_destroy(cuopus_batch *b);
 
+/* ---------------------------------------------------------------------------
+ * Serving: continuous batching (no counterpart in the reference).  Requests
+ * are demuxed on submit and queue in arrival order; cuopus_server_poll never
+ * blocks: it retires finished launches, admits queued requests into a new
+ * launch on a free stream while their device bytes (packets, chain table,
+ * int16 PCM, f32 output) fit the memory budget, and reports finished
+ * requests.  Launches on different streams run concurrently.  Outputs are
+ * written to the caller's device buffers given at submit; the caller keeps
+ * them alive until the request is reported.  Single-threaded use only.
+ * ------------------------------------------------------------------------- */
+typedef struct cuopus_server cuopus_server;
+typedef struct {
+    int n_streams;              /* launches in flight (0: 4) */
+    int n_waves;                /* persistent wavefronts per launch (0: default) */
+    long long mem_budget;       /* device bytes for admitted requests (0: 8 GiB) */
+    int max_chains_per_launch;  /* 0: unlimited */
+} cuopus_server_opts;
+
+int cuopus_server_create(const cuopus_server_opts *opts, cuopus_server **out, char *errbuf, size_t errcap);
+/* lost/mask_n: optional per-packet loss mask (FEC/PLC rule as the lossy decode). */
+int cuopus_server_submit(cuopus_server *srv, const uint8_t *ogg, size_t n, const uint8_t *lost, size_t mask_n,
+                         float *out_dev, size_t out_cap, uint64_t *req_id, char *errbuf, size_t errcap);
+/* Up to cap finished requests: ids[i] and rcs[i] (0 ok, else the request's
+ * error; errbuf then holds the last failing request's message). */
+int cuopus_server_poll(cuopus_server *srv, uint64_t *ids, int *rcs, int cap, int *n_done, char *errbuf,
+                       size_t errcap);
+/* s4: [0] queued requests [1] launches in flight [2] admitted device bytes
+ * [3] finished, not yet reported */
+int cuopus_server_stats(const cuopus_server *srv, int64_t *s4);
+void cuopus_server_destroy(cuopus_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -176,6 +176,9 @@ BENCH_SET_MODES = {
 BENCH_SETS = {
     # 250 files, durations exp(U(ln 5, ln 1800)) s
     "uneven_250": dict(n=250, seed=0x0C0FFEE3, dmin=5.0, dmax=1800.0),
+    # serving request pool: durations lognormal (median 45 s, sigma 1.0)
+    # clipped to [2 s, 7200 s]
+    "serve_pool": dict(n=60, seed=0x0C0FFEE4, dmin=2.0, dmax=7200.0, median=45.0, sigma=1.0),
 }
 
 
@@ -185,7 +188,11 @@ def bench_set_plan(name):
     modes = sorted(BENCH_SET_MODES)
     for i in range(cfg["n"]):
         rs = np.random.default_rng(cfg["seed"] + i)
-        dur = float(np.exp(rs.uniform(np.log(cfg["dmin"]), np.log(cfg["dmax"]))))
+        if "median" in cfg:
+            dur = float(np.clip(cfg["median"] * np.exp(cfg["sigma"] * rs.standard_normal()), cfg["dmin"],
+                                cfg["dmax"]))
+        else:
+            dur = float(np.exp(rs.uniform(np.log(cfg["dmin"]), np.log(cfg["dmax"]))))
         mode = modes[int(rs.integers(0, len(modes)))]
         args = "--seed %d --dur %.3f --signal speech %s --brange 12000 64000 --frame 960" % (
             cfg["seed"] + i, dur, BENCH_SET_MODES[mode])
