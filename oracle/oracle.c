@@ -195,6 +195,24 @@ void orc_taps(float *out)
 
 /* out[i] = sum_t taps[t] * x[3i - 24 + t] / 32768, zero outside [0, n48),
  * float32 accumulation in tap order (cuopus_kernels.cuh:441-456). */
+/* Outputs [o0, o1) of orc_resample (same arithmetic), so threads can split
+ * one stream. */
+void orc_resample_range(const int16_t *x, int64_t n48, float *out, int64_t o0, int64_t o1)
+{
+    float taps[RS_TAPS];
+    orc_taps(taps);
+    for (int64_t i = o0; i < o1; i++) {
+        int64_t c = 3 * i;
+        float acc = 0.f;
+        for (int t = 0; t < RS_TAPS; t++) {
+            int64_t li = c - RS_PAD + t;
+            float v = (li >= 0 && li < n48) ? (float)x[li] * (1.0f / 32768.0f) : 0.f;
+            acc += taps[t] * v;
+        }
+        out[i] = acc;
+    }
+}
+
 void orc_resample(const int16_t *x, int64_t n48, float *out, int64_t n_out)
 {
     float taps[RS_TAPS];

@@ -38,6 +38,8 @@ def lib():
         L.orc_decode_chains.argtypes = [P, P, P, P, P, P, ctypes.c_int, P, P, P, P, P, P, ctypes.c_int]
         L.orc_resample.restype = None
         L.orc_resample.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64]
+        L.orc_resample_range.restype = None
+        L.orc_resample_range.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64, ctypes.c_int64]
         L.orc_taps.restype = None
         L.orc_taps.argtypes = [P]
         L.orc_ec_schedule.restype = None
@@ -113,6 +115,20 @@ def resample(pcm48):
     out = np.zeros(n16, np.float32)
     if n16:
         lib().orc_resample(pcm48.ctypes.data, len(pcm48), out.ctypes.data, n16)
+    return out
+
+
+def resample_threads(pcm48, threads, chunk=1 << 20):
+    """orc_resample split over threads by output ranges (identical output)."""
+    from concurrent.futures import ThreadPoolExecutor
+    pcm48 = np.ascontiguousarray(pcm48, dtype=np.int16)
+    n16 = odemux.out16_len(len(pcm48))
+    out = np.zeros(n16, np.float32)
+    L = lib()
+    ranges = [(o, min(n16, o + chunk)) for o in range(0, n16, chunk)]
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda r: L.orc_resample_range(pcm48.ctypes.data, len(pcm48), out.ctypes.data, r[0], r[1]),
+                    ranges))
     return out
 
 

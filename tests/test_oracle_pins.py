@@ -146,3 +146,13 @@ def test_chains_cover_playable_exactly():
     for (first, npk, trim, keep) in chains:
         cap = sum(odemux.pkt_samples48(d.packets[first + k]) for k in range(npk))
         assert cap >= trim + keep
+
+
+def test_threaded_resampler_matches_serial():
+    """The CPU baseline's range-split resampler gives orc_resample's output."""
+    rs = np.random.default_rng(21)
+    for n in [0, 47, 48, 49, 1000, 123457]:
+        x = rs.integers(-32768, 32767, size=n).astype(np.int16)
+        a = ol.resample(x)
+        b = ol.resample_threads(x, 4, chunk=1000)
+        assert np.array_equal(a, b)
