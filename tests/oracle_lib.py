@@ -164,6 +164,9 @@ BENCH = os.path.join(ROOT, "corpus", "bench")
 BENCH_RECIPES = {
     "celt_fb_64k_2h": "--seed 0x0C0FFEE2 --dur 7200 --signal music --app lowdelay --bw fb --vbr 1 --bitrate 64000 "
                       "--frameset 120,240,480,960",
+    # the reference's CPU-runnable plain case: 60 s SILK-only WB speech, 16 kbps CBR, 20 ms
+    "silk_wb_16k_60s": "--seed 0x0C0FFEE1 --dur 60 --signal speech --app voip --bw wb --force silk --vbr 0 "
+                       "--bitrate 16000 --frame 960",
 }
 
 
@@ -195,6 +198,10 @@ BENCH_SETS = {
     # serving request pool: durations lognormal (median 45 s, sigma 1.0)
     # clipped to [2 s, 7200 s]
     "serve_pool": dict(n=60, seed=0x0C0FFEE4, dmin=2.0, dmax=7200.0, median=45.0, sigma=1.0),
+    # loss workload: 200 files x 1000 s hybrid FB 20 ms 32 kbps with in-band FEC (200k chains)
+    "hybrid_fec_200k": dict(n=200, seed=0x0C0FFEE5, fixed_dur=1000.0,
+                            args="--signal speech --app voip --bw fb --force hybrid --bitrate 32000 --frame 960 "
+                                 "--fec 1 --loss 2"),
 }
 
 
@@ -204,6 +211,10 @@ def bench_set_plan(name):
     modes = sorted(BENCH_SET_MODES)
     for i in range(cfg["n"]):
         rs = np.random.default_rng(cfg["seed"] + i)
+        if "fixed_dur" in cfg:
+            plan.append(("f%04d" % i, cfg["fixed_dur"], "hybrid_fb_fec",
+                         "--seed %d --dur %.3f %s" % (cfg["seed"] + i, cfg["fixed_dur"], cfg["args"])))
+            continue
         if "median" in cfg:
             dur = float(np.clip(cfg["median"] * np.exp(cfg["sigma"] * rs.standard_normal()), cfg["dmin"],
                                 cfg["dmax"]))

@@ -1,0 +1,96 @@
+"""GPU parity at the benchmark sizes: every packet's sample count and final
+range and every int16 sample of whole workloads equal fixed-point libopus
+1.5.2 (the oracle, decoding the identical chain list on the host's cores), and
+the float32 16 kHz outputs are within 1e-6 of the oracle's resampler.
+
+* silk_wb_16k_60s: the reference's CPU-runnable plain case (60 chains).
+* celt_fb_64k_2h: the headline workload (7200 chains, one launch).
+* uneven_250 / hybrid_fec_200k (2% seeded loss, FEC or concealment): run when
+  their inputs are present in corpus/bench.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+@pytest.fixture(scope="module")
+def cn():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from cuopus_amd import cuopus_native
+    return cuopus_native
+
+
+def _check_stream(cn, blob, lost=None, tag=""):
+    import torch
+    d, chains = ol.demux_chains(blob)
+    data, off, ln = d.blob()
+    print("case scale", tag, "packets", len(off), "chains", len(chains), flush=True)
+    pcm, nsamp, rng, stop, rc = cn.gate_chains(data, off, ln, chains, lost=lost)
+    ref = ol.decode_chains(data, off, ln, chains, lost=lost, threads=THREADS)
+    assert (ref["ret"] >= 0).all()
+    assert rc == 0 and not stop.any()
+    assert np.array_equal(nsamp, ref["nsamp"]), tag
+    bad = np.nonzero(rng != ref["rng"])[0]
+    assert bad.size == 0, (tag, "final range mismatch at", bad[:10])
+    diff = np.nonzero(pcm != ref["pcm"])[0]
+    assert diff.size == 0, (tag, "pcm mismatch at", diff[:10])
+    out = cn.decode_ogg_lossy(blob, lost) if lost is not None else cn.decode_ogg_to_cuda_tensor(blob)
+    torch.cuda.synchronize()
+    want = ol.resample_threads(ref["pcm"], THREADS)
+    got = out.cpu().numpy()
+    assert got.shape == want.shape
+    md = float(np.abs(got - want).max()) if got.size else 0.0
+    print("  f16k max abs diff %.3g over %d outputs" % (md, got.size), flush=True)
+    assert md <= 1e-6
+    return len(chains), len(off)
+
+
+def _bench_file(name):
+    p = os.path.join(ol.BENCH, name + ".opus")
+    if not os.path.exists(p):
+        pytest.skip("%s not present" % p)
+    return open(p, "rb").read()
+
+
+def test_plain_silk_wb_60s(cn):
+    _check_stream(cn, _bench_file("silk_wb_16k_60s"), tag="silk_wb_16k_60s")
+
+
+def test_celt_fb_2h_all_chains(cn):
+    nch, npk = _check_stream(cn, _bench_file("celt_fb_64k_2h"), tag="celt_fb_64k_2h")
+    assert nch == 7200
+
+
+def _set_files(name):
+    files = sorted(glob.glob(os.path.join(ol.BENCH, name, "f*.opus")))
+    if not files:
+        pytest.skip("%s set not present" % name)
+    return files
+
+
+def test_uneven_set_all_files(cn):
+    total = 0
+    for f in _set_files("uneven_250"):
+        total += _check_stream(cn, open(f, "rb").read(), tag=os.path.basename(f))[0]
+    print("uneven_250 chains", total)
+
+
+def test_hybrid_fec_loss_set(cn):
+    total = lost_total = 0
+    for i, f in enumerate(_set_files("hybrid_fec_200k")):
+        blob = open(f, "rb").read()
+        d, _ = ol.demux_chains(blob)
+        lost = ol.loss_mask(len(d.packets), 0.02, seed=0x0C0FFEE5 + i)
+        lost_total += int(lost.sum())
+        total += _check_stream(cn, blob, lost=lost, tag=os.path.basename(f))[0]
+    print("hybrid_fec_200k chains", total, "lost packets", lost_total)
