@@ -67,6 +67,8 @@ def _args():
     ap.add_argument("--mode", default="persistent", choices=["persistent", "one", "plain"])
     ap.add_argument("--slots", type=int, default=0, help="persistent wavefronts (0: tuned default)")
     ap.add_argument("--graph", action="store_true", help="replay the launch sequence as a HIP graph")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner on this workload first and use its best launch shape")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-file comparison")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -250,7 +252,15 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     modes = {"persistent": cn.MODE_PERSISTENT, "one": cn.MODE_ONE_LAUNCH, "plain": cn.MODE_PLAIN}
-    b = cn.Batch(blobs, mode=modes[a.mode], n_slots=a.slots, use_graph=a.graph, profile=not a.graph)
+    mode, slots, lpt = modes[a.mode], a.slots, True
+    tuned = None
+    if a.tune:
+        from cuopus_amd import autotune
+        tuned = autotune.tune(blobs, verbose=True)
+        mode, slots, lpt = tuned["mode"], tuned["n_slots"], tuned["lpt_order"]
+        a.mode = {v: k for k, v in modes.items()}[mode]
+        _log("tuned: %s" % {k: tuned[k] for k in ("mode", "n_slots", "lpt_order", "ms")})
+    b = cn.Batch(blobs, mode=mode, n_slots=slots, lpt_order=lpt, use_graph=a.graph, profile=not a.graph)
     _log("batch ready: %d files, %d chains, %.0f audio-s" % (len(blobs), b.chains, b.playable / 48000.0))
     wall, dec_ms, rs_ms = _time_batch(b, a.steps, a.warmup, torch)
     _log("%s: %.1f ms/step" % (a.mode, 1000.0 * wall / a.steps))
@@ -312,6 +322,7 @@ def main():
         "dtype": "int16",
         "data": "synthetic",
         "config": {"workload": a.workload, "desc": w["desc"], "mode": a.mode, "graph": a.graph,
+                   "n_slots": slots or "default", "tuned": tuned is not None,
                    "files": len(blobs), "audio_s_per_step": round(audio_s, 3)},
         "plain_value": plain,
         "batched_vs_plain": round(value / plain, 3) if plain else None,
