@@ -205,6 +205,7 @@ enum {
     kProfCeltDeemph, kProfSilk, kProfEmit, kProfCeltLost, kProfCeltEnergy, kProfCeltAlloc, kProfCeltTail,
     kProfPvq, kProfCwrsi, kProfExpRot, kProfTheta, kProfLeafFold, kProfBandPre, kProfBandPost, kProfEcUint,
     kProfAllocBisect, kProfAllocTail, kProfQpLeafBits, kProfBandsPrologue, kProfLowbandOut, kProfQpStage,
+    kProfSilkIndices, kProfSilkPulses, kProfSilkParams, kProfSilkCore, kProfSilkResamp, kProfSilkPlc,
     kProfCount
 };
 

@@ -542,6 +542,7 @@ CUO_FN int16_t *silk_iir_fir_interpol(int16_t *out, const int16_t *buf, int32_t 
 
 CUO_FN void silk_resampler_iir_fir(SilkResampler *S, int16_t *out, const int16_t *in, int32_t inLen, int16_t *buf)
 {
+    CUO_PROF_SCOPE(kProfSilkResamp);
     for (int i = 0; i < 8; i++) buf[i] = S->sFIR[i];
     int32_t inc = S->invRatio_Q16;
     int32_t nIn;
@@ -641,6 +642,7 @@ CUO_FN void silk_set_fs(SilkChannel *ch, int fs_kHz, int32_t fs_API_Hz)
 // ------------------------------------------------------------------ bitstream
 CUO_FN void silk_decode_indices(SilkChannel *ch, RangeDec *d, int FrameIndex, int decode_LBRR, int condCoding)
 {
+    CUO_PROF_SCOPE(kProfSilkIndices);
     SilkIndices &ix = ch->indices;
     int Ix;
     if (decode_LBRR || ch->VAD_flags[FrameIndex]) Ix = ec_icdf(d, tab::kTypeOffsetVadIcdf, 8) + 2;
@@ -726,6 +728,7 @@ CUO_FN void silk_shell_decoder(int16_t *p0, RangeDec *d, int pulses4)
 
 CUO_FN void silk_decode_pulses(RangeDec *d, int16_t *pulses, int signalType, int quantOffsetType, int frame_length)
 {
+    CUO_PROF_SCOPE(kProfSilkPulses);
     int32_t sum_pulses[20], nLshifts[20];
     int RateLevelIndex = ec_icdf(d, tab::kRateLevelsIcdf + 9 * (signalType >> 1), 8);
     int iter = frame_length >> 4;
@@ -776,6 +779,7 @@ CUO_FN void silk_decode_pulses(RangeDec *d, int16_t *pulses, int signalType, int
 
 CUO_FN void silk_decode_parameters(SilkChannel *ch, SilkCtrl *ctrl, int condCoding)
 {
+    CUO_PROF_SCOPE(kProfSilkParams);
     int16_t pNLSF_Q15[kSilkMaxOrder], pNLSF0_Q15[kSilkMaxOrder];
     silk_gains_dequant(ctrl->Gains_Q16, ch->indices.GainsIndices, &ch->LastGainIndex, condCoding == CODE_COND,
                        ch->nb_subfr);
@@ -825,6 +829,7 @@ CUO_FN int32_t lpc_pred_q10(const int32_t *s, const int16_t *A, int order)
 // silk_decode_core (decode_core.c:38-237)
 CUO_FN void silk_decode_core(SilkChannel *ch, SilkCtrl *ctrl, int16_t *xq, const int16_t *pulses, SilkScratch *sc)
 {
+    CUO_PROF_SCOPE(kProfSilkCore);
     int16_t *sLTP = sc->sLTP;
     int32_t *sLTP_Q15 = sc->sLTP_Q15;
     int32_t *res_Q14 = sc->res_Q14;
@@ -1085,6 +1090,7 @@ CUO_FN void silk_plc_glue_frames(SilkChannel *ch, int16_t *frame, int length)
 
 CUO_FN void silk_plc(SilkChannel *ch, SilkCtrl *ctrl, int16_t *frame, int lost, SilkScratch *sc)
 {
+    CUO_PROF_SCOPE(kProfSilkPlc);
     if (ch->fs_kHz != ch->plc.fs_kHz) {
         silk_plc_reset(ch);
         ch->plc.fs_kHz = ch->fs_kHz;

@@ -288,7 +288,8 @@ def release_workspaces():
 PROF_SCOPES = ["chain", "frame", "celt", "celt_head", "celt_bands", "celt_synth", "celt_comb", "celt_deemph",
                "silk", "emit", "celt_lost", "celt_energy", "celt_alloc", "celt_tail", "pvq", "cwrsi", "exp_rot",
                "theta", "leaf_fold", "band_pre", "band_post", "ec_uint", "alloc_bisect", "alloc_tail",
-               "qp_leaf_bits", "bands_prologue", "lowband_out", "qp_stage"]
+               "qp_leaf_bits", "bands_prologue", "lowband_out", "qp_stage", "silk_indices", "silk_pulses",
+               "silk_params", "silk_core", "silk_resamp", "silk_plc"]
 
 
 def prof_read():
