@@ -167,13 +167,14 @@ __host__ __device__ inline uint32_t wave_or(uint32_t v) { return v; }
 
 // Optional cycle accounting (built with -DCUO_PROF only): a scope adds the
 // shader clock cycles it spans to g_prof[id], once per wavefront.
+constexpr int kProfSlots = 64;  // one per lane; scope ids must stay below
 #if defined(CUO_PROF)
-__device__ unsigned long long g_prof[32];
+__device__ unsigned long long g_prof[kProfSlots];
 #endif
 #if defined(CUO_PROF) && defined(__HIP_DEVICE_COMPILE__)
 // Per-wavefront sums in LDS (lane 0 adds; one wavefront per workgroup), added
 // to g_prof once per chain by prof_flush().
-__shared__ unsigned long long g_prof_lds[32];
+__shared__ unsigned long long g_prof_lds[kProfSlots];
 struct ProfScope {
     int id;
     unsigned long long t0;
@@ -186,11 +187,11 @@ struct ProfScope {
 };
 __device__ inline void prof_reset()
 {
-    if (lane_id() < 32) g_prof_lds[lane_id()] = 0;
+    g_prof_lds[lane_id()] = 0;
 }
 __device__ inline void prof_flush()
 {
-    if (lane_id() < 32 && g_prof_lds[lane_id()]) atomicAdd(&g_prof[lane_id()], g_prof_lds[lane_id()]);
+    if (g_prof_lds[lane_id()]) atomicAdd(&g_prof[lane_id()], g_prof_lds[lane_id()]);
 }
 #define CUO_PROF_CAT2(a, b) a##b
 #define CUO_PROF_CAT(a, b) CUO_PROF_CAT2(a, b)
@@ -208,6 +209,7 @@ enum {
     kProfSilkIndices, kProfSilkPulses, kProfSilkParams, kProfSilkCore, kProfSilkResamp, kProfSilkPlc,
     kProfCount
 };
+static_assert(kProfCount <= kProfSlots, "profiling scope ids exceed the per-lane slots");
 
 // ---------------------------------------------------------------- basics
 CUO_FN int imin(int a, int b) { return a < b ? a : b; }

@@ -1,6 +1,7 @@
-// demux.cpp -- see demux.h.  Behaviour follows cuopus_demux.h:79-378.
+// demux.cpp -- see demux.h.  Default behaviour follows cuopus_demux.h:79-378.
 #include "demux.h"
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -24,38 +25,128 @@ int pkt_samples48(const uint8_t *p, uint32_t len)
 }
 
 namespace {
+struct CrcTable {
+    uint32_t t[256];
+    CrcTable()
+    {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t r = i << 24;
+            for (int k = 0; k < 8; k++) r = (r & 0x80000000u) ? (r << 1) ^ 0x04C11DB7u : (r << 1);
+            t[i] = r;
+        }
+    }
+};
+const CrcTable kCrc;
+}  // namespace
+
+uint32_t ogg_crc(const uint8_t *p, size_t n, uint32_t crc)
+{
+    for (size_t i = 0; i < n; i++) crc = (crc << 8) ^ kCrc.t[((crc >> 24) ^ p[i]) & 0xFF];
+    return crc;
+}
+
+DemuxOpts demux_opts_from_env()
+{
+    DemuxOpts o;
+    const char *c = getenv("CUOPUS_CHECK_CRC");
+    o.check_crc = c && *c && strcmp(c, "0") != 0;
+    const char *g = getenv("CUOPUS_GRANULE");
+    o.strict = !(g && (!strcmp(g, "warn") || !strcmp(g, "lenient")));
+    return o;
+}
+
+namespace {
 struct Span {
     uint64_t off;
     uint32_t len;
-    uint32_t page;
     bool done;  // ends in a lacing value < 255 (packet completes here)
 };
+
+struct Page {
+    uint64_t off;        // byte offset of the capture pattern
+    int64_t granule;
+    uint32_t serial, seq;
+    uint8_t flags;       // 1 continued, 2 BOS, 4 EOS
+    uint32_t span0, nspan;
+};
+
+// A packet assembled from spans of consecutive pages of one stream: its
+// pieces are parts[part0, part0 + nparts) of the link.
+struct Packet {
+    uint32_t part0 = 0, nparts = 0;
+    uint32_t len = 0;
+    uint32_t page = 0;  // page on which it completes
+};
+
+struct LinkBuild {
+    uint32_t serial = 0;
+    bool eos = false;
+    bool have_seq = false;
+    uint32_t last_seq = 0;
+    std::vector<Packet> packets;  // header packets included
+    std::vector<std::pair<uint64_t, uint32_t>> parts;  // (file offset, length)
+    Packet partial;
+    bool has_partial = false;
+    std::vector<uint32_t> pages;  // page indices of this link
+};
+
+bool starts_opus_head(const uint8_t *buf, const std::vector<Span> &spans, const Page &pg)
+{
+    if ((pg.flags & 1) || pg.nspan == 0) return false;
+    const Span &s = spans[pg.span0];
+    return s.len >= 19 && memcmp(buf + s.off, "OpusHead", 8) == 0;
+}
 }  // namespace
 
-bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err)
+bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const DemuxOpts &opts)
 {
     *d = Demuxed();
     if (!buf || n == 0) { *err = "empty stream"; return false; }
+    DemuxStats &st = d->stats;
+
+    // ---- pages
     std::vector<Span> spans;
-    std::vector<int64_t> page_granule;
+    std::vector<Page> pages;
     size_t pos = 0;
     static const uint8_t kCapture[4] = {'O', 'g', 'g', 'S'};
     while (pos < n) {
         const uint8_t *hit = std::search(buf + pos, buf + n, kCapture, kCapture + 4);
-        if (hit == buf + n) break;
+        if (hit == buf + n) { st.skipped_bytes += n - pos; break; }
         size_t idx = (size_t)(hit - buf);
-        if (idx + 27 > n) break;
-        int64_t granule;
-        memcpy(&granule, buf + idx + 6, 8);
+        st.skipped_bytes += idx - pos;
+        if (idx + 27 > n) { st.skipped_bytes += n - idx; break; }
         int nsegs = buf[idx + 26];
         const uint8_t *seg = buf + idx + 27;
         size_t body = idx + 27 + (size_t)nsegs;
-        if (body > n) break;
+        if (body > n) { st.skipped_bytes += n - idx; break; }
         uint32_t body_len = 0;
         for (int i = 0; i < nsegs; i++) body_len += seg[i];
-        if (body + body_len > n) break;
-        uint32_t pg = (uint32_t)page_granule.size();
-        page_granule.push_back(granule);
+        if (body + body_len > n) { st.skipped_bytes += n - idx; break; }
+        Page pg;
+        pg.off = idx;
+        memcpy(&pg.granule, buf + idx + 6, 8);
+        memcpy(&pg.serial, buf + idx + 14, 4);
+        memcpy(&pg.seq, buf + idx + 18, 4);
+        pg.flags = buf[idx + 5];
+        if (opts.check_crc) {
+            uint32_t want;
+            memcpy(&want, buf + idx + 22, 4);
+            static const uint8_t kZero[4] = {0, 0, 0, 0};
+            uint32_t c = ogg_crc(buf + idx, 22);
+            c = ogg_crc(kZero, 4, c);
+            c = ogg_crc(buf + idx + 26, 1 + (size_t)nsegs + body_len, c);
+            if (c != want || buf[idx + 4] != 0) {
+                st.crc_bad++;
+                if (opts.strict) {
+                    *err = "page at byte " + std::to_string(idx) + ": CRC mismatch";
+                    return false;
+                }
+                st.skipped_bytes += 4;
+                pos = idx + 4;  // resync at the next capture pattern
+                continue;
+            }
+        }
+        pg.span0 = (uint32_t)spans.size();
         uint64_t run_start = body;
         uint32_t run_len = 0;
         int last = 0;
@@ -63,48 +154,87 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err)
             run_len += seg[i];
             last = seg[i];
             if (seg[i] < 255) {
-                spans.push_back({run_start, run_len, pg, true});
+                spans.push_back({run_start, run_len, true});
                 run_start += run_len;
                 run_len = 0;
             }
         }
-        if (run_len > 0 || (nsegs > 0 && last == 255)) spans.push_back({run_start, run_len, pg, false});
+        if (run_len > 0 || (nsegs > 0 && last == 255)) spans.push_back({run_start, run_len, false});
+        pg.nspan = (uint32_t)spans.size() - pg.span0;
+        pages.push_back(pg);
         pos = body + body_len;
     }
-    if (page_granule.empty()) { *err = "no OGG pages found"; return false; }
-    d->n_pages = (uint32_t)page_granule.size();
+    if (pages.empty()) { *err = "no OGG pages found"; return false; }
 
-    // reassemble packets (a trailing partial packet is dropped)
-    std::vector<uint64_t> pk_begin_span;
-    std::vector<uint32_t> pk_len, pk_page;
-    {
-        uint64_t acc = 0;
-        size_t first_span = 0;
-        for (size_t i = 0; i < spans.size(); i++) {
-            acc += spans[i].len;
-            if (spans[i].done) {
-                pk_begin_span.push_back(first_span);
-                pk_len.push_back((uint32_t)acc);
-                pk_page.push_back(spans[i].page);
-                acc = 0;
-                first_span = i + 1;
+    // ---- logical streams and packets
+    std::vector<LinkBuild> links;
+    for (uint32_t pi = 0; pi < (uint32_t)pages.size(); pi++) {
+        const Page &pg = pages[pi];
+        const bool head = starts_opus_head(buf, spans, pg);
+        LinkBuild *L = links.empty() ? nullptr : &links.back();
+        if (head && (!L || ((pg.flags & 2) && (L->eos || L->packets.size() > 2)))) {
+            // first Opus stream, or a new link of a chained stream
+            if (L && L->has_partial) st.dropped_packets++;
+            links.emplace_back();
+            L = &links.back();
+            L->serial = pg.serial;
+        } else if (!L || pg.serial != L->serial) {
+            st.foreign_pages++;
+            continue;
+        }
+        L->pages.push_back(pi);
+        st.pages++;
+        if (L->has_partial && (!(pg.flags & 1) || (L->have_seq && pg.seq != L->last_seq + 1))) {
+            L->has_partial = false;  // its continuation is missing
+            L->parts.resize(L->partial.part0);
+            L->partial = Packet();
+            st.dropped_packets++;
+        }
+        L->have_seq = true;
+        L->last_seq = pg.seq;
+        uint32_t s = pg.span0, se = pg.span0 + pg.nspan;
+        if ((pg.flags & 1) && !L->has_partial && s < se) {
+            // continuation of a packet whose start is missing
+            while (s < se && !spans[s].done) s++;
+            if (s < se) { s++; st.dropped_packets++; }
+        }
+        for (; s < se; s++) {
+            Packet &p = L->partial;
+            if (!L->has_partial) p.part0 = (uint32_t)L->parts.size();
+            L->parts.push_back({spans[s].off, spans[s].len});
+            p.nparts++;
+            p.len += spans[s].len;
+            if (spans[s].done) {
+                p.page = pi;
+                L->packets.push_back(p);
+                L->partial = Packet();
+                L->has_partial = false;
+            } else {
+                L->has_partial = true;
             }
         }
+        if (pg.flags & 4) L->eos = true;
     }
-    size_t n_packets = pk_len.size();
-    if (n_packets < 2) { *err = "not an OGG Opus stream (fewer than 2 packets)"; return false; }
-    // the OpusHead header fits in its first span for any valid stream
-    const Span &h0 = spans[pk_begin_span[0]];
-    if (pk_len[0] < 19 || h0.len < 19 || memcmp(buf + h0.off, "OpusHead", 8) != 0) {
-        *err = "not an OGG Opus stream (no OpusHead)";
+    for (auto &L : links)
+        if (L.has_partial) st.dropped_packets++;  // trailing partial packet
+    d->n_pages = (uint32_t)pages.size();
+    if (links.empty()) {
+        // mirror the reference's messages for streams without an Opus head
+        size_t np = 0;
+        for (const Span &s : spans) np += s.done;
+        *err = np < 2 ? "not an OGG Opus stream (fewer than 2 packets)" : "not an OGG Opus stream (no OpusHead)";
         return false;
     }
-    d->channels = buf[h0.off + 9];
-    d->preskip = buf[h0.off + 10] | (buf[h0.off + 11] << 8);
+    if (links[0].packets.size() < 2) { *err = "not an OGG Opus stream (fewer than 2 packets)"; return false; }
 
-    size_t n_audio = n_packets - 2;
+    // ---- packet table
     uint64_t audio_bytes = 0;
-    for (size_t i = 2; i < n_packets; i++) audio_bytes += pk_len[i];
+    size_t n_audio = 0;
+    for (auto &L : links)
+        for (size_t i = 2; i < L.packets.size(); i++) {
+            audio_bytes += L.packets[i].len;
+            n_audio++;
+        }
     if (audio_bytes >= (1ull << 32)) { *err = "stream too large (>= 4 GiB of packets)"; return false; }
     d->data.resize(audio_bytes ? audio_bytes : 1);
     d->off.resize(n_audio);
@@ -112,48 +242,150 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err)
     d->sample_off.resize(n_audio);
     std::vector<uint32_t> pkt_page(n_audio);
     uint64_t w = 0;
-    for (size_t k = 0; k < n_audio; k++) {
-        size_t p = k + 2;
-        d->off[k] = (uint32_t)w;
-        d->len[k] = pk_len[p];
-        pkt_page[k] = pk_page[p];
-        uint64_t remaining = pk_len[p];
-        for (size_t s = pk_begin_span[p]; remaining > 0; s++) {
-            memcpy(d->data.data() + w, buf + spans[s].off, spans[s].len);
-            w += spans[s].len;
-            remaining -= spans[s].len;
+    size_t k = 0;
+    for (size_t li = 0; li < links.size(); li++) {
+        LinkBuild &LB = links[li];
+        Link lk;
+        lk.serial = LB.serial;
+        const Packet &h = LB.packets[0];
+        if (LB.packets.size() < 2 || h.nparts == 0 || LB.parts[h.part0].second < 19) {
+            // a link cut short before its headers completed carries no audio
+            if (li == 0) { *err = "not an OGG Opus stream (no OpusHead)"; return false; }
+            st.dropped_packets += (uint32_t)LB.packets.size();
+            continue;
         }
-    }
+        const uint8_t *hp = buf + LB.parts[h.part0].first;
+        lk.channels = hp[9];
+        lk.preskip = hp[10] | (hp[11] << 8);
+        lk.first = (uint32_t)k;
+        int64_t cum = 0;
+        for (size_t i = 2; i < LB.packets.size(); i++, k++) {
+            const Packet &p = LB.packets[i];
+            d->off[k] = (uint32_t)w;
+            d->len[k] = p.len;
+            pkt_page[k] = p.page;
+            for (uint32_t j = p.part0; j < p.part0 + p.nparts; j++) {
+                memcpy(d->data.data() + w, buf + LB.parts[j].first, LB.parts[j].second);
+                w += LB.parts[j].second;
+            }
+            d->sample_off[k] = cum;
+            cum += pkt_samples48(d->data.data() + d->off[k], d->len[k]);
+        }
+        lk.npkts = (uint32_t)k - lk.first;
+        lk.toc_total = cum;
 
-    // per-packet sample offsets and the per-page granule cross-check
-    int64_t cum = 0;
-    for (size_t k = 0; k < n_audio; k++) {
-        d->sample_off[k] = cum;
-        cum += pkt_samples48(d->data.data() + d->off[k], d->len[k]);
+        // ---- granule cross-check over this link's pages
+        // page -> end sample (link timeline) of the last packet completing on it
+        std::vector<std::pair<uint32_t, int64_t>> page_end;  // (page, end) in page order
+        int64_t page_begin_final = 0;
+        for (uint32_t q = lk.first; q < lk.first + lk.npkts; q++) {
+            int64_t end = d->sample_off[q] + pkt_samples48(d->data.data() + d->off[q], d->len[q]);
+            if (page_end.empty() || page_end.back().first != pkt_page[q]) {
+                page_begin_final = d->sample_off[q];
+                page_end.push_back({pkt_page[q], end});
+            } else {
+                page_end.back().second = end;
+            }
+        }
+        const int final_page = page_end.empty() ? -1 : (int)page_end.back().first;
+        const int last_dur = lk.npkts ? pkt_samples48(d->data.data() + d->off[lk.first + lk.npkts - 1],
+                                                      d->len[lk.first + lk.npkts - 1])
+                                      : 0;
+        int64_t last_granule = 0;
+        for (uint32_t pi : LB.pages)
+            if (pages[pi].granule != -1 && pages[pi].granule > last_granule) last_granule = pages[pi].granule;
+        lk.last_granule = last_granule;
+        uint32_t checked = 0, match = 0;
+        int64_t base = 0;  // lenient: granule minus TOC time, re-based at each mismatch
+        int64_t end_trim = 0;
+        for (auto &pe : page_end) {
+            int64_t g = pages[pe.first].granule;
+            if (g == -1) continue;
+            checked++;
+            const bool is_final = (int)pe.first == final_page;
+            if (g == pe.second) match++;
+            else if (is_final && pe.second - g >= 0 && pe.second - g < last_dur) match++;
+            if (opts.strict) continue;
+            int64_t want = pe.second + base;
+            if (g == want) continue;
+            if (is_final && g < want && want - g <= pe.second - page_begin_final) {
+                end_trim = want - g;
+                continue;
+            }
+            st.granule_warnings++;
+            if (!is_final) base = g - pe.second;
+        }
+        d->granule_checked += checked;
+        d->granule_match += match;
+        if (opts.strict) {
+            lk.play48 = last_granule - lk.preskip;
+        } else {
+            lk.end_trim = end_trim;
+            lk.play48 = lk.toc_total - end_trim - lk.preskip;
+            if (lk.play48 < 0) lk.play48 = 0;
+        }
+        d->links.push_back(lk);
     }
-    d->toc_total = cum;
-    std::vector<int64_t> page_end(page_granule.size(), -1);
-    int64_t final_data_page = -1;
-    for (size_t k = 0; k < n_audio; k++) {
-        uint32_t pg = pkt_page[k];
-        page_end[pg] = d->sample_off[k] + pkt_samples48(d->data.data() + d->off[k], d->len[k]);
-        final_data_page = std::max<int64_t>(final_data_page, pg);
+    if (opts.strict && d->granule_match != d->granule_checked) {
+        *err = "granule cross-check failed (" + std::to_string(d->granule_match) + "/" +
+               std::to_string(d->granule_checked) + ")";
+        return false;
     }
-    int last_dur = n_audio ? pkt_samples48(d->data.data() + d->off[n_audio - 1], d->len[n_audio - 1]) : 0;
-    uint32_t checked = 0, match = 0;
-    int64_t last_granule = 0;
-    for (size_t pg = 0; pg < page_granule.size(); pg++) {
-        int64_t g = page_granule[pg];
-        if (g != -1 && g > last_granule) last_granule = g;
-        if (g == -1 || page_end[pg] < 0) continue;
-        checked++;
-        if (g == page_end[pg]) match++;
-        else if ((int64_t)pg == final_data_page && page_end[pg] - g >= 0 && page_end[pg] - g < last_dur) match++;
+    const Link &l0 = d->links[0];
+    d->channels = l0.channels;
+    d->preskip = l0.preskip;
+    d->last_granule = l0.last_granule;
+    d->toc_total = 0;
+    d->pcm_samples_48k = 0;
+    for (const Link &lk : d->links) {
+        if (lk.channels != d->channels) { *err = "channel count changes between chained links"; return false; }
+        d->toc_total += lk.toc_total;
+        d->pcm_samples_48k += lk.play48 > 0 ? lk.play48 : 0;
     }
-    d->granule_checked = checked;
-    d->granule_match = match;
-    d->last_granule = last_granule;
-    d->pcm_samples_48k = last_granule - d->preskip;
+    if (d->links.size() == 1) d->pcm_samples_48k = l0.play48;  // may be <= 0 (nothing playable)
+    return true;
+}
+
+// Chains of one link: packets [lk.first, lk.first + lk.npkts), link timeline.
+static bool link_chains(const Demuxed &d, const Link &lk, int64_t step, int64_t preroll, std::vector<Chain> *out,
+                        std::string *err)
+{
+    const int64_t total_play = lk.play48;
+    if (total_play <= 0) return true;
+    const size_t n_pk = lk.npkts;
+    if (n_pk == 0) { *err = "no audio packets"; return false; }
+    const int64_t *so = d.sample_off.data() + lk.first;
+    std::vector<int64_t> cum(n_pk + 1, 0);
+    for (size_t k = 0; k < n_pk; k++)
+        cum[k + 1] = cum[k] + pkt_samples48(d.data.data() + d.off[lk.first + k], d.len[lk.first + k]);
+    int64_t kept = 0;
+    for (int64_t k = 0; k * step < total_play; k++) {
+        int64_t play_start = k * step;
+        int64_t play_end = std::min((k + 1) * step, total_play);
+        int64_t toc_start = play_start + lk.preskip;
+        int64_t toc_end = play_end + lk.preskip;
+        int64_t start_target = std::max<int64_t>(0, toc_start - preroll);
+        // first packet whose span reaches past the preroll start
+        int64_t first = (int64_t)(std::upper_bound(so, so + n_pk, start_target) - so) - 1;
+        if (first < 0) first = 0;
+        if (cum[first + 1] <= start_target) first++;
+        // last packet starting before the window end
+        int64_t last = (int64_t)(std::lower_bound(so, so + n_pk, toc_end) - so) - 1;
+        if (last < first) last = first;
+        if (first >= (int64_t)n_pk) { *err = "chain starts past the last packet"; return false; }
+        int64_t trim = toc_start - so[first];
+        int64_t keep = play_end - play_start;
+        if (trim < 0) { *err = "negative trim"; return false; }
+        int64_t capn = cum[last + 1] - cum[first];
+        if (capn < trim + keep) {
+            *err = "chain " + std::to_string(out->size()) + ": capacity " + std::to_string(capn) + " < trim+keep";
+            return false;
+        }
+        out->push_back({(uint32_t)(lk.first + first), (uint32_t)(last - first + 1), (uint32_t)trim,
+                        (uint32_t)keep});
+        kept += keep;
+    }
+    if (kept != total_play) { *err = "internal: sum(keep) != playable"; return false; }
     return true;
 }
 
@@ -162,41 +394,11 @@ bool build_chains(const Demuxed &d, double seconds, double preroll_ms, std::vect
     out->clear();
     const int64_t step = (int64_t)(seconds * 48000);
     const int64_t preroll = (int64_t)(preroll_ms * 48);
-    const int64_t total_play = d.pcm_samples_48k;
-    const size_t n_pk = d.off.size();
-    if (total_play <= 0) { *err = "no playable samples"; return false; }
+    if (d.pcm_samples_48k <= 0) { *err = "no playable samples"; return false; }
     if (step <= 0) { *err = "bad segment length"; return false; }
-    if (n_pk == 0) { *err = "no audio packets"; return false; }
-    std::vector<int64_t> cum(n_pk + 1, 0);
-    for (size_t k = 0; k < n_pk; k++) cum[k + 1] = cum[k] + pkt_samples48(d.data.data() + d.off[k], d.len[k]);
-    const std::vector<int64_t> &so = d.sample_off;
-    int64_t kept = 0;
-    for (int64_t k = 0; k * step < total_play; k++) {
-        int64_t play_start = k * step;
-        int64_t play_end = std::min((k + 1) * step, total_play);
-        int64_t toc_start = play_start + d.preskip;
-        int64_t toc_end = play_end + d.preskip;
-        int64_t start_target = std::max<int64_t>(0, toc_start - preroll);
-        // first packet whose span reaches past the preroll start
-        int64_t first = (int64_t)(std::upper_bound(so.begin(), so.end(), start_target) - so.begin()) - 1;
-        if (first < 0) first = 0;
-        if (cum[first + 1] <= start_target) first++;
-        // last packet starting before the window end
-        int64_t last = (int64_t)(std::lower_bound(so.begin(), so.end(), toc_end) - so.begin()) - 1;
-        if (last < first) last = first;
-        if (first >= (int64_t)n_pk) { *err = "chain starts past the last packet"; return false; }
-        int64_t trim = toc_start - so[first];
-        int64_t keep = play_end - play_start;
-        if (trim < 0) { *err = "negative trim"; return false; }
-        int64_t capn = cum[last + 1] - cum[first];
-        if (capn < trim + keep) {
-            *err = "chain " + std::to_string(k) + ": capacity " + std::to_string(capn) + " < trim+keep";
-            return false;
-        }
-        out->push_back({(uint32_t)first, (uint32_t)(last - first + 1), (uint32_t)trim, (uint32_t)keep});
-        kept += keep;
-    }
-    if (kept != total_play) { *err = "internal: sum(keep) != playable"; return false; }
+    if (d.off.empty()) { *err = "no audio packets"; return false; }
+    for (const Link &lk : d.links)
+        if (!link_chains(d, lk, step, preroll, out, err)) return false;
     return true;
 }
 

@@ -1,12 +1,33 @@
 // demux.h -- host-side Ogg/Opus demuxer and chain builder.
 //
-// Same contract as the reference's cu_demux_mem / cu_build_chains2
+// Default behaviour is the reference's cu_demux_mem / cu_build_chains2
 // (cuopus_demux.h:79-269, 293-378): granule-exact packet index (TOC durations
 // cross-checked against every page granule, RFC 7845 end-trim on the final
 // data page), OpusHead channels/pre-skip, and the chain table that cuts the
 // playable timeline into fixed windows decoded independently after a decoder
-// preroll.  Written as C++ over std::vector; errors return false with a
-// message, never abort.
+// preroll.  On streams the reference accepts the packet index and chain table
+// are identical.  Beyond it (the reference's known limitation, README.md:
+// 232-235, and its unimplemented CUOPUS_CHECK_CRC, BENCHMARKS.md:240):
+//
+//  * logical streams: pages of other serial numbers (multiplexed video or
+//    metadata streams) are skipped; a new Opus BOS page after audio has
+//    started opens a new link (chained stream, RFC 7845 section 3).  Each link
+//    has its own pre-skip and timeline; chains never cross a link, so the
+//    decoded 48 kHz PCM is the links' playable PCM back to back.
+//  * reassembly follows the continuation flag and page sequence numbers: a
+//    packet whose pages are not all present is dropped, never spliced.
+//  * check_crc: page CRC-32 (polynomial 0x04C11DB7, RFC 3533) verified; a bad
+//    page is an error under the strict policy and skipped (resync at the next
+//    capture pattern) under the lenient one.
+//  * lenient granule policy: mismatching page granules are counted as
+//    warnings instead of failing the stream; the timeline re-bases at each
+//    mismatch (so streams that start at a non-zero granule, or skip granules,
+//    decode), end-trim is taken from the final page when it lies within that
+//    page, and the playable length is the TOC total minus pre-skip and
+//    end-trim.  The per-chain capacity checks still apply.
+//
+// Written as C++ over std::vector; errors return false with a message, never
+// abort.
 #pragma once
 #include <stdint.h>
 #include <string>
@@ -14,17 +35,48 @@
 
 namespace cuo_host {
 
+struct DemuxOpts {
+    bool check_crc = false;   // CUOPUS_CHECK_CRC=1
+    bool strict = true;       // CUOPUS_GRANULE=strict (default) | warn
+};
+
+// Options from the environment (CUOPUS_CHECK_CRC, CUOPUS_GRANULE).
+DemuxOpts demux_opts_from_env();
+
+// One logical Opus stream of a (possibly chained) physical stream.
+struct Link {
+    uint32_t first = 0, npkts = 0;  // audio packet range in Demuxed::off/len
+    int channels = 0;
+    int preskip = 0;
+    uint32_t serial = 0;
+    int64_t toc_total = 0;          // sum of TOC durations of its packets
+    int64_t last_granule = 0;
+    int64_t end_trim = 0;
+    int64_t play48 = 0;             // playable samples of this link
+};
+
+struct DemuxStats {
+    uint32_t pages = 0;             // pages of the Opus stream(s) used
+    uint32_t crc_bad = 0;           // pages failing the CRC (check_crc only)
+    uint32_t foreign_pages = 0;     // pages of other logical streams
+    uint32_t dropped_packets = 0;   // incomplete packets (missing pages, truncation)
+    uint32_t granule_warnings = 0;  // mismatching granules accepted by the lenient policy
+    uint64_t skipped_bytes = 0;     // bytes outside accepted pages (garbage, bad pages)
+};
+
 struct Demuxed {
     std::vector<uint8_t> data;      // audio packets, contiguous
     std::vector<uint32_t> off, len; // per audio packet
-    std::vector<int64_t> sample_off;
+    std::vector<int64_t> sample_off;  // per packet, from the start of its link
+    std::vector<Link> links;
     int64_t toc_total = 0;
     int channels = 0;
-    int preskip = 0;
-    int64_t last_granule = 0;
-    int64_t pcm_samples_48k = 0;    // playable samples
+    int preskip = 0;                // of the first link
+    int64_t last_granule = 0;       // of the first link
+    int64_t pcm_samples_48k = 0;    // playable samples, all links
     uint32_t n_pages = 0;
     uint32_t granule_checked = 0, granule_match = 0;
+    DemuxStats stats;
 };
 
 struct Chain {
@@ -34,7 +86,11 @@ struct Chain {
 // Duration in 48 kHz samples of one Opus packet from its TOC (RFC 6716 3.1).
 int pkt_samples48(const uint8_t *p, uint32_t len);
 
-bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err);
+// Ogg page CRC-32: polynomial 0x04C11DB7, initial value 0, not reflected, no
+// final xor (RFC 3533 section 6; ogg_index.py:22-35).
+uint32_t ogg_crc(const uint8_t *p, size_t n, uint32_t crc = 0);
+
+bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const DemuxOpts &opts = DemuxOpts());
 
 bool build_chains(const Demuxed &d, double seconds, double preroll_ms, std::vector<Chain> *out,
                   std::string *err);

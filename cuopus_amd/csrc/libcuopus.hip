@@ -165,10 +165,9 @@ int demux_file(const uint8_t *ogg, size_t n, FilePlan *fp, bool with_chains, Err
 {
     std::string msg;
     if (!ogg || !n) return e.set(CUOPUS_RC_ARG, "empty input");
-    if (!cuo_host::demux(ogg, n, &fp->d, &msg)) return e.set(CUOPUS_RC_SOFT, "%s", msg.c_str());
+    if (!cuo_host::demux(ogg, n, &fp->d, &msg, cuo_host::demux_opts_from_env()))
+        return e.set(CUOPUS_RC_SOFT, "%s", msg.c_str());
     if (fp->d.channels != 1) return e.set(CUOPUS_RC_SOFT, "only mono supported (channels=%d)", fp->d.channels);
-    if (fp->d.granule_match != fp->d.granule_checked)
-        return e.set(CUOPUS_RC_SOFT, "granule cross-check failed (%u/%u)", fp->d.granule_match, fp->d.granule_checked);
     if (with_chains) {
         double seg_s = env_f("CUOPUS_SEG_S", 1.0);
         double preroll_ms = env_f("CUOPUS_PREROLL_MS", 250.0);
@@ -447,6 +446,50 @@ extern "C" int cuopus_dump_chains(const uint8_t *ogg, size_t n, uint32_t *out4, 
         out4[4 * i + 2] = fp.chains[i].trim;
         out4[4 * i + 3] = fp.chains[i].keep;
     }
+    return 0;
+}
+
+extern "C" int cuopus_dump_packets(const uint8_t *ogg, size_t n, uint8_t *data, size_t data_cap, uint32_t *lens,
+                                   size_t cap_packets, int64_t *n_packets, int64_t *n_bytes, char *errbuf,
+                                   size_t errcap)
+{
+    Err e(errbuf, errcap);
+    FilePlan fp;
+    int rc = demux_file(ogg, n, &fp, false, e);
+    if (rc) return rc;
+    const size_t np = fp.d.off.size();
+    const size_t nb = np ? (size_t)fp.d.off[np - 1] + fp.d.len[np - 1] : 0;
+    if (n_packets) *n_packets = (int64_t)np;
+    if (n_bytes) *n_bytes = (int64_t)nb;
+    if (lens) {
+        if (cap_packets < np) return e.set(CUOPUS_RC_ARG, "packet capacity %zu < %zu", cap_packets, np);
+        memcpy(lens, fp.d.len.data(), np * sizeof(uint32_t));
+    }
+    if (data) {
+        if (data_cap < nb) return e.set(CUOPUS_RC_ARG, "byte capacity %zu < %zu", data_cap, nb);
+        memcpy(data, fp.d.data.data(), nb);
+    }
+    return 0;
+}
+
+extern "C" int cuopus_demux_stats(const uint8_t *ogg, size_t n, int64_t *s10, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!s10) return e.set(CUOPUS_RC_ARG, "null output");
+    FilePlan fp;
+    int rc = demux_file(ogg, n, &fp, false, e);
+    if (rc) return rc;
+    const cuo_host::DemuxStats &st = fp.d.stats;
+    s10[0] = st.pages;
+    s10[1] = (int64_t)fp.d.links.size();
+    s10[2] = st.crc_bad;
+    s10[3] = st.foreign_pages;
+    s10[4] = st.dropped_packets;
+    s10[5] = st.granule_warnings;
+    s10[6] = fp.d.granule_checked;
+    s10[7] = (int64_t)st.skipped_bytes;
+    s10[8] = (int64_t)fp.d.off.size();
+    s10[9] = fp.d.pcm_samples_48k;
     return 0;
 }
 
@@ -1327,9 +1370,9 @@ extern "C" void cuopus_server_destroy(cuopus_server *srv)
 extern "C" int cuopus_prof_read(unsigned long long *out, int cap)
 {
 #if defined(CUO_PROF)
-    unsigned long long h[32];
+    unsigned long long h[kProfSlots];
     if (hipMemcpyFromSymbol(h, HIP_SYMBOL(g_prof), sizeof(h)) != hipSuccess) return -1;
-    unsigned long long z[32] = {0};
+    unsigned long long z[kProfSlots] = {0};
     if (hipMemcpyToSymbol(HIP_SYMBOL(g_prof), z, sizeof(z)) != hipSuccess) return -1;
     for (int i = 0; i < cap && i < kProfCount; i++) out[i] = h[i];
     return kProfCount;

@@ -79,6 +79,8 @@ def _load():
                                               P, P, P, P, P, ctypes.c_char_p, SZ]),
         "cuopus_resample_s16_to_f16k": (ctypes.c_int, [P, ctypes.c_int64, P, SZ, I64P, P, ctypes.c_char_p, SZ]),
         "cuopus_dump_chains": (ctypes.c_int, [ctypes.c_char_p, SZ, P, SZ, I64P, ctypes.c_char_p, SZ]),
+        "cuopus_dump_packets": (ctypes.c_int, [ctypes.c_char_p, SZ, P, SZ, P, SZ, I64P, I64P, ctypes.c_char_p, SZ]),
+        "cuopus_demux_stats": (ctypes.c_int, [ctypes.c_char_p, SZ, P, ctypes.c_char_p, SZ]),
         "cuopus_warmup": (ctypes.c_int, [ctypes.c_char_p, SZ]),
         "cuopus_last_arena_overflow": (ctypes.c_int, []),
         "cuopus_workspace_bytes": (ctypes.c_longlong, []),
@@ -152,6 +154,58 @@ def chain_table(path_or_bytes):
     out = np.zeros((n.value, 4), np.uint32)
     _check(lib.cuopus_dump_chains(data, len(data), out.ctypes.data, n.value, ctypes.byref(n), err, _ERRCAP), err)
     return out
+
+
+def packets(path_or_bytes):
+    """(data uint8, lens uint32) of the audio packets (host only), the
+    reference CLI's --dump-blob payload."""
+    lib = _load()
+    data = _as_bytes(path_or_bytes)
+    npk, nb = ctypes.c_int64(0), ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_dump_packets(data, len(data), None, 0, None, 0, ctypes.byref(npk), ctypes.byref(nb), err,
+                                   _ERRCAP), err)
+    buf = np.zeros(max(1, nb.value), np.uint8)
+    lens = np.zeros(max(1, npk.value), np.uint32)
+    _check(lib.cuopus_dump_packets(data, len(data), buf.ctypes.data, buf.size, lens.ctypes.data, lens.size,
+                                   ctypes.byref(npk), ctypes.byref(nb), err, _ERRCAP), err)
+    return buf[:nb.value], lens[:npk.value]
+
+
+DEMUX_STATS = ("pages", "links", "crc_bad", "foreign_pages", "dropped_packets", "granule_warnings",
+               "granule_checked", "skipped_bytes", "packets", "play48")
+
+
+def demux_stats(path_or_bytes):
+    """Demuxer report as a dict (keys DEMUX_STATS).  Honours CUOPUS_CHECK_CRC
+    and CUOPUS_GRANULE like every entry point."""
+    lib = _load()
+    data = _as_bytes(path_or_bytes)
+    s = (ctypes.c_int64 * 10)()
+    err = ctypes.create_string_buffer(_ERRCAP)
+    _check(lib.cuopus_demux_stats(data, len(data), s, err, _ERRCAP), err)
+    return dict(zip(DEMUX_STATS, list(s)))
+
+
+def dump_blob(path_or_bytes, out_path):
+    """Write the CUOP packet dump (cuopus_demux.h:387-397 layout: 'CUOP', u32
+    packet count, u32 lengths, packet bytes)."""
+    data, lens = packets(path_or_bytes)
+    with open(out_path, "wb") as f:
+        f.write(b"CUOP")
+        f.write(np.uint32(len(lens)).tobytes())
+        f.write(lens.astype("<u4").tobytes())
+        f.write(data.tobytes())
+
+
+def dump_chains(path_or_bytes, out_path):
+    """Write the CUC2 chain dump (cuopus_demux.h:399-411 layout: 'CUC2', u32
+    chain count, 4 x u32 first/npkts/trim/keep per chain)."""
+    ch = chain_table(path_or_bytes)
+    with open(out_path, "wb") as f:
+        f.write(b"CUC2")
+        f.write(np.uint32(len(ch)).tobytes())
+        f.write(ch.astype("<u4").tobytes())
 
 
 def _stream():
@@ -295,8 +349,8 @@ PROF_SCOPES = ["chain", "frame", "celt", "celt_head", "celt_bands", "celt_synth"
 def prof_read():
     """Per-scope shader-clock cycle sums of a -DCUO_PROF library (cleared on
     read); empty dict for a normal build."""
-    buf = (ctypes.c_ulonglong * 32)()
-    n = _load().cuopus_prof_read(buf, 32)
+    buf = (ctypes.c_ulonglong * 64)()
+    n = min(_load().cuopus_prof_read(buf, 64), len(PROF_SCOPES))
     return {PROF_SCOPES[i]: int(buf[i]) for i in range(max(n, 0)) if buf[i]}
 
 

@@ -106,6 +106,22 @@ int cuopus_probe_packets(const uint8_t *ogg, size_t n, int64_t *n_packets, int64
 int cuopus_dump_chains(const uint8_t *ogg, size_t n, uint32_t *out4, size_t cap_chains, int64_t *n_chains,
                        char *errbuf, size_t errcap);
 
+/* Audio packets of a stream (host only): the reference CLI's --dump-blob CUOP
+ * payload (cuopus_demux.h:387-397 / ogg_index.py cmd_extract): lens[i] per
+ * packet and the packet bytes back to back.  data or lens may be NULL to
+ * query *n_packets / *n_bytes. */
+int cuopus_dump_packets(const uint8_t *ogg, size_t n, uint8_t *data, size_t data_cap, uint32_t *lens,
+                        size_t cap_packets, int64_t *n_packets, int64_t *n_bytes, char *errbuf, size_t errcap);
+
+/* Demuxer report (host only; no counterpart in the reference, whose
+ * ogg_index.py cmd_index prints page / CRC / granule figures): s10[0] pages
+ * of the Opus stream(s) used, [1] chained links, [2] pages failing the CRC
+ * (CUOPUS_CHECK_CRC=1), [3] pages of other logical streams, [4] packets
+ * dropped as incomplete, [5] granule mismatches accepted (CUOPUS_GRANULE=warn),
+ * [6] granules checked, [7] bytes outside accepted pages, [8] audio packets,
+ * [9] playable 48 kHz samples. */
+int cuopus_demux_stats(const uint8_t *ogg, size_t n, int64_t *s10, char *errbuf, size_t errcap);
+
 /* Per-packet gate (the reference's harness2.cu:51-91 protocol, host
  * buffers): decode the chains of an already-demuxed packet table, each chain
  * from a fresh decoder.  Outputs per packet the decoded sample count and the

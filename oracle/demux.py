@@ -13,6 +13,8 @@ cuopus_demux.h (itself the C port of ogg_index.py):
   250 ms decoder preroll, bisect for first/last packet, trim/keep.
 * ``pkt_samples48``  -- TOC duration in 48 kHz samples.
 * ``out16_len``      -- libcuopus.cu:195-199, (play48 - 48) // 3 clamped at 0.
+* ``ogg_crc``        -- the page CRC of ogg_index.py:22-35 (polynomial
+  0x04C11DB7, initial value 0, MSB first, no final xor; RFC 3533 section 6).
 """
 import bisect
 import struct
@@ -36,6 +38,26 @@ def pkt_samples48(p):
             return 0
         nf = p[1] & 0x3F
     return _QMS[config] * 12 * nf
+
+
+def _crc_table():
+    t = []
+    for i in range(256):
+        r = i << 24
+        for _ in range(8):
+            r = ((r << 1) ^ 0x04C11DB7) & 0xFFFFFFFF if r & 0x80000000 else (r << 1) & 0xFFFFFFFF
+        t.append(r)
+    return t
+
+
+_CRC = _crc_table()
+
+
+def ogg_crc(data, crc=0):
+    """Ogg page CRC-32 of a byte string (ogg_index.py:22-35)."""
+    for b in bytes(data):
+        crc = ((crc << 8) & 0xFFFFFFFF) ^ _CRC[((crc >> 24) ^ b) & 0xFF]
+    return crc
 
 
 def out16_len(play48):

@@ -156,6 +156,11 @@ RECIPES = {
                   "--fec 1 --loss 2",
     "mixed": "--seed 18 --dur 20 --signal mixed --app audio --bw fb --brange 12000 64000",
     "silk_long": "--seed 19 --dur 12 --signal speech --app voip --bw wb --force silk --bitrate 20000 --frame 2880",
+    # high-rate CELT: packets above 255 bytes (multi-segment lacing, pages split inside packets)
+    "celt_hi": "--seed 20 --dur 8 --signal music --app lowdelay --bw fb --bitrate 256000 --frameset 120,240,480,960",
+    # CELT-only narrowband and wideband (TOC configs 16-23)
+    "celt_nb": "--seed 21 --dur 8 --signal speech --app lowdelay --bw nb --bitrate 24000 --frameset 120,240,480,960",
+    "celt_wb": "--seed 22 --dur 8 --signal mixed --app lowdelay --bw wb --bitrate 32000 --frame 480",
 }
 
 
