@@ -128,6 +128,25 @@ __host__ __device__ inline int32_t lane_get(int32_t v, int) { return v; }
 __host__ __device__ inline int32_t lane_set(int32_t, int, int32_t x) { return x; }
 #endif
 
+// 16-lane row primitives (DPP row shifts; device only).  row16_prefix_sum:
+// inclusive wrapping sums within each row, so lane 15 holds the sum of lanes
+// 0..15.  row16_shift_in: every row moves up by one lane and the row's first
+// lane takes fill.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ int32_t row16_prefix_sum(int32_t v)
+{
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false));
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false));
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false));
+    v = (int32_t)((uint32_t)v + (uint32_t)__builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false));
+    return v;
+}
+__device__ __forceinline__ int32_t row16_shift_in(int32_t v, int32_t fill)
+{
+    return __builtin_amdgcn_update_dpp(fill, v, 0x111, 0xF, 0xF, false);
+}
+#endif
+
 // Reductions over the lanes of per-lane partial results (identity on the
 // host).  Integer sums wrap like the serial loop's, so any order is exact.
 // Call with every lane active (outside CUO_PFOR bodies).
@@ -244,7 +263,12 @@ CUO_FN int32_t mul16_16_q13(int32_t a, int32_t b) { return mul16_16(a, b) >> 13;
 CUO_FN int32_t mul16_16_p15(int32_t a, int32_t b) { return (16384 + mul16_16(a, b)) >> 15; }
 CUO_FN int32_t mul16_16_p14(int32_t a, int32_t b) { return (8192 + mul16_16(a, b)) >> 14; }
 CUO_FN int32_t mul16_32_q15(int32_t a, int32_t b) { return (int32_t)(((int64_t)(int16_t)a * (int64_t)b) >> 15); }
+#if defined(__HIP_DEVICE_COMPILE__)
+// high word of b * (a16 << 16) == floor(a16 * b / 2^16)
+CUO_FN int32_t mul16_32_q16(int32_t a, int32_t b) { return __mulhi(b, (int32_t)((uint32_t)(uint16_t)a << 16)); }
+#else
 CUO_FN int32_t mul16_32_q16(int32_t a, int32_t b) { return (int32_t)(((int64_t)(int16_t)a * (int64_t)b) >> 16); }
+#endif
 CUO_FN int32_t mul16_32_p16(int32_t a, int32_t b) { return (int32_t)(((int64_t)(int16_t)a * (int64_t)b + 32768) >> 16); }
 CUO_FN int32_t mul32_32_q31(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)b) >> 31); }
 CUO_FN int32_t mul32_32_q16(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)b) >> 16); }
@@ -417,10 +441,26 @@ CUO_FN uint32_t lcg_jump(uint32_t seed, int n)
 }
 
 // ---------------------------------------------------------------- SILK layer
+#if defined(__HIP_DEVICE_COMPILE__)
+// (a * b16) >> 16 is the high word of a * (b16 << 16): one multiply instead
+// of a 64-bit product and shift.  The value is the same: floor(a*b16 / 2^16)
+// fits in 32 bits for 32-bit a and 16-bit b16.
+CUO_FN int32_t silk_smulwb(int32_t a, int32_t b) { return __mulhi(a, (int32_t)((uint32_t)(uint16_t)b << 16)); }
+CUO_FN int32_t silk_smlawb(int32_t a, int32_t b, int32_t c)
+{
+    return (int32_t)((uint32_t)a + (uint32_t)__mulhi(b, (int32_t)((uint32_t)(uint16_t)c << 16)));
+}
+CUO_FN int32_t silk_smulwt(int32_t a, int32_t b) { return __mulhi(a, (int32_t)((uint32_t)b & 0xFFFF0000u)); }
+CUO_FN int32_t silk_smlawt(int32_t a, int32_t b, int32_t c)
+{
+    return (int32_t)((uint32_t)a + (uint32_t)__mulhi(b, (int32_t)((uint32_t)c & 0xFFFF0000u)));
+}
+#else
 CUO_FN int32_t silk_smulwb(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)(int16_t)b) >> 16); }
 CUO_FN int32_t silk_smlawb(int32_t a, int32_t b, int32_t c) { return (int32_t)(a + (((int64_t)b * (int64_t)(int16_t)c) >> 16)); }
 CUO_FN int32_t silk_smulwt(int32_t a, int32_t b) { return (int32_t)(((int64_t)a * (int64_t)(b >> 16)) >> 16); }
 CUO_FN int32_t silk_smlawt(int32_t a, int32_t b, int32_t c) { return (int32_t)(a + (((int64_t)b * (int64_t)(c >> 16)) >> 16)); }
+#endif
 CUO_FN int32_t silk_smulbb(int32_t a, int32_t b) { return (int32_t)(int16_t)a * (int32_t)(int16_t)b; }
 CUO_FN int32_t silk_smlabb(int32_t a, int32_t b, int32_t c) { return a + (int32_t)(int16_t)b * (int32_t)(int16_t)c; }
 CUO_FN int32_t silk_smulbt(int32_t a, int32_t b) { return (int32_t)(int16_t)a * (b >> 16); }
@@ -432,11 +472,14 @@ CUO_FN int32_t silk_mla(int32_t a, int32_t b, int32_t c) { return (int32_t)((uin
 CUO_FN int32_t silk_rshift_round(int32_t a, int s) { return s == 1 ? (a >> 1) + (a & 1) : ((a >> (s - 1)) + 1) >> 1; }
 CUO_FN int64_t silk_rshift_round64(int64_t a, int s) { return s == 1 ? (a >> 1) + (a & 1) : ((a >> (s - 1)) + 1) >> 1; }
 CUO_FN int32_t silk_lshift(int32_t a, int s) { return (int32_t)((uint32_t)a << s); }
+// silk_ADD_SAT32: the wrapped sum unless both operands share a sign the sum
+// lacks, then the limit of that sign (written without branches).
 CUO_FN int32_t silk_add_sat32(int32_t a, int32_t b)
 {
-    uint32_t s = (uint32_t)a + (uint32_t)b;
-    if ((s & 0x80000000u) == 0) return ((uint32_t)a & (uint32_t)b & 0x80000000u) ? INT32_MIN : (int32_t)s;
-    return (((uint32_t)a | (uint32_t)b) & 0x80000000u) == 0 ? INT32_MAX : (int32_t)s;
+    const uint32_t s = (uint32_t)a + (uint32_t)b;
+    const bool ov = (int32_t)(((uint32_t)a ^ s) & ((uint32_t)b ^ s)) < 0;
+    const int32_t lim = (int32_t)((uint32_t)(a >> 31) ^ 0x7FFFFFFFu);
+    return ov ? lim : (int32_t)s;
 }
 CUO_FN int32_t silk_sub_sat32(int32_t a, int32_t b)
 {
@@ -451,7 +494,9 @@ CUO_FN int32_t silk_limit(int32_t a, int32_t l1, int32_t l2)
 }
 CUO_FN int32_t silk_lshift_sat32(int32_t a, int s)
 {
-    return silk_lshift(silk_limit(a, (int32_t)(INT32_MIN >> s), (int32_t)(INT32_MAX >> s)), s);
+    // silk_LIMIT_32 with lower limit below upper: a clamp
+    const int32_t lo = (int32_t)(INT32_MIN >> s), hi = (int32_t)(INT32_MAX >> s);
+    return silk_lshift(a < lo ? lo : (a > hi ? hi : a), s);
 }
 CUO_FN int32_t silk_clz32(int32_t a) { return a == 0 ? 32 : __builtin_clz((uint32_t)a); }
 CUO_FN int32_t silk_clz16(int16_t a) { return silk_clz32((int32_t)(uint16_t)a) - 16; }

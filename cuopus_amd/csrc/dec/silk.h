@@ -432,14 +432,14 @@ CUO_FN void silk_decode_pitch(int lagIndex, int contourIndex, int32_t *pitch_lag
 
 CUO_FN void silk_lpc_analysis_filter(int16_t *out, const int16_t *in, const int16_t *B, int len, int d)
 {
-    for (int ix = d; ix < len; ix++) {
+    CUO_PFOR(ix, d, len) {
         const int16_t *p = &in[ix - 1];
         int32_t acc = silk_smulbb(p[0], B[0]);
         for (int j = 1; j < d; j++) acc = add32_ovf(acc, (int32_t)p[-j] * (int32_t)B[j]);
         acc = sub32_ovf(silk_lshift((int32_t)p[1], 12), acc);
         out[ix] = (int16_t)silk_sat16(silk_rshift_round(acc, 12));
     }
-    for (int j = 0; j < d; j++) out[j] = 0;
+    CUO_PFOR(j, 0, d) out[j] = 0;
 }
 
 CUO_FN void silk_sum_sqr_shift(int32_t *energy, int32_t *shift, const int16_t *x, int len)
@@ -487,44 +487,65 @@ CUO_FN void silk_resampler_init(SilkResampler *S, int32_t Fs_in, int32_t Fs_out)
     while (silk_smulww(S->invRatio_Q16, Fs_out) < silk_lshift(Fs_in, 1)) S->invRatio_Q16++;
 }
 
+// 2x allpass upsampler (resampler_private_up2_HQ.c:38-113).  The recursion is
+// serial; inputs are read a block of kLanes at a time (one per lane) and the
+// even/odd outputs collected the same way, so the loop touches no memory.
 CUO_FN void silk_up2_hq(int32_t *S, int16_t *out, const int16_t *in, int len)
 {
-    const int16_t *h0 = tab::kResUp2HQ0, *h1 = tab::kResUp2HQ1;
-    for (int k = 0; k < len; k++) {
-        int32_t in32 = silk_lshift((int32_t)in[k], 10);
-        int32_t Y = in32 - S[0];
-        int32_t X = silk_smulwb(Y, h0[0]);
-        int32_t o1 = S[0] + X;
-        S[0] = in32 + X;
-        Y = o1 - S[1];
-        X = silk_smulwb(Y, h0[1]);
-        int32_t o2 = S[1] + X;
-        S[1] = o1 + X;
-        Y = o2 - S[2];
-        X = silk_smlawb(Y, Y, h0[2]);
-        o1 = S[2] + X;
-        S[2] = o2 + X;
-        out[2 * k] = (int16_t)silk_sat16(silk_rshift_round(o1, 10));
-        Y = in32 - S[3];
-        X = silk_smulwb(Y, h1[0]);
-        o1 = S[3] + X;
-        S[3] = in32 + X;
-        Y = o1 - S[4];
-        X = silk_smulwb(Y, h1[1]);
-        o2 = S[4] + X;
-        S[4] = o1 + X;
-        Y = o2 - S[5];
-        X = silk_smlawb(Y, Y, h1[2]);
-        o1 = S[5] + X;
-        S[5] = o2 + X;
-        out[2 * k + 1] = (int16_t)silk_sat16(silk_rshift_round(o1, 10));
+    const int32_t h00 = tab::kResUp2HQ0[0], h01 = tab::kResUp2HQ0[1], h02 = tab::kResUp2HQ0[2];
+    const int32_t h10 = tab::kResUp2HQ1[0], h11 = tab::kResUp2HQ1[1], h12 = tab::kResUp2HQ1[2];
+    int32_t s0 = uniform(S[0]), s1 = uniform(S[1]), s2 = uniform(S[2]);
+    int32_t s3 = uniform(S[3]), s4 = uniform(S[4]), s5 = uniform(S[5]);
+    for (int k0 = 0; k0 < len; k0 += kLanes) {
+        const int cnt = min32(kLanes, len - k0);
+        const int32_t iv = lane_id() < cnt ? (int32_t)in[k0 + lane_id()] : 0;
+        int32_t oe = 0, oo = 0;
+        for (int k = 0; k < cnt; k++) {
+            const int32_t in32 = silk_lshift(lane_get(iv, k), 10);
+            int32_t Y = in32 - s0;
+            int32_t X = silk_smulwb(Y, h00);
+            int32_t o1 = s0 + X;
+            s0 = in32 + X;
+            Y = o1 - s1;
+            X = silk_smulwb(Y, h01);
+            int32_t o2 = s1 + X;
+            s1 = o1 + X;
+            Y = o2 - s2;
+            X = silk_smlawb(Y, Y, h02);
+            o1 = s2 + X;
+            s2 = o2 + X;
+            oe = lane_set(oe, k, silk_sat16(silk_rshift_round(o1, 10)));
+            Y = in32 - s3;
+            X = silk_smulwb(Y, h10);
+            o1 = s3 + X;
+            s3 = in32 + X;
+            Y = o1 - s4;
+            X = silk_smulwb(Y, h11);
+            o2 = s4 + X;
+            s4 = o1 + X;
+            Y = o2 - s5;
+            X = silk_smlawb(Y, Y, h12);
+            o1 = s5 + X;
+            s5 = o2 + X;
+            oo = lane_set(oo, k, silk_sat16(silk_rshift_round(o1, 10)));
+        }
+        CUO_PFOR(k, 0, cnt) {
+            out[2 * (k0 + k)] = (int16_t)oe;
+            out[2 * (k0 + k) + 1] = (int16_t)oo;
+        }
     }
+    S[0] = s0; S[1] = s1; S[2] = s2;
+    S[3] = s3; S[4] = s4; S[5] = s5;
 }
 
+// Fractional FIR interpolation (resampler_private_IIR_FIR.c:36-61): every
+// output is independent, one per lane.
 CUO_FN int16_t *silk_iir_fir_interpol(int16_t *out, const int16_t *buf, int32_t max_index_Q16, int32_t inc_Q16)
 {
     const int16_t *F = tab::kResFracFir12;
-    for (int32_t index_Q16 = 0; index_Q16 < max_index_Q16; index_Q16 += inc_Q16) {
+    const int n = max_index_Q16 > 0 ? (max_index_Q16 + inc_Q16 - 1) / inc_Q16 : 0;
+    CUO_PFOR(j, 0, n) {
+        const int32_t index_Q16 = j * inc_Q16;
         int32_t ti = silk_smulwb(index_Q16 & 0xFFFF, 12);
         const int16_t *b = &buf[index_Q16 >> 16];
         int32_t r = silk_smulbb(b[0], F[ti * 4 + 0]);
@@ -535,15 +556,15 @@ CUO_FN int16_t *silk_iir_fir_interpol(int16_t *out, const int16_t *buf, int32_t 
         r = silk_smlabb(r, b[5], F[(11 - ti) * 4 + 2]);
         r = silk_smlabb(r, b[6], F[(11 - ti) * 4 + 1]);
         r = silk_smlabb(r, b[7], F[(11 - ti) * 4 + 0]);
-        *out++ = (int16_t)silk_sat16(silk_rshift_round(r, 15));
+        out[j] = (int16_t)silk_sat16(silk_rshift_round(r, 15));
     }
-    return out;
+    return out + n;
 }
 
 CUO_FN void silk_resampler_iir_fir(SilkResampler *S, int16_t *out, const int16_t *in, int32_t inLen, int16_t *buf)
 {
     CUO_PROF_SCOPE(kProfSilkResamp);
-    for (int i = 0; i < 8; i++) buf[i] = S->sFIR[i];
+    CUO_PFOR(i, 0, 8) buf[i] = S->sFIR[i];
     int32_t inc = S->invRatio_Q16;
     int32_t nIn;
     while (1) {
@@ -553,19 +574,19 @@ CUO_FN void silk_resampler_iir_fir(SilkResampler *S, int16_t *out, const int16_t
         out = silk_iir_fir_interpol(out, buf, max_index_Q16, inc);
         in += nIn;
         inLen -= nIn;
-        if (inLen > 0) { for (int i = 0; i < 8; i++) buf[i] = buf[(nIn << 1) + i]; }
+        if (inLen > 0) { CUO_PFOR(i, 0, 8) buf[i] = buf[(nIn << 1) + i]; }
         else break;
     }
-    for (int i = 0; i < 8; i++) S->sFIR[i] = buf[(nIn << 1) + i];
+    CUO_PFOR(i, 0, 8) S->sFIR[i] = buf[(nIn << 1) + i];
 }
 
 CUO_FN void silk_resample(SilkResampler *S, int16_t *out, const int16_t *in, int32_t inLen, int16_t *buf)
 {
     int nSamples = S->Fs_in_kHz - S->inputDelay;
-    for (int i = 0; i < nSamples; i++) S->delayBuf[S->inputDelay + i] = in[i];
+    CUO_PFOR(i, 0, nSamples) S->delayBuf[S->inputDelay + i] = in[i];
     silk_resampler_iir_fir(S, out, S->delayBuf, S->Fs_in_kHz, buf);
     silk_resampler_iir_fir(S, &out[S->Fs_out_kHz], &in[nSamples], inLen - S->Fs_in_kHz, buf);
-    for (int i = 0; i < S->inputDelay; i++) S->delayBuf[i] = in[inLen - S->inputDelay + i];
+    CUO_PFOR(i, 0, S->inputDelay) S->delayBuf[i] = in[inLen - S->inputDelay + i];
 }
 
 // ------------------------------------------------------------------ state management
@@ -826,9 +847,165 @@ CUO_FN int32_t lpc_pred_q10(const int32_t *s, const int16_t *A, int order)
     return p;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Excitation (decode_core.c:73-88) across the lanes.  The seed recurrence
+// s' = rand(s) + pulse = a*s + c + pulse (mod 2^32) is an affine map per
+// sample: each lane composes the maps of its block of consecutive samples, an
+// exclusive scan of the compositions over the lanes gives each block's
+// starting seed, and each lane then replays its block.  Same seeds, same
+// excitation as the serial loop.
+__device__ inline void silk_excitation_lanes(int32_t *exc, const int16_t *pulses, int L, int32_t seed,
+                                             int32_t offset_Q10)
+{
+    const uint32_t a = 196314165u, c = 907633515u;
+    const int lane = lane_id();
+    const int B = (L + kLanes - 1) / kLanes;
+    const int i0 = lane * B, i1 = min32(i0 + B, L);
+    uint32_t A = 1, C = 0;
+    for (int i = i0; i < i1; i++) {
+        A = a * A;
+        C = a * C + c + (uint32_t)(int32_t)pulses[i];
+    }
+    for (int o = 1; o < kLanes; o <<= 1) {
+        uint32_t Ap = (uint32_t)__shfl_up((int)A, o, 64), Cp = (uint32_t)__shfl_up((int)C, o, 64);
+        if (lane >= o) {
+            C = A * Cp + C;
+            A = A * Ap;
+        }
+    }
+    uint32_t Ae = (uint32_t)__shfl_up((int)A, 1, 64), Ce = (uint32_t)__shfl_up((int)C, 1, 64);
+    if (lane == 0) {
+        Ae = 1;
+        Ce = 0;
+    }
+    uint32_t sd = Ae * (uint32_t)seed + Ce;
+    for (int i = i0; i < i1; i++) {
+        sd = a * sd + c;
+        int32_t e = silk_lshift((int32_t)pulses[i], 14);
+        if (e > 0) e -= 80 << 4;
+        else if (e < 0) e += 80 << 4;
+        e += offset_Q10 << 4;
+        if ((int32_t)sd < 0) e = -e;
+        exc[i] = e;
+        sd += (uint32_t)(int32_t)pulses[i];
+    }
+}
+
+// silk_decode_core (decode_core.c:38-237) on the wavefront.  The LPC
+// synthesis recursion stays serial per sample, but its order-16 prediction is
+// one multiply per lane (lane j holds output n-1-j and coefficient j) and a
+// 16-lane DPP sum; the synthesis history lives in that register across
+// subframes.  The LTP synthesis reads its own output only lag-2 samples back,
+// so blocks of lag-2 outputs are computed one per lane.  The rewhitening
+// filter and state rescaling are lane-parallel loops.  Integer sums wrap like
+// the serial code's, so the order of summation does not change any value.
+__device__ inline void silk_decode_core_lanes(SilkChannel *ch, SilkCtrl *ctrl, int16_t *xq, const int16_t *pulses,
+                                              SilkScratch *sc)
+{
+    CUO_PROF_SCOPE(kProfSilkCore);
+    int16_t *sLTP = sc->sLTP;
+    int32_t *sLTP_Q15 = sc->sLTP_Q15;
+    int32_t *res_Q14 = sc->res_Q14;
+    const int lane = lane_id();
+    const int frame_length = uniform(ch->frame_length);
+    const int subfr = uniform(ch->subfr_length);
+    const int nb_subfr = uniform(ch->nb_subfr);
+    const int order = uniform(ch->LPC_order);
+    const int ltp_mem = uniform(ch->ltp_mem_length);
+    const int sig0 = uniform((int)ch->indices.signalType);
+    const int32_t offset_Q10 = tab::kQuantOffsets_Q10[(sig0 >> 1) * 2 + uniform((int)ch->indices.quantOffsetType)];
+    const int NLSF_interpolation_flag = uniform((int)ch->indices.NLSFInterpCoef_Q2) < 1 << 2;
+    silk_excitation_lanes(ch->exc_Q14, pulses, frame_length, uniform((int32_t)ch->indices.Seed), offset_Q10);
+    int32_t win = lane < kSilkMaxOrder ? ch->sLPC_Q14_buf[kSilkMaxOrder - 1 - lane] : 0;
+    const int32_t *pexc = ch->exc_Q14;
+    int16_t *pxq = xq;
+    int sLTP_buf_idx = ltp_mem;
+    int lag = 0;
+    for (int k = 0; k < nb_subfr; k++) {
+        const int16_t *A_Q12 = ctrl->PredCoef_Q12[k >> 1];
+        const int32_t a16 = lane < order ? (int32_t)((uint32_t)(uint16_t)A_Q12[lane] << 16) : 0;
+        int16_t *B_Q14 = &ctrl->LTPCoef_Q14[k * kLtpOrder];
+        int signalType = sig0;
+        const int32_t gain = uniform(ctrl->Gains_Q16[k]);
+        const int32_t prev_gain = uniform(ch->prev_gain_Q16);
+        const int32_t Gain_Q10 = gain >> 6;
+        int32_t inv_gain_Q31 = silk_inverse32_varq(gain, 47);
+        int32_t gain_adj_Q16 = 1 << 16;
+        if (gain != prev_gain) {
+            gain_adj_Q16 = silk_div32_varq(prev_gain, gain, 16);
+            win = silk_smulww(gain_adj_Q16, win);
+        }
+        ch->prev_gain_Q16 = gain;
+        if (uniform(ch->lossCnt) && uniform(ch->prevSignalType) == SIG_VOICED && sig0 != SIG_VOICED && k < 2) {
+            CUO_PFOR(i, 0, kLtpOrder) B_Q14[i] = i == kLtpOrder / 2 ? (int16_t)silk_fix(0.25, 14) : 0;
+            signalType = SIG_VOICED;
+            ctrl->pitchL[k] = ch->lagPrev;
+        }
+        const int32_t *pres = pexc;
+        if (signalType == SIG_VOICED) {
+            lag = uniform(ctrl->pitchL[k]);
+            if (k == 0 || (k == 2 && NLSF_interpolation_flag)) {
+                const int start_idx = ltp_mem - lag - order - kLtpOrder / 2;
+                if (k == 2) CUO_PFOR(i, 0, 2 * subfr) ch->outBuf[ltp_mem + i] = xq[i];
+                silk_lpc_analysis_filter(&sLTP[start_idx], &ch->outBuf[start_idx + k * subfr], A_Q12,
+                                         ltp_mem - start_idx, order);
+                if (k == 0) inv_gain_Q31 = silk_lshift(silk_smulwb(inv_gain_Q31, uniform(ctrl->LTP_scale_Q14)), 2);
+                CUO_PFOR(i, 0, lag + kLtpOrder / 2)
+                    sLTP_Q15[sLTP_buf_idx - i - 1] = silk_smulwb(inv_gain_Q31, sLTP[ltp_mem - i - 1]);
+            } else if (gain_adj_Q16 != 1 << 16) {
+                CUO_PFOR(i, 0, lag + kLtpOrder / 2)
+                    sLTP_Q15[sLTP_buf_idx - i - 1] = silk_smulww(gain_adj_Q16, sLTP_Q15[sLTP_buf_idx - i - 1]);
+            }
+            const int32_t b0 = uniform((int32_t)B_Q14[0]), b1 = uniform((int32_t)B_Q14[1]),
+                          b2 = uniform((int32_t)B_Q14[2]), b3 = uniform((int32_t)B_Q14[3]),
+                          b4 = uniform((int32_t)B_Q14[4]);
+            const int blk = max32(1, min32(lag - 2, kLanes));
+            for (int i0 = 0; i0 < subfr; i0 += blk) {
+                const int i = i0 + lane;
+                if (lane < blk && i < subfr) {
+                    const int32_t *pl = &sLTP_Q15[sLTP_buf_idx + i - lag + kLtpOrder / 2];
+                    int32_t p = 2;
+                    p = silk_smlawb(p, pl[0], b0);
+                    p = silk_smlawb(p, pl[-1], b1);
+                    p = silk_smlawb(p, pl[-2], b2);
+                    p = silk_smlawb(p, pl[-3], b3);
+                    p = silk_smlawb(p, pl[-4], b4);
+                    const int32_t r = pexc[i] + silk_lshift(p, 1);
+                    res_Q14[i] = r;
+                    sLTP_Q15[sLTP_buf_idx + i] = silk_lshift(r, 1);
+                }
+            }
+            sLTP_buf_idx += subfr;
+            pres = res_Q14;
+        }
+        const int32_t rnd = order >> 1;
+        for (int n0 = 0; n0 < subfr; n0 += kLanes) {
+            const int cnt = min32(kLanes, subfr - n0);
+            const int32_t rv = lane < cnt ? pres[n0 + lane] : 0;
+            int32_t outv = 0;
+            for (int i = 0; i < cnt; i++) {
+                const int32_t t = row16_prefix_sum(__mulhi(win, a16));
+                const int32_t p = (int32_t)((uint32_t)__builtin_amdgcn_readlane(t, 15) + (uint32_t)rnd);
+                const int32_t y = silk_add_sat32(__builtin_amdgcn_readlane(rv, i), silk_lshift_sat32(p, 4));
+                outv = lane == i ? silk_sat16(silk_rshift_round(silk_smulww(y, Gain_Q10), 8)) : outv;
+                win = row16_shift_in(win, y);
+            }
+            if (lane < cnt) pxq[n0 + lane] = (int16_t)outv;
+        }
+        pexc += subfr;
+        pxq += subfr;
+    }
+    if (lane < kSilkMaxOrder) ch->sLPC_Q14_buf[kSilkMaxOrder - 1 - lane] = win;
+}
+#endif
+
 // silk_decode_core (decode_core.c:38-237)
 CUO_FN void silk_decode_core(SilkChannel *ch, SilkCtrl *ctrl, int16_t *xq, const int16_t *pulses, SilkScratch *sc)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    silk_decode_core_lanes(ch, ctrl, xq, pulses, sc);
+    return;
+#endif
     CUO_PROF_SCOPE(kProfSilkCore);
     int16_t *sLTP = sc->sLTP;
     int32_t *sLTP_Q15 = sc->sLTP_Q15;
@@ -1179,8 +1356,8 @@ CUO_FN void silk_decode_frame(SilkChannel *ch, RangeDec *d, int16_t *pOut, int32
         silk_decode_parameters(ch, ctrl, condCoding);
         silk_decode_core(ch, ctrl, pOut, pulses, sc);
         int mv_len = ch->ltp_mem_length - ch->frame_length;
-        for (int i = 0; i < mv_len; i++) ch->outBuf[i] = ch->outBuf[ch->frame_length + i];
-        for (int i = 0; i < ch->frame_length; i++) ch->outBuf[mv_len + i] = pOut[i];
+        CUO_PFOR(i, 0, mv_len) ch->outBuf[i] = ch->outBuf[ch->frame_length + i];
+        CUO_PFOR(i, 0, ch->frame_length) ch->outBuf[mv_len + i] = pOut[i];
         silk_plc(ch, ctrl, pOut, 0, sc);
         ch->lossCnt = 0;
         ch->prevSignalType = ch->indices.signalType;
@@ -1188,8 +1365,8 @@ CUO_FN void silk_decode_frame(SilkChannel *ch, RangeDec *d, int16_t *pOut, int32
     } else {
         silk_plc(ch, ctrl, pOut, 1, sc);
         int mv_len = ch->ltp_mem_length - ch->frame_length;
-        for (int i = 0; i < mv_len; i++) ch->outBuf[i] = ch->outBuf[ch->frame_length + i];
-        for (int i = 0; i < ch->frame_length; i++) ch->outBuf[mv_len + i] = pOut[i];
+        CUO_PFOR(i, 0, mv_len) ch->outBuf[i] = ch->outBuf[ch->frame_length + i];
+        CUO_PFOR(i, 0, ch->frame_length) ch->outBuf[mv_len + i] = pOut[i];
     }
     silk_cng(ch, ctrl, pOut, L, sc);
     silk_plc_glue_frames(ch, pOut, L);
