@@ -17,6 +17,7 @@
 
 #define CUO_FN __host__ __device__ inline
 #define CUO_NOINLINE __host__ __device__ __attribute__((noinline))
+#define CUO_FI __host__ __device__ inline __attribute__((always_inline))
 
 namespace cuo {
 
@@ -226,6 +227,7 @@ enum {
     kProfPvq, kProfCwrsi, kProfExpRot, kProfTheta, kProfLeafFold, kProfBandPre, kProfBandPost, kProfEcUint,
     kProfAllocBisect, kProfAllocTail, kProfQpLeafBits, kProfBandsPrologue, kProfLowbandOut, kProfQpStage,
     kProfSilkIndices, kProfSilkPulses, kProfSilkParams, kProfSilkCore, kProfSilkResamp, kProfSilkPlc,
+    kProfSilkNlsf, kProfSilkNlsf2a, kProfSilkCng, kProfSilkPulseCounts, kProfSilkShell, kProfSilkSigns,
     kProfCount
 };
 static_assert(kProfCount <= kProfSlots, "profiling scope ids exceed the per-lane slots");

@@ -34,7 +34,7 @@ constexpr int kEcCodeExtra = 7;  // (32-2)%8+1
 
 // Every function loads the fields it needs as wave-uniform values (scalar
 // registers), updates them and stores them back.
-CUO_FN const uint8_t *ec_buf(const RangeDec *d)
+CUO_FI const uint8_t *ec_buf(const RangeDec *d)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     return (const uint8_t *)(uintptr_t)uniform_u64((uint64_t)(uintptr_t)d->buf);
@@ -43,8 +43,28 @@ CUO_FN const uint8_t *ec_buf(const RangeDec *d)
 #endif
 }
 
+// A register copy of a range decoder (every field wave-uniform): hot symbol
+// loops decode on the copy and store it back once.
+CUO_FI RangeDec ec_regs(const RangeDec *d)
+{
+    RangeDec r;
+    r.buf = ec_buf(d);
+    r.storage = uniform(d->storage);
+    r.end_offs = uniform(d->end_offs);
+    r.end_window = uniform(d->end_window);
+    r.nend_bits = uniform(d->nend_bits);
+    r.nbits_total = uniform(d->nbits_total);
+    r.offs = uniform(d->offs);
+    r.rng = uniform(d->rng);
+    r.val = uniform(d->val);
+    r.ext = uniform(d->ext);
+    r.rem = uniform(d->rem);
+    r.error = uniform(d->error);
+    return r;
+}
+
 // Renormalise after a symbol: rng and val are the updated values.
-CUO_FN void ec_normalize_rv(RangeDec *d, uint32_t rng, uint32_t val)
+CUO_FI void ec_normalize_rv(RangeDec *d, uint32_t rng, uint32_t val)
 {
     if (rng <= kEcCodeBot) {
         int nbits = uniform(d->nbits_total);
@@ -66,7 +86,7 @@ CUO_FN void ec_normalize_rv(RangeDec *d, uint32_t rng, uint32_t val)
     d->rng = rng;
     d->val = val;
 }
-CUO_FN void ec_normalize(RangeDec *d) { ec_normalize_rv(d, uniform(d->rng), uniform(d->val)); }
+CUO_FI void ec_normalize(RangeDec *d) { ec_normalize_rv(d, uniform(d->rng), uniform(d->val)); }
 
 CUO_FN void ec_init(RangeDec *d, const uint8_t *buf, uint32_t storage)
 {
@@ -85,7 +105,7 @@ CUO_FN void ec_init(RangeDec *d, const uint8_t *buf, uint32_t storage)
     ec_normalize_rv(d, rng, rng - 1 - (uint32_t)(rem >> (8 - kEcCodeExtra)));
 }
 
-CUO_FN uint32_t ec_decode(RangeDec *d, uint32_t ft)
+CUO_FI uint32_t ec_decode(RangeDec *d, uint32_t ft)
 {
     uint32_t ext = uniform(d->rng) / ft;
     d->ext = ext;
@@ -94,7 +114,7 @@ CUO_FN uint32_t ec_decode(RangeDec *d, uint32_t ft)
     return ft - (t < ft ? t : ft);
 }
 
-CUO_FN uint32_t ec_decode_bin(RangeDec *d, int bits)
+CUO_FI uint32_t ec_decode_bin(RangeDec *d, int bits)
 {
     uint32_t ext = uniform(d->rng) >> bits;
     d->ext = ext;
@@ -103,7 +123,7 @@ CUO_FN uint32_t ec_decode_bin(RangeDec *d, int bits)
     return ft - (t < ft ? t : ft);
 }
 
-CUO_FN void ec_update(RangeDec *d, uint32_t fl, uint32_t fh, uint32_t ft)
+CUO_FI void ec_update(RangeDec *d, uint32_t fl, uint32_t fh, uint32_t ft)
 {
     uint32_t ext = uniform(d->ext), rng = uniform(d->rng), val = uniform(d->val);
     uint32_t s = ext * (ft - fh);
@@ -112,7 +132,7 @@ CUO_FN void ec_update(RangeDec *d, uint32_t fl, uint32_t fh, uint32_t ft)
     ec_normalize_rv(d, rng, val);
 }
 
-CUO_FN int ec_bit_logp(RangeDec *d, int logp)
+CUO_FI int ec_bit_logp(RangeDec *d, int logp)
 {
     uint32_t r = uniform(d->rng), v = uniform(d->val);
     uint32_t s = r >> logp;
@@ -125,16 +145,41 @@ CUO_FN int ec_bit_logp(RangeDec *d, int logp)
 template <typename T>
 CUO_FN uint32_t icdf_at(const T *icdf, int i);
 template <>
-CUO_FN uint32_t icdf_at<uint8_t>(const uint8_t *icdf, int i) { return ld_u8(icdf, i); }
+CUO_FI uint32_t icdf_at<uint8_t>(const uint8_t *icdf, int i) { return ld_u8(icdf, i); }
 
 // icdf must point into a constant table (it is read with scalar loads).
 template <typename T>
-CUO_FN int ec_icdf_t(RangeDec *d, const T *icdf, int ftb)
+CUO_FI int ec_icdf_t(RangeDec *d, const T *icdf, int ftb)
 {
     uint32_t s = uniform(d->rng), v = uniform(d->val);
     uint32_t r = s >> ftb;
     int ret = -1;
     uint32_t t;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (sizeof(T) == 1) {
+        // same search, the table read a 32-bit word (four entries) per load
+        const uint8_t *p = (const uint8_t *)icdf;
+        const uint32_t mis = (uint32_t)(uintptr_t)p & 3u;
+        uint32_t w = const_word(p) >> (8 * mis);
+        int left = 4 - (int)mis;
+        ret = 0;
+        for (;;) {
+            t = s;
+            s = r * (w & 0xFFu);
+            if (v >= s) break;
+            ret++;
+            if (--left == 0) {
+                p += 4 - (int)((uintptr_t)p & 3u);
+                w = const_word(p);
+                left = 4;
+            } else {
+                w >>= 8;
+            }
+        }
+        ec_normalize_rv(d, t - s, v - s);
+        return ret;
+    }
+#endif
     do {
         t = s;
         s = r * icdf_at<T>(icdf, ++ret);
@@ -142,11 +187,11 @@ CUO_FN int ec_icdf_t(RangeDec *d, const T *icdf, int ftb)
     ec_normalize_rv(d, t - s, v - s);
     return ret;
 }
-CUO_FN int ec_icdf(RangeDec *d, const uint8_t *icdf, int ftb) { return ec_icdf_t<uint8_t>(d, icdf, ftb); }
+CUO_FI int ec_icdf(RangeDec *d, const uint8_t *icdf, int ftb) { return ec_icdf_t<uint8_t>(d, icdf, ftb); }
 
 // ec_icdf with the two-entry table {icdf0, 0}: symbol 0 takes [r*icdf0, rng),
 // symbol 1 takes [0, r*icdf0).
-CUO_FN int ec_icdf2(RangeDec *d, uint32_t icdf0, int ftb)
+CUO_FI int ec_icdf2(RangeDec *d, uint32_t icdf0, int ftb)
 {
     uint32_t rng = uniform(d->rng), v = uniform(d->val);
     uint32_t s = (rng >> ftb) * icdf0;
@@ -158,7 +203,7 @@ CUO_FN int ec_icdf2(RangeDec *d, uint32_t icdf0, int ftb)
     return 1;
 }
 
-CUO_FN uint32_t ec_bits(RangeDec *d, int bits)
+CUO_FI uint32_t ec_bits(RangeDec *d, int bits)
 {
     uint32_t window = uniform(d->end_window);
     int available = uniform(d->nend_bits);
@@ -201,11 +246,11 @@ CUO_FN uint32_t ec_uint(RangeDec *d, uint32_t ft)
     return s;
 }
 
-CUO_FN int ec_tell(const RangeDec *d) { return uniform(d->nbits_total) - ilog(uniform(d->rng)); }
+CUO_FI int ec_tell(const RangeDec *d) { return uniform(d->nbits_total) - ilog(uniform(d->rng)); }
 
 static constexpr uint32_t kTellCorrection[8] = {35733, 38967, 42495, 46340, 50535, 55109, 60097, 65535};
 
-CUO_FN uint32_t ec_tell_frac(const RangeDec *d)
+CUO_FI uint32_t ec_tell_frac(const RangeDec *d)
 {
     // 1/8-bit resolution via a linear estimate plus threshold table
     // (vendor/celt/entcode.c:69-86)

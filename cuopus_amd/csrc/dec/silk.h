@@ -113,6 +113,7 @@ struct SilkScratch {
     int16_t exc_buf[160];
     int16_t rs_buf[2 * 160 + 8];
     int32_t cng_sig[kSilkMaxFrame + kSilkMaxOrder];
+    int16_t nlsf_tmp[kSilkMaxOrder];
 };
 
 // ------------------------------------------------------------------ codebook access
@@ -267,10 +268,14 @@ CUO_FN void nlsf2a_find_poly(int32_t *out, const int32_t *cLSF, int dd)
 
 static constexpr uint8_t kNlsfOrdering16[16] = {0, 15, 8, 7, 4, 11, 12, 3, 2, 13, 10, 5, 6, 9, 14, 1};
 static constexpr uint8_t kNlsfOrdering10[10] = {0, 9, 6, 3, 4, 5, 8, 1, 2, 7};
+// inverse permutations: position j of cos_LSF_QA holds NLSF kNlsfInv*[j]
+static constexpr uint8_t kNlsfInv16[16] = {0, 15, 8, 7, 4, 11, 12, 3, 2, 13, 10, 5, 6, 9, 14, 1};
+static constexpr uint8_t kNlsfInv10[10] = {0, 7, 8, 3, 4, 5, 2, 9, 6, 1};
 
 // silk_NLSF2A (NLSF2A.c:66-141)
 CUO_FN void silk_nlsf2a(int16_t *a_Q12, const int16_t *NLSF, int d)
 {
+    CUO_PROF_SCOPE(kProfSilkNlsf2a);
     const uint8_t *ordering = d == 16 ? kNlsfOrdering16 : kNlsfOrdering10;
     int32_t cos_LSF_QA[kSilkMaxOrder];
     int32_t P[kSilkMaxOrder / 2 + 1], Q[kSilkMaxOrder / 2 + 1];
@@ -357,6 +362,7 @@ CUO_FN void silk_nlsf_stabilize(int16_t *NLSF_Q15, const int16_t *dmin, int L)
 
 CUO_FN void silk_nlsf_decode(int16_t *pNLSF_Q15, const int8_t *NLSFIndices, const NlsfCb &cb)
 {
+    CUO_PROF_SCOPE(kProfSilkNlsf);
     uint8_t pred_Q8[kSilkMaxOrder];
     int16_t ec_ix[kSilkMaxOrder];
     int16_t res_Q10[kSilkMaxOrder];
@@ -661,9 +667,13 @@ CUO_FN void silk_set_fs(SilkChannel *ch, int fs_kHz, int32_t fs_API_Hz)
 }
 
 // ------------------------------------------------------------------ bitstream
-CUO_FN void silk_decode_indices(SilkChannel *ch, RangeDec *d, int FrameIndex, int decode_LBRR, int condCoding)
+CUO_FN void silk_decode_indices(SilkChannel *ch, RangeDec *dp, int FrameIndex, int decode_LBRR, int condCoding)
 {
     CUO_PROF_SCOPE(kProfSilkIndices);
+    ch = lds_uniform(ch);
+    dp = lds_uniform(dp);
+    RangeDec dl = ec_regs(dp);  // decode on a register copy of the range decoder
+    RangeDec *d = &dl;
     SilkIndices &ix = ch->indices;
     int Ix;
     if (decode_LBRR || ch->VAD_flags[FrameIndex]) Ix = ec_icdf(d, tab::kTypeOffsetVadIcdf, 8) + 2;
@@ -678,12 +688,14 @@ CUO_FN void silk_decode_indices(SilkChannel *ch, RangeDec *d, int FrameIndex, in
     }
     for (int i = 1; i < ch->nb_subfr; i++) ix.GainsIndices[i] = (int8_t)ec_icdf(d, tab::kDeltaGainIcdf, 8);
     NlsfCb cb = nlsf_cb(ch->fs_kHz == 16);
-    ix.NLSFIndices[0] = (int8_t)ec_icdf(d, &cb.CB1_iCDF[(ix.signalType >> 1) * cb.nVectors], 8);
-    int16_t ec_ix[kSilkMaxOrder];
-    uint8_t pred_Q8[kSilkMaxOrder];
-    silk_nlsf_unpack(ec_ix, pred_Q8, cb, ix.NLSFIndices[0]);
+    const int cb1 = ec_icdf(d, &cb.CB1_iCDF[(ix.signalType >> 1) * cb.nVectors], 8);
+    ix.NLSFIndices[0] = (int8_t)cb1;
+    // entropy-table selector of coefficient i (silk_NLSF_unpack, NLSF_unpack.c:35-55)
+    const uint8_t *sel = &cb.ec_sel[cb1 * cb.order / 2];
     for (int i = 0; i < cb.order; i++) {
-        Ix = ec_icdf(d, &cb.ec_iCDF[ec_ix[i]], 8);
+        const int entry = (int)ld_u8(sel, i >> 1);
+        const int ec_ix = silk_smulbb((entry >> ((i & 1) ? 5 : 1)) & 7, 2 * 4 + 1);
+        Ix = ec_icdf(d, &cb.ec_iCDF[ec_ix], 8);
         if (Ix == 0) Ix -= ec_icdf(d, tab::kNlsfExtIcdf, 8);
         else if (Ix == 2 * 4) Ix += ec_icdf(d, tab::kNlsfExtIcdf, 8);
         ix.NLSFIndices[i + 1] = (int8_t)(Ix - 4);
@@ -713,6 +725,7 @@ CUO_FN void silk_decode_indices(SilkChannel *ch, RangeDec *d, int FrameIndex, in
     }
     ch->ec_prevSignalType = ix.signalType;
     ix.Seed = (int8_t)ec_icdf(d, tab::kUniform4Icdf, 8);
+    *dp = dl;
 }
 
 CUO_FN void silk_shell_split(int16_t *c1, int16_t *c2, RangeDec *d, int p, const uint8_t *tabl)
@@ -747,8 +760,96 @@ CUO_FN void silk_shell_decoder(int16_t *p0, RangeDec *d, int pulses4)
     silk_shell_split(&p0[14], &p0[15], d, p1[7], t0);
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// silk_decode_pulses on the wavefront (decode_pulses.c:37-115, code_signs.c:
+// 75-115).  The range decoder runs on a register copy of its state; per-block
+// pulse counts live one per lane; a block's 16 magnitudes are read and written
+// with one access per lane, and only the nonzero ones (a ballot) cost a sign
+// symbol, in ascending position order as in the serial loop.
+__device__ inline void silk_decode_pulses_lanes(RangeDec *dp, int16_t *pulses, int signalType, int quantOffsetType,
+                                                int frame_length)
+{
+    CUO_PROF_SCOPE(kProfSilkPulses);
+    dp = lds_uniform(dp);
+    pulses = lds_uniform(pulses);
+    RangeDec dl = ec_regs(dp);
+    RangeDec *d = &dl;
+    const int lane = lane_id();
+    const int RateLevelIndex = ec_icdf(d, tab::kRateLevelsIcdf + 9 * (signalType >> 1), 8);
+    const int iter = (frame_length + 15) >> 4;
+    const uint8_t *cdf = tab::kPulsesPerBlockIcdf + 18 * RateLevelIndex;
+    int32_t sumv = 0, lsv = 0;
+    int any_ls = 0;
+    {
+    CUO_PROF_SCOPE(kProfSilkPulseCounts);
+    for (int i = 0; i < iter; i++) {
+        int nls = 0;
+        int sp = ec_icdf(d, cdf, 8);
+        while (sp == 16 + 1) {
+            nls++;
+            sp = ec_icdf(d, tab::kPulsesPerBlockIcdf + 18 * 9 + (nls == 10), 8);
+        }
+        sumv = lane_set(sumv, i, sp);
+        lsv = lane_set(lsv, i, nls);
+        any_ls |= nls;
+    }
+    }
+    {
+    CUO_PROF_SCOPE(kProfSilkShell);
+    for (int i = 0; i < iter; i++) {
+        const int sp = lane_get(sumv, i);
+        if (sp > 0) silk_shell_decoder(&pulses[i * 16], d, sp);
+        else if (lane < 16) pulses[i * 16 + lane] = 0;
+    }
+    }
+    if (any_ls) {
+        for (int i = 0; i < iter; i++) {
+            const int nLS = lane_get(lsv, i);
+            if (nLS > 0) {
+                int16_t *pp = &pulses[i * 16];
+                int32_t qv = lane < 16 ? (int32_t)pp[lane] : 0;
+                for (int k = 0; k < 16; k++) {
+                    int32_t abs_q = lane_get(qv, k);
+                    for (int j = 0; j < nLS; j++) {
+                        abs_q = silk_lshift(abs_q, 1);
+                        abs_q += ec_icdf(d, tab::kLsbIcdf, 8);
+                    }
+                    qv = lane_set(qv, k, abs_q);
+                }
+                if (lane < 16) pp[lane] = (int16_t)qv;
+                sumv = lane == i ? (sumv | (nLS << 5)) : sumv;
+            }
+        }
+    }
+    const uint8_t *icdf_ptr = &tab::kSignIcdf[silk_smulbb(7, quantOffsetType + (signalType << 1))];
+    const int length = (frame_length + 16 / 2) >> 4;
+    CUO_PROF_SCOPE(kProfSilkSigns);
+    for (int i = 0; i < length; i++) {
+        const int p = lane_get(sumv, i);
+        if (p > 0) {
+            const uint32_t icdf0 = ld_u8(icdf_ptr, imin(p & 0x1F, 6));
+            int16_t *q = &pulses[i * 16];
+            const int32_t qv = lane < 16 ? (int32_t)q[lane] : 0;
+            uint64_t nz = __ballot(qv > 0);
+            uint32_t neg = 0;
+            while (nz) {
+                const int j = __builtin_ctzll(nz);
+                nz &= nz - 1;
+                if (!ec_icdf2(d, icdf0, 8)) neg |= 1u << j;
+            }
+            if (lane < 16 && ((neg >> lane) & 1u)) q[lane] = (int16_t)-qv;
+        }
+    }
+    *dp = dl;
+}
+#endif
+
 CUO_FN void silk_decode_pulses(RangeDec *d, int16_t *pulses, int signalType, int quantOffsetType, int frame_length)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    silk_decode_pulses_lanes(d, pulses, signalType, quantOffsetType, frame_length);
+    return;
+#endif
     CUO_PROF_SCOPE(kProfSilkPulses);
     int32_t sum_pulses[20], nLshifts[20];
     int RateLevelIndex = ec_icdf(d, tab::kRateLevelsIcdf + 9 * (signalType >> 1), 8);
@@ -797,6 +898,251 @@ CUO_FN void silk_decode_pulses(RangeDec *d, int16_t *pulses, int signalType, int
         q += 16;
     }
 }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// ---- parameter decoding on the wavefront.  Coefficient vectors (NLSFs,
+// polynomial and predictor coefficients) live one element per lane; the
+// serial recurrences of NLSF2A.c, LPC_fit.c, LPC_inv_pred_gain.c and
+// NLSF_stabilize.c become one lane-parallel step per stage.  Each stage
+// computes every element from the previous stage's values exactly as the
+// serial loop does, so the results are identical.
+
+// silk_bwexpander_32 (bwexpander_32.c:38-50): the chirp powers are a scalar
+// recurrence; element i is scaled by power i.
+__device__ inline int32_t bwexpander32_lanes(int32_t a, int d, int32_t chirp_Q16)
+{
+    const int32_t cm1 = chirp_Q16 - 65536;
+    int32_t cv = 0;
+    for (int i = 0; i < d; i++) {
+        cv = lane_set(cv, i, chirp_Q16);
+        chirp_Q16 += silk_rshift_round(silk_mul(chirp_Q16, cm1), 16);
+    }
+    return lane_id() < d ? silk_smulww(cv, a) : a;
+}
+
+// LPC_inv_pred_gain.c:45-121 on Q12 coefficients a (lane k < d); 0 if unstable
+__device__ inline int32_t inv_pred_gain_lanes(int32_t aQ12, int d)
+{
+    const int lane = lane_id();
+    const int32_t A_LIMIT = silk_fix(0.99975, 24);
+    const int32_t MIN_INV = silk_fix(1.0f / 1e4f, 30);
+    if (wave_sum(lane < d ? aQ12 : 0) >= 4096) return 0;
+    int32_t A = lane < d ? silk_lshift(aQ12, 24 - 12) : 0;
+    int32_t invGain = 1 << 30;
+    for (int k = d - 1; k > 0; k--) {
+        const int32_t Ak = lane_get(A, k);
+        if (Ak > A_LIMIT || Ak < -A_LIMIT) return 0;
+        const int32_t rc_Q31 = -silk_lshift(Ak, 31 - 24);
+        const int32_t rc_mult1_Q30 = (1 << 30) - silk_smmul(rc_Q31, rc_Q31);
+        invGain = silk_lshift(silk_smmul(invGain, rc_mult1_Q30), 2);
+        if (invGain < MIN_INV) return 0;
+        const int mult2Q = 32 - silk_clz32(rc_mult1_Q30 < 0 ? -rc_mult1_Q30 : rc_mult1_Q30);
+        const int32_t rc_mult2 = silk_inverse32_varq(rc_mult1_Q30, mult2Q + 30);
+        const int32_t mirror = __shfl(A, k - 1 - lane, 64);
+        int32_t nv = A;
+        int bad = 0;
+        if (lane < k) {
+            const int64_t t64 = silk_rshift_round64(
+                silk_smull(silk_sub_sat32(A, (int32_t)silk_rshift_round64(silk_smull(mirror, rc_Q31), 31)), rc_mult2),
+                mult2Q);
+            bad = t64 > INT32_MAX || t64 < INT32_MIN;
+            nv = (int32_t)t64;
+        }
+        if (__ballot(bad)) return 0;
+        A = nv;
+    }
+    const int32_t A0 = lane_get(A, 0);
+    if (A0 > A_LIMIT || A0 < -A_LIMIT) return 0;
+    const int32_t rc_Q31 = -silk_lshift(A0, 31 - 24);
+    const int32_t rc_mult1_Q30 = (1 << 30) - silk_smmul(rc_Q31, rc_Q31);
+    invGain = silk_lshift(silk_smmul(invGain, rc_mult1_Q30), 2);
+    if (invGain < MIN_INV) return 0;
+    return invGain;
+}
+
+// NLSF_stabilize.c:47-142 on NLSFs v (lane i < L); fb is LDS scratch for the
+// rare fallback after 20 rounds, which runs the serial code.
+__device__ inline int32_t nlsf_stabilize_lanes(int32_t v, const int16_t *dmin, int L, int16_t *fb)
+{
+    const int lane = lane_id();
+    const int32_t dm = lane <= L ? (int32_t)dmin[lane] : 0;
+    const int32_t dmin0 = ld_s16(dmin, 0), dminL = ld_s16(dmin, L);
+    for (int loops = 0; loops < 20; loops++) {
+        const int32_t pv = __shfl_up(v, 1, 64);
+        const int32_t base = lane < L ? v : (1 << 15);
+        const int32_t diff = lane <= L ? base - ((lane == 0 ? 0 : pv) + dm) : INT32_MAX;
+        const int32_t min_diff = wave_min(diff);
+        if (min_diff >= 0) return v;
+        const int I = __builtin_ctzll(__ballot(diff == min_diff));
+        if (I == 0) {
+            v = lane == 0 ? dmin0 : v;
+        } else if (I == L) {
+            v = lane == L - 1 ? (int32_t)(int16_t)((1 << 15) - dminL) : v;
+        } else {
+            const int32_t dI = lane_get(dm, I);
+            const int32_t min_center = wave_sum(lane < I ? dm : 0) + (dI >> 1);
+            const int32_t max_center = (1 << 15) - wave_sum(lane > I ? dm : 0) - (dI >> 1);
+            const int32_t center = (int16_t)silk_limit(silk_rshift_round(lane_get(v, I - 1) + lane_get(v, I), 1),
+                                                       min_center, max_center);
+            const int32_t n1 = (int16_t)(center - (dI >> 1));
+            const int32_t n2 = (int16_t)(n1 + dI);
+            v = lane == I - 1 ? n1 : (lane == I ? n2 : v);
+        }
+    }
+    if (lane < L) fb[lane] = (int16_t)v;
+    for (int i = 1; i < L; i++) {  // insertion sort, increasing
+        int16_t x = fb[i];
+        int j;
+        for (j = i - 1; j >= 0 && x < fb[j]; j--) fb[j + 1] = fb[j];
+        fb[j + 1] = x;
+    }
+    fb[0] = (int16_t)imax(fb[0], dmin0);
+    for (int i = 1; i < L; i++) fb[i] = (int16_t)imax(fb[i], silk_sat16((int32_t)fb[i - 1] + ld_s16(dmin, i)));
+    fb[L - 1] = (int16_t)imin(fb[L - 1], (1 << 15) - dminL);
+    for (int i = L - 2; i >= 0; i--) fb[i] = (int16_t)imin(fb[i], fb[i + 1] - ld_s16(dmin, i + 1));
+    return lane < L ? (int32_t)fb[lane] : 0;
+}
+
+// NLSF_decode.c:63-93: NLSFs one per lane
+__device__ inline int32_t nlsf_decode_lanes(const int8_t *NLSFIndices, const NlsfCb &cb, int16_t *fb)
+{
+    CUO_PROF_SCOPE(kProfSilkNlsf);
+    const int lane = lane_id();
+    const int order = cb.order;
+    const int cb1 = uniform((int)NLSFIndices[0]);
+    const uint8_t *sel = &cb.ec_sel[cb1 * order / 2];
+    int32_t out_Q10 = 0, res = 0;
+    for (int i = order - 1; i >= 0; i--) {
+        const int entry = (int)ld_u8(sel, i >> 1);
+        const int pred = (int)ld_u8(cb.pred_Q8, i + ((entry >> ((i & 1) ? 4 : 0)) & 1) * (order - 1));
+        const int32_t pred_Q10 = silk_smulbb(out_Q10, (int16_t)pred) >> 8;
+        out_Q10 = silk_lshift(uniform((int)NLSFIndices[1 + i]), 10);
+        if (out_Q10 > 0) out_Q10 = out_Q10 - silk_fix(0.1, 10);
+        else if (out_Q10 < 0) out_Q10 = out_Q10 + silk_fix(0.1, 10);
+        out_Q10 = silk_smlawb(pred_Q10, out_Q10, cb.quantStep_Q16);
+        res = lane_set(res, i, (int16_t)out_Q10);
+    }
+    int32_t v = 0;
+    if (lane < order) {
+        const int32_t w = cb.CB1_Wght_Q9[cb1 * order + lane];
+        const int32_t c = cb.CB1_Q8[cb1 * order + lane];
+        v = silk_limit(silk_lshift(res, 14) / w + silk_lshift((int16_t)c, 7), 0, 32767);
+    }
+    return nlsf_stabilize_lanes(v, cb.deltaMin_Q15, order, fb);
+}
+
+// NLSF2A.c:66-141 for order D; a_Q12 written to LDS
+template <int D>
+__device__ inline void nlsf2a_lanes(int16_t *a_Q12, int32_t nlsf)
+{
+    CUO_PROF_SCOPE(kProfSilkNlsf2a);
+    constexpr int dd = D / 2;
+    const int lane = lane_id();
+    // cos(NLSF) by table interpolation (NLSF2A.c:89-100), lane k
+    int32_t cosv = 0;
+    if (lane < D) {
+        const int32_t f_int = nlsf >> (15 - 7);
+        const int32_t f_frac = nlsf - silk_lshift(f_int, 15 - 7);
+        const int32_t cos_val = tab::kLsfCos_Q12[f_int];
+        const int32_t delta = tab::kLsfCos_Q12[f_int + 1] - cos_val;
+        cosv = silk_rshift_round(silk_lshift(cos_val, 8) + silk_mul(delta, f_frac), 20 - 16);
+    }
+    // cos_LSF_QA[ordering[k]] = cos of NLSF k, so position j holds lane inv[j]
+    const uint8_t *inv = D == 16 ? kNlsfInv16 : kNlsfInv10;
+    // P (row 0, lanes 0..dd) from positions 0,2,4..; Q (row 1, lanes 16..16+dd) from 1,3,5..
+    const int row = lane >> 4, n = lane & 15;
+    int32_t poly = n == 0 ? (1 << 16) : (n == 1 ? -(row == 0 ? lane_get(cosv, inv[0]) : lane_get(cosv, inv[1])) : 0);
+#pragma unroll
+    for (int k = 1; k < dd; k++) {
+        const int32_t fP = lane_get(cosv, inv[2 * k]), fQ = lane_get(cosv, inv[2 * k + 1]);
+        const int32_t ftmp = row == 0 ? fP : fQ;
+        const int32_t p1 = __builtin_amdgcn_update_dpp(0, poly, 0x111, 0xF, 0xF, false);  // out[n-1]
+        const int32_t p2 = __builtin_amdgcn_update_dpp(0, poly, 0x112, 0xF, 0xF, false);  // out[n-2]
+        const int32_t t = (int32_t)silk_rshift_round64(silk_smull(ftmp, p1), 16);
+        if (n >= 1 && n <= k) poly = poly + p2 - t;
+        else if (n == k + 1) poly = silk_lshift(p2, 1) - t;
+    }
+    // a32_QA1 (NLSF2A.c:114-120): lane i < dd takes -Qtmp - Ptmp of i, lane i >= dd
+    // takes Qtmp - Ptmp of D - 1 - i
+    const int m = lane < dd ? lane : D - 1 - lane;
+    const int mi = m < 0 ? 0 : m;
+    const int32_t P0 = __shfl(poly, mi, 64), P1 = __shfl(poly, mi + 1, 64);
+    const int32_t Q0 = __shfl(poly, 16 + mi, 64), Q1 = __shfl(poly, 16 + mi + 1, 64);
+    const int32_t Ptmp = P1 + P0, Qtmp = Q1 - Q0;
+    int32_t a = lane < dd ? -Qtmp - Ptmp : (lane < D ? Qtmp - Ptmp : 0);
+    // silk_LPC_fit (LPC_fit.c:36-81), QOUT 12, QIN 17
+    int it;
+    for (it = 0; it < 10; it++) {
+        const int32_t absv = lane < D ? (a > 0 ? a : -a) : 0;
+        int32_t maxabs = wave_max(absv);
+        const int idx = __builtin_ctzll(__ballot(lane < D && absv == maxabs));
+        maxabs = silk_rshift_round(maxabs, 17 - 12);
+        if (maxabs > 32767) {
+            maxabs = min32(maxabs, 163838);
+            const int32_t chirp_Q16 = silk_fix(0.999, 16) - silk_lshift(maxabs - 32767, 14) / ((maxabs * (idx + 1)) >> 2);
+            a = bwexpander32_lanes(a, D, chirp_Q16);
+        } else {
+            break;
+        }
+    }
+    int32_t q12;
+    if (it == 10) {
+        q12 = silk_sat16(silk_rshift_round(a, 17 - 12));
+        a = silk_lshift(q12, 17 - 12);
+    } else {
+        q12 = (int16_t)silk_rshift_round(a, 17 - 12);
+    }
+    for (int i = 0; inv_pred_gain_lanes(q12, D) == 0 && i < 16; i++) {
+        a = bwexpander32_lanes(a, D, 65536 - silk_lshift(2, i));
+        q12 = (int16_t)silk_rshift_round(a, 17 - 12);
+    }
+    if (lane < D) a_Q12[lane] = (int16_t)q12;
+}
+
+// silk_decode_parameters (decode_parameters.c:35-115) on the wavefront
+__device__ inline void silk_decode_parameters_lanes(SilkChannel *ch, SilkCtrl *ctrl, int condCoding, int16_t *fb)
+{
+    CUO_PROF_SCOPE(kProfSilkParams);
+    const int lane = lane_id();
+    silk_gains_dequant(ctrl->Gains_Q16, ch->indices.GainsIndices, &ch->LastGainIndex, condCoding == CODE_COND,
+                       ch->nb_subfr);
+    const NlsfCb cb = nlsf_cb(uniform(ch->fs_kHz) == 16);
+    const int order = uniform(ch->LPC_order);
+    const int32_t nlsf = nlsf_decode_lanes(ch->indices.NLSFIndices, cb, fb);
+    if (order == 16) nlsf2a_lanes<16>(ctrl->PredCoef_Q12[1], nlsf);
+    else nlsf2a_lanes<10>(ctrl->PredCoef_Q12[1], nlsf);
+    if (uniform(ch->first_frame_after_reset) == 1) ch->indices.NLSFInterpCoef_Q2 = 4;
+    const int icoef = uniform((int)ch->indices.NLSFInterpCoef_Q2);
+    if (icoef < 4) {
+        const int32_t prev = lane < order ? (int32_t)ch->prevNLSF_Q15[lane] : 0;
+        const int32_t nlsf0 = (int16_t)(prev + (silk_mul(icoef, nlsf - prev) >> 2));
+        if (order == 16) nlsf2a_lanes<16>(ctrl->PredCoef_Q12[0], nlsf0);
+        else nlsf2a_lanes<10>(ctrl->PredCoef_Q12[0], nlsf0);
+    } else {
+        CUO_PFOR(i, 0, order) ctrl->PredCoef_Q12[0][i] = ctrl->PredCoef_Q12[1][i];
+    }
+    if (lane < order) ch->prevNLSF_Q15[lane] = (int16_t)nlsf;
+    if (uniform(ch->lossCnt)) {
+        silk_bwexpander(ctrl->PredCoef_Q12[0], order, 63570);
+        silk_bwexpander(ctrl->PredCoef_Q12[1], order, 63570);
+    }
+    const int nb_subfr = uniform(ch->nb_subfr);
+    if (uniform((int)ch->indices.signalType) == SIG_VOICED) {
+        silk_decode_pitch(ch->indices.lagIndex, ch->indices.contourIndex, ctrl->pitchL, ch->fs_kHz, nb_subfr);
+        const int8_t *cbk = ltp_vq(uniform((int)ch->indices.PERIndex));
+        CUO_PFOR(j, 0, nb_subfr * kLtpOrder) {
+            const int k = j / kLtpOrder, i = j - k * kLtpOrder;
+            ctrl->LTPCoef_Q14[j] = (int16_t)silk_lshift(cbk[ch->indices.LTPIndex[k] * kLtpOrder + i], 7);
+        }
+        ctrl->LTP_scale_Q14 = tab::kLtpScales_Q14[ch->indices.LTP_scaleIndex];
+    } else {
+        CUO_PFOR(k, 0, nb_subfr) ctrl->pitchL[k] = 0;
+        CUO_PFOR(k, 0, kLtpOrder * nb_subfr) ctrl->LTPCoef_Q14[k] = 0;
+        ch->indices.PERIndex = 0;
+        ctrl->LTP_scale_Q14 = 0;
+    }
+}
+#endif
 
 CUO_FN void silk_decode_parameters(SilkChannel *ch, SilkCtrl *ctrl, int condCoding)
 {
@@ -1283,6 +1629,7 @@ CUO_FN void silk_plc(SilkChannel *ch, SilkCtrl *ctrl, int16_t *frame, int lost, 
 // ------------------------------------------------------------------ CNG (CNG.c)
 CUO_FN void silk_cng(SilkChannel *ch, const SilkCtrl *ctrl, int16_t *frame, int length, SilkScratch *sc)
 {
+    CUO_PROF_SCOPE(kProfSilkCng);
     SilkCngState *cng = &ch->cng;
     if (ch->fs_kHz != cng->fs_kHz) {
         silk_cng_reset(ch);
@@ -1353,7 +1700,11 @@ CUO_FN void silk_decode_frame(SilkChannel *ch, RangeDec *d, int16_t *pOut, int32
         int16_t *pulses = sc->pulses;
         silk_decode_indices(ch, d, ch->nFramesDecoded, lostFlag, condCoding);
         silk_decode_pulses(d, pulses, ch->indices.signalType, ch->indices.quantOffsetType, ch->frame_length);
+#if defined(__HIP_DEVICE_COMPILE__)
+        silk_decode_parameters_lanes(ch, ctrl, condCoding, sc->nlsf_tmp);
+#else
         silk_decode_parameters(ch, ctrl, condCoding);
+#endif
         silk_decode_core(ch, ctrl, pOut, pulses, sc);
         int mv_len = ch->ltp_mem_length - ch->frame_length;
         CUO_PFOR(i, 0, mv_len) ch->outBuf[i] = ch->outBuf[ch->frame_length + i];

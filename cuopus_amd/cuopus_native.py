@@ -343,7 +343,8 @@ PROF_SCOPES = ["chain", "frame", "celt", "celt_head", "celt_bands", "celt_synth"
                "silk", "emit", "celt_lost", "celt_energy", "celt_alloc", "celt_tail", "pvq", "cwrsi", "exp_rot",
                "theta", "leaf_fold", "band_pre", "band_post", "ec_uint", "alloc_bisect", "alloc_tail",
                "qp_leaf_bits", "bands_prologue", "lowband_out", "qp_stage", "silk_indices", "silk_pulses",
-               "silk_params", "silk_core", "silk_resamp", "silk_plc"]
+               "silk_params", "silk_core", "silk_resamp", "silk_plc", "silk_nlsf", "silk_nlsf2a", "silk_cng",
+               "silk_pulse_counts", "silk_shell", "silk_signs"]
 
 
 def prof_read():
