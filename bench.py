@@ -48,6 +48,13 @@ WORKLOADS = {
         "desc": "CELT-only FB mono 64 kbps VBR, frame size per 1 s from {2.5,5,10,20} ms, one 2 h file, "
                 "1 s chains + 250 ms preroll, one launch, f32 16 kHz out",
     },
+    "uneven_50": {
+        "glob": "corpus/bench/uneven_50/f*.opus",
+        "desc": "the first 50 files of the uneven set (3.5 h, about 12.6k chains): mono speech, durations "
+                "log-uniform 5 s-30 min, mode per file from SILK-NB/WB, hybrid-SWB/FB, CELT-FB, bitrate "
+                "redrawn every 1 s in 12-64 kbps, 20 ms frames; all chains in one persistent kernel vs one "
+                "launch per file",
+    },
     "uneven_250": {
         "glob": "corpus/bench/uneven_250/f*.opus",
         "desc": "250 mono speech files, durations log-uniform 5 s-30 min (18.1 h), mode per file from "
@@ -71,6 +78,9 @@ def _args():
                     help="run the autotuner on this workload first and use its best launch shape")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-file comparison")
+    ap.add_argument("--no-batching", action="store_true",
+                    help="skip the many-file batching comparison (persistent vs one launch per file on the "
+                         "uneven_50 set, when present)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-audio-s", type=float, default=7200.0,
                     help="playable audio-seconds of the workload the CPU baseline decodes")
@@ -167,6 +177,29 @@ def _cpu_baseline(blobs, threads, max_audio_s=7200.0):
         "sample": "%d files, %d chains, %.0f audio-s: fixed-point libopus 1.5.2 decode (%.2f s) + float32 "
                   "resampler (%.2f s), %d threads" % (nfiles, len(chains), secs, t1 - t0, t2 - t1, threads),
     }
+
+
+def _batching(cn, torch, steps, name="uneven_50"):
+    """Many files of uneven length (config 3 shape, >= 10k chains): every chain
+    in one persistent launch vs one decode + one resample launch per file."""
+    import glob
+    paths = sorted(glob.glob(os.path.join(ROOT, WORKLOADS[name]["glob"])))
+    if not paths:
+        return None
+    blobs = [open(p, "rb").read() for p in paths]
+    out = {"workload": name, "files": len(blobs)}
+    for key, mode in (("persistent", cn.MODE_PERSISTENT), ("plain", cn.MODE_PLAIN)):
+        b = cn.Batch(blobs, mode=mode)
+        wall, _, _ = _time_batch(b, steps, 1, torch)
+        out["chains"] = b.chains
+        out["audio_s"] = round(b.playable / 48000.0, 1)
+        out[key + "_value"] = round(b.playable / 48000.0 * steps / wall, 1)
+        out[key + "_ms_per_step"] = round(1000.0 * wall / steps, 2)
+        b.close()
+        del b
+        _log("batching %s %s: %.1f ms/step" % (name, key, out[key + "_ms_per_step"]))
+    out["batched_vs_plain"] = round(out["persistent_value"] / out["plain_value"], 3)
+    return out
 
 
 def _serve(cn, torch, qps_list, seconds, seed=0x5E7E):
@@ -298,6 +331,10 @@ def main():
         bp.close()
         del bp
 
+    batching = None
+    if not a.no_batching and not a.workload.startswith("uneven"):
+        batching = _batching(cn, torch, max(1, min(a.steps, 3)))
+
     serve = None
     if a.serve_qps:
         serve = _serve(cn, torch, [float(q) for q in a.serve_qps.split(",")], a.serve_seconds)
@@ -329,6 +366,7 @@ def main():
         "roofline": roof,
         "resampler_roofline": rs_roof,
         "cpu_baseline": cpu,
+        "batching": batching,
         "serve": serve,
     }
     print(json.dumps(line), flush=True)

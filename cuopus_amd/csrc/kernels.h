@@ -11,8 +11,8 @@
 //   persistent variant, or one per workgroup in the plain variant.
 // k_resample_48to16: 49-tap Kaiser polyphase decimator, f32 out
 //   (cuopus_kernels.cuh:441-456; gpu_resample.py:20-49).  A workgroup stages
-//   its int16 input tile plus the 24-sample halo on each side in LDS once and
-//   every output reads its taps from LDS.
+//   its int16 input tile plus the 24-sample halo on each side in LDS once;
+//   each thread computes 4 consecutive outputs from a register window.
 #pragma once
 #include "dec/opus_dec.h"
 
@@ -189,15 +189,28 @@ struct RsFile {
     uint64_t tile0;    // first global tile index of this file
 };
 
-constexpr int kRsTile = 256;  // outputs per workgroup
+constexpr int kRsThreads = 256;                   // threads per workgroup
+constexpr int kRsPerThread = 4;                   // consecutive outputs per thread
+constexpr int kRsTile = kRsThreads * kRsPerThread;  // outputs per workgroup
+constexpr int kRsIn = 3 * kRsTile + 2 * kRsPad;     // staged input samples per workgroup
+constexpr int kRsWin = (3 * (kRsPerThread - 1) + kRsTaps + 3) & ~3;  // one thread's window, float4 reads
+static_assert(kRsIn % 8 == 0, "staging works in 16-byte chunks of 8 samples");
+static_assert(3 * kRsPerThread * (kRsThreads - 1) + kRsWin <= kRsIn, "the last window stays inside the tile");
 
 // Batched resampler: workgroup b handles tile b of the concatenated per-file
 // tile list; out[i] = sum_t taps[t] * x[3 i - 24 + t] / 32768 with zeros
-// outside [0, in_len), accumulated in tap order.
-__global__ __launch_bounds__(kRsTile) void k_resample_48to16(const int16_t *__restrict__ in, const RsFile *__restrict__ files,
-                                                               int n_files, RsTaps taps)
+// outside [0, in_len), each output accumulated in tap order with fused
+// multiply-adds.  The tile's input (3 * 1024 samples plus the 24-sample halo
+// on each side) is staged in LDS as f32, with 16-byte loads where the tile
+// lies inside the file and its start is 16-byte aligned; thread t then reads
+// its 60-sample window as 15 float4 (lanes 48 bytes apart: every group of 16
+// lanes covers all LDS banks once) and produces 4 consecutive outputs, stored
+// as one float4 when aligned.
+__global__ __launch_bounds__(kRsThreads) void k_resample_48to16(const int16_t *__restrict__ in,
+                                                                const RsFile *__restrict__ files, int n_files,
+                                                                RsTaps taps)
 {
-    __shared__ float xs[3 * kRsTile + 2 * kRsPad + 4];
+    __shared__ __align__(16) float xs[kRsIn];
     const uint64_t b = blockIdx.x;
     // find the file of this tile (files are few per launch; binary search)
     int lo = 0, hi = n_files - 1;
@@ -208,20 +221,58 @@ __global__ __launch_bounds__(kRsTile) void k_resample_48to16(const int16_t *__re
     const RsFile f = files[lo];
     const uint64_t o0 = (b - f.tile0) * kRsTile;
     const int64_t base = (int64_t)(3 * o0) - kRsPad;
-    const int nin = 3 * kRsTile + 2 * kRsPad;
     const int16_t *x = in + f.in_off;
-    for (int i = threadIdx.x; i < nin; i += kRsTile) {
-        int64_t gi = base + i;
-        xs[i] = (gi >= 0 && gi < (int64_t)f.in_len) ? (float)x[gi] * (1.0f / 32768.0f) : 0.f;
+    const int tid = (int)threadIdx.x;
+    const bool inside = base >= 0 && base + kRsIn <= (int64_t)f.in_len;
+    if (inside && ((uintptr_t)(x + base) & 15) == 0) {
+        const int4 *src = (const int4 *)(x + base);
+        for (int c = tid; c < kRsIn / 8; c += kRsThreads) {
+            const int4 v = src[c];
+            const int32_t w4[4] = {v.x, v.y, v.z, v.w};
+            float y[8];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                y[2 * k] = (float)(int16_t)(w4[k] & 0xFFFF) * (1.0f / 32768.0f);
+                y[2 * k + 1] = (float)(int16_t)((uint32_t)w4[k] >> 16) * (1.0f / 32768.0f);
+            }
+            float4 *d = (float4 *)&xs[8 * c];
+            d[0] = make_float4(y[0], y[1], y[2], y[3]);
+            d[1] = make_float4(y[4], y[5], y[6], y[7]);
+        }
+    } else {
+        for (int i = tid; i < kRsIn; i += kRsThreads) {
+            const int64_t gi = base + i;
+            xs[i] = (gi >= 0 && gi < (int64_t)f.in_len) ? (float)x[gi] * (1.0f / 32768.0f) : 0.f;
+        }
     }
     __syncthreads();
-    const uint64_t o = o0 + threadIdx.x;
+    const uint64_t o = o0 + (uint64_t)(kRsPerThread * tid);
     if (o >= f.out_len) return;
-    const float *w = &xs[3 * threadIdx.x];
-    float acc = 0.f;
+    float w[kRsWin];
+    const float4 *wv = (const float4 *)&xs[3 * kRsPerThread * tid];
 #pragma unroll
-    for (int t = 0; t < kRsTaps; t++) acc += taps.t[t] * w[t];
-    f.out[o] = acc;
+    for (int m = 0; m < kRsWin / 4; m++) {
+        const float4 v = wv[m];
+        w[4 * m] = v.x;
+        w[4 * m + 1] = v.y;
+        w[4 * m + 2] = v.z;
+        w[4 * m + 3] = v.w;
+    }
+    float acc[kRsPerThread];
+#pragma unroll
+    for (int j = 0; j < kRsPerThread; j++) acc[j] = 0.f;
+#pragma unroll
+    for (int t = 0; t < kRsTaps; t++)
+#pragma unroll
+        for (int j = 0; j < kRsPerThread; j++) acc[j] = __builtin_fmaf(taps.t[t], w[3 * j + t], acc[j]);
+    float *dst = f.out + o;
+    if (o + kRsPerThread <= f.out_len && ((uintptr_t)dst & 15) == 0) {
+        *(float4 *)dst = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kRsPerThread; j++)
+            if (o + j < f.out_len) dst[j] = acc[j];
+    }
 }
 
 }  // namespace cuo

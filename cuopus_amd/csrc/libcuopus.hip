@@ -158,7 +158,8 @@ struct Plan {
     std::vector<uint8_t> lost;
     std::vector<ChainDesc> chains;
     std::vector<uint64_t> cap;   // per chain: sum of TOC durations
-    uint64_t total_keep = 0;
+    uint64_t total_keep = 0;  // playable samples of all files
+    uint64_t cat_len = 0;     // length of the compacted PCM buffer (file starts padded to 8 samples)
 };
 
 int demux_file(const uint8_t *ogg, size_t n, FilePlan *fp, bool with_chains, Err &e)
@@ -194,7 +195,7 @@ int merge(Plan &pl, Err &e)
     pl.len.resize(npk);
     pl.chains.resize(nch);
     pl.cap.resize(nch);
-    uint64_t db = 0, pb = 0, cb = 0, cat = 0;
+    uint64_t db = 0, pb = 0, cb = 0, cat = 0, keep_sum = 0;
     for (auto &f : pl.files) {
         memcpy(pl.data.data() + db, f.d.data.data(), f.d.data.size());
         for (size_t k = 0; k < f.d.off.size(); k++) {
@@ -228,12 +229,15 @@ int merge(Plan &pl, Err &e)
             fk += c.keep;
         }
         if (fk != f.keep) return e.set(CUOPUS_RC_SOFT, "internal: keep mismatch");
-        cat += fk;
+        // each file's PCM starts 16-byte aligned (the resampler's wide loads)
+        cat += (fk + 7) & ~7ull;
+        keep_sum += fk;
         db += f.d.data.size();
         pb += f.d.off.size();
         cb += f.chains.size();
     }
-    pl.total_keep = cat;
+    pl.total_keep = keep_sum;
+    pl.cat_len = cat;
     return 0;
 }
 
@@ -284,7 +288,7 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     if (!pl.lost.empty() && (rc = grow(W.lost, pl.lost.size(), e))) return rc;
     int16_t *out = o.s16_out;
     if (!out) {
-        if ((rc = grow(W.cat, 2 * (pl.total_keep ? pl.total_keep : 1), e))) return rc;
+        if ((rc = grow(W.cat, 2 * (pl.cat_len ? pl.cat_len : 1), e))) return rc;
         out = (int16_t *)W.cat.p;
     }
     const uint32_t n_waves = std::max<uint32_t>(1, std::min<uint32_t>(env_u("CUOPUS_SLOTS", kDefaultWaves), nch));
@@ -345,7 +349,7 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
             if ((rc = grow(W.files, sizeof(RsFile) * rf.size(), e))) return rc;
             HIPCHK(e, hipMemcpyAsync(W.files.p, rf.data(), sizeof(RsFile) * rf.size(), hipMemcpyHostToDevice, st));
             static RsTaps taps = make_taps();
-            hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)tiles), dim3(kRsTile), 0, st, out,
+            hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)tiles), dim3(kRsThreads), 0, st, out,
                                (const RsFile *)W.files.p, (int)rf.size(), taps);
             HIPCHK(e, hipGetLastError());
         }
@@ -639,6 +643,7 @@ extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint
         cat += keep[c];
     }
     pl.total_keep = cat;
+    pl.cat_len = cat;
     if (n_chains == 0) return 0;
     if ((rc = grow(W.nsamp, 4ull * n_packets, e))) return rc;
     if ((rc = grow(W.rng, 4ull * n_packets, e))) return rc;
@@ -682,7 +687,7 @@ extern "C" int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, f
     HIPCHK(e, hipMemcpyAsync(W.files.p, &r, sizeof(RsFile), hipMemcpyHostToDevice, st));
     static RsTaps taps = make_taps();
     unsigned tiles = (unsigned)((n16 + kRsTile - 1) / kRsTile);
-    hipLaunchKernelGGL(k_resample_48to16, dim3(tiles), dim3(kRsTile), 0, st, in_dev, (const RsFile *)W.files.p, 1, taps);
+    hipLaunchKernelGGL(k_resample_48to16, dim3(tiles), dim3(kRsThreads), 0, st, in_dev, (const RsFile *)W.files.p, 1, taps);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipStreamSynchronize(st));
     if (out_len) *out_len = n16;
@@ -761,7 +766,7 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
             }
             const int r = b->file_rs[f];
             if (r >= 0) {
-                hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)b->file_tiles[r]), dim3(kRsTile), 0, st,
+                hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)b->file_tiles[r]), dim3(kRsThreads), 0, st,
                                    (const int16_t *)b->cat.p, (const RsFile *)b->files.p + r, 1, b->taps);
                 HIPCHK(e, hipGetLastError());
             }
@@ -789,7 +794,7 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
     if (prof) HIPCHK(e, hipEventRecord(b->ev[1], st));
     if (b->tiles) {
         if (prof) HIPCHK(e, hipEventRecord(b->ev[2], st));
-        hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)b->tiles), dim3(kRsTile), 0, st,
+        hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)b->tiles), dim3(kRsThreads), 0, st,
                            (const int16_t *)b->cat.p, (const RsFile *)b->files.p, (int)b->rsf.size(), b->taps);
         HIPCHK(e, hipGetLastError());
         if (prof) HIPCHK(e, hipEventRecord(b->ev[3], st));
@@ -898,7 +903,7 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
     if ((rc = grow(b->data, pl.data.size() + 4, e)) || (rc = grow(b->off, 4 * npk, e)) ||
         (rc = grow(b->len, 4 * npk, e)) || (rc = grow(b->chains, sizeof(ChainDesc) * nch, e)) ||
         (rc = grow(b->order, 4ull * nch, e)) || (rc = grow(b->res, sizeof(ChainResult) * nch, e)) ||
-        (rc = grow(b->cat, 2 * (pl.total_keep ? pl.total_keep : 1), e)) || (rc = grow(b->queue, 256, e)) ||
+        (rc = grow(b->cat, 2 * (pl.cat_len ? pl.cat_len : 1), e)) || (rc = grow(b->queue, 256, e)) ||
         (rc = grow(b->heap, sizeof(DecHeap) * (size_t)b->n_heaps, e)) ||
         (rc = grow(b->files, sizeof(RsFile) * (rsf_upload.size() ? rsf_upload.size() : 1), e)) ||
         (!pl.lost.empty() && (rc = grow(b->lost, pl.lost.size(), e)))) {
@@ -1128,7 +1133,7 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
     if ((rc = grow(L.data, pl.data.size() + 4, e)) || (rc = grow(L.off, 4 * npk, e)) ||
         (rc = grow(L.len, 4 * npk, e)) || (rc = grow(L.chains, sizeof(ChainDesc) * nch, e)) ||
         (rc = grow(L.order, 4ull * nch, e)) || (rc = grow(L.res, sizeof(ChainResult) * nch, e)) ||
-        (rc = grow(L.cat, 2 * (pl.total_keep ? pl.total_keep : 1), e)) || (rc = grow(L.queue, 256, e)) ||
+        (rc = grow(L.cat, 2 * (pl.cat_len ? pl.cat_len : 1), e)) || (rc = grow(L.queue, 256, e)) ||
         (rc = grow(L.files, sizeof(RsFile) * (rsf.empty() ? 1 : rsf.size()), e)) ||
         (rc = grow(L.heap, sizeof(DecHeap) * (size_t)waves, e)) ||
         (any_lost && (rc = grow(L.lost, npk, e))))
@@ -1178,7 +1183,7 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
     launch_persistent(waves, st, a);
     HIPCHK(e, hipGetLastError());
     if (tiles) {
-        hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)tiles), dim3(kRsTile), 0, st, (const int16_t *)L.cat.p,
+        hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)tiles), dim3(kRsThreads), 0, st, (const int16_t *)L.cat.p,
                            (const RsFile *)L.files.p, (int)rsf.size(), srv->taps);
         HIPCHK(e, hipGetLastError());
     }

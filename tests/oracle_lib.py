@@ -200,6 +200,9 @@ BENCH_SET_MODES = {
 BENCH_SETS = {
     # 250 files, durations exp(U(ln 5, ln 1800)) s
     "uneven_250": dict(n=250, seed=0x0C0FFEE3, dmin=5.0, dmax=1800.0),
+    # the first 50 files of uneven_250 (identical files: same per-file seeds),
+    # about 12.6k chains: small enough to ship with every GPU run
+    "uneven_50": dict(n=50, seed=0x0C0FFEE3, dmin=5.0, dmax=1800.0),
     # serving request pool: durations lognormal (median 45 s, sigma 1.0)
     # clipped to [2 s, 7200 s]
     "serve_pool": dict(n=60, seed=0x0C0FFEE4, dmin=2.0, dmax=7200.0, median=45.0, sigma=1.0),
