@@ -33,6 +33,8 @@ struct CeltState {
     // the device: it is only touched by lane-parallel loops and concealment)
     int32_t *mem;
     struct CeltPlcScratch *plc;  // pitch-concealment buffers, caller-owned
+    int32_t *freq;               // synthesis spectrum (960), caller-owned: only lane-parallel loops touch it
+    int32_t *etmp;               // prefilter_and_fold output (kOverlap), caller-owned
     int32_t start, end;
     // ---- cleared by reset (OPUS_RESET_STATE) from here on
     uint32_t rng;
@@ -66,7 +68,6 @@ struct CeltPlcScratch {
 struct CeltScratch {
     // frame decode and noise-based concealment
     int16_t X[960];
-    int32_t freq[960];
     int16_t norm[8 * 78];
     int16_t tmp[176];
     int32_t iy[176];
@@ -74,7 +75,6 @@ struct CeltScratch {
     int32_t cap[kNbBands], offsets[kNbBands], tf_res[kNbBands];
     int32_t bits1[kNbBands], bits2[kNbBands], thresh[kNbBands], trim_offset[kNbBands];
     uint8_t collapse[kNbBands];
-    int32_t etmp[kOverlap];
     RangeDec dec;  // decoder of a frame opened here (redundancy frames)
     // explicit stack of the band splitter (quant_partition), 6 frames
     alignas(8) unsigned char qp_stack[6 * 96];
@@ -545,6 +545,113 @@ CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
     return yy;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Lane-parallel PVQ codeword decode (device).  Lane j holds U(n, j) for the
+// current dimension n: the first row is one vector gather from the U table
+// (U(j, n) = U(n, j); the row of min(j, n) holds it), each later row follows
+// from the one above (pvq_row_down).  The table walks of cwrs.c:463-537 --
+// "largest k' below a bound with U(n, k') <= i" -- become one ballot over the
+// lanes: U(n, .) is non-decreasing, so the lanes that pass form a prefix and
+// the highest passing lane is the walk's result.  Needs k + 1 < 64; lanes
+// beyond the ones a step reads are masked out of every ballot, and the
+// gather index is clamped into the table.
+__device__ __forceinline__ uint32_t pvq_row_lanes(int n, int row_lane, int row_n)
+{
+    const int j = lane_id();
+    int idx = j < n ? row_lane + n : row_n + j;
+    idx = idx < 0 ? 0 : (idx > 1271 ? 1271 : idx);
+    return tab::kPvqU[idx];
+}
+__device__ __forceinline__ int pvq_row_off(int r) { return tab::kPvqURow[r < 14 ? r : 14]; }
+
+// Next row down from the current one, without memory: the recurrence
+// U(n,j) = U(n-1,j) + U(n,j-1) + U(n-1,j-1) gives
+// U(n-1,j) + U(n-1,j-1) = U(n,j) - U(n,j-1), and with U(n-1,0) = 0
+// (n-1 >= 1) that telescopes to U(n-1,j) = (-1)^j (2 P(j) - F(j)), where
+// F(t) = (-1)^t U(n,t) and P is the inclusive prefix sum of F.  P is one
+// 16-lane DPP scan per row (row_shr 1, 2, 4, 8) plus the totals of the rows
+// below, read from lanes 15, 31 and 47.  The arithmetic wraps mod 2^32, so
+// every lane whose true value fits 32 bits -- all lanes up to the highest one
+// a decode step reads -- is exact, and a lane only depends on lanes below it.
+__device__ __forceinline__ uint32_t pvq_row_down(uint32_t row)
+{
+    const int l = lane_id();
+    const uint32_t f = (l & 1) ? 0u - row : row;
+    const uint32_t v = (uint32_t)row16_prefix_sum((int32_t)f);
+    const uint32_t t0 = (uint32_t)lane_get((int32_t)v, LaneK<15>());
+    const uint32_t t1 = (uint32_t)lane_get((int32_t)v, LaneK<31>());
+    const uint32_t t2 = (uint32_t)lane_get((int32_t)v, LaneK<47>());
+    const uint32_t below = l < 16 ? 0u : (l < 32 ? t0 : (l < 48 ? t0 + t1 : t0 + t1 + t2));
+    const uint32_t sv = 2u * (v + below) - f;
+    return (l & 1) ? 0u - sv : sv;
+}
+__device__ __forceinline__ int highest_lane(uint64_t m) { return m ? 63 - __builtin_clzll(m) : 0; }
+
+// cwrsi with the row of the first dimension already gathered (row = U(n, .))
+__device__ __noinline__ int32_t cwrsi_lanes(int n, int k, uint32_t i, int32_t *y, uint32_t row)
+{
+    n = uniform(n);
+    k = uniform(k);
+    i = uniform(i);
+    y = lds_uniform(y);
+    const int l = lane_id();
+    int32_t yy = 0;
+    while (n > 2) {
+        const uint32_t next = pvq_row_down(row);
+        uint32_t p;
+        int s, k0;
+        int16_t val;
+        if (k >= n) {
+            p = (uint32_t)lane_get((int32_t)row, k + 1);
+            s = -(i >= p);
+            i -= p & (uint32_t)s;
+            k0 = k;
+            const uint32_t q = (uint32_t)lane_get((int32_t)row, n);
+            const int bound = q > i ? n - 1 : k;  // largest k' <= bound with U(n, k') <= i
+            k = highest_lane(__ballot(l <= bound && row <= i));
+            p = (uint32_t)lane_get((int32_t)row, k);
+            i -= p;
+            val = (int16_t)((k0 - k + s) ^ s);
+            *y++ = val;
+            yy = mac16_16(yy, val, val);
+        } else {
+            p = (uint32_t)lane_get((int32_t)row, k);
+            const uint32_t q = (uint32_t)lane_get((int32_t)row, k + 1);
+            if (p <= i && i < q) {
+                i -= p;
+                *y++ = 0;
+            } else {
+                s = -(i >= q);
+                i -= q & (uint32_t)s;
+                k0 = k;
+                k = highest_lane(__ballot(l < k && row <= i));
+                p = (uint32_t)lane_get((int32_t)row, k);
+                i -= p;
+                val = (int16_t)((k0 - k + s) ^ s);
+                *y++ = val;
+                yy = mac16_16(yy, val, val);
+            }
+        }
+        n--;
+        row = next;
+    }
+    uint32_t p = 2 * (uint32_t)k + 1;
+    int s = -(i >= p);
+    i -= p & (uint32_t)s;
+    int k0 = k;
+    k = (int)((i + 1) >> 1);
+    if (k) i -= 2 * (uint32_t)k - 1;
+    int16_t val = (int16_t)((k0 - k + s) ^ s);
+    *y++ = val;
+    yy = mac16_16(yy, val, val);
+    s = -(int)i;
+    val = (int16_t)((k + s) ^ s);
+    *y = val;
+    yy = mac16_16(yy, val, val);
+    return yy;
+}
+#endif
+
 // exp_rotation1 (vq.c:74-95): a forward then a backward pass of 2x2
 // rotations between X[i] and X[i+stride].  Residue classes mod stride are
 // independent chains; stride 1 is one serial chain, run register-blocked.
@@ -555,20 +662,25 @@ CUO_FN void exp_rotation1(int16_t *X, int len, int stride, int16_t c, int16_t s)
     len = uniform(len);
     const int16_t ms = (int16_t)-s;
     if (stride == 1) {
+        // Only the carried value is serial: step i's carry is
+        // (c*X[i+1] + s*carry + 2^14) >> 15, so c*X[i+1] + 2^14 is formed for
+        // the whole block first, the serial pass records the carry entering
+        // each step in that step's lane, and the block's outputs are then
+        // formed lane-parallel from it.
+        const int l = lane_id();
         // forward: carry = X[i] as updated by step i-1
         if (len >= 2) {
             int32_t carry = uniform((int32_t)X[0]);
             for (int i0 = 0; i0 < len - 1; i0 += kLanes) {
                 const int nb = uniform(imin(kLanes, len - 1 - i0));
-                const int l = lane_id();
-                int32_t v = l < nb ? X[i0 + l + 1] : 0;
-                int32_t o = 0;
-                for (int k = 0; k < nb; k++) {
-                    int32_t x1 = carry, x2 = lane_get(v, k);
-                    o = lane_set(o, k, (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15));
-                    carry = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
-                }
-                if (l < nb) X[i0 + l] = (int16_t)o;
+                const int32_t x2 = l < nb ? X[i0 + l + 1] : 0;
+                const int32_t p = mul16_16(c, x2) + 16384;
+                int32_t x1 = 0;
+                serial_lanes(nb, [&](auto k) {
+                    x1 = lane_set(x1, k, carry);
+                    carry = (int16_t)((lane_get(p, k) + mul16_16(s, carry)) >> 15);
+                });
+                if (l < nb) X[i0 + l] = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
             }
             X[len - 1] = (int16_t)carry;
         }
@@ -577,15 +689,14 @@ CUO_FN void exp_rotation1(int16_t *X, int len, int stride, int16_t c, int16_t s)
             int32_t carry = uniform((int32_t)X[len - 2]);
             for (int ih = len - 3; ih >= 0; ih -= kLanes) {
                 const int nb = uniform(imin(kLanes, ih + 1));
-                const int l = lane_id();
-                int32_t v = l < nb ? X[ih - l] : 0;
-                int32_t o = 0;
-                for (int k = 0; k < nb; k++) {
-                    int32_t x1 = lane_get(v, k), x2 = carry;
-                    o = lane_set(o, k, (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15));
-                    carry = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
-                }
-                if (l < nb) X[ih - l + 1] = (int16_t)o;
+                const int32_t x1 = l < nb ? X[ih - l] : 0;
+                const int32_t q = mul16_16(c, x1) + 16384;
+                int32_t x2 = 0;
+                serial_lanes(nb, [&](auto k) {
+                    x2 = lane_set(x2, k, carry);
+                    carry = (int16_t)((lane_get(q, k) + mul16_16(ms, carry)) >> 15);
+                });
+                if (l < nb) X[ih - l + 1] = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
             }
             X[0] = (int16_t)carry;
         }
@@ -639,12 +750,28 @@ CUO_FN unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDe
 {
     CUO_PROF_SCOPE(kProfPvq);
     uint32_t idx;
-    {
-        CUO_PROF_SCOPE(kProfEcUint);
-        idx = ec_uint(d, pvq_v(N, K));
-    }
     int32_t Ryy;
+#if defined(__HIP_DEVICE_COMPILE__)
+    N = uniform(N);
+    K = uniform(K);
+    if (K + 1 < 64 && N > 2) {
+        // V(N, K) = U(N, K) + U(N, K + 1) from the first dimension's row
+        const int row_lane = pvq_row_off(lane_id());
+        const uint32_t row = pvq_row_lanes(N, row_lane, lane_get(row_lane, imin(N, 14)));
+        const uint32_t v = (uint32_t)lane_get((int32_t)row, K) + (uint32_t)lane_get((int32_t)row, K + 1);
+        {
+            CUO_PROF_SCOPE(kProfEcUint);
+            idx = ec_uint(d, v);
+        }
+        CUO_PROF_SCOPE(kProfCwrsi);
+        Ryy = cwrsi_lanes(N, K, idx, iy, row);
+    } else
+#endif
     {
+        {
+            CUO_PROF_SCOPE(kProfEcUint);
+            idx = ec_uint(d, pvq_v(N, K));
+        }
         CUO_PROF_SCOPE(kProfCwrsi);
         Ryy = cwrsi(N, K, idx, iy);
     }
@@ -1367,7 +1494,6 @@ CUO_FN void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *ol
     CUO_PROF_SCOPE(kProfCeltSynth);
     X = lds_uniform(X);
     oldBandE = lds_uniform(oldBandE);
-    freq = lds_uniform(freq);
     const int N = 120 << LM, M = 1 << LM;
     int B, NB, shift;
     if (isTransient) { B = M; NB = 120; shift = 3; }
@@ -1453,11 +1579,11 @@ CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int 
         const int l = lane_id();
         int32_t v = l < nb ? x[j0 + l] : 0;
         int32_t t = 0;
-        for (int k = 0; k < nb; k++) {
+        serial_lanes(nb, [&](auto k) {
             int32_t tmp = lane_get(v, k) + m;
             m = mul16_32_q15(coef0, tmp);
             t = lane_set(t, k, tmp);
-        }
+        });
         if (l < nb) {
             int16_t y = sig2word16(t);
             pcm[j0 + l] = accum ? sat16(pcm[j0 + l] + y) : y;
@@ -1494,7 +1620,7 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
         int end = st->end;
         int effEnd = imax(start, imin(end, kNbBands));
         shift_history(mem, N);
-        if (st->prefilter_and_fold) prefilter_and_fold(st, N, sc->etmp);
+        if (st->prefilter_and_fold) prefilter_and_fold(st, N, st->etmp);
         int16_t decay = loss_duration == 0 ? 1536 : 512;
         for (int i = start; i < end; i++)
             st->oldBandE[i] = max16(st->backgroundLogE[i], (int16_t)(st->oldBandE[i] - decay));
@@ -1510,7 +1636,7 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
             renormalise_vector(X + boffs, blen, kQ15One);
         }
         st->rng = seed;
-        celt_synthesis(X, out_syn, st->oldBandE, start, effEnd, 0, LM, 0, sc->freq);
+        celt_synthesis(X, out_syn, st->oldBandE, start, effEnd, 0, LM, 0, st->freq);
         st->prefilter_and_fold = 0;
         st->skip_plc = 1;
     } else {
@@ -1665,8 +1791,8 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
         anti_collapse(X, sc->collapse, LM, start, end, oldBandE, oldLogE, oldLogE2, pulses, st->rng);
     if (silence)
         for (int i = 0; i < kNbBands; i++) oldBandE[i] = -(28 << kDbShift);
-    if (st->prefilter_and_fold) prefilter_and_fold(st, N, sc->etmp);
-    celt_synthesis(X, out_syn, oldBandE, start, effEnd, isTransient, LM, silence, sc->freq);
+    if (st->prefilter_and_fold) prefilter_and_fold(st, N, st->etmp);
+    celt_synthesis(X, out_syn, oldBandE, start, effEnd, isTransient, LM, silence, st->freq);
 
     st->pf_period = imax(st->pf_period, kCombMinPeriod);
     st->pf_period_old = imax(st->pf_period_old, kCombMinPeriod);

@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #define CUO_FN __host__ __device__ inline
 #define CUO_NOINLINE __host__ __device__ __attribute__((noinline))
@@ -29,7 +30,7 @@ namespace cuo {
 // is the same in every lane: accesses become LDS instructions and the values
 // loaded are uniform (scalar registers, scalar ALU).  The host build
 // (same-source CPU gate) uses the pointer as it is.
-constexpr size_t kWaveLdsBytes = 19 * 1024;
+constexpr size_t kWaveLdsBytes = 13 * 1024;
 __shared__ __attribute__((aligned(16))) char g_wave_lds[kWaveLdsBytes];
 #if defined(__HIP_DEVICE_COMPILE__)
 
@@ -120,13 +121,62 @@ __host__ __device__ inline int lane_id() { return 0; }
 // is loaded one per lane, the recurrence walks the block reading element k
 // with lane_get(v, k) (a scalar register) and collecting results with
 // lane_set(v, k, x); the block is then stored one per lane.  The serial
-// order of the arithmetic is unchanged.  k must be wave-uniform.
+// order of the arithmetic is unchanged.  k and x must be wave-uniform and
+// every lane active.
+//
+// serial_lanes(nb, f) calls f(k) for k = 0 .. nb-1 in order.  On the device
+// the steps come in unrolled chunks of 8 where k is a compile-time LaneK<k>,
+// so lane_get / lane_set take immediate lane selects (lane_set is then one
+// v_writelane_b32) and the reads of later elements can be issued ahead of
+// the recurrence; the last nb % 8 steps take a runtime k.
+template <int K>
+using LaneK = std::integral_constant<int, K>;
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ __forceinline__ int32_t lane_get(int32_t v, int k) { return __builtin_amdgcn_readlane(v, k); }
 __device__ __forceinline__ int32_t lane_set(int32_t v, int k, int32_t x) { return lane_id() == k ? x : v; }
+template <int K>
+__device__ __forceinline__ int32_t lane_get(int32_t v, LaneK<K>)
+{
+    return __builtin_amdgcn_readlane(v, K);
+}
+template <int K>
+__device__ __forceinline__ int32_t lane_set(int32_t v, LaneK<K>, int32_t x)
+{
+    asm("v_writelane_b32 %0, %1, %2" : "=v"(v) : "s"(__builtin_amdgcn_readfirstlane(x)), "i"(K), "0"(v));
+    return v;
+}
+template <int K, int E, class F>
+__device__ __forceinline__ void serial_unrolled(F &f)
+{
+    if constexpr (K < E) {
+        f(LaneK<K>());
+        serial_unrolled<K + 1, E>(f);
+    }
+}
+template <int C, class F>
+__device__ __forceinline__ int serial_chunks(int nb, F &f)
+{
+    if constexpr (C < 64) {
+        if (C + 8 <= nb) {
+            serial_unrolled<C, C + 8>(f);
+            return serial_chunks<C + 8>(nb, f);
+        }
+    }
+    return C;
+}
+template <class F>
+__device__ __forceinline__ void serial_lanes(int nb, F &&f)
+{
+    for (int k = serial_chunks<0>(nb, f); k < nb; k++) f(k);
+}
 #else
 __host__ __device__ inline int32_t lane_get(int32_t v, int) { return v; }
 __host__ __device__ inline int32_t lane_set(int32_t, int, int32_t x) { return x; }
+template <class F>
+inline void serial_lanes(int nb, F &&f)
+{
+    for (int k = 0; k < nb; k++) f(k);
+}
 #endif
 
 // 16-lane row primitives (DPP row shifts; device only).  row16_prefix_sum:

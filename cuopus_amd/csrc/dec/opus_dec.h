@@ -62,12 +62,18 @@ struct DecHeap {
     int32_t celt_mem[2048 + 120];
     int16_t frame_out[kMaxFrameOut];
     CeltPlcScratch plc;
+    int32_t celt_freq[960];
+    int32_t celt_etmp[kOverlap];
+    int32_t silk_cng_sig[kSilkMaxFrame + kSilkMaxOrder];
 };
 
 CUO_FN void dec_bind(OpusDecState *st, DecScratch *sc, DecHeap *h)
 {
     st->celt.mem = h->celt_mem;
     st->celt.plc = &h->plc;
+    st->celt.freq = h->celt_freq;
+    st->celt.etmp = h->celt_etmp;
+    st->silk.cng_sig = h->silk_cng_sig;
     sc->frame_out = h->frame_out;
 }
 

@@ -92,6 +92,7 @@ struct SilkDecoder {
     SilkChannel ch;
     int16_t sMid[2];
     int32_t nChannelsAPI, nChannelsInternal, prev_decode_only_middle;
+    int32_t *cng_sig;  // caller-owned comfort-noise buffer (kept by init and reset)
 };
 
 struct SilkCtrl {
@@ -112,7 +113,7 @@ struct SilkScratch {
     int16_t frame_out[kSilkMaxFrame + 2];     // internal-rate frame + 2-sample history
     int16_t exc_buf[160];
     int16_t rs_buf[2 * 160 + 8];
-    int32_t cng_sig[kSilkMaxFrame + kSilkMaxOrder];
+    int32_t *cng_sig;  // comfort-noise synthesis buffer (kSilkMaxFrame + kSilkMaxOrder), caller-owned
     int16_t nlsf_tmp[kSilkMaxOrder];
 };
 
@@ -761,6 +762,72 @@ CUO_FN void silk_shell_decoder(int16_t *p0, RangeDec *d, int pulses4)
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
+// A byte table held in one VGPR: lane w holds bytes 4w..4w+3 (tables of at
+// most 256 bytes).  Loaded once per use site by one vector load per lane.
+__device__ __forceinline__ int32_t lane_table(const uint8_t *t, int nbytes)
+{
+    const int w = lane_id();
+    return 4 * w < nbytes ? (int32_t)((const uint32_t *)t)[w] : 0;
+}
+// ec_icdf over a lane-held table starting at byte off: the same search as
+// ec_icdf_t, its words read with v_readlane instead of scalar loads.
+__device__ __forceinline__ int ec_icdf_lt(RangeDec *d, int32_t tabv, int off, int ftb)
+{
+    uint32_t s = uniform(d->rng), v = uniform(d->val);
+    const uint32_t r = s >> ftb;
+    int ret = 0;
+    uint32_t t;
+    int w = off >> 2;
+    uint32_t word = (uint32_t)lane_get(tabv, w) >> (8 * (off & 3));
+    int left = 4 - (off & 3);
+    for (;;) {
+        t = s;
+        s = r * (word & 0xFFu);
+        if (v >= s) break;
+        ret++;
+        if (--left == 0) {
+            word = (uint32_t)lane_get(tabv, ++w);
+            left = 4;
+        } else {
+            word >>= 8;
+        }
+    }
+    ec_normalize_rv(d, t - s, v - s);
+    return ret;
+}
+// silk_shell_decoder (shell_coder.c:119-151) over lane-held tables: split of
+// p pulses uses table bytes from offset p(p+1)/2 - 1.
+__device__ __forceinline__ void shell_split_lt(int16_t *c1, int16_t *c2, RangeDec *d, int p, int32_t tabv)
+{
+    if (p > 0) {
+        c1[0] = (int16_t)ec_icdf_lt(d, tabv, (p * (p + 1) >> 1) - 1, 8);
+        c2[0] = (int16_t)(p - c1[0]);
+    } else {
+        c1[0] = 0;
+        c2[0] = 0;
+    }
+}
+__device__ __forceinline__ void silk_shell_decoder_lt(int16_t *p0, RangeDec *d, int pulses4, int32_t t0, int32_t t1,
+                                                      int32_t t2, int32_t t3)
+{
+    int16_t p3[2], p2[4], p1[8];
+    shell_split_lt(&p3[0], &p3[1], d, pulses4, t3);
+    shell_split_lt(&p2[0], &p2[1], d, p3[0], t2);
+    shell_split_lt(&p1[0], &p1[1], d, p2[0], t1);
+    shell_split_lt(&p0[0], &p0[1], d, p1[0], t0);
+    shell_split_lt(&p0[2], &p0[3], d, p1[1], t0);
+    shell_split_lt(&p1[2], &p1[3], d, p2[1], t1);
+    shell_split_lt(&p0[4], &p0[5], d, p1[2], t0);
+    shell_split_lt(&p0[6], &p0[7], d, p1[3], t0);
+    shell_split_lt(&p2[2], &p2[3], d, p3[1], t2);
+    shell_split_lt(&p1[4], &p1[5], d, p2[2], t1);
+    shell_split_lt(&p0[8], &p0[9], d, p1[4], t0);
+    shell_split_lt(&p0[10], &p0[11], d, p1[5], t0);
+    shell_split_lt(&p1[6], &p1[7], d, p2[3], t1);
+    shell_split_lt(&p0[12], &p0[13], d, p1[6], t0);
+    shell_split_lt(&p0[14], &p0[15], d, p1[7], t0);
+}
+
 // silk_decode_pulses on the wavefront (decode_pulses.c:37-115, code_signs.c:
 // 75-115).  The range decoder runs on a register copy of its state; per-block
 // pulse counts live one per lane; a block's 16 magnitudes are read and written
@@ -775,19 +842,25 @@ __device__ inline void silk_decode_pulses_lanes(RangeDec *dp, int16_t *pulses, i
     RangeDec dl = ec_regs(dp);
     RangeDec *d = &dl;
     const int lane = lane_id();
+    // the pulse-count and shell tables, one VGPR each
+    const int32_t tpb = lane_table(tab::kPulsesPerBlockIcdf, (int)sizeof(tab::kPulsesPerBlockIcdf));
+    const int32_t ts0 = lane_table(tab::kShellTable0, (int)sizeof(tab::kShellTable0));
+    const int32_t ts1 = lane_table(tab::kShellTable1, (int)sizeof(tab::kShellTable1));
+    const int32_t ts2 = lane_table(tab::kShellTable2, (int)sizeof(tab::kShellTable2));
+    const int32_t ts3 = lane_table(tab::kShellTable3, (int)sizeof(tab::kShellTable3));
     const int RateLevelIndex = ec_icdf(d, tab::kRateLevelsIcdf + 9 * (signalType >> 1), 8);
     const int iter = (frame_length + 15) >> 4;
-    const uint8_t *cdf = tab::kPulsesPerBlockIcdf + 18 * RateLevelIndex;
+    const int cdf = 18 * RateLevelIndex;
     int32_t sumv = 0, lsv = 0;
     int any_ls = 0;
     {
     CUO_PROF_SCOPE(kProfSilkPulseCounts);
     for (int i = 0; i < iter; i++) {
         int nls = 0;
-        int sp = ec_icdf(d, cdf, 8);
+        int sp = ec_icdf_lt(d, tpb, cdf, 8);
         while (sp == 16 + 1) {
             nls++;
-            sp = ec_icdf(d, tab::kPulsesPerBlockIcdf + 18 * 9 + (nls == 10), 8);
+            sp = ec_icdf_lt(d, tpb, 18 * 9 + (nls == 10), 8);
         }
         sumv = lane_set(sumv, i, sp);
         lsv = lane_set(lsv, i, nls);
@@ -798,7 +871,7 @@ __device__ inline void silk_decode_pulses_lanes(RangeDec *dp, int16_t *pulses, i
     CUO_PROF_SCOPE(kProfSilkShell);
     for (int i = 0; i < iter; i++) {
         const int sp = lane_get(sumv, i);
-        if (sp > 0) silk_shell_decoder(&pulses[i * 16], d, sp);
+        if (sp > 0) silk_shell_decoder_lt(&pulses[i * 16], d, sp, ts0, ts1, ts2, ts3);
         else if (lane < 16) pulses[i * 16 + lane] = 0;
     }
     }
@@ -1742,6 +1815,7 @@ CUO_FN int silk_decode(SilkDecoder *dec, const SilkControl *ctl, int lostFlag, i
     sc = lds_uniform(sc);
     SilkChannel *ch = &dec->ch;
     int ret = 0;
+    sc->cng_sig = dec->cng_sig;
     if (newPacketFlag) ch->nFramesDecoded = 0;
     if (1 > dec->nChannelsInternal) {
         // growing to 1 internal channel only touches channel_state[1] in

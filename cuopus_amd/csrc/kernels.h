@@ -19,9 +19,9 @@
 namespace cuo {
 
 constexpr int kMaxPktSamples = 5760;
-// Resident wavefronts of the persistent decoder: two per SIMD (LDS < 20 KB
-// and <= 256 VGPRs per wavefront), 8 per CU, 256 CUs.
-constexpr uint32_t kDefaultWaves = 2048;
+// Resident wavefronts of the persistent decoder: three per SIMD (13 KB of
+// LDS and <= 168 VGPRs per wavefront), 12 per CU, 256 CUs.
+constexpr uint32_t kDefaultWaves = 3072;
 constexpr int kRsTaps = 49;
 constexpr int kRsPad = 24;
 
@@ -41,17 +41,18 @@ struct ChainResult {
 };
 
 // Decoder memory of one wavefront.  The chain's state and per-call scratch
-// (WaveState, under 19 KB) sit in LDS, so two wavefronts fit per SIMD; the
-// CELT history, one frame of output and the pitch-concealment buffers
-// (DecHeap, 29 KB) sit in a per-wavefront HBM slot, where only lane-parallel
-// loops and the rare concealment touch them.  Chains restart from a fresh
-// decoder, so nothing is carried between chains.
+// (WaveState, under 13 KB) sit in LDS, so three wavefronts fit per SIMD; the
+// CELT history and synthesis spectrum, one frame of output, the
+// concealment and comfort-noise buffers (DecHeap, 34 KB) sit in a
+// per-wavefront HBM slot, where only lane-parallel loops and the rare
+// concealment touch them.  Chains restart from a fresh decoder, so nothing is
+// carried between chains.
 struct WaveState {
     OpusDecState st;
     DecScratch sc;
 };
 static_assert(sizeof(WaveState) <= kWaveLdsBytes, "wavefront decoder memory exceeds its LDS block");
-static_assert(kWaveLdsBytes * 8 <= 160 * 1024, "two wavefronts per SIMD need <= 20 KB of LDS each");
+static_assert(kWaveLdsBytes * 12 <= 160 * 1024, "three wavefronts per SIMD need <= 13.3 KB of LDS each");
 
 struct DecodeArgs {
     const uint8_t *data;
@@ -97,6 +98,11 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
     {
     CUO_PROF_SCOPE(kProfChain);
     const ChainDesc ch = a.chains[c];
+    const uint8_t *data = a.data;
+    const uint32_t *off = a.off, *len = a.len;
+    const uint8_t *lost = a.lost;
+    int32_t *pkt_nsamp = a.pkt_nsamp;
+    uint32_t *pkt_rng = a.pkt_rng;
     OpusDecState *st = lds_uniform(&ws->st);
     DecScratch *sc = lds_uniform(&ws->sc);
     dec_bind(st, sc, heap);
@@ -112,28 +118,28 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
     int32_t stop = 0;
     for (uint32_t k = 0; k < ch.npkts; k++) {
         const uint32_t p = ch.first + k;
-        const uint8_t *pk = a.data + a.off[p];
+        const uint8_t *pk = data + off[p];
         int r;
-        if (a.lost && a.lost[p]) {
+        if (lost && lost[p]) {
             // loss rule: FEC from the next packet of the chain when it carries
             // LBRR, else concealment of this packet's duration
-            int dur = packet_nb_samples(pk, (int)a.len[p]);
+            int dur = packet_nb_samples(pk, (int)len[p]);
             if (dur <= 0) { stop = dur ? dur : OPUS_INVALID_PACKET; break; }
             bool fec = false;
-            if (k + 1 < ch.npkts && !a.lost[p + 1])
-                fec = packet_has_lbrr(a.data + a.off[p + 1], (int)a.len[p + 1]) > 0;
-            if (fec) r = opus_decode_packet(st, a.data + a.off[p + 1], (int)a.len[p + 1], out, dur, 1, sc);
+            if (k + 1 < ch.npkts && !lost[p + 1])
+                fec = packet_has_lbrr(data + off[p + 1], (int)len[p + 1]) > 0;
+            if (fec) r = opus_decode_packet(st, data + off[p + 1], (int)len[p + 1], out, dur, 1, sc);
             else r = opus_decode_packet(st, nullptr, 0, out, dur, 0, sc);
         } else {
-            r = opus_decode_packet(st, pk, (int)a.len[p], out, kMaxPktSamples, 0, sc);
+            r = opus_decode_packet(st, pk, (int)len[p], out, kMaxPktSamples, 0, sc);
         }
         if (r < 0) { stop = r; break; }
         // a packet held by several chains (preroll overlap) is recorded by the
         // last chain that holds it, which owns packets below the next chain's
         // first packet
-        if (a.pkt_nsamp && lane == 0 && p < ch.rec_end) {
-            a.pkt_nsamp[p] = r;
-            a.pkt_rng[p] = st->rangeFinal;
+        if (pkt_nsamp && lane == 0 && p < ch.rec_end) {
+            pkt_nsamp[p] = r;
+            pkt_rng[p] = st->rangeFinal;
         }
     }
     if (lane == 0) {
@@ -144,33 +150,50 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
     prof_flush();
 }
 
-// Plain variant: workgroup g decodes chain order[g].
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_decode_chains(DecodeArgs a)
+// The launch's DecodeArgs, read in place from the kernel-argument segment (a
+// decode kernel's only argument).  Passing the by-value argument on by
+// reference would copy it to per-lane scratch memory, and a kernel that keeps
+// values live across the out-of-line decoder needs more registers than three
+// wavefronts per SIMD allow; the kernels below only forward this pointer.
+__device__ __forceinline__ const DecodeArgs *launch_args()
 {
-    const uint32_t g = blockIdx.x;
-    if (g >= a.n_chains) return;
-    const uint32_t c = a.order ? a.order[g] : g;
-    decode_one_chain(a, c, (WaveState *)g_wave_lds, a.heap + g, (int)threadIdx.x);
+    return (const DecodeArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
+// Plain variant: workgroup g decodes chain order[g].
+__device__ __noinline__ void plain_worker(const DecodeArgs *ap, uint32_t g, int lane)
+{
+    if (g >= ap->n_chains) return;
+    const uint32_t c = ap->order ? ap->order[g] : g;
+    decode_one_chain(*ap, c, (WaveState *)g_wave_lds, ap->heap + g, lane);
+}
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_decode_chains(DecodeArgs)
+{
+    plain_worker(launch_args(), blockIdx.x, (int)threadIdx.x);
 }
 
 // Persistent variant: the grid's wavefronts pull chains from an atomic queue
 // until it is drained (every wavefront exits once the counter passes n_chains).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_decode_persistent(DecodeArgs a)
+__device__ __noinline__ void persistent_worker(const DecodeArgs *ap, DecHeap *heap, int lane)
 {
     WaveState *ws = (WaveState *)g_wave_lds;
-    const int lane = (int)threadIdx.x;
     while (true) {
         // The fetched index is made wave-uniform with readfirstlane (not a
         // lane shuffle) so the loop exit is a scalar branch: with a shuffle the
         // compiler treats the exit as divergent and may let lanes run different
         // iterations, which a wavefront-per-chain decoder cannot allow.
         uint32_t w = 0;
-        if (lane == 0) w = atomicAdd(a.queue, 1u);
+        if (lane == 0) w = atomicAdd(ap->queue, 1u);
         w = (uint32_t)__builtin_amdgcn_readfirstlane((int)w);
-        if (w >= a.n_chains) break;
-        const uint32_t c = a.order ? a.order[w] : w;
-        decode_one_chain(a, c, ws, a.heap + blockIdx.x, lane);
+        if (w >= ap->n_chains) break;
+        const uint32_t c = ap->order ? ap->order[w] : w;
+        decode_one_chain(*ap, c, ws, heap, lane);
     }
+}
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_decode_persistent(DecodeArgs)
+{
+    const DecodeArgs *ap = launch_args();
+    persistent_worker(ap, ap->heap + blockIdx.x, (int)threadIdx.x);
 }
 
 // Zero a work-queue counter (a kernel rather than a memset so the sequence can
