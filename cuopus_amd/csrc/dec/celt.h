@@ -193,8 +193,9 @@ CUO_FN int pulses2bits(int band, int LM, int pulses)
     return pulses == 0 ? 0 : (int)ld_u8(cache, pulses) + 1;
 }
 
+
 // ------------------------------------------------------------------ coarse/fine energy
-CUO_FN int laplace_decode(RangeDec *d, uint32_t fs, int decay)
+CUO_FI int laplace_decode(RangeDec *d, uint32_t fs, int decay)
 {
     int val = 0;
     uint32_t fl = 0;
@@ -222,7 +223,7 @@ CUO_FN int laplace_decode(RangeDec *d, uint32_t fs, int decay)
     return val;
 }
 
-CUO_FN void unquant_coarse_energy(int start, int end, int16_t *oldE, int intra, RangeDec *d, int LM)
+CUO_FI void unquant_coarse_energy(int start, int end, int16_t *oldE, int intra, RangeDec *d, int LM)
 {
     const int32_t *prob = kEProb32.v + (LM * 2 + intra) * 42;
     int32_t prev = 0;
@@ -253,7 +254,7 @@ CUO_FN void unquant_coarse_energy(int start, int end, int16_t *oldE, int intra, 
     }
 }
 
-CUO_FN void unquant_fine_energy(int start, int end, int16_t *oldE, const int32_t *fine_quant, RangeDec *d)
+CUO_FI void unquant_fine_energy(int start, int end, int16_t *oldE, const int32_t *fine_quant, RangeDec *d)
 {
     for (int i = start; i < end; i++) {
         if (fine_quant[i] <= 0) continue;
@@ -263,7 +264,7 @@ CUO_FN void unquant_fine_energy(int start, int end, int16_t *oldE, const int32_t
     }
 }
 
-CUO_FN void unquant_energy_finalise(int start, int end, int16_t *oldE, const int32_t *fine_quant,
+CUO_FI void unquant_energy_finalise(int start, int end, int16_t *oldE, const int32_t *fine_quant,
                                     const int32_t *fine_priority, int bits_left, RangeDec *d)
 {
     for (int prio = 0; prio < 2; prio++) {
@@ -278,7 +279,7 @@ CUO_FN void unquant_energy_finalise(int start, int end, int16_t *oldE, const int
 }
 
 // ------------------------------------------------------------------ tf / allocation
-CUO_FN void tf_decode(int start, int end, int isTransient, int32_t *tf_res, int LM, RangeDec *d)
+CUO_FI void tf_decode(int start, int end, int isTransient, int32_t *tf_res, int LM, RangeDec *d)
 {
     uint32_t budget = d->storage * 8;
     uint32_t tell = (uint32_t)ec_tell(d);
@@ -303,7 +304,7 @@ CUO_FN void tf_decode(int start, int end, int isTransient, int32_t *tf_res, int 
 }
 
 // clt_compute_allocation + interp_bits2pulses, mono decoder (rate.c:245-646)
-CUO_FN int compute_allocation(int start, int end, const int32_t *offsets, const int32_t *cap, int alloc_trim,
+CUO_FI int compute_allocation(int start, int end, const int32_t *offsets, const int32_t *cap, int alloc_trim,
                               int32_t total, int32_t *balance_out, int32_t *bits, int32_t *ebits,
                               int32_t *fine_priority, int LM, RangeDec *d, CeltScratch *sc)
 {
@@ -746,7 +747,7 @@ CUO_FN void exp_rotation(int16_t *X, int len, int dir, int stride, int K, int sp
     }
 }
 
-CUO_FN unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDec *d, int16_t gain, int32_t *iy)
+CUO_FI unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDec *d, int16_t gain, int32_t *iy)
 {
     CUO_PROF_SCOPE(kProfPvq);
     uint32_t idx;
@@ -859,7 +860,7 @@ CUO_FN int compute_qn(int N, int b, int offset, int pulse_cap)
 // One level of the mono theta split (bands.c:700-903 with stereo = 0).
 struct SplitOut { int imid, iside, delta, itheta, qalloc; };
 
-CUO_FN SplitOut compute_theta_mono(BandCtx *ctx, int N, int *b, int B, int B0, int LM, int *fill)
+CUO_FI SplitOut compute_theta_mono(BandCtx *ctx, int N, int *b, int B, int B0, int LM, int *fill)
 {
     SplitOut o;
     int i = ctx->band;
@@ -918,7 +919,7 @@ static_assert(sizeof(QpFrame) <= 96, "QpFrame exceeds its stack slot");
 
 // Leaf of the band splitter: PVQ decode of the pulses the bits buy, or
 // folding / noise when there are none (bands.c:1046-1105).
-CUO_FN unsigned qp_leaf(BandCtx *ctx, int16_t *fX, int N, int b, int B, int16_t *flow, int LM, int16_t gain,
+CUO_FI unsigned qp_leaf(BandCtx *ctx, int16_t *fX, int N, int b, int B, int16_t *flow, int LM, int16_t gain,
                         int fill_in)
 {
     int q;
@@ -972,7 +973,7 @@ CUO_FN bool qp_splits(const BandCtx *ctx, int N, int b, int LM)
     return LM != -1 && b > (int)ld_u8(cache, (int)ld_u8(cache, 0)) + 12 && N > 2;
 }
 
-CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B0, int16_t *lowband0,
+CUO_FI unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B0, int16_t *lowband0,
                                 int LM0, int16_t gain0, int fill0)
 {
     // most partitions are leaves: decode those without the explicit stack
@@ -1027,8 +1028,12 @@ CUO_FN unsigned quant_partition(BandCtx *ctx, int16_t *X0, int N0, int b0, int B
                 }
                 continue;
             }
-            unsigned cm = qp_leaf(ctx, lds_uniform(f.X), f.N, f.b, f.B,
-                                  f.lowband ? lds_uniform(f.lowband) : nullptr, f.LM, f.gain, f.fill);
+            unsigned cm;
+            {
+                CUO_PROF_SCOPE(kProfQpLeaf);
+                cm = qp_leaf(ctx, lds_uniform(f.X), f.N, f.b, f.B, f.lowband ? lds_uniform(f.lowband) : nullptr,
+                             f.LM, f.gain, f.fill);
+            }
             ret = cm;
             sp--;
             continue;
@@ -1068,7 +1073,7 @@ static constexpr uint8_t kBitDeinterleave[16] = {0x00, 0x03, 0x0C, 0x0F, 0x30, 0
                                                  0xC0, 0xC3, 0xCC, 0xCF, 0xF0, 0xF3, 0xFC, 0xFF};
 
 // quant_band, decoder, mono (bands.c:1109-1234)
-CUO_FN unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_t *lowband, int LM,
+CUO_FI unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_t *lowband, int LM,
                            int16_t *lowband_out, int16_t gain, int16_t *lowband_scratch, int fill, int16_t *tmp)
 {
     int N0 = N, N_B = N, B0 = B, time_divide = 0, recombine = 0;
@@ -1138,7 +1143,7 @@ CUO_FN unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_
 }
 
 // quant_all_bands, decoder, mono (bands.c:1398-1672)
-CUO_FN void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, const int32_t *pulses,
+CUO_FI void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, const int32_t *pulses,
                             int shortBlocks, int spread, const int32_t *tf_res, int32_t total_bits,
                             int32_t balance, RangeDec *ec, int LM, int codedBands, uint32_t *seed,
                             CeltScratch *sc)
@@ -1678,6 +1683,13 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     }
     if (st->loss_duration == 0) st->skip_plc = 0;
     if (dec == nullptr) { ec_init(&sc->dec, data, (uint32_t)len); dec = &sc->dec; }
+    // The frame decodes on a register copy of the range decoder (every
+    // function below that takes it is inlined, so its fields stay in scalar
+    // registers); the copy goes back to its home once the frame is decoded,
+    // where the caller reads the final state.
+    RangeDec *const dec_home = dec;
+    RangeDec dec_regs = ec_regs(dec_home);
+    dec = &dec_regs;
     CUO_PFOR(i, 0, kNbBands) oldBandE[i] = max16(oldBandE[i], oldBandE[kNbBands + i]);
 
     int32_t total_bits = len * 8;
@@ -1833,6 +1845,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
         }
     }
     st->rng = dec->rng;
+    *dec_home = dec_regs;
     deemphasis(out_syn, pcm, N, &st->preemph_mem, accum);
     st->loss_duration = 0;
     st->prefilter_and_fold = 0;

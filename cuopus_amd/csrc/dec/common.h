@@ -198,6 +198,26 @@ __device__ __forceinline__ int32_t row16_shift_in(int32_t v, int32_t fill)
 }
 #endif
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// A byte table held in one VGPR: lane w holds bytes 4w..4w+3 of a 4-byte
+// aligned table of at most 256 bytes (the last word's bytes past the end read
+// as zero).  One vector load per lane, then every lookup is a v_readlane.
+__device__ __forceinline__ int32_t lane_table(const uint8_t *t, int nbytes)
+{
+    const int w = lane_id();
+    if (4 * w + 4 <= nbytes) return (int32_t)((const uint32_t *)t)[w];
+    uint32_t v = 0;
+    for (int b = 0; b < 3; b++)
+        if (4 * w + b < nbytes) v |= (uint32_t)t[4 * w + b] << (8 * b);
+    return (int32_t)v;
+}
+// byte i (wave-uniform) of a lane-held table
+__device__ __forceinline__ int lane_byte(int32_t tv, int i)
+{
+    return (int)(((uint32_t)lane_get(tv, i >> 2) >> (8 * (i & 3))) & 0xFFu);
+}
+#endif
+
 // Reductions over the lanes of per-lane partial results (identity on the
 // host).  Integer sums wrap like the serial loop's, so any order is exact.
 // Call with every lane active (outside CUO_PFOR bodies).
@@ -278,7 +298,7 @@ enum {
     kProfAllocBisect, kProfAllocTail, kProfQpLeafBits, kProfBandsPrologue, kProfLowbandOut, kProfQpStage,
     kProfSilkIndices, kProfSilkPulses, kProfSilkParams, kProfSilkCore, kProfSilkResamp, kProfSilkPlc,
     kProfSilkNlsf, kProfSilkNlsf2a, kProfSilkCng, kProfSilkPulseCounts, kProfSilkShell, kProfSilkSigns,
-    kProfCount
+    kProfQpLeaf, kProfCount
 };
 static_assert(kProfCount <= kProfSlots, "profiling scope ids exceed the per-lane slots");
 

@@ -226,7 +226,7 @@ CUO_FI uint32_t ec_bits(RangeDec *d, int bits)
     return ret;
 }
 
-CUO_FN uint32_t ec_uint(RangeDec *d, uint32_t ft)
+CUO_FI uint32_t ec_uint(RangeDec *d, uint32_t ft)
 {
     ft--;
     int ftb = ilog(ft);

@@ -762,13 +762,6 @@ CUO_FN void silk_shell_decoder(int16_t *p0, RangeDec *d, int pulses4)
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-// A byte table held in one VGPR: lane w holds bytes 4w..4w+3 (tables of at
-// most 256 bytes).  Loaded once per use site by one vector load per lane.
-__device__ __forceinline__ int32_t lane_table(const uint8_t *t, int nbytes)
-{
-    const int w = lane_id();
-    return 4 * w < nbytes ? (int32_t)((const uint32_t *)t)[w] : 0;
-}
 // ec_icdf over a lane-held table starting at byte off: the same search as
 // ec_icdf_t, its words read with v_readlane instead of scalar loads.
 __device__ __forceinline__ int ec_icdf_lt(RangeDec *d, int32_t tabv, int off, int ftb)

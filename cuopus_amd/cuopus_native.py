@@ -344,7 +344,7 @@ PROF_SCOPES = ["chain", "frame", "celt", "celt_head", "celt_bands", "celt_synth"
                "theta", "leaf_fold", "band_pre", "band_post", "ec_uint", "alloc_bisect", "alloc_tail",
                "qp_leaf_bits", "bands_prologue", "lowband_out", "qp_stage", "silk_indices", "silk_pulses",
                "silk_params", "silk_core", "silk_resamp", "silk_plc", "silk_nlsf", "silk_nlsf2a", "silk_cng",
-               "silk_pulse_counts", "silk_shell", "silk_signs"]
+               "silk_pulse_counts", "silk_shell", "silk_signs", "qp_leaf"]
 
 
 def prof_read():
