@@ -55,6 +55,13 @@ WORKLOADS = {
                 "redrawn every 1 s in 12-64 kbps, 20 ms frames; all chains in one persistent kernel vs one "
                 "launch per file",
     },
+    "hybrid_fec_200k": {
+        "glob": "corpus/bench/hybrid_fec_200k/f*.opus",
+        "loss": (0.02, 0x0C0FFEE5),
+        "desc": "200 files x 1000 s (200k chains, 55.6 h) hybrid FB 20 ms 32 kbps with in-band FEC; 2% of "
+                "packets lost by a seeded mask per file, decoded from the next packet's LBRR when present, "
+                "concealed otherwise; all chains in one persistent kernel, f32 16 kHz out",
+    },
     "uneven_250": {
         "glob": "corpus/bench/uneven_250/f*.opus",
         "desc": "250 mono speech files, durations log-uniform 5 s-30 min (18.1 h), mode per file from "
@@ -269,6 +276,7 @@ def main():
         if not os.path.exists(p):
             sys.exit("workload input %s missing: generate it with __graft_entry__.build()" % p)
     blobs = [open(p, "rb").read() for p in paths]
+    masks = None
 
     import torch
     from cuopus_amd import cuopus_native as cn
@@ -293,7 +301,14 @@ def main():
         mode, slots, lpt = tuned["mode"], tuned["n_slots"], tuned["lpt_order"]
         a.mode = {v: k for k, v in modes.items()}[mode]
         _log("tuned: %s" % {k: tuned[k] for k in ("mode", "n_slots", "lpt_order", "ms")})
-    b = cn.Batch(blobs, mode=mode, n_slots=slots, lpt_order=lpt, use_graph=a.graph, profile=not a.graph)
+    if "loss" in w:
+        # the seeded loss mask of each file, as the parity test draws it
+        from tests import oracle_lib as ol
+        p_loss, seed0 = w["loss"]
+        masks = [ol.loss_mask(cn.packet_count(bl)[0], p_loss, seed0 + i) for i, bl in enumerate(blobs)]
+        _log("loss masks: %d of %d packets lost" % (sum(int(m.sum()) for m in masks), sum(m.size for m in masks)))
+    b = cn.Batch(blobs, lost_masks=masks, mode=mode, n_slots=slots, lpt_order=lpt, use_graph=a.graph,
+                 profile=not a.graph)
     _log("batch ready: %d files, %d chains, %.0f audio-s" % (len(blobs), b.chains, b.playable / 48000.0))
     wall, dec_ms, rs_ms = _time_batch(b, a.steps, a.warmup, torch)
     _log("%s: %.1f ms/step" % (a.mode, 1000.0 * wall / a.steps))
@@ -324,7 +339,7 @@ def main():
 
     plain = None
     if not a.no_plain and a.mode != "plain":
-        bp = cn.Batch(blobs, mode=cn.MODE_PLAIN, profile=True)
+        bp = cn.Batch(blobs, lost_masks=masks, mode=cn.MODE_PLAIN, profile=True)
         pw, _, _ = _time_batch(bp, max(1, min(a.steps, 5)), 1, torch)
         plain = round(bp.playable / 48000.0 * max(1, min(a.steps, 5)) / pw, 1)
         _log("plain: %.1f ms/step" % (1000.0 * pw / max(1, min(a.steps, 5))))

@@ -30,7 +30,7 @@ namespace cuo {
 // is the same in every lane: accesses become LDS instructions and the values
 // loaded are uniform (scalar registers, scalar ALU).  The host build
 // (same-source CPU gate) uses the pointer as it is.
-constexpr size_t kWaveLdsBytes = 13 * 1024;
+constexpr size_t kWaveLdsBytes = 10 * 1024;
 __shared__ __attribute__((aligned(16))) char g_wave_lds[kWaveLdsBytes];
 #if defined(__HIP_DEVICE_COMPILE__)
 

@@ -47,9 +47,12 @@ struct DecScratch {
         CeltScratch celt;
         SilkScratch silk;
     };
-    int16_t pcm_silk[480];
-    int16_t pcm_transition[240];
-    int16_t redundant_audio[240];
+    // SILK output, transition and redundancy frames (caller-owned, HBM on the
+    // device: only lane-parallel loops and the frame decoders' output touch
+    // them)
+    int16_t *pcm_silk;
+    int16_t *pcm_transition;
+    int16_t *redundant_audio;
     RangeDec dec;  // the frame's range decoder (shared by SILK and CELT)
     int16_t frame_sizes[48];  // packet_parse output
     // One frame of output, caller-owned (HBM on the device): a frame decode
@@ -62,6 +65,10 @@ struct DecHeap {
     int32_t celt_mem[2048 + 120];
     int16_t frame_out[kMaxFrameOut];
     CeltPlcScratch plc;
+    int16_t pcm_silk[480];
+    int16_t pcm_transition[240];
+    int16_t redundant_audio[240];
+    int32_t silk_cng_exc[kSilkMaxFrame];
     int32_t celt_freq[960];
     int32_t celt_etmp[kOverlap];
     int32_t silk_cng_sig[kSilkMaxFrame + kSilkMaxOrder];
@@ -74,6 +81,10 @@ CUO_FN void dec_bind(OpusDecState *st, DecScratch *sc, DecHeap *h)
     st->celt.freq = h->celt_freq;
     st->celt.etmp = h->celt_etmp;
     st->silk.cng_sig = h->silk_cng_sig;
+    st->silk.cng_exc = h->silk_cng_exc;
+    sc->pcm_silk = h->pcm_silk;
+    sc->pcm_transition = h->pcm_transition;
+    sc->redundant_audio = h->redundant_audio;
     sc->frame_out = h->frame_out;
 }
 

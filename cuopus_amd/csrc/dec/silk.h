@@ -55,7 +55,7 @@ struct SilkPlcState {
 };
 
 struct SilkCngState {
-    int32_t exc_buf_Q14[kSilkMaxFrame];
+    // CNG_exc_buf_Q14 lives outside (SilkDecoder::cng_exc, caller-owned)
     int16_t smth_NLSF_Q15[kSilkMaxOrder];
     int32_t synth_state[kSilkMaxOrder];
     int32_t smth_Gain_Q16;
@@ -93,6 +93,7 @@ struct SilkDecoder {
     int16_t sMid[2];
     int32_t nChannelsAPI, nChannelsInternal, prev_decode_only_middle;
     int32_t *cng_sig;  // caller-owned comfort-noise buffer (kept by init and reset)
+    int32_t *cng_exc;  // caller-owned CNG excitation history (kSilkMaxFrame), part of the channel state
 };
 
 struct SilkCtrl {
@@ -114,6 +115,7 @@ struct SilkScratch {
     int16_t exc_buf[160];
     int16_t rs_buf[2 * 160 + 8];
     int32_t *cng_sig;  // comfort-noise synthesis buffer (kSilkMaxFrame + kSilkMaxOrder), caller-owned
+    int32_t *cng_exc;  // CNG excitation history (kSilkMaxFrame), caller-owned
     int16_t nlsf_tmp[kSilkMaxOrder];
 };
 
@@ -625,9 +627,18 @@ CUO_FN void silk_reset_channel(SilkChannel *ch)
     silk_plc_reset(ch);
 }
 
+// The CNG excitation history is channel state kept outside the channel
+// struct: a channel reset clears it too.
+CUO_FN void silk_reset_cng_exc(SilkDecoder *d)
+{
+    int32_t *e = d->cng_exc;
+    CUO_PFOR(i, 0, kSilkMaxFrame) e[i] = 0;
+}
+
 CUO_FN void silk_init_decoder(SilkDecoder *d)
 {
     silk_reset_channel(&d->ch);
+    silk_reset_cng_exc(d);
     d->sMid[0] = d->sMid[1] = 0;
     d->nChannelsAPI = 0;
     d->nChannelsInternal = 0;
@@ -639,6 +650,7 @@ CUO_FN void silk_init_decoder(SilkDecoder *d)
 CUO_FN void silk_reset_decoder(SilkDecoder *d)
 {
     silk_reset_channel(&d->ch);
+    silk_reset_cng_exc(d);
     d->sMid[0] = d->sMid[1] = 0;
     d->prev_decode_only_middle = 0;
 }
@@ -1709,9 +1721,20 @@ CUO_FN void silk_cng(SilkChannel *ch, const SilkCtrl *ctrl, int16_t *frame, int 
         int subfr = 0;
         for (int i = 0; i < ch->nb_subfr; i++)
             if (ctrl->Gains_Q16[i] > max_Gain_Q16) { max_Gain_Q16 = ctrl->Gains_Q16[i]; subfr = i; }
-        for (int i = (ch->nb_subfr - 1) * ch->subfr_length - 1; i >= 0; i--)
-            cng->exc_buf_Q14[ch->subfr_length + i] = cng->exc_buf_Q14[i];
-        for (int i = 0; i < ch->subfr_length; i++) cng->exc_buf_Q14[i] = ch->exc_Q14[subfr * ch->subfr_length + i];
+        // exc_buf[L + i] = exc_buf[i] for i from (nb_subfr - 1) L - 1 down to
+        // 0 (a move up by one subframe), then the loudest subframe to the
+        // front.  Blocks of kLanes from the top: each block is read whole
+        // before it is written, and what is left below it lies under the
+        // block's destination.
+        int32_t *exc_buf = sc->cng_exc;
+        const int L = ch->subfr_length;
+        for (int top = (ch->nb_subfr - 1) * L; top > 0; top -= kLanes) {
+            const int lo = imax(0, top - kLanes);
+            const int i = lo + lane_id();
+            const int32_t v = i < top ? exc_buf[i] : 0;
+            if (i < top) exc_buf[L + i] = v;
+        }
+        CUO_PFOR(i, 0, L) exc_buf[i] = ch->exc_Q14[subfr * L + i];
         for (int i = 0; i < ch->nb_subfr; i++) {
             cng->smth_Gain_Q16 += silk_smulwb(ctrl->Gains_Q16[i] - cng->smth_Gain_Q16, 4634);
             if (silk_smulww(cng->smth_Gain_Q16, 46396) > ctrl->Gains_Q16[i]) cng->smth_Gain_Q16 = ctrl->Gains_Q16[i];
@@ -1735,10 +1758,16 @@ CUO_FN void silk_cng(SilkChannel *ch, const SilkCtrl *ctrl, int16_t *frame, int 
         int exc_mask = 255;
         while (exc_mask > length) exc_mask >>= 1;
         int32_t seed = cng->rand_seed;
-        for (int i = 0; i < length; i++) {
-            seed = silk_rand(seed);
-            int idx = (seed >> 24) & exc_mask;
-            sig[kSilkMaxOrder + i] = cng->exc_buf_Q14[idx];
+        const int32_t *exc_buf = sc->cng_exc;
+        for (int i0 = 0; i0 < length; i0 += kLanes) {
+            // the serial seed sequence picks kLanes indices, then one gather
+            const int nb = imin(kLanes, length - i0);
+            int32_t idxv = 0;
+            serial_lanes(nb, [&](auto k) {
+                seed = silk_rand(seed);
+                idxv = lane_set(idxv, k, (seed >> 24) & exc_mask);
+            });
+            if (lane_id() < nb) sig[kSilkMaxOrder + i0 + lane_id()] = exc_buf[idxv];
         }
         cng->rand_seed = seed;
         silk_nlsf2a(A_Q12, cng->smth_NLSF_Q15, ch->LPC_order);
@@ -1809,6 +1838,7 @@ CUO_FN int silk_decode(SilkDecoder *dec, const SilkControl *ctl, int lostFlag, i
     SilkChannel *ch = &dec->ch;
     int ret = 0;
     sc->cng_sig = dec->cng_sig;
+    sc->cng_exc = dec->cng_exc;
     if (newPacketFlag) ch->nFramesDecoded = 0;
     if (1 > dec->nChannelsInternal) {
         // growing to 1 internal channel only touches channel_state[1] in

@@ -19,9 +19,9 @@
 namespace cuo {
 
 constexpr int kMaxPktSamples = 5760;
-// Resident wavefronts of the persistent decoder: three per SIMD (13 KB of
-// LDS and <= 168 VGPRs per wavefront), 12 per CU, 256 CUs.
-constexpr uint32_t kDefaultWaves = 3072;
+// Resident wavefronts of the persistent decoder: four per SIMD (10 KB of
+// LDS and <= 128 VGPRs per wavefront), 16 per CU, 256 CUs.
+constexpr uint32_t kDefaultWaves = 4096;
 constexpr int kRsTaps = 49;
 constexpr int kRsPad = 24;
 
@@ -41,9 +41,10 @@ struct ChainResult {
 };
 
 // Decoder memory of one wavefront.  The chain's state and per-call scratch
-// (WaveState, under 13 KB) sit in LDS, so three wavefronts fit per SIMD; the
-// CELT history and synthesis spectrum, one frame of output, the
-// concealment and comfort-noise buffers (DecHeap, 34 KB) sit in a
+// (WaveState, under 10 KB) sit in LDS, so four wavefronts fit per SIMD; the
+// CELT history and synthesis spectrum, the output, SILK, transition and
+// redundancy frames, the concealment and comfort-noise buffers (DecHeap,
+// 37 KB) sit in a
 // per-wavefront HBM slot, where only lane-parallel loops and the rare
 // concealment touch them.  Chains restart from a fresh decoder, so nothing is
 // carried between chains.
@@ -52,7 +53,7 @@ struct WaveState {
     DecScratch sc;
 };
 static_assert(sizeof(WaveState) <= kWaveLdsBytes, "wavefront decoder memory exceeds its LDS block");
-static_assert(kWaveLdsBytes * 12 <= 160 * 1024, "three wavefronts per SIMD need <= 13.3 KB of LDS each");
+static_assert(kWaveLdsBytes * 16 <= 160 * 1024, "four wavefronts per SIMD need <= 10 KB of LDS each");
 
 struct DecodeArgs {
     const uint8_t *data;
@@ -167,7 +168,7 @@ __device__ __noinline__ void plain_worker(const DecodeArgs *ap, uint32_t g, int 
     const uint32_t c = ap->order ? ap->order[g] : g;
     decode_one_chain(*ap, c, (WaveState *)g_wave_lds, ap->heap + g, lane);
 }
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_decode_chains(DecodeArgs)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_decode_chains(DecodeArgs)
 {
     plain_worker(launch_args(), blockIdx.x, (int)threadIdx.x);
 }
@@ -190,7 +191,7 @@ __device__ __noinline__ void persistent_worker(const DecodeArgs *ap, DecHeap *he
         decode_one_chain(*ap, c, ws, heap, lane);
     }
 }
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_decode_persistent(DecodeArgs)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_decode_persistent(DecodeArgs)
 {
     const DecodeArgs *ap = launch_args();
     persistent_worker(ap, ap->heap + blockIdx.x, (int)threadIdx.x);
