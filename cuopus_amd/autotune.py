@@ -2,7 +2,7 @@
 
 For a workload (a list of Ogg/Opus streams) it times the resident batch over
 a grid of launch shapes -- the persistent kernel with a given number of
-resident wavefronts (one to three per SIMD, or oversubscribed), with and
+resident wavefronts (two to four per SIMD), with and
 without longest-chain-first ordering, and the one-workgroup-per-chain launch
 -- checks that every shape produces bit-identical outputs, and keeps the
 fastest.  Results are cached per device name and workload size class in a
@@ -22,12 +22,10 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 
 # (mode, n_slots, lpt_order)
 DEFAULT_GRID = [
-    (cn.MODE_PERSISTENT, 1024, True),
     (cn.MODE_PERSISTENT, 2048, True),
-    (cn.MODE_PERSISTENT, 2560, True),
     (cn.MODE_PERSISTENT, 3072, True),
-    (cn.MODE_PERSISTENT, 3072, False),
     (cn.MODE_PERSISTENT, 4096, True),
+    (cn.MODE_PERSISTENT, 4096, False),
     (cn.MODE_ONE_LAUNCH, 0, True),
 ]
 

@@ -793,44 +793,42 @@ CUO_FI unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDe
     return wave_or(cm);
 }
 
+// haar1 and the (de)interleavers (bands.c:560-640): every (i, j) step reads
+// and writes its own elements, so the steps are spread over the lanes; the
+// interleavers gather into tmp in one pass and copy back in a second (LDS
+// accesses of a wavefront complete in program order).
 CUO_FN void haar1(int16_t *X, int N0, int stride)
 {
     N0 >>= 1;
-    for (int i = 0; i < stride; i++)
-        for (int j = 0; j < N0; j++) {
-            int32_t t1 = mul16_16(23170, X[stride * 2 * j + i]);
-            int32_t t2 = mul16_16(23170, X[stride * (2 * j + 1) + i]);
-            X[stride * 2 * j + i] = (int16_t)pshr32(t1 + t2, 15);
-            X[stride * (2 * j + 1) + i] = (int16_t)pshr32(t1 - t2, 15);
-        }
+    CUO_PFOR(ij, 0, stride * N0) {
+        const int i = ij / N0, j = ij - i * N0;
+        int32_t t1 = mul16_16(23170, X[stride * 2 * j + i]);
+        int32_t t2 = mul16_16(23170, X[stride * (2 * j + 1) + i]);
+        X[stride * 2 * j + i] = (int16_t)pshr32(t1 + t2, 15);
+        X[stride * (2 * j + 1) + i] = (int16_t)pshr32(t1 - t2, 15);
+    }
 }
 
 CUO_FN void deinterleave_hadamard(int16_t *X, int N0, int stride, int hadamard, int16_t *tmp)
 {
     int N = N0 * stride;
-    if (hadamard) {
-        const int16_t *ordery = tab::kOrderyTable + stride - 2;
-        for (int i = 0; i < stride; i++)
-            for (int j = 0; j < N0; j++) tmp[ordery[i] * N0 + j] = X[j * stride + i];
-    } else {
-        for (int i = 0; i < stride; i++)
-            for (int j = 0; j < N0; j++) tmp[i * N0 + j] = X[j * stride + i];
+    const int16_t *ordery = hadamard ? tab::kOrderyTable + stride - 2 : nullptr;
+    CUO_PFOR(ij, 0, N) {
+        const int i = ij / N0, j = ij - i * N0;
+        tmp[(hadamard ? ordery[i] : i) * N0 + j] = X[j * stride + i];
     }
-    for (int i = 0; i < N; i++) X[i] = tmp[i];
+    CUO_PFOR(i, 0, N) X[i] = tmp[i];
 }
 
 CUO_FN void interleave_hadamard(int16_t *X, int N0, int stride, int hadamard, int16_t *tmp)
 {
     int N = N0 * stride;
-    if (hadamard) {
-        const int16_t *ordery = tab::kOrderyTable + stride - 2;
-        for (int i = 0; i < stride; i++)
-            for (int j = 0; j < N0; j++) tmp[j * stride + i] = X[ordery[i] * N0 + j];
-    } else {
-        for (int i = 0; i < stride; i++)
-            for (int j = 0; j < N0; j++) tmp[j * stride + i] = X[i * N0 + j];
+    const int16_t *ordery = hadamard ? tab::kOrderyTable + stride - 2 : nullptr;
+    CUO_PFOR(ij, 0, N) {
+        const int i = ij / N0, j = ij - i * N0;
+        tmp[j * stride + i] = X[(hadamard ? ordery[i] : i) * N0 + j];
     }
-    for (int i = 0; i < N; i++) X[i] = tmp[i];
+    CUO_PFOR(i, 0, N) X[i] = tmp[i];
 }
 
 struct BandCtx {
@@ -1096,7 +1094,7 @@ CUO_FI unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_
     {
     CUO_PROF_SCOPE(kProfBandPre);
     if (lowband_scratch && lowband && (recombine || ((N_B & 1) == 0 && tf_change < 0) || B0 > 1)) {
-        for (int j = 0; j < N; j++) lowband_scratch[j] = lowband[j];
+        CUO_PFOR(j, 0, N) lowband_scratch[j] = lowband[j];
         lowband = lowband_scratch;
     }
     for (int k = 0; k < recombine; k++) {
@@ -1170,7 +1168,10 @@ CUO_FI void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, 
         ctx.remaining_bits = remaining_bits;
         int b;
         if (i <= codedBands - 1) {
-            int32_t curr_balance = balance / imin(3, codedBands - i);
+            // balance / imin(3, codedBands - i), the three divisors spelled out
+            // so each division is a constant one
+            const int d = imin(3, codedBands - i);
+            int32_t curr_balance = d == 3 ? balance / 3 : (d == 2 ? balance / 2 : balance);
             b = imax(0, imin(16383, imin(remaining_bits + 1, pulses[i] + curr_balance)));
         } else {
             b = 0;
@@ -1180,7 +1181,7 @@ CUO_FI void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, 
         if (i == start + 1) {
             int n1 = M * (eB[start + 1] - eB[start]);
             int n2 = M * (eB[start + 2] - eB[start + 1]);
-            for (int j = 0; j < n2 - n1; j++) norm[n1 + j] = norm[2 * n1 - n2 + j];
+            CUO_PFOR(j, 0, n2 - n1) norm[n1 + j] = norm[2 * n1 - n2 + j];  // source ends where target starts
         }
         int tf_change = tf_res[i];
         ctx.tf_change = tf_change;
@@ -1440,8 +1441,11 @@ CUO_FN void fft_impl(int k, Cpx *fout)
 }
 
 // clt_mdct_backward (mdct.c:242-342): in has stride `stride`, out gets the
-// overlap-added time signal.
-CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride)
+// overlap-added time signal.  The N/4-point FFT runs in fbuf (LDS, N/4
+// complex values) rather than in place in the history buffer: the
+// pre-rotation writes fbuf, the butterflies work on LDS, the post-rotation
+// reads fbuf and writes the history.
+CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride, Cpx *fbuf)
 {
     int N = tab::kMdctN;
     const int16_t *trig = tab::kMdctTrig;
@@ -1456,21 +1460,23 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
             int32_t x1 = in[2 * stride * i], x2 = in[stride * (N2 - 1) - 2 * stride * i];
             int32_t yr = add32_ovf(smul(x2, trig[i]), smul(x1, trig[N4 + i]));
             int32_t yi = sub32_ovf(smul(x1, trig[i]), smul(x2, trig[N4 + i]));
-            yp[2 * rev + 1] = yr;
-            yp[2 * rev] = yi;
+            fbuf[rev].i = yr;
+            fbuf[rev].r = yi;
         }
     }
-    fft_impl(shift, (Cpx *)yp);
+    fft_impl(shift, fbuf);
     {
         // post-rotation: iteration i owns the pairs at 2i and N2-2-2i
+        const int32_t *fb = (const int32_t *)fbuf;
         CUO_PFOR(i, 0, (N4 + 1) >> 1) {
+            const int32_t *fb0 = fb + 2 * i, *fb1 = fb + N2 - 2 - 2 * i;
             int32_t *yp0 = yp + 2 * i, *yp1 = yp + N2 - 2 - 2 * i;
-            int32_t re = yp0[1], im = yp0[0];
+            int32_t re = fb0[1], im = fb0[0];
             int16_t t0 = trig[i], t1 = trig[N4 + i];
             int32_t yr = add32_ovf(smul(re, t0), smul(im, t1));
             int32_t yi = sub32_ovf(smul(re, t1), smul(im, t0));
-            re = yp1[1];
-            im = yp1[0];
+            re = fb1[1];
+            im = fb1[0];
             yp0[0] = yr;
             yp1[1] = yi;
             t0 = trig[N4 - i - 1];
@@ -1493,18 +1499,24 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
     }
 }
 
-CUO_FN void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *oldBandE, int start, int effEnd,
-                           int isTransient, int LM, int silence, int32_t *freq)
+// The inverse MDCTs run their FFTs in the band decoder's scratch (X, norm,
+// tmp, iy: dead once denormalise_bands has read X), which holds the largest
+// FFT (480 complex values).
+static_assert(offsetof(CeltScratch, iy) + sizeof(CeltScratch::iy) - offsetof(CeltScratch, X) >= 480 * 8,
+              "band scratch too small for the synthesis FFT");
+CUO_FI void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *oldBandE, int start, int effEnd,
+                           int isTransient, int LM, int silence, int32_t *freq, CeltScratch *sc)
 {
     CUO_PROF_SCOPE(kProfCeltSynth);
     X = lds_uniform(X);
     oldBandE = lds_uniform(oldBandE);
+    Cpx *fbuf = lds_uniform((Cpx *)sc->X);
     const int N = 120 << LM, M = 1 << LM;
     int B, NB, shift;
     if (isTransient) { B = M; NB = 120; shift = 3; }
     else { B = 1; NB = 120 << LM; shift = 3 - LM; }
     denormalise_bands(X, freq, oldBandE, start, effEnd, M, silence);
-    for (int b = 0; b < B; b++) mdct_backward(&freq[b], out_syn + NB * b, shift, B);
+    for (int b = 0; b < B; b++) mdct_backward(&freq[b], out_syn + NB * b, shift, B, fbuf);
     CUO_PFOR(i, 0, N) out_syn[i] = saturate(out_syn[i], kSigSat);
 }
 
@@ -1641,7 +1653,7 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
             renormalise_vector(X + boffs, blen, kQ15One);
         }
         st->rng = seed;
-        celt_synthesis(X, out_syn, st->oldBandE, start, effEnd, 0, LM, 0, st->freq);
+        celt_synthesis(X, out_syn, st->oldBandE, start, effEnd, 0, LM, 0, st->freq, sc);
         st->prefilter_and_fold = 0;
         st->skip_plc = 1;
     } else {
@@ -1804,7 +1816,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     if (silence)
         for (int i = 0; i < kNbBands; i++) oldBandE[i] = -(28 << kDbShift);
     if (st->prefilter_and_fold) prefilter_and_fold(st, N, st->etmp);
-    celt_synthesis(X, out_syn, oldBandE, start, effEnd, isTransient, LM, silence, st->freq);
+    celt_synthesis(X, out_syn, oldBandE, start, effEnd, isTransient, LM, silence, st->freq, sc);
 
     st->pf_period = imax(st->pf_period, kCombMinPeriod);
     st->pf_period_old = imax(st->pf_period_old, kCombMinPeriod);

@@ -222,31 +222,34 @@ __device__ __forceinline__ int lane_byte(int32_t tv, int i)
 // host).  Integer sums wrap like the serial loop's, so any order is exact.
 // Call with every lane active (outside CUO_PFOR bodies).
 #if defined(__HIP_DEVICE_COMPILE__)
+// Each is an inclusive 16-lane DPP scan (row_shr 1, 2, 4, 8; lanes shifted in
+// from outside the row read the operation's identity) whose row results, in
+// lanes 15, 31, 47 and 63, are combined in scalar registers.
+template <class Op>
+__device__ __forceinline__ int32_t wave_reduce(int32_t v, int32_t ident, Op op)
+{
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x111, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x112, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x114, 0xF, 0xF, false));
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x118, 0xF, 0xF, false));
+    return op(op(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
+              op(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
+}
 __device__ __forceinline__ int32_t wave_sum(int32_t v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = (int32_t)((uint32_t)v + (uint32_t)__shfl_xor(v, o, 64));
-    return v;
+    return wave_reduce(v, 0, [](int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); });
 }
 __device__ __forceinline__ int32_t wave_max(int32_t v)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        int32_t w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
+    return wave_reduce(v, INT32_MIN, [](int32_t a, int32_t b) { return a > b ? a : b; });
 }
 __device__ __forceinline__ int32_t wave_min(int32_t v)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        int32_t w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
+    return wave_reduce(v, INT32_MAX, [](int32_t a, int32_t b) { return a < b ? a : b; });
 }
 __device__ __forceinline__ uint32_t wave_or(uint32_t v)
 {
-    for (int o = 32; o > 0; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o, 64);
-    return v;
+    return (uint32_t)wave_reduce((int32_t)v, 0, [](int32_t a, int32_t b) { return a | b; });
 }
 #else
 __host__ __device__ inline int32_t wave_sum(int32_t v) { return v; }

@@ -249,6 +249,14 @@ CUO_FI uint32_t ec_uint(RangeDec *d, uint32_t ft)
 CUO_FI int ec_tell(const RangeDec *d) { return uniform(d->nbits_total) - ilog(uniform(d->rng)); }
 
 static constexpr uint32_t kTellCorrection[8] = {35733, 38967, 42495, 46340, 50535, 55109, 60097, 65535};
+constexpr uint64_t kTellCorrLo = 0xB504A5FF98378B95ull, kTellCorrHi = 0xFFFFEAC1D745C567ull;
+constexpr bool tell_corr_packed()
+{
+    for (int b = 0; b < 8; b++)
+        if ((((b < 4 ? kTellCorrLo : kTellCorrHi) >> (16 * (b & 3))) & 0xFFFFu) != kTellCorrection[b]) return false;
+    return true;
+}
+static_assert(tell_corr_packed(), "packed tell-correction constants differ from the table");
 
 CUO_FI uint32_t ec_tell_frac(const RangeDec *d)
 {
@@ -259,7 +267,9 @@ CUO_FI uint32_t ec_tell_frac(const RangeDec *d)
     int l = ilog(rng);
     uint32_t r = rng >> (l - 16);
     uint32_t b = (r >> 12) - 8;
-    b += r > kTellCorrection[b];
+    // kTellCorrection[b] from two 64-bit constants (four 16-bit entries each)
+    const uint64_t corr = b < 4 ? kTellCorrLo : kTellCorrHi;
+    b += r > (uint32_t)((corr >> (16 * (b & 3))) & 0xFFFFu);
     l = (l << 3) + (int)b;
     return nbits - (uint32_t)l;
 }

@@ -484,6 +484,7 @@ CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, Ou
     if (count < 0) return count;
     if (packet_stream_channels != 1) return OPUS_UNIMPLEMENTED;  // mono streams only
     data += offset;
+    int nframes = count;
     if (decode_fec) {
         if (frame_size < packet_frame_size || packet_mode == MODE_CELT_ONLY || st->mode == MODE_CELT_ONLY)
             return decode_plc(st, out, frame_size, sc);
@@ -492,31 +493,25 @@ CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, Ou
             int ret = decode_plc(st, out, frame_size - packet_frame_size, sc);
             if (ret < 0) { st->last_packet_duration = duration_copy; return ret; }
         }
-        st->mode = packet_mode;
-        st->bandwidth = packet_bandwidth;
-        st->frame_size = packet_frame_size;
-        st->stream_channels = packet_stream_channels;
-        int ret = frame_decode<kFrameData>(st, data, size[0], out.buf, packet_frame_size, 1, sc);
-        if (ret < 0) return ret;
-        out.emit(ret);
-        st->last_packet_duration = frame_size;
-        return frame_size;
+        nframes = 1;  // the LBRR of the first frame, over packet_frame_size samples
+    } else if (count * packet_frame_size > frame_size) {
+        return OPUS_BUFFER_TOO_SMALL;
     }
-    if (count * packet_frame_size > frame_size) return OPUS_BUFFER_TOO_SMALL;
     st->mode = packet_mode;
     st->bandwidth = packet_bandwidth;
     st->frame_size = packet_frame_size;
     st->stream_channels = packet_stream_channels;
     int nb_samples = 0;
-    for (int i = 0; i < count; i++) {
-        int ret = frame_decode<kFrameData>(st, data, size[i], out.buf, frame_size - nb_samples, 0, sc);
+    for (int i = 0; i < nframes; i++) {
+        const int limit = decode_fec ? packet_frame_size : frame_size - nb_samples;
+        const int ret = frame_decode<kFrameData>(st, data, size[i], out.buf, limit, decode_fec, sc);
         if (ret < 0) return ret;
         out.emit(ret);
         data += size[i];
         nb_samples += ret;
     }
-    st->last_packet_duration = nb_samples;
-    return nb_samples;
+    st->last_packet_duration = decode_fec ? frame_size : nb_samples;
+    return decode_fec ? frame_size : nb_samples;
 }
 
 }  // namespace cuo
