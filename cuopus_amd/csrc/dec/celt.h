@@ -569,27 +569,64 @@ __device__ __forceinline__ int pvq_row_off(int r) { return tab::kPvqURow[r < 14 
 // U(n,j) = U(n-1,j) + U(n,j-1) + U(n-1,j-1) gives
 // U(n-1,j) + U(n-1,j-1) = U(n,j) - U(n,j-1), and with U(n-1,0) = 0
 // (n-1 >= 1) that telescopes to U(n-1,j) = (-1)^j (2 P(j) - F(j)), where
-// F(t) = (-1)^t U(n,t) and P is the inclusive prefix sum of F.  P is one
-// 16-lane DPP scan per row (row_shr 1, 2, 4, 8) plus the totals of the rows
-// below, read from lanes 15, 31 and 47.  The arithmetic wraps mod 2^32, so
+// F(t) = (-1)^t U(n,t) and P is the inclusive prefix sum of F over the
+// lanes.  The arithmetic wraps mod 2^32, so
 // every lane whose true value fits 32 bits -- all lanes up to the highest one
 // a decode step reads -- is exact, and a lane only depends on lanes below it.
-__device__ __forceinline__ uint32_t pvq_row_down(uint32_t row)
+// Inclusive prefix sum over all 64 lanes (wrapping): a 16-lane DPP scan per
+// row plus the totals of the rows below, read from lanes 15, 31 and 47.
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
 {
     const int l = lane_id();
-    const uint32_t f = (l & 1) ? 0u - row : row;
-    const uint32_t v = (uint32_t)row16_prefix_sum((int32_t)f);
+    const uint32_t v = (uint32_t)row16_prefix_sum((int32_t)x);
     const uint32_t t0 = (uint32_t)lane_get((int32_t)v, LaneK<15>());
     const uint32_t t1 = (uint32_t)lane_get((int32_t)v, LaneK<31>());
     const uint32_t t2 = (uint32_t)lane_get((int32_t)v, LaneK<47>());
     const uint32_t below = l < 16 ? 0u : (l < 32 ? t0 : (l < 48 ? t0 + t1 : t0 + t1 + t2));
-    const uint32_t sv = 2u * (v + below) - f;
+    return v + below;
+}
+__device__ __forceinline__ uint32_t pvq_row_down(uint32_t row)
+{
+    const int l = lane_id();
+    const uint32_t f = (l & 1) ? 0u - row : row;
+    const uint32_t sv = 2u * wave_incl_sum(f) - f;
     return (l & 1) ? 0u - sv : sv;
 }
 __device__ __forceinline__ int highest_lane(uint64_t m) { return m ? 63 - __builtin_clzll(m) : 0; }
 
-// cwrsi with the row of the first dimension already gathered (row = U(n, .))
-__device__ __noinline__ int32_t cwrsi_lanes(int n, int k, uint32_t i, int32_t *y, uint32_t row)
+// Runs of zeros (cwrs.c:513-520 taken for several dimensions in a row): with
+// k < n, dimension d decodes to 0 while U(d, k) <= i_d < U(d, k + 1), where
+// i_d is i less the U(d', k) of the dimensions d' > d before it.  Lane m takes
+// dimension n - m: it gathers U(n - m, k) and U(n - m, k + 1) (row k and k + 1
+// of the table, column n - m > k), an exclusive prefix sum gives its i, and
+// the lowest lane that fails the test ends the run.  Lanes are limited to
+// dimensions above max(k, 2), which the row-by-row steps handle.  The sums are
+// exact up to the first failing lane (each is at most i there).  Writes the
+// zeros, lowers i and returns the run length (>= 1 when dimension n is a zero).
+__device__ __forceinline__ int pvq_zero_run(int n, int k, uint32_t &i, int32_t *y)
+{
+    const int m = lane_id();
+    const int d = n - m;
+    const int lim = k > 2 ? k : 2;
+    const bool valid = d > lim;
+    const int ia = valid ? ld_s16(tab::kPvqURow, k) + d : 0;
+    const int ib = valid ? ld_s16(tab::kPvqURow, k + 1) + d : 0;
+    const uint32_t a = valid ? tab::kPvqU[ia] : 0u;
+    const uint32_t b = tab::kPvqU[ib];
+    const uint32_t incl = wave_incl_sum(a);
+    const uint32_t im = i - (incl - a);
+    const uint64_t fail = __ballot(valid && !(a <= im && im < b));
+    const int run = fail ? __builtin_ctzll(fail) : imin(n - lim, 64);
+    i -= (uint32_t)lane_get((int32_t)incl, run - 1);
+    if (m < run) y[m] = 0;
+    return run;
+}
+// Zero runs shorter than this stay on the row-by-row steps.
+constexpr int kPvqRunMin = 8;
+
+// cwrsi with the row of the first dimension already gathered (row = U(n, .);
+// row_lane = the table offset of row min(lane, 14))
+__device__ __noinline__ int32_t cwrsi_lanes(int n, int k, uint32_t i, int32_t *y, uint32_t row, int row_lane)
 {
     n = uniform(n);
     k = uniform(k);
@@ -619,6 +656,13 @@ __device__ __noinline__ int32_t cwrsi_lanes(int n, int k, uint32_t i, int32_t *y
             p = (uint32_t)lane_get((int32_t)row, k);
             const uint32_t q = (uint32_t)lane_get((int32_t)row, k + 1);
             if (p <= i && i < q) {
+                if (n - k >= kPvqRunMin) {
+                    const int run = pvq_zero_run(n, k, i, y);
+                    y += run;
+                    n -= run;
+                    if (n > 2) row = pvq_row_lanes(n, row_lane, lane_get(row_lane, imin(n, 14)));
+                    continue;
+                }
                 i -= p;
                 *y++ = 0;
             } else {
@@ -765,7 +809,7 @@ CUO_FI unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDe
             idx = ec_uint(d, v);
         }
         CUO_PROF_SCOPE(kProfCwrsi);
-        Ryy = cwrsi_lanes(N, K, idx, iy, row);
+        Ryy = cwrsi_lanes(N, K, idx, iy, row, row_lane);
     } else
 #endif
     {
