@@ -22,12 +22,17 @@ written, 40 B of chain descriptor + result per chain; achieved = those bytes
 over the kernel's average duration measured with HIP events on its launch
 stream.  The resampler's own roofline (10 B per 16 kHz output) is reported
 beside it.  traffic comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-in profiles/ for the same workload when present.
+in profiles/ for the same workload when present, and issue from the same
+summary's SQ counters: the fraction of a resident wavefront's cycles in which
+it issues a VALU / scalar / any instruction or waits on a memory counter (four
+wavefronts share each SIMD), the measure that bounds this integer decoder.
 
 cpu_baseline: fixed-point libopus (built from the reference's vendored 1.5.2
 sources) decoding the identical chain list with a fresh decoder per chain on
-the host's cores, plus the float32 resampler restated in C, timed on the
-whole workload.
+the host's cores, plus the float32 resampler restated in C, timed on a
+bounded sample of the workload; beside it (stream_1thread) the same libopus
+decoding the start of the first file as one stream without chains or preroll
+on one thread, the per-core rate of a CPU pool that decodes whole files.
 """
 import argparse
 import json
@@ -186,6 +191,46 @@ def _cpu_baseline(blobs, threads, max_audio_s=7200.0):
     }
 
 
+def _cpu_stream(blob, max_audio_s=1800.0):
+    """libopus decoding the start of the first file as ONE stream (one decoder,
+    no chains, no preroll: what a CPU pool does per file), one thread."""
+    import numpy as np
+    from tests import oracle_lib as ol
+    L = ol.lib()
+    d, _ = ol.demux_chains(blob)
+    data, off, ln = d.blob()
+    total = 0
+    n = 0
+    limit = int(max_audio_s * 48000) + d.preskip
+    while n < off.size and total < limit:
+        total += L.orc_packet_nb_samples(data[int(off[n]):].ctypes.data, int(ln[n]))
+        n += 1
+    keep = max(total - d.preskip, 0)
+    t0 = time.perf_counter()
+    ref = ol.decode_chains(data, off, ln, [(0, n, d.preskip, keep)], threads=1, caps=[total + 5760])
+    t1 = time.perf_counter()
+    assert int(ref["ret"][0]) == total
+    secs = keep / 48000.0
+    return {"value": round(secs / (t1 - t0), 1), "unit": "audio-s/s", "cores": 1, "kind": "reference",
+            "sample": "first %d packets (%.0f audio-s) of the first file as one stream, no preroll, 1 thread: "
+                      "%.2f s" % (n, secs, t1 - t0)}
+
+
+def _read_issue(workload):
+    """Per-wavefront issue and wait fractions of the decode kernel from the
+    committed rocprofv3 SQ counter summary (line 'issue_fractions: k=v ...')."""
+    path = os.path.join(ROOT, "profiles", "decode_%s_pmc.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("issue_fractions:"):
+            try:
+                return {k: float(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
+            except ValueError:
+                return None
+    return None
+
+
 def _batching(cn, torch, steps, name="uneven_50"):
     """Many files of uneven length (config 3 shape, >= 10k chains): every chain
     in one persistent launch vs one decode + one resample launch per file."""
@@ -328,7 +373,7 @@ def main():
         roof = {"bound": "hbm", "achieved": round(ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": "k_decode_persistent"
                 if a.mode == "persistent" else "k_decode_chains", "kernel_ms": round(d_avg, 3),
-                "algorithmic_bytes": int(alg_dec)}
+                "algorithmic_bytes": int(alg_dec), "issue": _read_issue(a.workload)}
         rs_bytes = 10 * b.out16
         rs_ach = rs_bytes / (r_avg * 1e-3) / 1e9
         rs_roof = {"bound": "hbm", "achieved": round(rs_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -359,6 +404,8 @@ def main():
         th = a.cpu_threads or min(BOX_CPU_SHARE, os.cpu_count() or 1)
         cpu = _cpu_baseline(blobs, th, a.cpu_audio_s)
         _log("cpu baseline: %s" % cpu["sample"])
+        cpu["stream_1thread"] = _cpu_stream(blobs[0], min(a.cpu_audio_s, 1800.0))
+        _log("cpu stream: %s" % cpu["stream_1thread"]["sample"])
 
     line = {
         "metric": "audio-sec decoded/s on 1 MI355X, bit-exact: batched vs plain; serve p99 ms",

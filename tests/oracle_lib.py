@@ -68,9 +68,10 @@ def demux_chains(blob, seconds=1.0, preroll_ms=250.0):
     return d, odemux.build_chains(d, seconds, preroll_ms)
 
 
-def decode_chains(data, off, ln, chains, lost=None, threads=1):
+def decode_chains(data, off, ln, chains, lost=None, threads=1, caps=None):
     """Decode every chain with a fresh fixed-point libopus decoder.
 
+    caps: PCM capacity per chain (default 5760 per packet + 5760).
     Returns dict: pcm (compacted keep windows, int16), nsamp/rng per packet,
     ret per chain, full (list of per-chain full PCM arrays)."""
     L = lib()
@@ -78,7 +79,9 @@ def decode_chains(data, off, ln, chains, lost=None, threads=1):
     n = ch.shape[0]
     first = np.ascontiguousarray(ch[:, 0], dtype=np.uint32)
     npk = np.ascontiguousarray(ch[:, 1], dtype=np.uint32)
-    caps = (npk.astype(np.uint64) + 1) * 5760
+    if caps is None:
+        caps = (npk.astype(np.uint64) + 1) * 5760
+    caps = np.ascontiguousarray(caps, dtype=np.uint64)
     pcm_off = np.zeros(n, np.uint64)
     if n > 1:
         pcm_off[1:] = np.cumsum(caps[:-1])
