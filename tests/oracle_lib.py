@@ -164,6 +164,11 @@ RECIPES = {
     # CELT-only narrowband and wideband (TOC configs 16-23)
     "celt_nb": "--seed 21 --dur 8 --signal speech --app lowdelay --bw nb --bitrate 24000 --frameset 120,240,480,960",
     "celt_wb": "--seed 22 --dur 8 --signal mixed --app lowdelay --bw wb --bitrate 32000 --frame 480",
+    # multi-frame packets: CELT 60 and 120 ms (code 3, several 20 ms frames per packet), SILK 40 ms,
+    # hybrid 10 ms frames
+    "celt_multi": "--seed 23 --dur 8 --signal music --app lowdelay --bw fb --bitrate 48000 --frameset 2880,5760",
+    "silk_40ms": "--seed 24 --dur 8 --signal speech --app voip --bw wb --force silk --bitrate 18000 --frame 1920",
+    "hybrid_10ms": "--seed 25 --dur 8 --signal speech --app voip --bw swb --force hybrid --bitrate 28000 --frame 480",
 }
 
 
