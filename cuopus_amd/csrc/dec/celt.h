@@ -621,7 +621,9 @@ __device__ __forceinline__ int pvq_zero_run(int n, int k, uint32_t &i, int32_t *
     if (m < run) y[m] = 0;
     return run;
 }
-// Zero runs shorter than this stay on the row-by-row steps.
+// The run step is taken when n - k is at least this; shorter stretches are
+// cheaper row by row.  k + 1 <= 14 there: the allocation only yields (N, K)
+// with V(N, K) < 2^32, and k < n >= 15 with k >= 14 would exceed it.
 constexpr int kPvqRunMin = 8;
 
 // cwrsi with the row of the first dimension already gathered (row = U(n, .);
