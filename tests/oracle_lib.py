@@ -172,6 +172,20 @@ RECIPES = {
 }
 
 
+# More seeds of the mode-switching mixes (SILK/hybrid/CELT transitions, redundancy, FEC): decoded with
+# seeded loss by one CPU-gate test and one GPU test each
+SEED_SWEEP = {
+    "sweep_%d" % sd: "--seed %d --dur 10 --signal mixed --app %s --bw %s --brange 10000 72000%s"
+                     % (sd, app, bw, " --fec 1 --loss 4" if sd % 2 else "")
+    for sd, app, bw in ((31, "audio", "fb"), (32, "voip", "fb"), (33, "audio", "swb"), (34, "voip", "wb"),
+                        (35, "audio", "fb"), (36, "voip", "swb"))
+}
+
+
+def recipe(name):
+    return RECIPES[name] if name in RECIPES else SEED_SWEEP[name]
+
+
 # Benchmark inputs (minutes to generate; written once under corpus/bench)
 BENCH = os.path.join(ROOT, "corpus", "bench")
 BENCH_RECIPES = {

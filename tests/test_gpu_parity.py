@@ -24,7 +24,7 @@ def cn():
 
 
 def _load(name):
-    path = ol.small_corpus(name, ol.RECIPES[name])
+    path = ol.small_corpus(name, ol.recipe(name))
     blob = open(path, "rb").read()
     d, chains = ol.demux_chains(blob)
     return blob, d, chains
@@ -58,6 +58,13 @@ def test_gate_loss_bitexact(cn, name):
     _, d, _ = _load(name)
     lost = ol.loss_mask(len(d.packets), 0.05, seed=sum(name.encode()))
     _check_gate(cn, name, lost)
+
+
+@pytest.mark.parametrize("name", sorted(ol.SEED_SWEEP))
+def test_gate_seed_sweep_loss_bitexact(cn, name):
+    _, d, _ = _load(name)
+    lost = ol.loss_mask(len(d.packets), 0.05, seed=sum(name.encode()))
+    _check_gate(cn, name, lost=lost)
 
 
 @pytest.mark.parametrize("name", ["silk_wb", "celt_fb", "hybrid_fb", "mixed"])

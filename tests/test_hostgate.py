@@ -82,6 +82,20 @@ def test_hostgate_loss_bitexact(hg, name, p):
     assert np.array_equal(pcm, ref["pcm"])
 
 
+@pytest.mark.parametrize("name", sorted(ol.SEED_SWEEP))
+def test_hostgate_seed_sweep_loss_bitexact(hg, name):
+    blob = open(ol.small_corpus(name, ol.recipe(name)), "rb").read()
+    d, chains = ol.demux_chains(blob)
+    data, off, ln = d.blob()
+    lost = ol.loss_mask(len(off), 0.05, seed=sum(name.encode()))
+    pcm, nsamp, rng, ret = _hg_chains(hg, data, off, ln, chains, lost)
+    ref = ol.decode_chains(data, off, ln, chains, lost=lost)
+    assert (ret >= 0).all()
+    assert np.array_equal(nsamp, ref["nsamp"])
+    assert np.array_equal(rng, ref["rng"])
+    assert np.array_equal(pcm, ref["pcm"])
+
+
 def test_range_decoder_schedule(hg):
     """The 144-symbol schedule (schedule.inc:19-37) through the product's
     range decoder vs libopus entdec, on random and real packets."""
