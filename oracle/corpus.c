@@ -19,7 +19,7 @@
  * usage: mkcorpus --out F --seed S --dur SEC [--signal speech|music]
  *        [--signal speech|music|mixed] [--app voip|audio|lowdelay] [--bw nb|mb|wb|swb|fb]
  *        [--force auto|silk|hybrid|celt] [--vbr 0|1] [--bitrate B]
- *        [--brange LO HI] [--frame N] [--frameset a,b,c] [--fec 0|1]
+ *        [--brange LO HI] [--frame N] [--frameset a,b,c] [--fec 0|1] [--dtx 0|1] [--gaps P]
  *        [--loss P] [--complexity C] [--pcm F]
  * --brange redraws the bitrate every second from U(LO,HI); --frameset draws
  * the frame size (48 kHz samples) per second.
@@ -232,7 +232,8 @@ int main(int argc, char **argv)
     const char *bw = "fb", *force = "auto", *frameset = NULL;
     uint64_t seed = 1;
     double dur = 10.0;
-    int vbr = 1, bitrate = 32000, blo = 0, bhi = 0, frame = 960, fec = 0, loss = 0, cx = 10;
+    int vbr = 1, bitrate = 32000, blo = 0, bhi = 0, frame = 960, fec = 0, loss = 0, cx = 10, dtx = 0;
+    double gaps = 0.0;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
 #define NEXT (i + 1 < argc ? argv[++i] : "")
@@ -252,6 +253,8 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--fec")) fec = atoi(NEXT);
         else if (!strcmp(a, "--loss")) loss = atoi(NEXT);
         else if (!strcmp(a, "--complexity")) cx = atoi(NEXT);
+        else if (!strcmp(a, "--dtx")) dtx = atoi(NEXT);
+        else if (!strcmp(a, "--gaps")) gaps = atof(NEXT);
         else { fprintf(stderr, "unknown arg %s\n", a); return 2; }
 #undef NEXT
     }
@@ -282,6 +285,14 @@ int main(int argc, char **argv)
         for (long i = 0; i < n; i++) if (is_music[i / FS]) x[i] = y[i];
         free(y);
     } else gen_speech(x, n);
+    /* --gaps P: each 500 ms segment is silent with probability P (only the
+       int16 dither remains), so the encoder's DTX has inactive stretches */
+    if (gaps > 0.0) {
+        const long seg = FS / 2;
+        for (long s0 = 0; s0 < n; s0 += seg)
+            if (runif() < gaps)
+                for (long i = s0; i < n && i < s0 + seg; i++) x[i] = 0.0;
+    }
 
     int application = !strcmp(app, "voip") ? OPUS_APPLICATION_VOIP
                     : !strcmp(app, "lowdelay") ? OPUS_APPLICATION_RESTRICTED_LOWDELAY
@@ -295,6 +306,7 @@ int main(int argc, char **argv)
     opus_encoder_ctl(enc, OPUS_SET_BITRATE(bitrate));
     opus_encoder_ctl(enc, OPUS_SET_INBAND_FEC(fec));
     opus_encoder_ctl(enc, OPUS_SET_PACKET_LOSS_PERC(loss));
+    opus_encoder_ctl(enc, OPUS_SET_DTX(dtx));
     if (!strcmp(force, "silk")) opus_encoder_ctl(enc, SET_FORCE_MODE, MODE_SILK);
     else if (!strcmp(force, "hybrid")) opus_encoder_ctl(enc, SET_FORCE_MODE, MODE_HYB);
     else if (!strcmp(force, "celt")) opus_encoder_ctl(enc, SET_FORCE_MODE, MODE_CELT);

@@ -169,6 +169,10 @@ RECIPES = {
     "celt_multi": "--seed 23 --dur 8 --signal music --app lowdelay --bw fb --bitrate 48000 --frameset 2880,5760",
     "silk_40ms": "--seed 24 --dur 8 --signal speech --app voip --bw wb --force silk --bitrate 18000 --frame 1920",
     "hybrid_10ms": "--seed 25 --dur 8 --signal speech --app voip --bw swb --force hybrid --bitrate 28000 --frame 480",
+    # discontinuous transmission over silent stretches: TOC-only packets (0-byte frames) decoded as
+    # concealment / comfort noise
+    "silk_dtx": "--seed 26 --dur 10 --signal speech --app voip --bw wb --force silk --bitrate 16000 --dtx 1 --gaps 0.5",
+    "mixed_dtx": "--seed 27 --dur 10 --signal speech --app voip --bw fb --brange 12000 40000 --dtx 1 --gaps 0.5",
 }
 
 

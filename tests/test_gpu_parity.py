@@ -53,7 +53,7 @@ def test_gate_bitexact(cn, name):
 
 
 @pytest.mark.parametrize("name", ["hybrid_fec", "silk_wb", "hybrid_fb", "silk_nb", "celt_fb", "mixed", "celt_multi",
-                                  "silk_40ms", "hybrid_10ms"])
+                                  "silk_40ms", "hybrid_10ms", "silk_dtx", "mixed_dtx"])
 def test_gate_loss_bitexact(cn, name):
     _, d, _ = _load(name)
     lost = ol.loss_mask(len(d.packets), 0.05, seed=sum(name.encode()))

@@ -68,7 +68,7 @@ def test_hostgate_bitexact(hg, name):
 @pytest.mark.parametrize("name,p", [("hybrid_fec", 0.05), ("silk_wb", 0.05), ("silk_nb", 0.1),
                                     ("hybrid_swb", 0.05), ("silk_long", 0.05), ("celt_fb", 0.1),
                                     ("mixed", 0.05), ("celt_multi", 0.1), ("silk_40ms", 0.1),
-                                    ("hybrid_10ms", 0.05)])
+                                    ("hybrid_10ms", 0.05), ("silk_dtx", 0.05), ("mixed_dtx", 0.05)])
 def test_hostgate_loss_bitexact(hg, name, p):
     blob = open(ol.small_corpus(name, ol.RECIPES[name]), "rb").read()
     d, chains = ol.demux_chains(blob)
