@@ -173,6 +173,12 @@ RECIPES = {
     # concealment / comfort noise
     "silk_dtx": "--seed 26 --dur 10 --signal speech --app voip --bw wb --force silk --bitrate 16000 --dtx 1 --gaps 0.5",
     "mixed_dtx": "--seed 27 --dur 10 --signal speech --app voip --bw fb --brange 12000 40000 --dtx 1 --gaps 0.5",
+    # rate extremes: CELT at 6 kbps (most bands skipped or folded), CELT at 510 kbps (packets up to
+    # 1275 bytes), SILK WB at 40 kbps CBR (large pulse counts, LSB escapes)
+    "celt_lo": "--seed 28 --dur 8 --signal music --app lowdelay --bw fb --bitrate 6000 --frameset 120,240,480,960",
+    "celt_max": "--seed 29 --dur 4 --signal music --app lowdelay --bw fb --bitrate 510000 --vbr 0 "
+                "--frameset 120,240,480,960",
+    "silk_hi": "--seed 30 --dur 8 --signal speech --app voip --bw wb --force silk --vbr 0 --bitrate 40000",
 }
 
 
