@@ -179,6 +179,10 @@ RECIPES = {
     "celt_max": "--seed 29 --dur 4 --signal music --app lowdelay --bw fb --bitrate 510000 --vbr 0 "
                 "--frameset 120,240,480,960",
     "silk_hi": "--seed 30 --dur 8 --signal speech --app voip --bw wb --force silk --vbr 0 --bitrate 40000",
+    # lengths at the chain protocol's edges: shorter than one chain, exactly one chain, one sample past it
+    "len_short": "--seed 37 --dur 0.1 --signal speech --app audio --bw fb --bitrate 32000",
+    "len_1s": "--seed 38 --dur 1 --signal mixed --app audio --bw fb --bitrate 32000",
+    "len_1s_plus": "--seed 39 --dur 1.0000208333 --signal music --app lowdelay --bw fb --bitrate 48000",
 }
 
 

@@ -67,7 +67,7 @@ def test_gate_seed_sweep_loss_bitexact(cn, name):
     _check_gate(cn, name, lost=lost)
 
 
-@pytest.mark.parametrize("name", ["silk_wb", "celt_fb", "hybrid_fb", "mixed"])
+@pytest.mark.parametrize("name", ["silk_wb", "celt_fb", "hybrid_fb", "mixed", "len_short", "len_1s", "len_1s_plus"])
 def test_f16k_matches_oracle(cn, name):
     import torch
     blob, d, chains = _load(name)
