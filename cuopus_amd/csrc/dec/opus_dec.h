@@ -36,6 +36,7 @@ struct OpusDecState {
     int32_t prev_redundancy;
     int32_t last_packet_duration;
     uint32_t rangeFinal;
+    int32_t decode_gain;  // OPUS_SET_GAIN value, Q8 dB (0: none)
 };
 
 // Per-call temporaries.  SILK and CELT decode one after the other inside a
@@ -102,6 +103,7 @@ CUO_FN void opus_dec_init(OpusDecState *st)
     st->prev_redundancy = 0;
     st->last_packet_duration = 0;
     st->rangeFinal = 0;
+    st->decode_gain = 0;
 }
 
 // ------------------------------------------------------------------ TOC helpers
@@ -233,10 +235,18 @@ CUO_FN int packet_has_lbrr(const uint8_t *p, int len)
     int off;
     int ret = packet_parse(p, len, size, &off);
     if (ret <= 0) return ret;
-    int lbrr = (p[off] >> (7 - nb_frames)) & 0x1;
-    if (p[0] & 0x4) lbrr = lbrr || ((p[off] >> (6 - 2 * nb_frames)) & 0x1);
+    // libopus reads the first byte of the first frame even when that frame is
+    // empty (then the byte lies past the packet): such a byte reads as 0 here,
+    // as in the oracle's zero-padded packet copy, so the result never depends
+    // on what follows the packet in memory
+    const int b0 = off < len ? p[off] : 0;
+    int lbrr = (b0 >> (7 - nb_frames)) & 0x1;
+    if (p[0] & 0x4) lbrr = lbrr || ((b0 >> (6 - 2 * nb_frames)) & 0x1);
     return lbrr;
 }
+
+// log2(10) / (20 * 256) in Q25: Q8 dB -> Q10 log2 (QCONST16(6.48814081e-4f, 25))
+constexpr int32_t kGainLog2Q25 = 21771;
 
 CUO_FN void smooth_fade(const int16_t *in1, const int16_t *in2, int16_t *out, int overlap)
 {
@@ -437,6 +447,12 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
         } else {
             smooth_fade(pcm_transition, pcm, pcm, F2_5);
         }
+    }
+    if (st->decode_gain) {
+        // OPUS_SET_GAIN (opus_decoder.c:639-648): gain = 2^(decode_gain / (20 log10(2) 256)), applied
+        // as a Q16 multiply with saturation
+        const int32_t gain = celt_exp2((int16_t)mul16_16_p15(kGainLog2Q25, st->decode_gain));
+        CUO_PFOR(i, 0, frame_size) pcm[i] = (int16_t)saturate(mul16_32_p16(pcm[i], gain), 32767);
     }
     st->rangeFinal = len <= 1 ? 0 : (HasData ? dec.rng ^ redundant_rng : 0);
     st->prev_mode = mode;

@@ -257,6 +257,7 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
         const uint8_t *hp = buf + LB.parts[h.part0].first;
         lk.channels = hp[9];
         lk.preskip = hp[10] | (hp[11] << 8);
+        lk.gain_q8 = (int16_t)(uint16_t)(hp[16] | (hp[17] << 8));
         lk.first = (uint32_t)k;
         int64_t cum = 0;
         for (size_t i = 2; i < LB.packets.size(); i++, k++) {

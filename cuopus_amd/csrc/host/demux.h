@@ -48,6 +48,7 @@ struct Link {
     uint32_t first = 0, npkts = 0;  // audio packet range in Demuxed::off/len
     int channels = 0;
     int preskip = 0;
+    int gain_q8 = 0;                // OpusHead output gain (RFC 7845 5.1), Q7.8 dB
     uint32_t serial = 0;
     int64_t toc_total = 0;          // sum of TOC durations of its packets
     int64_t last_granule = 0;

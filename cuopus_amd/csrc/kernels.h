@@ -32,7 +32,7 @@ struct ChainDesc {
     uint32_t keep;    // samples kept
     uint64_t dst;     // offset of the keep window in the compacted output
     uint32_t rec_end; // per-packet records (gate): only packets below this index
-    uint32_t pad;
+    int32_t gain;     // output gain, Q8 dB (OpusHead; libopus OPUS_SET_GAIN semantics)
 };
 
 struct ChainResult {
@@ -108,6 +108,7 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
     DecScratch *sc = lds_uniform(&ws->sc);
     dec_bind(st, sc, heap);
     opus_dec_init(st);
+    st->decode_gain = ch.gain;
     ChainOut out;
     out.buf = sc->frame_out;
     out.out = a.out;
@@ -213,28 +214,35 @@ struct RsFile {
     uint64_t tile0;    // first global tile index of this file
 };
 
-constexpr int kRsThreads = 256;                   // threads per workgroup
-constexpr int kRsPerThread = 4;                   // consecutive outputs per thread
-constexpr int kRsTile = kRsThreads * kRsPerThread;  // outputs per workgroup
-constexpr int kRsIn = 3 * kRsTile + 2 * kRsPad;     // staged input samples per workgroup
-constexpr int kRsWin = (3 * (kRsPerThread - 1) + kRsTaps + 3) & ~3;  // one thread's window, float4 reads
-static_assert(kRsIn % 8 == 0, "staging works in 16-byte chunks of 8 samples");
-static_assert(3 * kRsPerThread * (kRsThreads - 1) + kRsWin <= kRsIn, "the last window stays inside the tile");
+// Resampler launch shapes (threads per workgroup x consecutive outputs per
+// thread); the autotuner picks one per device.  Outputs per thread stay a
+// multiple of 4 so each thread's window starts 16-byte aligned in LDS.
+constexpr int kRsVariants = 5;
+constexpr int kRsDefaultVariant = 0;
+constexpr int kRsShape[kRsVariants][2] = {{256, 4}, {256, 8}, {128, 8}, {512, 4}, {128, 4}};
+__host__ __device__ constexpr int rs_tile(int v) { return kRsShape[v][0] * kRsShape[v][1]; }
 
 // Batched resampler: workgroup b handles tile b of the concatenated per-file
 // tile list; out[i] = sum_t taps[t] * x[3 i - 24 + t] / 32768 with zeros
 // outside [0, in_len), each output accumulated in tap order with fused
-// multiply-adds.  The tile's input (3 * 1024 samples plus the 24-sample halo
-// on each side) is staged in LDS as f32, with 16-byte loads where the tile
-// lies inside the file and its start is 16-byte aligned; thread t then reads
-// its 60-sample window as 15 float4 (lanes 48 bytes apart: every group of 16
-// lanes covers all LDS banks once) and produces 4 consecutive outputs, stored
-// as one float4 when aligned.
-__global__ __launch_bounds__(kRsThreads) void k_resample_48to16(const int16_t *__restrict__ in,
-                                                                const RsFile *__restrict__ files, int n_files,
-                                                                RsTaps taps)
+// multiply-adds.  The tile's input (3 * Threads * PerThread samples plus the
+// 24-sample halo on each side) is staged in LDS as f32, with 16-byte loads
+// where the tile lies inside the file and its start is 16-byte aligned;
+// thread t then reads its window as float4s (lanes 12 * PerThread bytes
+// apart) and produces PerThread consecutive outputs, stored as float4s when
+// aligned.
+template <int Threads, int PerThread>
+__global__ __launch_bounds__(Threads) void k_resample_48to16(const int16_t *__restrict__ in,
+                                                             const RsFile *__restrict__ files, int n_files,
+                                                             RsTaps taps)
 {
-    __shared__ __align__(16) float xs[kRsIn];
+    constexpr int Tile = Threads * PerThread;
+    constexpr int In = 3 * Tile + 2 * kRsPad;                      // staged input samples
+    constexpr int Win = (3 * (PerThread - 1) + kRsTaps + 3) & ~3;  // one thread's window
+    static_assert(PerThread % 4 == 0, "windows must start 16-byte aligned");
+    static_assert(In % 8 == 0, "staging works in 16-byte chunks of 8 samples");
+    static_assert(3 * PerThread * (Threads - 1) + Win <= In, "the last window stays inside the tile");
+    __shared__ __align__(16) float xs[In];
     const uint64_t b = blockIdx.x;
     // find the file of this tile (files are few per launch; binary search)
     int lo = 0, hi = n_files - 1;
@@ -243,14 +251,14 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_48to16(const int16_t *_
         if (files[mid].tile0 <= b) lo = mid; else hi = mid - 1;
     }
     const RsFile f = files[lo];
-    const uint64_t o0 = (b - f.tile0) * kRsTile;
+    const uint64_t o0 = (b - f.tile0) * Tile;
     const int64_t base = (int64_t)(3 * o0) - kRsPad;
     const int16_t *x = in + f.in_off;
     const int tid = (int)threadIdx.x;
-    const bool inside = base >= 0 && base + kRsIn <= (int64_t)f.in_len;
+    const bool inside = base >= 0 && base + In <= (int64_t)f.in_len;
     if (inside && ((uintptr_t)(x + base) & 15) == 0) {
         const int4 *src = (const int4 *)(x + base);
-        for (int c = tid; c < kRsIn / 8; c += kRsThreads) {
+        for (int c = tid; c < In / 8; c += Threads) {
             const int4 v = src[c];
             const int32_t w4[4] = {v.x, v.y, v.z, v.w};
             float y[8];
@@ -264,37 +272,39 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_48to16(const int16_t *_
             d[1] = make_float4(y[4], y[5], y[6], y[7]);
         }
     } else {
-        for (int i = tid; i < kRsIn; i += kRsThreads) {
+        for (int i = tid; i < In; i += Threads) {
             const int64_t gi = base + i;
             xs[i] = (gi >= 0 && gi < (int64_t)f.in_len) ? (float)x[gi] * (1.0f / 32768.0f) : 0.f;
         }
     }
     __syncthreads();
-    const uint64_t o = o0 + (uint64_t)(kRsPerThread * tid);
+    const uint64_t o = o0 + (uint64_t)(PerThread * tid);
     if (o >= f.out_len) return;
-    float w[kRsWin];
-    const float4 *wv = (const float4 *)&xs[3 * kRsPerThread * tid];
+    float w[Win];
+    const float4 *wv = (const float4 *)&xs[3 * PerThread * tid];
 #pragma unroll
-    for (int m = 0; m < kRsWin / 4; m++) {
+    for (int m = 0; m < Win / 4; m++) {
         const float4 v = wv[m];
         w[4 * m] = v.x;
         w[4 * m + 1] = v.y;
         w[4 * m + 2] = v.z;
         w[4 * m + 3] = v.w;
     }
-    float acc[kRsPerThread];
+    float acc[PerThread];
 #pragma unroll
-    for (int j = 0; j < kRsPerThread; j++) acc[j] = 0.f;
+    for (int j = 0; j < PerThread; j++) acc[j] = 0.f;
 #pragma unroll
     for (int t = 0; t < kRsTaps; t++)
 #pragma unroll
-        for (int j = 0; j < kRsPerThread; j++) acc[j] = __builtin_fmaf(taps.t[t], w[3 * j + t], acc[j]);
+        for (int j = 0; j < PerThread; j++) acc[j] = __builtin_fmaf(taps.t[t], w[3 * j + t], acc[j]);
     float *dst = f.out + o;
-    if (o + kRsPerThread <= f.out_len && ((uintptr_t)dst & 15) == 0) {
-        *(float4 *)dst = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    if (o + PerThread <= f.out_len && ((uintptr_t)dst & 15) == 0) {
+#pragma unroll
+        for (int j = 0; j < PerThread; j += 4)
+            *(float4 *)(dst + j) = make_float4(acc[j], acc[j + 1], acc[j + 2], acc[j + 3]);
     } else {
 #pragma unroll
-        for (int j = 0; j < kRsPerThread; j++)
+        for (int j = 0; j < PerThread; j++)
             if (o + j < f.out_len) dst[j] = acc[j];
     }
 }

@@ -9,6 +9,7 @@
 // code 0 and decoded samples == the chain's TOC capacity) before reporting
 // success (libcuopus.cu:436-452).
 #include <hip/hip_runtime.h>
+#include <dlfcn.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -18,11 +19,13 @@
 #include <algorithm>
 #include <string>
 #include <vector>
+#include <array>
 #include <deque>
 
 #include "../../include/cuopus.h"
 #include "host/demux.h"
 #include "kernels.h"
+#include "logmel.h"
 
 using namespace cuo;
 using cuo_host::Chain;
@@ -141,6 +144,96 @@ RsTaps make_taps()
     return t;
 }
 
+// ---------------------------------------------------------------- tuned defaults
+// Per-device launch defaults written by the autotuner (cuopus_amd/autotune.py)
+// to cuopus_tuned.txt beside libcuopus.so, or the file CUOPUS_TUNED names.
+// One line per device: "<gcnArchName>/<CU count> slots=<n> rs=<variant>".
+// Read once per process and device; absent file or device: built-in defaults.
+struct TunedDefaults {
+    uint32_t slots = 0;  // persistent wavefronts (0: kDefaultWaves)
+    int rs = -1;         // resampler variant (-1: kRsDefaultVariant)
+};
+
+std::string lib_dir()
+{
+    Dl_info info;
+    if (dladdr((void *)&lib_dir, &info) && info.dli_fname) {
+        std::string s = info.dli_fname;
+        size_t k = s.rfind('/');
+        return k == std::string::npos ? std::string(".") : s.substr(0, k);
+    }
+    return ".";
+}
+
+const TunedDefaults &tuned_defaults()
+{
+    static TunedDefaults cache[64];
+    static bool loaded[64] = {false};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (loaded[dev]) return cache[dev];
+    loaded[dev] = true;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return cache[dev];
+    char key[160];
+    std::string arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    snprintf(key, sizeof(key), "%s/%d", arch.c_str(), prop.multiProcessorCount);
+    const char *env = getenv("CUOPUS_TUNED");
+    std::string path = env && *env ? std::string(env) : lib_dir() + "/cuopus_tuned.txt";
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return cache[dev];
+    char line[512];
+    while (fgets(line, sizeof(line), f)) {
+        char k[160];
+        if (sscanf(line, "%159s", k) != 1 || strcmp(k, key) != 0) continue;
+        const char *s = strstr(line, "slots=");
+        if (s) cache[dev].slots = (uint32_t)strtoul(s + 6, nullptr, 10);
+        const char *r = strstr(line, "rs=");
+        if (r) cache[dev].rs = atoi(r + 3);
+    }
+    fclose(f);
+    return cache[dev];
+}
+
+// Per-frame decode costs of the work-order model, by TOC (about 0.1 us of
+// one wavefront each; measured on MI355X with homogeneous streams).
+// CELT by frame size 2.5/5/10/20 ms, hybrid by 10/20 ms, SILK by 10/20/40/60 ms.
+constexpr uint32_t kCostCelt[4] = {95, 115, 150, 215};
+constexpr uint32_t kCostHybrid[2] = {250, 330};
+constexpr uint32_t kCostSilk[4] = {120, 180, 340, 500};
+constexpr uint32_t kCostLost = 100;
+
+// Resampler launch: variant v picks outputs per thread (rs_variants table).
+int resample_variant()
+{
+    unsigned v = env_u("CUOPUS_RS", 0);
+    if (v) return (int)v - 1;
+    const int t = tuned_defaults().rs;
+    return t >= 0 && t < kRsVariants ? t : kRsDefaultVariant;
+}
+
+uint64_t rs_tiles(uint64_t out_len, int v) { return (out_len + rs_tile(v) - 1) / rs_tile(v); }
+
+void launch_resample(uint64_t tiles, hipStream_t st, const int16_t *in, const RsFile *files, int nfiles,
+                     const RsTaps &taps, int v)
+{
+    switch (v) {
+#define CUO_RS(V)                                                                                              \
+    case V:                                                                                                    \
+        hipLaunchKernelGGL((k_resample_48to16<kRsShape[V][0], kRsShape[V][1]>), dim3((unsigned)tiles),          \
+                           dim3(kRsShape[V][0]), 0, st, in, files, nfiles, taps);                              \
+        break;
+        CUO_RS(1)
+        CUO_RS(2)
+        CUO_RS(3)
+        CUO_RS(4)
+        CUO_RS(0)
+#undef CUO_RS
+    default: break;
+    }
+}
+
 // ---------------------------------------------------------------- planning
 struct FilePlan {
     Demuxed d;
@@ -151,16 +244,30 @@ struct FilePlan {
     int64_t out16 = 0;
 };
 
+// The decoder reads packet bytes as aligned 32-bit words and never depends
+// on a byte past a packet; the packet buffer still ends in this many zero
+// bytes so the last word read is inside it and holds nothing stale.
+constexpr size_t kPktPad = 4;
+
 struct Plan {
     std::vector<FilePlan> files;
-    std::vector<uint8_t> data;
+    std::vector<uint8_t> data;   // every file's packets back to back + kPktPad zero bytes
     std::vector<uint32_t> off, len;
     std::vector<uint8_t> lost;
     std::vector<ChainDesc> chains;
     std::vector<uint64_t> cap;   // per chain: sum of TOC durations
+    std::vector<uint32_t> cost;  // per chain: estimated decode cost (work order)
     uint64_t total_keep = 0;  // playable samples of all files
     uint64_t cat_len = 0;     // length of the compacted PCM buffer (file starts padded to 8 samples)
 };
+
+// OpusHead output gain (RFC 7845 5.1, Q7.8 dB) is applied unless
+// CUOPUS_OUTPUT_GAIN=0 (the reference's decode path ignores it).
+bool apply_output_gain()
+{
+    const char *v = getenv("CUOPUS_OUTPUT_GAIN");
+    return !(v && v[0] == '0');
+}
 
 int demux_file(const uint8_t *ogg, size_t n, FilePlan *fp, bool with_chains, Err &e)
 {
@@ -180,6 +287,29 @@ int demux_file(const uint8_t *ogg, size_t n, FilePlan *fp, bool with_chains, Err
     return 0;
 }
 
+// Decode cost of one packet in units of about 0.1 us of one wavefront,
+// from its TOC: per-frame costs by mode and frame duration, measured on
+// MI355X with homogeneous streams (CELT frames cost far more per second of
+// audio at 2.5 ms than at 20 ms; SILK and hybrid frames cost more than CELT
+// frames of the same length).
+uint32_t packet_cost(const uint8_t *p, uint32_t len)
+{
+    if (len == 0) return kCostLost;
+    const uint8_t toc = p[0];
+    int frames = 1;
+    if ((toc & 3) == 1 || (toc & 3) == 2) frames = 2;
+    else if ((toc & 3) == 3) frames = len > 1 ? (p[1] & 0x3F) : 1;
+    uint32_t per;
+    if (toc & 0x80) {
+        per = kCostCelt[(toc >> 3) & 3];
+    } else if ((toc & 0x60) == 0x60) {
+        per = kCostHybrid[(toc >> 3) & 1];
+    } else {
+        per = kCostSilk[(toc >> 3) & 3];
+    }
+    return per * (uint32_t)(frames > 0 ? frames : 1);
+}
+
 // Merge the demuxed files into one packet table and one chain table.
 int merge(Plan &pl, Err &e)
 {
@@ -189,15 +319,17 @@ int merge(Plan &pl, Err &e)
         npk += f.d.off.size();
         nch += f.chains.size();
     }
-    if (nbytes >= (1ull << 32)) return e.set(CUOPUS_RC_ARG, "merged packet data exceeds 4 GiB; split the call");
-    pl.data.resize(nbytes ? nbytes : 1);
+    if (nbytes + kPktPad >= (1ull << 32)) return e.set(CUOPUS_RC_ARG, "merged packet data exceeds 4 GiB; split the call");
+    pl.data.assign(nbytes + kPktPad, 0);
     pl.off.resize(npk);
     pl.len.resize(npk);
     pl.chains.resize(nch);
     pl.cap.resize(nch);
+    pl.cost.resize(nch);
+    const bool gain_on = apply_output_gain();
     uint64_t db = 0, pb = 0, cb = 0, cat = 0, keep_sum = 0;
     for (auto &f : pl.files) {
-        memcpy(pl.data.data() + db, f.d.data.data(), f.d.data.size());
+        if (!f.d.data.empty()) memcpy(pl.data.data() + db, f.d.data.data(), f.d.data.size());
         for (size_t k = 0; k < f.d.off.size(); k++) {
             pl.off[pb + k] = f.d.off[k] + (uint32_t)db;
             pl.len[pb + k] = f.d.len[k];
@@ -209,14 +341,20 @@ int merge(Plan &pl, Err &e)
             const Chain &c = f.chains[i];
             if ((uint64_t)c.first + c.npkts > f.d.off.size())
                 return e.set(CUOPUS_RC_SOFT, "chain %zu out of packet range", i);
-            uint64_t s = 0;
+            uint64_t s = 0, cost = 0;
             for (uint32_t k = 0; k < c.npkts; k++) {
                 uint32_t p = c.first + k;
-                s += (uint64_t)cuo_host::pkt_samples48(f.d.data.data() + f.d.off[p], f.d.len[p]);
+                const uint8_t *pk = f.d.data.data() + f.d.off[p];
+                s += (uint64_t)cuo_host::pkt_samples48(pk, f.d.len[p]);
+                cost += packet_cost(pk, f.d.len[p]);
             }
             if (s < (uint64_t)c.trim + c.keep)
                 return e.set(CUOPUS_RC_SOFT, "chain %zu capacity %llu < trim %u + keep %u", i,
                              (unsigned long long)s, c.trim, c.keep);
+            // chains never cross a link: the link holding the first packet gives the gain
+            int gain = 0;
+            for (const cuo_host::Link &lk : f.d.links)
+                if (c.first >= lk.first && c.first < lk.first + lk.npkts) gain = lk.gain_q8;
             ChainDesc &cd = pl.chains[cb + i];
             cd.first = c.first + (uint32_t)pb;
             cd.npkts = c.npkts;
@@ -224,8 +362,9 @@ int merge(Plan &pl, Err &e)
             cd.keep = c.keep;
             cd.dst = cat + fk;
             cd.rec_end = 0;  // no per-packet records outside the gate
-            cd.pad = 0;
+            cd.gain = gain_on ? gain : 0;
             pl.cap[cb + i] = s;
+            pl.cost[cb + i] = (uint32_t)std::min<uint64_t>(cost, 0xFFFFFFFFu);
             fk += c.keep;
         }
         if (fk != f.keep) return e.set(CUOPUS_RC_SOFT, "internal: keep mismatch");
@@ -251,16 +390,75 @@ int refuse_capture(hipStream_t st, Err &e)
     return 0;
 }
 
-// Longest-first work order by estimated cost (packets x frame cost).
-void lpt_order(const Plan &pl, std::vector<uint32_t> *order)
+// Work order: most expensive chains first (longest processing time first)
+// by the per-packet cost model, so the persistent kernel's last chains are
+// short ones and its tail stays short.  cost_model 0 keeps file order.
+void work_order(const Plan &pl, int cost_model, std::vector<uint32_t> *order)
 {
     const size_t n = pl.chains.size();
     order->resize(n);
-    std::vector<std::pair<uint64_t, uint32_t>> key(n);
-    for (size_t i = 0; i < n; i++) key[i] = {pl.cap[i] + 960ull * pl.chains[i].npkts, (uint32_t)i};
-    std::stable_sort(key.begin(), key.end(), [](const std::pair<uint64_t, uint32_t> &a,
-                                                const std::pair<uint64_t, uint32_t> &b) { return a.first > b.first; });
-    for (size_t i = 0; i < n; i++) (*order)[i] = key[i].second;
+    for (size_t i = 0; i < n; i++) (*order)[i] = (uint32_t)i;
+    if (cost_model == 0) return;
+    std::vector<uint64_t> key(n);
+    for (size_t i = 0; i < n; i++)
+        key[i] = cost_model == 2 ? pl.cap[i] + 960ull * pl.chains[i].npkts : pl.cost[i];
+    std::stable_sort(order->begin(), order->end(), [&](uint32_t a, uint32_t b) { return key[a] > key[b]; });
+}
+
+// ---------------------------------------------------------------- device buffers of one call
+struct CallBufs {
+    uint8_t *data = nullptr;
+    uint32_t *off = nullptr, *len = nullptr;
+    uint8_t *lost = nullptr;
+    ChainDesc *chains = nullptr;
+    uint32_t *order = nullptr;
+    ChainResult *res = nullptr;
+    int16_t *cat = nullptr;
+    RsFile *files = nullptr;
+    uint32_t *queue = nullptr;
+    DecHeap *heap = nullptr;
+};
+
+struct CallNeed {
+    size_t data = 0, npk = 0, lost = 0, nch = 0, cat = 0, nfiles = 0, nheaps = 0;
+};
+
+int bufs_from_workspace(const CallNeed &n, CallBufs *b, Err &e)
+{
+    int rc;
+    if ((rc = grow(W.data, n.data, e)) || (rc = grow(W.off, 4 * n.npk, e)) || (rc = grow(W.len, 4 * n.npk, e)) ||
+        (rc = grow(W.chains, sizeof(ChainDesc) * n.nch, e)) || (rc = grow(W.order, 4 * n.nch, e)) ||
+        (rc = grow(W.res, sizeof(ChainResult) * n.nch, e)) || (rc = grow(W.queue, 256, e)) ||
+        (rc = grow(W.heap, sizeof(DecHeap) * n.nheaps, e)))
+        return rc;
+    if (n.lost && (rc = grow(W.lost, n.lost, e))) return rc;
+    if (n.cat && (rc = grow(W.cat, 2 * n.cat, e))) return rc;
+    if (n.nfiles && (rc = grow(W.files, sizeof(RsFile) * n.nfiles, e))) return rc;
+    b->data = (uint8_t *)W.data.p;
+    b->off = (uint32_t *)W.off.p;
+    b->len = (uint32_t *)W.len.p;
+    b->lost = n.lost ? (uint8_t *)W.lost.p : nullptr;
+    b->chains = (ChainDesc *)W.chains.p;
+    b->order = (uint32_t *)W.order.p;
+    b->res = (ChainResult *)W.res.p;
+    b->cat = n.cat ? (int16_t *)W.cat.p : nullptr;
+    b->files = n.nfiles ? (RsFile *)W.files.p : nullptr;
+    b->queue = (uint32_t *)W.queue.p;
+    b->heap = (DecHeap *)W.heap.p;
+    return 0;
+}
+
+constexpr int kMaxDevices = 64;
+
+// Resident wavefronts of the persistent decoder when the caller does not
+// choose: CUOPUS_SLOTS, else the tuned value for this device, else every
+// resident slot of the chip.
+uint32_t default_slots()
+{
+    unsigned s = env_u("CUOPUS_SLOTS", 0);
+    if (s) return s;
+    const TunedDefaults &t = tuned_defaults();
+    return t.slots ? t.slots : kDefaultWaves;
 }
 
 struct RunOpts {
@@ -268,8 +466,18 @@ struct RunOpts {
     std::vector<float *> f16_out;        // per file resample targets (empty: no resample)
     int32_t *pkt_nsamp = nullptr;        // device, per packet (gate)
     uint32_t *pkt_rng = nullptr;
-    bool persistent = false;
+    bool persistent = true;              // persistent work-queue kernel (else one workgroup per chain)
+    uint32_t n_slots = 0;                // persistent wavefronts (0: default_slots())
 };
+
+// Kernel selection of the drop-in entry points: the persistent work-queue
+// kernel with min(chains, slots) wavefronts (its heap memory is O(slots));
+// CUOPUS_KERNEL=plain forces one workgroup per chain.
+bool use_persistent()
+{
+    const char *v = getenv("CUOPUS_KERNEL");
+    return !(v && strcmp(v, "plain") == 0);
+}
 
 // Upload, decode, resample, read back and validate.  results receives the
 // per-chain results (host).
@@ -277,63 +485,10 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
 {
     const uint32_t nch = (uint32_t)pl.chains.size();
     const uint64_t npk = pl.off.size();
-    int rc;
-    // +4: the decoder reads packet bytes as aligned 32-bit words
-    if ((rc = grow(W.data, pl.data.size() + 4, e))) return rc;
-    if ((rc = grow(W.off, 4 * npk, e))) return rc;
-    if ((rc = grow(W.len, 4 * npk, e))) return rc;
-    if ((rc = grow(W.chains, sizeof(ChainDesc) * nch, e))) return rc;
-    if ((rc = grow(W.order, 4ull * nch, e))) return rc;
-    if ((rc = grow(W.res, sizeof(ChainResult) * nch, e))) return rc;
-    if (!pl.lost.empty() && (rc = grow(W.lost, pl.lost.size(), e))) return rc;
-    int16_t *out = o.s16_out;
-    if (!out) {
-        if ((rc = grow(W.cat, 2 * (pl.cat_len ? pl.cat_len : 1), e))) return rc;
-        out = (int16_t *)W.cat.p;
-    }
-    const uint32_t n_waves = std::max<uint32_t>(1, std::min<uint32_t>(env_u("CUOPUS_SLOTS", kDefaultWaves), nch));
-    if ((rc = grow(W.queue, 256, e))) return rc;
-    // one HBM heap per workgroup of the launch
-    if ((rc = grow(W.heap, sizeof(DecHeap) * (size_t)(o.persistent ? n_waves : nch), e))) return rc;
-
-    std::vector<uint32_t> order;
-    lpt_order(pl, &order);
-    HIPCHK(e, hipMemcpyAsync(W.data.p, pl.data.data(), pl.data.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(e, hipMemcpyAsync(W.off.p, pl.off.data(), 4 * npk, hipMemcpyHostToDevice, st));
-    HIPCHK(e, hipMemcpyAsync(W.len.p, pl.len.data(), 4 * npk, hipMemcpyHostToDevice, st));
-    HIPCHK(e, hipMemcpyAsync(W.chains.p, pl.chains.data(), sizeof(ChainDesc) * nch, hipMemcpyHostToDevice, st));
-    HIPCHK(e, hipMemcpyAsync(W.order.p, order.data(), 4ull * nch, hipMemcpyHostToDevice, st));
-    if (!pl.lost.empty())
-        HIPCHK(e, hipMemcpyAsync(W.lost.p, pl.lost.data(), pl.lost.size(), hipMemcpyHostToDevice, st));
-
-    DecodeArgs a;
-    a.data = (const uint8_t *)W.data.p;
-    a.off = (const uint32_t *)W.off.p;
-    a.len = (const uint32_t *)W.len.p;
-    a.lost = pl.lost.empty() ? nullptr : (const uint8_t *)W.lost.p;
-    a.chains = (const ChainDesc *)W.chains.p;
-    a.out = out;
-    a.res = (ChainResult *)W.res.p;
-    a.heap = (DecHeap *)W.heap.p;
-    a.pkt_nsamp = o.pkt_nsamp;
-    a.pkt_rng = o.pkt_rng;
-    a.queue = (uint32_t *)W.queue.p;
-    if (o.persistent) {
-        HIPCHK(e, hipMemsetAsync(W.queue.p, 0, 4, st));
-        a.order = (const uint32_t *)W.order.p;
-        a.n_chains = nch;
-        launch_persistent(n_waves, st, a);
-        HIPCHK(e, hipGetLastError());
-    } else {
-        a.order = (const uint32_t *)W.order.p;
-        a.n_chains = nch;
-        launch_chains(nch, st, a);
-        HIPCHK(e, hipGetLastError());
-    }
-    // one resampler launch over every file's tiles
+    std::vector<RsFile> rf;
+    uint64_t tiles = 0;
+    const int rsv = resample_variant();
     if (!o.f16_out.empty()) {
-        std::vector<RsFile> rf;
-        uint64_t tiles = 0;
         for (size_t f = 0; f < pl.files.size(); f++) {
             if (pl.files[f].out16 <= 0) continue;
             RsFile r;
@@ -342,20 +497,69 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
             r.out_len = (uint64_t)pl.files[f].out16;
             r.out = o.f16_out[f];
             r.tile0 = tiles;
-            tiles += (r.out_len + kRsTile - 1) / kRsTile;
+            tiles += rs_tiles(r.out_len, rsv);
             rf.push_back(r);
         }
-        if (!rf.empty()) {
-            if ((rc = grow(W.files, sizeof(RsFile) * rf.size(), e))) return rc;
-            HIPCHK(e, hipMemcpyAsync(W.files.p, rf.data(), sizeof(RsFile) * rf.size(), hipMemcpyHostToDevice, st));
-            static RsTaps taps = make_taps();
-            hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)tiles), dim3(kRsThreads), 0, st, out,
-                               (const RsFile *)W.files.p, (int)rf.size(), taps);
-            HIPCHK(e, hipGetLastError());
-        }
+    }
+    const uint32_t n_waves = std::max<uint32_t>(1, std::min<uint32_t>(o.n_slots ? o.n_slots : default_slots(), nch));
+    CallNeed need;
+    need.data = pl.data.size();
+    need.npk = npk;
+    need.lost = pl.lost.empty() ? 0 : pl.lost.size();
+    need.nch = nch;
+    need.cat = o.s16_out ? 0 : (pl.cat_len ? pl.cat_len : 1);
+    need.nfiles = rf.size();
+    need.nheaps = o.persistent ? n_waves : nch;
+    CallBufs b;
+    int rc = bufs_from_workspace(need, &b, e);
+    if (rc) return rc;
+    int16_t *out = o.s16_out ? o.s16_out : b.cat;
+
+    std::vector<uint32_t> order;
+    work_order(pl, 1, &order);
+    HIPCHK(e, hipMemcpyAsync(b.data, pl.data.data(), pl.data.size(), hipMemcpyHostToDevice, st));
+    if (npk) {
+        HIPCHK(e, hipMemcpyAsync(b.off, pl.off.data(), 4 * npk, hipMemcpyHostToDevice, st));
+        HIPCHK(e, hipMemcpyAsync(b.len, pl.len.data(), 4 * npk, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(e, hipMemcpyAsync(b.chains, pl.chains.data(), sizeof(ChainDesc) * nch, hipMemcpyHostToDevice, st));
+    HIPCHK(e, hipMemcpyAsync(b.order, order.data(), 4ull * nch, hipMemcpyHostToDevice, st));
+    if (b.lost) HIPCHK(e, hipMemcpyAsync(b.lost, pl.lost.data(), pl.lost.size(), hipMemcpyHostToDevice, st));
+    if (!rf.empty()) HIPCHK(e, hipMemcpyAsync(b.files, rf.data(), sizeof(RsFile) * rf.size(), hipMemcpyHostToDevice, st));
+
+    DecodeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.data = b.data;
+    a.off = b.off;
+    a.len = b.len;
+    a.lost = b.lost;
+    a.chains = b.chains;
+    a.order = b.order;
+    a.n_chains = nch;
+    a.out = out;
+    a.res = b.res;
+    a.heap = b.heap;
+    a.pkt_nsamp = o.pkt_nsamp;
+    a.pkt_rng = o.pkt_rng;
+    a.queue = b.queue;
+    if (o.persistent) {
+        // the queue counter is zeroed by a kernel (device buffers keep the last
+        // call's contents; no memset node if a caller captures the sequence)
+        hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b.queue);
+        HIPCHK(e, hipGetLastError());
+        launch_persistent(n_waves, st, a);
+    } else {
+        launch_chains(nch, st, a);
+    }
+    HIPCHK(e, hipGetLastError());
+    // one resampler launch over every file's tiles
+    if (!rf.empty()) {
+        static RsTaps taps = make_taps();
+        launch_resample(tiles, st, out, b.files, (int)rf.size(), taps, rsv);
+        HIPCHK(e, hipGetLastError());
     }
     results.resize(nch);
-    HIPCHK(e, hipMemcpyAsync(results.data(), W.res.p, sizeof(ChainResult) * nch, hipMemcpyDeviceToHost, st));
+    HIPCHK(e, hipMemcpyAsync(results.data(), b.res, sizeof(ChainResult) * nch, hipMemcpyDeviceToHost, st));
     HIPCHK(e, hipStreamSynchronize(st));
     for (uint32_t c = 0; c < nch; c++) {
         if (results[c].stop != 0) return e.set(CUOPUS_RC_SOFT, "chain %u: device decode error %d", c, results[c].stop);
@@ -366,18 +570,11 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     return 0;
 }
 
-int decode_files(const uint8_t *const *oggs, const size_t *ns, int nfiles, const uint8_t *lost, size_t mask_n,
-                 RunOpts &o, hipStream_t st, Plan &pl, Err &e)
+// Merge already-demuxed files (pl.files), attach the loss mask and run.
+int decode_plan(Plan &pl, const uint8_t *lost, size_t mask_n, RunOpts &o, hipStream_t st, Err &e)
 {
     int rc = refuse_capture(st, e);
     if (rc) return rc;
-    pl.files.resize((size_t)nfiles);
-    for (int f = 0; f < nfiles; f++) {
-        char sub[256];
-        Err fe(sub, sizeof(sub));
-        if ((rc = demux_file(oggs[f], ns[f], &pl.files[f], true, fe)))
-            return nfiles > 1 ? e.set(rc, "file %d: %s", f, sub) : e.set(rc, "%s", sub);
-    }
     if ((rc = merge(pl, e))) return rc;
     if (lost) {
         if (mask_n != pl.off.size())
@@ -394,6 +591,110 @@ int last_ovf = 0;
 }  // namespace
 
 // =====================================================================================
+// Demuxed files: demux once, size the output, then decode (the Python
+// binding's path, so a file is parsed once per call).
+// =====================================================================================
+struct cuopus_file {
+    FilePlan fp;
+};
+
+extern "C" int cuopus_file_open(const uint8_t *ogg, size_t n, cuopus_file **out, int64_t *info6, char *errbuf,
+                                size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!out) return e.set(CUOPUS_RC_ARG, "out is NULL");
+    *out = nullptr;
+    cuopus_file *f = new cuopus_file();
+    int rc = demux_file(ogg, n, &f->fp, true, e);
+    if (rc) { delete f; return rc; }
+    if (info6) {
+        info6[0] = f->fp.out16;
+        info6[1] = (int64_t)f->fp.keep;
+        info6[2] = f->fp.d.channels;
+        info6[3] = (int64_t)f->fp.d.off.size();
+        info6[4] = (int64_t)f->fp.chains.size();
+        info6[5] = f->fp.d.links.empty() ? 0 : f->fp.d.links[0].gain_q8;
+    }
+    *out = f;
+    return 0;
+}
+
+extern "C" void cuopus_file_close(cuopus_file *f) { delete f; }
+
+namespace {
+
+int files_f16k(cuopus_file *const *fs, int nfiles, const uint8_t *lost, size_t mask_n, float *const *out_devs,
+               const size_t *out_caps, int64_t *out_lens, void *stream, Err &e)
+{
+    last_ovf = 0;
+    if (out_lens)
+        for (int f = 0; f < nfiles; f++) out_lens[f] = 0;
+    if (nfiles <= 0) return 0;
+    if (!fs || !out_devs || !out_caps) return e.set(CUOPUS_RC_ARG, "NULL argument array");
+    Plan pl;
+    RunOpts o;
+    o.persistent = use_persistent();
+    pl.files.resize((size_t)nfiles);
+    for (int f = 0; f < nfiles; f++) {
+        if (!fs[f]) return e.set(CUOPUS_RC_ARG, "file %d is NULL", f);
+        const FilePlan &fp = fs[f]->fp;
+        if ((uint64_t)fp.out16 > out_caps[f])
+            return nfiles > 1 ? e.set(CUOPUS_RC_ARG, "file %d: output capacity %zu < required %lld", f, out_caps[f],
+                                      (long long)fp.out16)
+                              : e.set(CUOPUS_RC_ARG, "output capacity %zu < required %lld samples", out_caps[f],
+                                      (long long)fp.out16);
+        if (fp.out16 > 0 && !out_devs[f])
+            return nfiles > 1 ? e.set(CUOPUS_RC_ARG, "file %d: out_dev is NULL", f)
+                              : e.set(CUOPUS_RC_ARG, "out_dev is NULL");
+        pl.files[f] = fp;
+        o.f16_out.push_back(out_devs[f]);
+    }
+    int rc = decode_plan(pl, lost, mask_n, o, (hipStream_t)stream, e);
+    if (rc) return rc;
+    if (out_lens)
+        for (int f = 0; f < nfiles; f++) out_lens[f] = pl.files[f].out16;
+    return 0;
+}
+
+int files_f16k_from_ogg(const uint8_t *const *oggs, const size_t *ns, int nfiles, const uint8_t *lost,
+                        size_t mask_n, float *const *out_devs, const size_t *out_caps, int64_t *out_lens,
+                        void *stream, Err &e)
+{
+    if (out_lens)
+        for (int f = 0; f < nfiles; f++) out_lens[f] = 0;
+    if (nfiles <= 0) return 0;
+    if (!oggs || !ns) return e.set(CUOPUS_RC_ARG, "NULL argument array");
+    std::vector<cuopus_file> files((size_t)nfiles);
+    std::vector<cuopus_file *> fs((size_t)nfiles);
+    for (int f = 0; f < nfiles; f++) {
+        char sub[256];
+        Err fe(sub, sizeof(sub));
+        int rc = demux_file(oggs[f], ns[f], &files[f].fp, true, fe);
+        if (rc) return nfiles > 1 ? e.set(rc, "file %d: %s", f, sub) : e.set(rc, "%s", sub);
+        fs[f] = &files[f];
+    }
+    return files_f16k(fs.data(), nfiles, lost, mask_n, out_devs, out_caps, out_lens, stream, e);
+}
+
+}  // namespace
+
+extern "C" int cuopus_file_decode_f16k_on(cuopus_file *f, float *out_dev, size_t out_cap, int64_t *out_len,
+                                          void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (!f) return e.set(CUOPUS_RC_ARG, "file is NULL");
+    return files_f16k(&f, 1, nullptr, 0, &out_dev, &out_cap, out_len, stream, e);
+}
+
+extern "C" int cuopus_files_decode_f16k_multi(cuopus_file *const *fs, int nfiles, float *const *out_devs,
+                                              const size_t *out_caps, int64_t *out_lens, void *stream,
+                                              char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    return files_f16k(fs, nfiles, nullptr, 0, out_devs, out_caps, out_lens, stream, e);
+}
+
+// =====================================================================================
 extern "C" int cuopus_last_arena_overflow(void) { return last_ovf; }
 
 extern "C" long long cuopus_workspace_bytes(void) { return g_dev_bytes; }
@@ -403,11 +704,13 @@ extern "C" void cuopus_release_workspaces(void)
     DevBuf *all[] = {&W.data, &W.off, &W.len, &W.lost, &W.chains, &W.order, &W.heap,
                      &W.cat, &W.res, &W.nsamp, &W.rng, &W.files, &W.queue};
     for (DevBuf *b : all) {
-        if (b->p) hipFree(b->p);
+        if (b->p) {
+            hipFree(b->p);
+            g_dev_bytes -= (long long)b->cap;
+        }
         b->p = nullptr;
         b->cap = 0;
     }
-    g_dev_bytes = 0;
 }
 
 extern "C" int cuopus_probe(const uint8_t *ogg, size_t n, int64_t *samples16k, char *errbuf, size_t errcap)
@@ -497,38 +800,11 @@ extern "C" int cuopus_demux_stats(const uint8_t *ogg, size_t n, int64_t *s10, ch
     return 0;
 }
 
-static int f16k_common(const uint8_t *ogg, size_t n, const uint8_t *lost, size_t mask_n, float *out_dev,
-                       size_t out_cap, int64_t *out_len, void *stream, char *errbuf, size_t errcap)
-{
-    Err e(errbuf, errcap);
-    last_ovf = 0;
-    if (out_len) *out_len = 0;
-    hipStream_t st = (hipStream_t)stream;
-    // size check before any GPU work
-    FilePlan probe;
-    int rc = demux_file(ogg, n, &probe, false, e);
-    if (rc) return rc;
-    if ((uint64_t)probe.out16 > out_cap)
-        return e.set(CUOPUS_RC_ARG, "output capacity %zu < required %lld samples", out_cap, (long long)probe.out16);
-    if (probe.out16 > 0 && !out_dev) return e.set(CUOPUS_RC_ARG, "out_dev is NULL");
-    if (probe.out16 == 0) {
-        if ((rc = refuse_capture(st, e))) return rc;
-        return 0;
-    }
-    Plan pl;
-    RunOpts o;
-    o.f16_out.push_back(out_dev);
-    o.persistent = env_u("CUOPUS_PERSISTENT", 0) != 0;
-    rc = decode_files(&ogg, &n, 1, lost, mask_n, o, st, pl, e);
-    if (rc) return rc;
-    if (out_len) *out_len = probe.out16;
-    return 0;
-}
-
 extern "C" int cuopus_decode_f16k_on(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap, int64_t *out_len,
                                      void *stream, char *errbuf, size_t errcap)
 {
-    return f16k_common(ogg, n, nullptr, 0, out_dev, out_cap, out_len, stream, errbuf, errcap);
+    Err e(errbuf, errcap);
+    return files_f16k_from_ogg(&ogg, &n, 1, nullptr, 0, &out_dev, &out_cap, out_len, stream, e);
 }
 
 extern "C" int cuopus_decode_f16k(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap, int64_t *out_len,
@@ -541,11 +817,9 @@ extern "C" int cuopus_decode_lossy_f16k_on(const uint8_t *ogg, size_t n, const u
                                            float *out_dev, size_t out_cap, int64_t *out_len, void *stream,
                                            char *errbuf, size_t errcap)
 {
-    if (!lost) {
-        Err e(errbuf, errcap);
-        return e.set(CUOPUS_RC_ARG, "lost mask is NULL");
-    }
-    return f16k_common(ogg, n, lost, mask_n, out_dev, out_cap, out_len, stream, errbuf, errcap);
+    Err e(errbuf, errcap);
+    if (!lost) return e.set(CUOPUS_RC_ARG, "lost mask is NULL");
+    return files_f16k_from_ogg(&ogg, &n, 1, lost, mask_n, &out_dev, &out_cap, out_len, stream, e);
 }
 
 extern "C" int cuopus_decode_s16_on(const uint8_t *ogg, size_t n, int16_t *out_dev, size_t out_cap, int64_t *out_len,
@@ -554,21 +828,20 @@ extern "C" int cuopus_decode_s16_on(const uint8_t *ogg, size_t n, int16_t *out_d
     Err e(errbuf, errcap);
     last_ovf = 0;
     if (out_len) *out_len = 0;
-    hipStream_t st = (hipStream_t)stream;
-    FilePlan probe;
-    int rc = demux_file(ogg, n, &probe, false, e);
-    if (rc) return rc;
-    if (probe.keep > out_cap)
-        return e.set(CUOPUS_RC_ARG, "output capacity %zu < required %llu samples", out_cap,
-                     (unsigned long long)probe.keep);
-    if (probe.keep > 0 && !out_dev) return e.set(CUOPUS_RC_ARG, "out_dev is NULL");
     Plan pl;
+    pl.files.resize(1);
+    int rc = demux_file(ogg, n, &pl.files[0], true, e);
+    if (rc) return rc;
+    const uint64_t keep = pl.files[0].keep;
+    if (keep > out_cap)
+        return e.set(CUOPUS_RC_ARG, "output capacity %zu < required %llu samples", out_cap, (unsigned long long)keep);
+    if (keep > 0 && !out_dev) return e.set(CUOPUS_RC_ARG, "out_dev is NULL");
     RunOpts o;
     o.s16_out = out_dev;
-    o.persistent = env_u("CUOPUS_PERSISTENT", 0) != 0;
-    rc = decode_files(&ogg, &n, 1, nullptr, 0, o, st, pl, e);
+    o.persistent = use_persistent();
+    rc = decode_plan(pl, nullptr, 0, o, (hipStream_t)stream, e);
     if (rc) return rc;
-    if (out_len) *out_len = (int64_t)probe.keep;
+    if (out_len) *out_len = (int64_t)keep;
     return 0;
 }
 
@@ -577,35 +850,7 @@ extern "C" int cuopus_decode_f16k_multi(const uint8_t *const *oggs, const size_t
                                         void *stream, char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);
-    last_ovf = 0;
-    if (out_lens)
-        for (int f = 0; f < nfiles; f++) out_lens[f] = 0;
-    if (nfiles <= 0) return 0;
-    if (!oggs || !ns || !out_devs || !out_caps) return e.set(CUOPUS_RC_ARG, "NULL argument array");
-    hipStream_t st = (hipStream_t)stream;
-    Plan pl;
-    RunOpts o;
-    o.persistent = env_u("CUOPUS_PERSISTENT", 0) != 0;
-    // validate capacities first (host only)
-    std::vector<int64_t> need((size_t)nfiles);
-    for (int f = 0; f < nfiles; f++) {
-        FilePlan fp;
-        char sub[256];
-        Err fe(sub, sizeof(sub));
-        int rc = demux_file(oggs[f], ns[f], &fp, false, fe);
-        if (rc) return e.set(rc, "file %d: %s", f, sub);
-        need[f] = fp.out16;
-        if ((uint64_t)fp.out16 > out_caps[f])
-            return e.set(CUOPUS_RC_ARG, "file %d: output capacity %zu < required %lld", f, out_caps[f],
-                         (long long)fp.out16);
-        if (fp.out16 > 0 && !out_devs[f]) return e.set(CUOPUS_RC_ARG, "file %d: out_dev is NULL", f);
-        o.f16_out.push_back(out_devs[f]);
-    }
-    int rc = decode_files(oggs, ns, nfiles, nullptr, 0, o, st, pl, e);
-    if (rc) return rc;
-    if (out_lens)
-        for (int f = 0; f < nfiles; f++) out_lens[f] = need[f];
-    return 0;
+    return files_f16k_from_ogg(oggs, ns, nfiles, nullptr, 0, out_devs, out_caps, out_lens, stream, e);
 }
 
 extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, const uint32_t *len,
@@ -621,10 +866,10 @@ extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint
     if (!data || !off || !len || !first || !npkts || !trim || !keep || !pcm_out || !pkt_nsamp || !pkt_rng ||
         !chain_stop)
         return e.set(CUOPUS_RC_ARG, "NULL argument");
-    if (nbytes >= (1ull << 32)) return e.set(CUOPUS_RC_ARG, "packet data exceeds 4 GiB");
+    if (nbytes + kPktPad >= (1ull << 32)) return e.set(CUOPUS_RC_ARG, "packet data exceeds 4 GiB");
     Plan pl;
-    pl.data.assign(data, data + (nbytes ? nbytes : 0));
-    if (pl.data.empty()) pl.data.push_back(0);
+    pl.data.assign(nbytes + kPktPad, 0);
+    if (nbytes) memcpy(pl.data.data(), data, nbytes);
     pl.off.assign(off, off + n_packets);
     pl.len.assign(len, len + n_packets);
     for (uint32_t p = 0; p < n_packets; p++)
@@ -632,14 +877,26 @@ extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint
     if (lost) pl.lost.assign(lost, lost + n_packets);
     pl.chains.resize(n_chains);
     pl.cap.resize(n_chains);
+    pl.cost.resize(n_chains);
     uint64_t cat = 0;
     for (uint32_t c = 0; c < n_chains; c++) {
         if ((uint64_t)first[c] + npkts[c] > n_packets) return e.set(CUOPUS_RC_ARG, "chain %u out of range", c);
-        pl.chains[c] = {first[c], npkts[c], trim[c], keep[c], cat, c + 1 < n_chains ? first[c + 1] : 0xFFFFFFFFu, 0};
-        uint64_t s = 0;
-        for (uint32_t k = 0; k < npkts[c]; k++)
-            s += (uint64_t)cuo_host::pkt_samples48(data + off[first[c] + k], len[first[c] + k]);
+        ChainDesc &cd = pl.chains[c];
+        cd.first = first[c];
+        cd.npkts = npkts[c];
+        cd.trim = trim[c];
+        cd.keep = keep[c];
+        cd.dst = cat;
+        cd.rec_end = c + 1 < n_chains ? first[c + 1] : 0xFFFFFFFFu;
+        cd.gain = 0;
+        uint64_t s = 0, cost = 0;
+        for (uint32_t k = 0; k < npkts[c]; k++) {
+            const uint32_t p = first[c] + k;
+            s += (uint64_t)cuo_host::pkt_samples48(data + off[p], len[p]);
+            cost += packet_cost(data + off[p], len[p]);
+        }
         pl.cap[c] = s;
+        pl.cost[c] = (uint32_t)std::min<uint64_t>(cost, 0xFFFFFFFFu);
         cat += keep[c];
     }
     pl.total_keep = cat;
@@ -652,7 +909,9 @@ extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint
     RunOpts o;
     o.pkt_nsamp = (int32_t *)W.nsamp.p;
     o.pkt_rng = (uint32_t *)W.rng.p;
-    o.persistent = env_u("CUOPUS_PERSISTENT", 0) != 0;
+    o.persistent = use_persistent();
+    const unsigned slots = env_u("CUOPUS_GATE_SLOTS", 0);
+    if (slots) o.n_slots = slots;
     std::vector<ChainResult> res;
     rc = run_plan(pl, st, o, res, e);
     // results are copied back even when a chain failed validation
@@ -686,14 +945,128 @@ extern "C" int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, f
     if ((rc = grow(W.files, sizeof(RsFile), e))) return rc;
     HIPCHK(e, hipMemcpyAsync(W.files.p, &r, sizeof(RsFile), hipMemcpyHostToDevice, st));
     static RsTaps taps = make_taps();
-    unsigned tiles = (unsigned)((n16 + kRsTile - 1) / kRsTile);
-    hipLaunchKernelGGL(k_resample_48to16, dim3(tiles), dim3(kRsThreads), 0, st, in_dev, (const RsFile *)W.files.p, 1, taps);
+    const int rsv = resample_variant();
+    launch_resample(rs_tiles((uint64_t)n16, rsv), st, in_dev, (const RsFile *)W.files.p, 1, taps, rsv);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipStreamSynchronize(st));
     if (out_len) *out_len = n16;
     return 0;
 }
 
+
+// =====================================================================================
+// Log-mel front-end over a resident f32 16 kHz buffer
+// =====================================================================================
+namespace {
+
+// Slaney mel scale (linear below 1 kHz, logarithmic above; 15 mel at 1 kHz).
+double hz_to_mel(double f)
+{
+    const double f_sp = 200.0 / 3.0, min_log_hz = 1000.0, min_log_mel = min_log_hz / f_sp;
+    const double logstep = log(6.4) / 27.0;
+    return f < min_log_hz ? f / f_sp : min_log_mel + log(f / min_log_hz) / logstep;
+}
+double mel_to_hz(double m)
+{
+    const double f_sp = 200.0 / 3.0, min_log_hz = 1000.0, min_log_mel = min_log_hz / f_sp;
+    const double logstep = log(6.4) / 27.0;
+    return m < min_log_mel ? f_sp * m : min_log_hz * exp(logstep * (m - min_log_mel));
+}
+
+// Window, twiddles and the n_mels triangular filters (0-8 kHz at 16 kHz,
+// Slaney area normalisation 2 / (f_hi - f_lo)), in double, stored as float.
+bool mel_tables(int n_mels, MelTables *t)
+{
+    memset(t, 0, sizeof(*t));
+    for (int j = 0; j < kMelWin; j++) t->window[j] = (float)(0.5 - 0.5 * cos(2.0 * M_PI * j / kMelWin));
+    for (int e = 0; e < 256; e++) {
+        t->tw256[2 * e] = (float)cos(2.0 * M_PI * e / 256.0);
+        t->tw256[2 * e + 1] = (float)-sin(2.0 * M_PI * e / 256.0);
+    }
+    for (int k = 0; k < kMelBins; k++) {
+        t->tw512[2 * k] = (float)cos(2.0 * M_PI * k / 512.0);
+        t->tw512[2 * k + 1] = (float)-sin(2.0 * M_PI * k / 512.0);
+    }
+    const double sr = 16000.0, fmax = sr / 2;
+    std::vector<double> edge((size_t)n_mels + 2);
+    const double m0 = hz_to_mel(0.0), m1 = hz_to_mel(fmax);
+    for (int i = 0; i < n_mels + 2; i++) edge[i] = mel_to_hz(m0 + (m1 - m0) * i / (n_mels + 1));
+    int nw = 0;
+    for (int m = 0; m < n_mels; m++) {
+        const double lo = edge[m], mid = edge[m + 1], hi = edge[m + 2];
+        const double enorm = 2.0 / (hi - lo);
+        int first = -1, count = 0;
+        t->woff[m] = nw;
+        for (int k = 0; k < kMelBins; k++) {
+            const double f = k * sr / kMelFft;
+            const double w = std::max(0.0, std::min((f - lo) / (mid - lo), (hi - f) / (hi - mid)));
+            if (w <= 0.0) {
+                if (first >= 0) break;
+                continue;
+            }
+            if (first < 0) first = k;
+            if (nw >= kMelMaxWeights) return false;
+            t->weight[nw++] = (float)(w * enorm);
+            count++;
+        }
+        t->first[m] = first < 0 ? 0 : first;
+        t->count[m] = count;
+    }
+    return true;
+}
+
+struct MelCache {
+    DevBuf tab;
+    int n_mels = 0;
+};
+MelCache g_mel[kMaxDevices];
+
+}  // namespace
+
+extern "C" int cuopus_mel_filters(int n_mels, float *out, size_t cap)
+{
+    if (n_mels < 1 || n_mels > kMelMaxMels || !out || cap < (size_t)n_mels * kMelBins) return CUOPUS_RC_ARG;
+    MelTables t;
+    if (!mel_tables(n_mels, &t)) return CUOPUS_RC_ARG;
+    for (size_t i = 0; i < (size_t)n_mels * kMelBins; i++) out[i] = 0.f;
+    for (int m = 0; m < n_mels; m++)
+        for (int i = 0; i < t.count[m]; i++) out[(size_t)m * kMelBins + t.first[m] + i] = t.weight[t.woff[m] + i];
+    return 0;
+}
+
+extern "C" int cuopus_log_mel(const float *in_dev, int64_t n16, float *out_dev, size_t out_cap, int64_t *frames,
+                              int n_mels, void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    if (frames) *frames = 0;
+    if (n_mels < 1 || n_mels > kMelMaxMels) return e.set(CUOPUS_RC_ARG, "n_mels %d outside 1..%d", n_mels, kMelMaxMels);
+    if (n16 < 0) return e.set(CUOPUS_RC_ARG, "negative input length");
+    const int64_t nf = n16 >= kMelWin ? 1 + (n16 - kMelWin) / kMelHop : 0;
+    if ((uint64_t)(nf * n_mels) > out_cap)
+        return e.set(CUOPUS_RC_ARG, "output capacity %zu < %lld frames x %d mels", out_cap, (long long)nf, n_mels);
+    if (nf == 0) return 0;
+    if (!in_dev || !out_dev) return e.set(CUOPUS_RC_ARG, "NULL pointer");
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0;
+    HIPCHK(e, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return e.set(CUOPUS_RC_ARG, "device %d out of range", dev);
+    MelCache &mc = g_mel[dev];
+    if (mc.n_mels != n_mels) {
+        MelTables t;
+        if (!mel_tables(n_mels, &t)) return e.set(CUOPUS_RC_ARG, "mel filter table overflow");
+        int rc = grow(mc.tab, sizeof(MelTables), e);
+        if (rc) return rc;
+        HIPCHK(e, hipMemcpy(mc.tab.p, &t, sizeof(MelTables), hipMemcpyHostToDevice));
+        mc.n_mels = n_mels;
+    }
+    const uint64_t blocks = ((uint64_t)nf + kMelWaves - 1) / kMelWaves;
+    hipLaunchKernelGGL(k_log_mel, dim3((unsigned)blocks), dim3(64 * kMelWaves), 0, st, in_dev, nf,
+                       (const MelTables *)mc.tab.p, n_mels, out_dev);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipStreamSynchronize(st));
+    if (frames) *frames = nf;
+    return 0;
+}
 
 // =====================================================================================
 // Resident batches
@@ -715,6 +1088,7 @@ struct cuopus_batch {
     hipGraph_t graph = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ran = false;
+    int rs_variant = 0;
 };
 
 namespace {
@@ -766,8 +1140,8 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
             }
             const int r = b->file_rs[f];
             if (r >= 0) {
-                hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)b->file_tiles[r]), dim3(kRsThreads), 0, st,
-                                   (const int16_t *)b->cat.p, (const RsFile *)b->files.p + r, 1, b->taps);
+                launch_resample(b->file_tiles[r], st, (const int16_t *)b->cat.p, (const RsFile *)b->files.p + r, 1,
+                                b->taps, b->rs_variant);
                 HIPCHK(e, hipGetLastError());
             }
         }
@@ -794,8 +1168,8 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
     if (prof) HIPCHK(e, hipEventRecord(b->ev[1], st));
     if (b->tiles) {
         if (prof) HIPCHK(e, hipEventRecord(b->ev[2], st));
-        hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)b->tiles), dim3(kRsThreads), 0, st,
-                           (const int16_t *)b->cat.p, (const RsFile *)b->files.p, (int)b->rsf.size(), b->taps);
+        launch_resample(b->tiles, st, (const int16_t *)b->cat.p, (const RsFile *)b->files.p, (int)b->rsf.size(),
+                        b->taps, b->rs_variant);
         HIPCHK(e, hipGetLastError());
         if (prof) HIPCHK(e, hipEventRecord(b->ev[3], st));
     } else if (prof) {
@@ -856,16 +1230,11 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
         b->file_chain0[f] = (uint32_t)pl.files[f].chain0;
         b->file_nchain[f] = (uint32_t)pl.files[f].chains.size();
     }
-    if (b->opts.lpt_order && b->opts.mode != 0) {
-        lpt_order(pl, &order);
-    } else {
-        order.resize(pl.chains.size());
-        for (size_t i = 0; i < order.size(); i++) order[i] = (uint32_t)i;
-    }
+    work_order(pl, b->opts.mode != 0 ? b->opts.lpt_order : 0, &order);
     const uint32_t nch = (uint32_t)pl.chains.size();
     // persistent grid: n_slots wavefronts (default: the chip's resident
     // capacity at the kernel's LDS footprint), never more than the chains
-    b->n_slots = b->opts.n_slots > 0 ? (uint32_t)b->opts.n_slots : kDefaultWaves;
+    b->n_slots = b->opts.n_slots > 0 ? (uint32_t)b->opts.n_slots : default_slots();
     if (b->n_slots > nch) b->n_slots = nch;
     if (b->opts.mode == 2) {
         b->n_heaps = b->n_slots;
@@ -876,6 +1245,8 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
         for (int f = 0; f < nfiles; f++) b->n_heaps = std::max<uint64_t>(b->n_heaps, b->file_nchain[f]);
     }
     b->taps = make_taps();
+    b->rs_variant = b->opts.rs_variant > 0 && b->opts.rs_variant <= kRsVariants ? b->opts.rs_variant - 1
+                                                                               : resample_variant();
     b->file_rs.assign(pl.files.size(), -1);
     for (size_t f = 0; f < pl.files.size(); f++) {
         if (!out_devs || pl.files[f].out16 <= 0) continue;
@@ -886,7 +1257,7 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
         r.out_len = (uint64_t)pl.files[f].out16;
         r.out = out_devs[f];
         r.tile0 = b->tiles;
-        uint64_t t = (r.out_len + kRsTile - 1) / kRsTile;
+        uint64_t t = rs_tiles(r.out_len, b->rs_variant);
         b->file_tiles.push_back(t);
         b->tiles += t;
         b->rsf.push_back(r);
@@ -1026,29 +1397,56 @@ extern "C" void cuopus_batch_destroy(cuopus_batch *b)
 // =====================================================================================
 // Requests are demuxed on submit and queue in arrival order.  poll() retires
 // finished launches (event query, no blocking), then admits queued requests
-// into a new launch on a free stream while the bytes they need on the device
-// (packets, chain table, int16 PCM, f32 output) fit under the memory budget.
-// Several launches run at once on their own streams, so requests that arrive
-// while others decode start as soon as a stream and memory are free and their
-// work overlaps on the device.  Each launch is the batch path: one persistent
-// decode kernel over the admitted requests' chains (longest first) and one
-// resample launch over their files.
+// into a new launch on a free launch slot while the device memory the server
+// holds stays within the budget.  Several launches run at once on their own
+// streams, so requests that arrive while others decode start as soon as a
+// slot and memory are free and their work overlaps on the device.  Each
+// launch is the batch path: one persistent decode kernel over the admitted
+// requests' chains (costliest first) and one resample launch over their files.
+//
+// Memory: each launch slot keeps its device buffers (packets, tables, int16
+// PCM, decoder heaps, ...) for reuse; the budget bounds the sum of every
+// slot's buffers, busy or idle.  Admission computes what a launch would add
+// to its slot's buffers; when that does not fit, idle slots' buffers are
+// released first.
+//
+// Ordering: submit records an event on the caller's stream; the launch waits
+// for it before it writes the request's output, so work the caller queued on
+// that memory earlier (the allocator reusing a block, a fill) finishes first.
 struct SrvReq {
     uint64_t id = 0;
     FilePlan fp;
     std::vector<uint8_t> lost;
     float *out = nullptr;
     int64_t out16 = 0;
-    uint64_t mem = 0;
+    hipEvent_t ready = nullptr;   // recorded on the caller's stream at submit
     int rc = 0;
     char err[160] = {0};
+    ~SrvReq()
+    {
+        if (ready) hipEventDestroy(ready);
+    }
+};
+
+// Device bytes of one launch, per buffer (what srv_launch grows).
+struct SrvNeed {
+    uint64_t data = kPktPad, npk = 0, nch = 0, cat = 0, files = 0, lost = 0;
+    uint32_t waves = 0;
+    void add(const SrvReq *r)
+    {
+        data += r->fp.d.data.size();
+        npk += r->fp.d.off.size();
+        nch += r->fp.chains.size();
+        cat += (r->fp.keep + 7) & ~7ull;
+        files += r->out16 > 0;
+        lost = lost || !r->lost.empty();
+    }
 };
 
 struct SrvLaunch {
     hipStream_t st = nullptr;
     hipEvent_t done = nullptr;
     bool busy = false;
-    uint64_t mem = 0;
     std::vector<SrvReq *> reqs;
     std::vector<uint64_t> cap;         // per chain expected samples
     DevBuf data, off, len, lost, chains, order, cat, res, files, queue, heap;
@@ -1056,6 +1454,29 @@ struct SrvLaunch {
     size_t h_stage_cap = 0;
     ChainResult *h_res = nullptr;      // pinned results
     size_t h_res_cap = 0;
+    // the device buffers in srv_extra's order
+    std::array<DevBuf *, 11> bufs() { return {&data, &off, &len, &lost, &chains, &order, &cat, &res, &files, &queue, &heap}; }
+    std::array<const DevBuf *, 11> bufs() const
+    {
+        return {&data, &off, &len, &lost, &chains, &order, &cat, &res, &files, &queue, &heap};
+    }
+    uint64_t held() const
+    {
+        uint64_t s = 0;
+        for (const DevBuf *b : bufs()) s += b->cap;
+        return s;
+    }
+    void release()
+    {
+        for (DevBuf *b : bufs()) {
+            if (b->p) {
+                hipFree(b->p);
+                g_dev_bytes -= (long long)b->cap;
+            }
+            b->p = nullptr;
+            b->cap = 0;
+        }
+    }
 };
 
 struct cuopus_server {
@@ -1063,9 +1484,16 @@ struct cuopus_server {
     std::deque<SrvReq *> pending;
     std::vector<SrvLaunch> launches;
     std::deque<SrvReq *> finished;
-    uint64_t mem_in_use = 0;
     uint64_t next_id = 1;
     RsTaps taps;
+    uint32_t waves = 0;
+    int rs_variant = 0;
+    uint64_t held() const
+    {
+        uint64_t s = 0;
+        for (const SrvLaunch &L : launches) s += L.held();
+        return s;
+    }
 };
 
 namespace {
@@ -1083,11 +1511,19 @@ int pinned_grow(uint8_t **p, size_t *cap, size_t need, Err &e)
     return 0;
 }
 
-uint64_t req_device_bytes(const SrvReq *r)
+// What a launch of `n` adds to slot L's device buffers (grow() replaces a
+// buffer by one of exactly the needed size when it is too small).
+uint64_t srv_extra(const cuopus_server *srv, const SrvLaunch &L, const SrvNeed &n)
 {
-    uint64_t nch = r->fp.chains.size(), npk = r->fp.d.off.size();
-    return r->fp.d.data.size() + 8 * npk + r->lost.size() + (sizeof(ChainDesc) + sizeof(ChainResult) + 4) * nch +
-           2 * r->fp.keep + 4 * (uint64_t)(r->out16 > 0 ? r->out16 : 0) + 64;
+    const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(srv->waves, n.nch));
+    const uint64_t need[11] = {n.data, 4 * n.npk, 4 * n.npk, n.lost ? n.npk : 0, sizeof(ChainDesc) * n.nch, 4 * n.nch,
+                               2 * std::max<uint64_t>(1, n.cat), sizeof(ChainResult) * n.nch,
+                               sizeof(RsFile) * std::max<uint64_t>(1, n.files), 256, sizeof(DecHeap) * waves};
+    const auto have = L.bufs();
+    uint64_t extra = 0;
+    for (int i = 0; i < 11; i++)
+        if (need[i] > have[i]->cap) extra += need[i];
+    return extra;
 }
 
 // Build and enqueue one launch for the requests in L.reqs (no host waits).
@@ -1113,7 +1549,7 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
         }
     }
     std::vector<uint32_t> order;
-    lpt_order(pl, &order);
+    work_order(pl, 1, &order);
     std::vector<RsFile> rsf;
     uint64_t tiles = 0;
     for (size_t f = 0; f < pl.files.size(); f++) {
@@ -1124,13 +1560,12 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
         r.out_len = (uint64_t)L.reqs[f]->out16;
         r.out = L.reqs[f]->out;
         r.tile0 = tiles;
-        tiles += (r.out_len + kRsTile - 1) / kRsTile;
+        tiles += rs_tiles(r.out_len, srv->rs_variant);
         rsf.push_back(r);
     }
-    const uint32_t waves = std::max<uint32_t>(1, std::min<uint32_t>(
-        srv->opts.n_waves > 0 ? (uint32_t)srv->opts.n_waves : kDefaultWaves, nch));
-    // device buffers (grow-only per launch slot)
-    if ((rc = grow(L.data, pl.data.size() + 4, e)) || (rc = grow(L.off, 4 * npk, e)) ||
+    const uint32_t waves = std::max<uint32_t>(1, std::min<uint32_t>(srv->waves, nch));
+    // device buffers (kept per launch slot; sizes as srv_extra computes them)
+    if ((rc = grow(L.data, pl.data.size(), e)) || (rc = grow(L.off, 4 * npk, e)) ||
         (rc = grow(L.len, 4 * npk, e)) || (rc = grow(L.chains, sizeof(ChainDesc) * nch, e)) ||
         (rc = grow(L.order, 4ull * nch, e)) || (rc = grow(L.res, sizeof(ChainResult) * nch, e)) ||
         (rc = grow(L.cat, 2 * (pl.cat_len ? pl.cat_len : 1), e)) || (rc = grow(L.queue, 256, e)) ||
@@ -1157,6 +1592,9 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
         L.h_res = (ChainResult *)hr;
     }
     hipStream_t st = L.st;
+    // the caller's stream reached each request's submit point
+    for (SrvReq *r : L.reqs)
+        if (r->ready) HIPCHK(e, hipStreamWaitEvent(st, r->ready, 0));
     HIPCHK(e, hipMemcpyAsync(L.data.p, L.h_stage + o_data, pl.data.size(), hipMemcpyHostToDevice, st));
     HIPCHK(e, hipMemcpyAsync(L.off.p, L.h_stage + o_off, 4 * npk, hipMemcpyHostToDevice, st));
     HIPCHK(e, hipMemcpyAsync(L.len.p, L.h_stage + o_len, 4 * npk, hipMemcpyHostToDevice, st));
@@ -1183,8 +1621,8 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
     launch_persistent(waves, st, a);
     HIPCHK(e, hipGetLastError());
     if (tiles) {
-        hipLaunchKernelGGL(k_resample_48to16, dim3((unsigned)tiles), dim3(kRsThreads), 0, st, (const int16_t *)L.cat.p,
-                           (const RsFile *)L.files.p, (int)rsf.size(), srv->taps);
+        launch_resample(tiles, st, (const int16_t *)L.cat.p, (const RsFile *)L.files.p, (int)rsf.size(), srv->taps,
+                        srv->rs_variant);
         HIPCHK(e, hipGetLastError());
     }
     HIPCHK(e, hipMemcpyAsync(L.h_res, L.res.p, rneed, hipMemcpyDeviceToHost, st));
@@ -1213,8 +1651,6 @@ void srv_retire(cuopus_server *srv, SrvLaunch &L)
         c0 += nc;
         srv->finished.push_back(r);
     }
-    srv->mem_in_use -= L.mem;
-    L.mem = 0;
     L.reqs.clear();
     L.busy = false;
 }
@@ -1234,6 +1670,8 @@ extern "C" int cuopus_server_create(const cuopus_server_opts *opts, cuopus_serve
     if (s->opts.mem_budget <= 0) s->opts.mem_budget = 8ll << 30;
     if (s->opts.max_chains_per_launch <= 0) s->opts.max_chains_per_launch = 1 << 20;
     s->taps = make_taps();
+    s->waves = s->opts.n_waves > 0 ? (uint32_t)s->opts.n_waves : default_slots();
+    s->rs_variant = resample_variant();
     s->launches.resize((size_t)s->opts.n_streams);
     for (auto &L : s->launches) {
         hipError_t a = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking);
@@ -1248,8 +1686,8 @@ extern "C" int cuopus_server_create(const cuopus_server_opts *opts, cuopus_serve
 }
 
 extern "C" int cuopus_server_submit(cuopus_server *srv, const uint8_t *ogg, size_t n, const uint8_t *lost,
-                                    size_t mask_n, float *out_dev, size_t out_cap, uint64_t *req_id, char *errbuf,
-                                    size_t errcap)
+                                    size_t mask_n, float *out_dev, size_t out_cap, void *stream, uint64_t *req_id,
+                                    char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);
     if (!srv || !req_id) return e.set(CUOPUS_RC_ARG, "NULL argument");
@@ -1265,12 +1703,20 @@ extern "C" int cuopus_server_submit(cuopus_server *srv, const uint8_t *ogg, size
     if (rc) { delete r; return rc; }
     r->out = out_dev;
     r->out16 = r->fp.out16;
-    r->mem = req_device_bytes(r);
-    if (r->mem > (uint64_t)srv->opts.mem_budget) {
-        unsigned long long need = (unsigned long long)r->mem;
+    SrvNeed one;
+    one.add(r);
+    SrvLaunch empty;
+    const uint64_t alone = srv_extra(srv, empty, one);
+    if (alone > (uint64_t)srv->opts.mem_budget) {
         delete r;
-        return e.set(CUOPUS_RC_ARG, "request needs %llu device bytes, budget is %lld", need,
+        return e.set(CUOPUS_RC_ARG, "request needs %llu device bytes, budget is %lld", (unsigned long long)alone,
                      (long long)srv->opts.mem_budget);
+    }
+    hipError_t s = hipEventCreateWithFlags(&r->ready, hipEventDisableTiming);
+    if (s == hipSuccess) s = hipEventRecord(r->ready, (hipStream_t)stream);
+    if (s != hipSuccess) {
+        delete r;
+        return e.set(CUOPUS_RC_GPU, "submit event: %s", hipGetErrorString(s));
     }
     r->id = srv->next_id++;
     *req_id = r->id;
@@ -1291,22 +1737,40 @@ extern "C" int cuopus_server_poll(cuopus_server *srv, uint64_t *ids, int *rcs, i
         if (q != hipSuccess) return e.set(CUOPUS_RC_GPU, "launch: %s", hipGetErrorString(q));
         srv_retire(srv, L);
     }
-    // 2. admit queued requests onto free streams while memory allows
+    // 2. admit queued requests onto free launch slots while the device bytes
+    //    the server holds stay within the budget
+    const uint64_t budget = (uint64_t)srv->opts.mem_budget;
     for (auto &L : srv->launches) {
         if (L.busy || srv->pending.empty()) continue;
-        uint64_t mem = 0, nch = 0;
+        SrvNeed need;
+        std::vector<SrvReq *> take;
+        bool released = false;
         while (!srv->pending.empty()) {
-            SrvReq *r = srv->pending.front();
-            if (srv->mem_in_use + mem + r->mem > (uint64_t)srv->opts.mem_budget) break;
-            if (!L.reqs.empty() && nch + r->fp.chains.size() > (uint64_t)srv->opts.max_chains_per_launch) break;
-            L.reqs.push_back(r);
-            mem += r->mem;
-            nch += r->fp.chains.size();
-            srv->pending.pop_front();
+            SrvReq *r = srv->pending[take.size()];
+            SrvNeed next = need;
+            next.add(r);
+            if (!take.empty() && next.nch > (uint64_t)srv->opts.max_chains_per_launch) break;
+            uint64_t extra = srv_extra(srv, L, next);
+            if (srv->held() + extra > budget && !released) {
+                // give back what idle slots keep, then try again
+                for (auto &M : srv->launches)
+                    if (!M.busy && &M != &L) M.release();
+                released = true;
+                extra = srv_extra(srv, L, next);
+            }
+            if (srv->held() + extra > budget && L.held() > 0 && take.empty()) {
+                // this slot's own buffers are the wrong shape: drop them
+                L.release();
+                extra = srv_extra(srv, L, next);
+            }
+            if (srv->held() + extra > budget) break;
+            need = next;
+            take.push_back(r);
+            if (take.size() == srv->pending.size()) break;
         }
-        if (L.reqs.empty()) break;  // head request must wait for memory
-        L.mem = mem;
-        srv->mem_in_use += mem;
+        if (take.empty()) break;  // head request must wait for memory
+        for (size_t i = 0; i < take.size(); i++) srv->pending.pop_front();
+        L.reqs = take;
         int rc = srv_launch(srv, L, e);
         if (rc) {
             for (SrvReq *r : L.reqs) {
@@ -1314,8 +1778,6 @@ extern "C" int cuopus_server_poll(cuopus_server *srv, uint64_t *ids, int *rcs, i
                 snprintf(r->err, sizeof(r->err), "%s", errbuf ? errbuf : "launch failed");
                 srv->finished.push_back(r);
             }
-            srv->mem_in_use -= L.mem;
-            L.mem = 0;
             L.reqs.clear();
             if (rc == CUOPUS_RC_GPU) return rc;
         }
@@ -1342,7 +1804,7 @@ extern "C" int cuopus_server_stats(const cuopus_server *srv, int64_t *s4)
     for (auto &L : srv->launches) busy += L.busy;
     s4[0] = (int64_t)srv->pending.size();
     s4[1] = busy;
-    s4[2] = (int64_t)srv->mem_in_use;
+    s4[2] = (int64_t)srv->held();
     s4[3] = (int64_t)srv->finished.size();
     return 0;
 }
@@ -1352,13 +1814,7 @@ extern "C" void cuopus_server_destroy(cuopus_server *srv)
     if (!srv) return;
     for (auto &L : srv->launches) {
         if (L.st) hipStreamSynchronize(L.st);
-        DevBuf *all[] = {&L.data, &L.off, &L.len, &L.lost, &L.chains, &L.order, &L.cat, &L.res, &L.files,
-                         &L.queue, &L.heap};
-        for (DevBuf *d : all) {
-            if (d->p) { hipFree(d->p); g_dev_bytes -= (long long)d->cap; }
-            d->p = nullptr;
-            d->cap = 0;
-        }
+        L.release();
         if (L.h_stage) hipHostFree(L.h_stage);
         if (L.h_res) hipHostFree(L.h_res);
         for (SrvReq *r : L.reqs) delete r;

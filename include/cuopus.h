@@ -133,10 +133,43 @@ int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, 
                        int32_t *pkt_nsamp, uint32_t *pkt_rng, int32_t *chain_stop, void *stream, char *errbuf,
                        size_t errcap);
 
+/* Parsed files: demux + chain table once (host only), then decode.  info6,
+ * when non-NULL: [0] f32 16 kHz output length, [1] playable 48 kHz samples,
+ * [2] channels, [3] audio packets, [4] chains, [5] OpusHead output gain
+ * (Q7.8 dB).  The decode calls behave as cuopus_decode_f16k_on /
+ * cuopus_decode_f16k_multi on the same bytes (same output, same return
+ * codes) without parsing the stream again; the Python binding uses them so a
+ * file is demuxed once per call.  A handle is freed by cuopus_file_close and
+ * may be decoded any number of times. */
+typedef struct cuopus_file cuopus_file;
+int cuopus_file_open(const uint8_t *ogg, size_t n, cuopus_file **out, int64_t *info6, char *errbuf, size_t errcap);
+int cuopus_file_decode_f16k_on(cuopus_file *f, float *out_dev, size_t out_cap, int64_t *out_len, void *stream,
+                               char *errbuf, size_t errcap);
+int cuopus_files_decode_f16k_multi(cuopus_file *const *fs, int nfiles, float *const *out_devs, const size_t *out_caps,
+                                   int64_t *out_lens, void *stream, char *errbuf, size_t errcap);
+void cuopus_file_close(cuopus_file *f);
+
 /* Resample playable int16 48 kHz device PCM to f32 16 kHz (length
  * max(0,(n48-48)/3)) with the library's resampler kernel. */
 int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, float *out_dev, size_t out_cap,
                                 int64_t *out_len, void *stream, char *errbuf, size_t errcap);
+
+/* Log-mel features of a resident f32 16 kHz device buffer (the consumer the
+ * reference keeps its PCM on the device for, ROADMAP.md:149-156; no
+ * counterpart entry point).  Frame t covers in_dev[160 t, 160 t + 400): 25 ms
+ * every 10 ms, no edge padding, *frames = n16 >= 400 ? 1 + (n16 - 400) / 160
+ * : 0.  Periodic Hann window (400), zero-padded 512-point power spectrum,
+ * n_mels (1..128) triangular filters on the Slaney mel scale from 0 to
+ * 8000 Hz with Slaney area normalisation, out = ln(max(mel, 1e-10)) as
+ * row-major (frames, n_mels) float32; out_cap in floats.  Synchronises the
+ * stream before returning. */
+int cuopus_log_mel(const float *in_dev, int64_t n16, float *out_dev, size_t out_cap, int64_t *frames, int n_mels,
+                   void *stream, char *errbuf, size_t errcap);
+
+/* The filter weights cuopus_log_mel applies (host only): out[m * 257 + k],
+ * row-major (n_mels, 257) float32, cap in floats.  Returns 0 or
+ * CUOPUS_RC_ARG. */
+int cuopus_mel_filters(int n_mels, float *out, size_t cap);
 
 /* ---------------------------------------------------------------------------
  * Resident batches: demux + upload once, then decode from HBM-resident
@@ -153,11 +186,16 @@ typedef struct {
                         workgroup per chain; 2: persistent kernel, n_slots
                         wavefronts pull chains from a device work queue */
     int n_slots;     /* mode 2: resident wavefronts (0 = library default) */
-    int lpt_order;   /* 1: longest chains first (modes 1-2) */
+    int lpt_order;   /* work order (modes 1-2): 0 file order, 1 costliest
+                        chains first by the per-packet cost model (TOC mode
+                        and frame duration), 2 longest chains first by
+                        samples + 960 x packets */
     int use_graph;   /* 1: capture the launch sequence into a HIP graph on
                         the first run and replay it afterwards */
     int profile;     /* 1: time the decode and resample kernels with events */
-
+    int rs_variant;  /* resampler launch shape: 0 = library default (tuned),
+                        v + 1 = variant v (threads x outputs per thread:
+                        0: 256x4, 1: 256x8, 2: 128x8, 3: 512x4, 4: 128x4) */
 } cuopus_batch_opts;
 
 /* lost_masks may be NULL (no loss) or hold one mask per file (NULL entries
@@ -190,20 +228,27 @@ typedef struct cuopus_server cuopus_server;
 typedef struct {
     int n_streams;              /* launches in flight (0: 4) */
     int n_waves;                /* persistent wavefronts per launch (0: default) */
-    long long mem_budget;       /* device bytes for admitted requests (0: 8 GiB) */
+    long long mem_budget;       /* device bytes the server may hold: every launch
+                                   slot's buffers (packets, tables, int16 PCM,
+                                   decoder heaps), busy or idle (0: 8 GiB) */
     int max_chains_per_launch;  /* 0: unlimited */
 } cuopus_server_opts;
 
 int cuopus_server_create(const cuopus_server_opts *opts, cuopus_server **out, char *errbuf, size_t errcap);
-/* lost/mask_n: optional per-packet loss mask (FEC/PLC rule as the lossy decode). */
+/* lost/mask_n: optional per-packet loss mask (FEC/PLC rule as the lossy
+ * decode).  stream: the caller's stream (hipStream_t) that last used out_dev;
+ * the request's launch waits for the point of this call on that stream
+ * before it writes out_dev.  The output is complete when poll reports the
+ * request. */
 int cuopus_server_submit(cuopus_server *srv, const uint8_t *ogg, size_t n, const uint8_t *lost, size_t mask_n,
-                         float *out_dev, size_t out_cap, uint64_t *req_id, char *errbuf, size_t errcap);
+                         float *out_dev, size_t out_cap, void *stream, uint64_t *req_id, char *errbuf,
+                         size_t errcap);
 /* Up to cap finished requests: ids[i] and rcs[i] (0 ok, else the request's
  * error; errbuf then holds the last failing request's message). */
 int cuopus_server_poll(cuopus_server *srv, uint64_t *ids, int *rcs, int cap, int *n_done, char *errbuf,
                        size_t errcap);
-/* s4: [0] queued requests [1] launches in flight [2] admitted device bytes
- * [3] finished, not yet reported */
+/* s4: [0] queued requests [1] launches in flight [2] device bytes the server
+ * holds (never above mem_budget) [3] finished, not yet reported */
 int cuopus_server_stats(const cuopus_server *srv, int64_t *s4);
 void cuopus_server_destroy(cuopus_server *srv);
 

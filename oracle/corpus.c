@@ -20,9 +20,13 @@
  *        [--signal speech|music|mixed] [--app voip|audio|lowdelay] [--bw nb|mb|wb|swb|fb]
  *        [--force auto|silk|hybrid|celt] [--vbr 0|1] [--bitrate B]
  *        [--brange LO HI] [--frame N] [--frameset a,b,c] [--fec 0|1] [--dtx 0|1] [--gaps P]
- *        [--loss P] [--complexity C] [--pcm F]
+ *        [--loss P] [--complexity C] [--pcm F] [--gain Q8] [--channels 1|2] [--monoflip P]
  * --brange redraws the bitrate every second from U(LO,HI); --frameset draws
- * the frame size (48 kHz samples) per second.
+ * the frame size (48 kHz samples) per second.  --gain writes the OpusHead
+ * output gain (Q7.8 dB).  --channels 2 encodes a stereo stream: left is the
+ * signal, right is 0.7 x the signal 20 samples later plus 0.5 x a second,
+ * independent signal of the same kind; --monoflip P forces mono packets for
+ * each second with probability P (stereo <-> mono switches in one stream).
  */
 #include <math.h>
 #include <stdint.h>
@@ -233,7 +237,8 @@ int main(int argc, char **argv)
     uint64_t seed = 1;
     double dur = 10.0;
     int vbr = 1, bitrate = 32000, blo = 0, bhi = 0, frame = 960, fec = 0, loss = 0, cx = 10, dtx = 0;
-    double gaps = 0.0;
+    double gaps = 0.0, monoflip = 0.0;
+    int gain_q8 = 0, channels = 1;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
 #define NEXT (i + 1 < argc ? argv[++i] : "")
@@ -255,6 +260,9 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "--complexity")) cx = atoi(NEXT);
         else if (!strcmp(a, "--dtx")) dtx = atoi(NEXT);
         else if (!strcmp(a, "--gaps")) gaps = atof(NEXT);
+        else if (!strcmp(a, "--gain")) gain_q8 = atoi(NEXT);
+        else if (!strcmp(a, "--channels")) channels = atoi(NEXT) == 2 ? 2 : 1;
+        else if (!strcmp(a, "--monoflip")) monoflip = atof(NEXT);
         else { fprintf(stderr, "unknown arg %s\n", a); return 2; }
 #undef NEXT
     }
@@ -285,20 +293,32 @@ int main(int argc, char **argv)
         for (long i = 0; i < n; i++) if (is_music[i / FS]) x[i] = y[i];
         free(y);
     } else gen_speech(x, n);
+    double *xr = NULL;
+    if (channels == 2) {
+        double *y = (double *)calloc((size_t)n + 1, sizeof(double));
+        if (!strcmp(signal, "music") || mixed) gen_music(y, n);
+        else gen_speech(y, n);
+        xr = (double *)calloc((size_t)n + 1, sizeof(double));
+        for (long i = 0; i < n; i++) xr[i] = 0.7 * (i >= 20 ? x[i - 20] : 0.0) + 0.5 * y[i];
+        free(y);
+    }
     /* --gaps P: each 500 ms segment is silent with probability P (only the
        int16 dither remains), so the encoder's DTX has inactive stretches */
     if (gaps > 0.0) {
         const long seg = FS / 2;
         for (long s0 = 0; s0 < n; s0 += seg)
             if (runif() < gaps)
-                for (long i = s0; i < n && i < s0 + seg; i++) x[i] = 0.0;
+                for (long i = s0; i < n && i < s0 + seg; i++) {
+                    x[i] = 0.0;
+                    if (xr) xr[i] = 0.0;
+                }
     }
 
     int application = !strcmp(app, "voip") ? OPUS_APPLICATION_VOIP
                     : !strcmp(app, "lowdelay") ? OPUS_APPLICATION_RESTRICTED_LOWDELAY
                     : OPUS_APPLICATION_AUDIO;
     int err = 0;
-    OpusEncoder *enc = opus_encoder_create(FS, 1, application, &err);
+    OpusEncoder *enc = opus_encoder_create(FS, channels, application, &err);
     if (!enc) { fprintf(stderr, "encoder: %s\n", opus_strerror(err)); return 1; }
     opus_encoder_ctl(enc, OPUS_SET_COMPLEXITY(cx));
     opus_encoder_ctl(enc, OPUS_SET_VBR(vbr));
@@ -317,18 +337,21 @@ int main(int argc, char **argv)
 
     /* int16 with TPDF dither; padded with silence to flush the lookahead */
     long total_in = n + lookahead + 2 * 2880;
-    int16_t *pcm = (int16_t *)calloc((size_t)total_in, sizeof(int16_t));
+    int16_t *pcm = (int16_t *)calloc((size_t)total_in * channels, sizeof(int16_t));
     for (long i = 0; i < n; i++) {
-        double v = x[i] * 32768.0 + (runif() - runif());
-        v = floor(v + 0.5);
-        if (v > 32767) v = 32767;
-        if (v < -32768) v = -32768;
-        pcm[i] = (int16_t)v;
+        for (int c = 0; c < channels; c++) {
+            double v = (c ? xr[i] : x[i]) * 32768.0 + (runif() - runif());
+            v = floor(v + 0.5);
+            if (v > 32767) v = 32767;
+            if (v < -32768) v = -32768;
+            pcm[i * channels + c] = (int16_t)v;
+        }
     }
     free(x);
+    free(xr);
     if (pcm_out) {
         FILE *pf = fopen(pcm_out, "wb");
-        if (pf) { fwrite(pcm, 2, (size_t)n, pf); fclose(pf); }
+        if (pf) { fwrite(pcm, 2, (size_t)n * channels, pf); fclose(pf); }
     }
 
     static ogg_w w;
@@ -337,10 +360,10 @@ int main(int argc, char **argv)
     w.serial = (uint32_t)(seed * 2654435761u) | 1u;
     uint8_t head[19];
     memcpy(head, "OpusHead", 8);
-    head[8] = 1; head[9] = 1;
+    head[8] = 1; head[9] = (uint8_t)channels;
     head[10] = (uint8_t)(lookahead & 0xFF); head[11] = (uint8_t)(lookahead >> 8);
     head[12] = FS & 0xFF; head[13] = (FS >> 8) & 0xFF; head[14] = (FS >> 16) & 0xFF; head[15] = 0;
-    head[16] = 0; head[17] = 0; head[18] = 0;
+    head[16] = (uint8_t)(gain_q8 & 0xFF); head[17] = (uint8_t)((gain_q8 >> 8) & 0xFF); head[18] = 0;
     ogg_packet(&w, head, 19, 0);
     ogg_flush(&w, 0x02);
     uint8_t tags[8 + 4 + 8 + 4];
@@ -373,12 +396,14 @@ int main(int argc, char **argv)
             if (nfs > 0) fsz = fs_list[(int)(runif() * nfs) % nfs];
             if (mixed)
                 opus_encoder_ctl(enc, OPUS_SET_SIGNAL(is_music[s] ? OPUS_SIGNAL_MUSIC : OPUS_SIGNAL_VOICE));
+            if (channels == 2 && monoflip > 0.0)
+                opus_encoder_ctl(enc, OPUS_SET_FORCE_CHANNELS(runif() < monoflip ? 1 : OPUS_AUTO));
             /* keep frames inside the second so the draw applies per segment */
         }
         int this_fs = fsz;
         int64_t to_boundary = (int64_t)(sec + 1) * FS - done;
         if (to_boundary < this_fs) this_fs = (int)to_boundary;
-        int nb = opus_encode(enc, pcm + done, this_fs, pkt, (opus_int32)sizeof(pkt));
+        int nb = opus_encode(enc, pcm + done * channels, this_fs, pkt, (opus_int32)sizeof(pkt));
         if (nb < 0) { fprintf(stderr, "encode: %s (fs %d)\n", opus_strerror(nb), this_fs); return 1; }
         done += this_fs;
         int64_t g = done >= end_granule ? end_granule : done;

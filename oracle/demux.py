@@ -75,6 +75,7 @@ class Demuxed:
         self.sample_off = []
         self.channels = 0
         self.preskip = 0
+        self.gain_q8 = 0
         self.last_granule = 0
         self.pcm_samples_48k = 0
         self.granule_checked = 0
@@ -141,6 +142,7 @@ def demux(buf):
     d = Demuxed()
     d.channels = head[9]
     d.preskip = head[10] | (head[11] << 8)
+    d.gain_q8 = int.from_bytes(head[16:18], "little", signed=True)  # output gain, Q7.8 dB (RFC 7845 5.1)
     d.packets = packets[2:]
     d.pkt_page = ppage[2:]
     d.n_pages = len(page_granule)

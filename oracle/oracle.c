@@ -38,33 +38,52 @@
 
 int orc_version(void) { return 1; }
 
-/* Decode one chain.  Packet k is data[off[k] .. off[k]+len[k]).  lost may be
- * NULL.  pcm receives every decoded int16 sample (capacity cap); nsamp[k] and
- * rng[k] receive opus_decode's return and OPUS_GET_FINAL_RANGE after packet k
- * (either may be NULL).  Returns total samples, or a negative opus error. */
-long orc_decode_chain(const unsigned char *data, const uint32_t *off,
-                      const uint32_t *len, const uint8_t *lost, int npk,
-                      int16_t *pcm, long cap, int32_t *nsamp, uint32_t *rng)
+/* opus_packet_has_lbrr on a copy of the packet followed by zero bytes:
+ * libopus reads the first byte of the first frame even when that frame is
+ * empty, i.e. possibly past the packet; in the copy that byte is 0, so the
+ * FEC decision never depends on what follows the packet in memory. */
+static int orc_has_lbrr(const unsigned char *q, opus_int32 len)
+{
+    unsigned char small[1536];
+    unsigned char *buf = (size_t)len + 8 <= sizeof(small) ? small : (unsigned char *)malloc((size_t)len + 8);
+    if (!buf) return OPUS_ALLOC_FAIL;
+    memcpy(buf, q, (size_t)len);
+    memset(buf + len, 0, 8);
+    int r = opus_packet_has_lbrr(buf, len);
+    if (buf != small) free(buf);
+    return r;
+}
+
+/* Decode one chain with a fresh decoder of `channels` output channels and
+ * OPUS_SET_GAIN(gain) (Q8 dB; 0: none).  Packet k is data[off[k] ..
+ * off[k]+len[k]).  lost may be NULL.  pcm receives every decoded int16
+ * sample, interleaved when channels == 2 (capacity cap samples per channel);
+ * nsamp[k] and rng[k] receive opus_decode's return and OPUS_GET_FINAL_RANGE
+ * after packet k (either may be NULL).  Returns total samples per channel, or
+ * a negative opus error. */
+long orc_decode_chain_cfg(const unsigned char *data, const uint32_t *off, const uint32_t *len, const uint8_t *lost,
+                          int npk, int16_t *pcm, long cap, int32_t *nsamp, uint32_t *rng, int channels, int gain)
 {
     int err = 0;
-    OpusDecoder *dec = opus_decoder_create(48000, 1, &err);
+    OpusDecoder *dec = opus_decoder_create(48000, channels, &err);
     if (!dec) return err ? err : -1;
+    if (gain) opus_decoder_ctl(dec, OPUS_SET_GAIN(gain));
     long total = 0;
     for (int k = 0; k < npk; k++) {
         const unsigned char *p = data + off[k];
+        int16_t *o = pcm + total * channels;
         int ret;
         if (cap - total < ORC_MAX_PKT_SAMPLES) { ret = OPUS_BUFFER_TOO_SMALL; goto done; }
         if (lost && lost[k]) {
             int dur = opus_packet_get_nb_samples(p, (opus_int32)len[k], 48000);
             if (dur <= 0) { ret = dur ? dur : OPUS_INVALID_PACKET; goto done; }
             const unsigned char *q = (k + 1 < npk) ? data + off[k + 1] : NULL;
-            if (q && !lost[k + 1] && opus_packet_has_lbrr(q, (opus_int32)len[k + 1]) > 0)
-                ret = opus_decode(dec, q, (opus_int32)len[k + 1], pcm + total, dur, 1);
+            if (q && !lost[k + 1] && orc_has_lbrr(q, (opus_int32)len[k + 1]) > 0)
+                ret = opus_decode(dec, q, (opus_int32)len[k + 1], o, dur, 1);
             else
-                ret = opus_decode(dec, NULL, 0, pcm + total, dur, 0);
+                ret = opus_decode(dec, NULL, 0, o, dur, 0);
         } else {
-            ret = opus_decode(dec, p, (opus_int32)len[k], pcm + total,
-                              ORC_MAX_PKT_SAMPLES, 0);
+            ret = opus_decode(dec, p, (opus_int32)len[k], o, ORC_MAX_PKT_SAMPLES, 0);
         }
         if (ret < 0) goto done;
         if (nsamp) nsamp[k] = ret;
@@ -83,6 +102,13 @@ done:
     return total;
 }
 
+/* The chain protocol's default decoder: mono, no gain. */
+long orc_decode_chain(const unsigned char *data, const uint32_t *off, const uint32_t *len, const uint8_t *lost,
+                      int npk, int16_t *pcm, long cap, int32_t *nsamp, uint32_t *rng)
+{
+    return orc_decode_chain_cfg(data, off, len, lost, npk, pcm, cap, nsamp, rng, 1, 0);
+}
+
 /* ---- many chains, optionally on a thread pool (CPU baseline) ---- */
 typedef struct {
     const unsigned char *data;
@@ -96,6 +122,8 @@ typedef struct {
     uint32_t *rng;
     int64_t *chain_ret;
     int n_chains;
+    int channels;            /* decoder output channels */
+    const int32_t *gain;     /* per chain OPUS_SET_GAIN value, or NULL */
     volatile int next;
     pthread_mutex_t mu;
 } orc_job;
@@ -113,9 +141,10 @@ static void *orc_worker(void *arg)
             uint32_t f = j->first[c], np = j->npkts[c];
             int32_t *ns = (int32_t *)calloc(np ? np : 1, sizeof(int32_t));
             uint32_t *rg = (uint32_t *)calloc(np ? np : 1, sizeof(uint32_t));
-            j->chain_ret[c] = orc_decode_chain(
+            j->chain_ret[c] = orc_decode_chain_cfg(
                 j->data, j->off + f, j->len + f, j->lost ? j->lost + f : NULL,
-                (int)np, j->pcm + j->pcm_off[c], (long)j->pcm_cap[c], ns, rg);
+                (int)np, j->pcm + j->pcm_off[c] * (uint64_t)j->channels, (long)j->pcm_cap[c], ns, rg,
+                j->channels, j->gain ? j->gain[c] : 0);
             /* A packet inside several chains (preroll overlap) is recorded by
              * the last chain that holds it: chain c owns packets below the
              * first packet of chain c + 1.  Deterministic on any thread count. */
@@ -131,14 +160,14 @@ static void *orc_worker(void *arg)
     return NULL;
 }
 
-/* Decode n_chains chains with nthreads workers (dynamic chunks of 4).
+/* Decode n_chains chains with nthreads workers (dynamic chunks of 4), each
+ * with a fresh decoder of `channels` output channels and the chain's gain
+ * (gain may be NULL).  pcm_off / pcm_cap are in samples per channel.
  * Returns 0 if every chain decoded, else the first negative chain return. */
-int orc_decode_chains(const unsigned char *data, const uint32_t *off,
-                      const uint32_t *len, const uint8_t *lost,
-                      const uint32_t *first, const uint32_t *npkts, int n_chains,
-                      const uint64_t *pcm_off, const uint64_t *pcm_cap,
-                      int16_t *pcm, int32_t *nsamp, uint32_t *rng,
-                      int64_t *chain_ret, int nthreads)
+int orc_decode_chains_cfg(const unsigned char *data, const uint32_t *off, const uint32_t *len, const uint8_t *lost,
+                          const uint32_t *first, const uint32_t *npkts, int n_chains, const uint64_t *pcm_off,
+                          const uint64_t *pcm_cap, int16_t *pcm, int32_t *nsamp, uint32_t *rng, int64_t *chain_ret,
+                          int nthreads, int channels, const int32_t *gain)
 {
     orc_job j;
     memset(&j, 0, sizeof(j));
@@ -146,6 +175,8 @@ int orc_decode_chains(const unsigned char *data, const uint32_t *off,
     j.first = first; j.npkts = npkts; j.pcm_off = pcm_off; j.pcm_cap = pcm_cap;
     j.pcm = pcm; j.nsamp = nsamp; j.rng = rng; j.chain_ret = chain_ret;
     j.n_chains = n_chains; j.next = 0;
+    j.channels = channels == 2 ? 2 : 1;
+    j.gain = gain;
     pthread_mutex_init(&j.mu, NULL);
     if (nthreads < 1) nthreads = 1;
     if (nthreads == 1) {
@@ -160,6 +191,15 @@ int orc_decode_chains(const unsigned char *data, const uint32_t *off,
     for (int c = 0; c < n_chains; c++)
         if (chain_ret[c] < 0) return (int)chain_ret[c];
     return 0;
+}
+
+int orc_decode_chains(const unsigned char *data, const uint32_t *off, const uint32_t *len, const uint8_t *lost,
+                      const uint32_t *first, const uint32_t *npkts, int n_chains, const uint64_t *pcm_off,
+                      const uint64_t *pcm_cap, int16_t *pcm, int32_t *nsamp, uint32_t *rng, int64_t *chain_ret,
+                      int nthreads)
+{
+    return orc_decode_chains_cfg(data, off, len, lost, first, npkts, n_chains, pcm_off, pcm_cap, pcm, nsamp, rng,
+                                 chain_ret, nthreads, 1, NULL);
 }
 
 /* ---- resampler ---- */

@@ -24,9 +24,10 @@ struct HostOut {
     }
 };
 
-extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const uint32_t *len,
-                                const uint8_t *lost, int npk, int16_t *pcm, long cap,
-                                int32_t *nsamp, uint32_t *rng)
+// One chain from a fresh decoder with OPUS_SET_GAIN(gain) (Q8 dB).
+extern "C" long hg_decode_chain_gain(const uint8_t *data, const uint32_t *off, const uint32_t *len,
+                                     const uint8_t *lost, int npk, int16_t *pcm, long cap,
+                                     int32_t *nsamp, uint32_t *rng, int gain)
 {
     OpusDecState *st = (OpusDecState *)malloc(sizeof(OpusDecState));
     DecScratch *sc = (DecScratch *)malloc(sizeof(DecScratch));
@@ -36,6 +37,7 @@ extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const 
     memset(heap, 0x3C, sizeof(DecHeap));
     dec_bind(st, sc, heap);
     opus_dec_init(st);
+    st->decode_gain = gain;
     HostOut out{sc->frame_out, pcm, 0};
     long total = 0;
     long ret = 0;
@@ -64,6 +66,13 @@ extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const 
     free(sc);
     free(heap);
     return ret < 0 ? ret : total;
+}
+
+extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const uint32_t *len,
+                                const uint8_t *lost, int npk, int16_t *pcm, long cap,
+                                int32_t *nsamp, uint32_t *rng)
+{
+    return hg_decode_chain_gain(data, off, len, lost, npk, pcm, cap, nsamp, rng, 0);
 }
 
 extern "C" void hg_ec_schedule(const uint8_t *buf, uint32_t n, uint32_t *out)

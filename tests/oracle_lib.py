@@ -36,6 +36,9 @@ def lib():
         L.orc_decode_chain.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_long, P, P]
         L.orc_decode_chains.restype = ctypes.c_int
         L.orc_decode_chains.argtypes = [P, P, P, P, P, P, ctypes.c_int, P, P, P, P, P, P, ctypes.c_int]
+        L.orc_decode_chains_cfg.restype = ctypes.c_int
+        L.orc_decode_chains_cfg.argtypes = [P, P, P, P, P, P, ctypes.c_int, P, P, P, P, P, P, ctypes.c_int,
+                                            ctypes.c_int, P]
         L.orc_resample.restype = None
         L.orc_resample.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64]
         L.orc_resample_range.restype = None
@@ -68,12 +71,15 @@ def demux_chains(blob, seconds=1.0, preroll_ms=250.0):
     return d, odemux.build_chains(d, seconds, preroll_ms)
 
 
-def decode_chains(data, off, ln, chains, lost=None, threads=1, caps=None):
-    """Decode every chain with a fresh fixed-point libopus decoder.
+def decode_chains(data, off, ln, chains, lost=None, threads=1, caps=None, channels=1, gain=0):
+    """Decode every chain with a fresh fixed-point libopus decoder of
+    `channels` output channels and OPUS_SET_GAIN(gain) (Q8 dB; an int for
+    every chain or one per chain).
 
-    caps: PCM capacity per chain (default 5760 per packet + 5760).
-    Returns dict: pcm (compacted keep windows, int16), nsamp/rng per packet,
-    ret per chain, full (list of per-chain full PCM arrays)."""
+    caps: PCM capacity per chain in samples per channel (default 5760 per
+    packet + 5760).  Returns dict: pcm (compacted keep windows, int16,
+    interleaved for 2 channels), nsamp/rng per packet, ret per chain, full
+    (list of per-chain full PCM arrays)."""
     L = lib()
     ch = np.asarray(chains, dtype=np.int64).reshape(-1, 4)
     n = ch.shape[0]
@@ -85,7 +91,7 @@ def decode_chains(data, off, ln, chains, lost=None, threads=1, caps=None):
     pcm_off = np.zeros(n, np.uint64)
     if n > 1:
         pcm_off[1:] = np.cumsum(caps[:-1])
-    pcm = np.zeros(int(caps.sum()), np.int16)
+    pcm = np.zeros(int(caps.sum()) * channels, np.int16)
     nsamp = np.zeros(len(off), np.int32)
     rng = np.zeros(len(off), np.uint32)
     ret = np.zeros(n, np.int64)
@@ -96,17 +102,20 @@ def decode_chains(data, off, ln, chains, lost=None, threads=1, caps=None):
     if lost is not None:
         lost = np.ascontiguousarray(lost, dtype=np.uint8)
         lp = lost.ctypes.data
-    L.orc_decode_chains(data.ctypes.data, off.ctypes.data, ln.ctypes.data, lp, first.ctypes.data, npk.ctypes.data,
-                        n, pcm_off.ctypes.data, caps.ctypes.data, pcm.ctypes.data, nsamp.ctypes.data,
-                        rng.ctypes.data, ret.ctypes.data, threads)
+    gains = np.ascontiguousarray(np.broadcast_to(np.asarray(gain, dtype=np.int32), (max(n, 1),)))
+    L.orc_decode_chains_cfg(data.ctypes.data, off.ctypes.data, ln.ctypes.data, lp, first.ctypes.data,
+                            npk.ctypes.data, n, pcm_off.ctypes.data, caps.ctypes.data, pcm.ctypes.data,
+                            nsamp.ctypes.data, rng.ctypes.data, ret.ctypes.data, threads, channels,
+                            gains.ctypes.data)
     kept = []
     full = []
+    C = channels
     for i in range(n):
         r = int(ret[i])
-        seg = pcm[int(pcm_off[i]): int(pcm_off[i]) + max(r, 0)]
+        seg = pcm[C * int(pcm_off[i]): C * (int(pcm_off[i]) + max(r, 0))]
         full.append(seg)
         trim, keep = int(ch[i, 2]), int(ch[i, 3])
-        kept.append(seg[trim: trim + keep])
+        kept.append(seg[C * trim: C * (trim + keep)])
     return {"pcm": np.concatenate(kept) if kept else np.zeros(0, np.int16), "nsamp": nsamp, "rng": rng,
             "ret": ret, "full": full}
 
@@ -186,6 +195,17 @@ RECIPES = {
 }
 
 
+# OpusHead output gain (Q7.8 dB): applied as libopus OPUS_SET_GAIN, with saturation
+GAIN_RECIPES = {
+    "gain_up6": "--seed 40 --dur 6 --signal speech --app voip --bw wb --force silk --bitrate 20000 --gain 1536",
+    "gain_down10": "--seed 41 --dur 6 --signal music --app lowdelay --bw fb --bitrate 64000 "
+                   "--frameset 120,240,480,960 --gain -2560",
+    "gain_mixed": "--seed 42 --dur 8 --signal mixed --app audio --bw fb --brange 12000 64000 --gain 700",
+    # a large boost: many samples saturate at +-32767
+    "gain_clip": "--seed 43 --dur 4 --signal music --app audio --bw fb --bitrate 48000 --gain 6000",
+}
+
+
 # More seeds of the mode-switching mixes (SILK/hybrid/CELT transitions, redundancy, FEC): decoded with
 # seeded loss by one CPU-gate test and one GPU test each
 SEED_SWEEP = {
@@ -197,7 +217,10 @@ SEED_SWEEP = {
 
 
 def recipe(name):
-    return RECIPES[name] if name in RECIPES else SEED_SWEEP[name]
+    for table in (RECIPES, SEED_SWEEP, GAIN_RECIPES):
+        if name in table:
+            return table[name]
+    raise KeyError(name)
 
 
 # Benchmark inputs (minutes to generate; written once under corpus/bench)
@@ -234,7 +257,9 @@ BENCH_SET_MODES = {
     "celt_fb": "--app lowdelay --bw fb",
 }
 BENCH_SETS = {
-    # 250 files, durations exp(U(ln 5, ln 1800)) s
+    # the batching config at its stated size: 1000 files, durations exp(U(ln 5, ln 1800)) s
+    "uneven_1000": dict(n=1000, seed=0x0C0FFEE3, dmin=5.0, dmax=1800.0),
+    # its first 250 files (identical files: same per-file seeds)
     "uneven_250": dict(n=250, seed=0x0C0FFEE3, dmin=5.0, dmax=1800.0),
     # the first 50 files of uneven_250 (identical files: same per-file seeds),
     # about 12.6k chains: small enough to ship with every GPU run
@@ -271,14 +296,18 @@ def bench_set_plan(name):
     return plan
 
 
-def bench_set(name, jobs=8):
+def bench_set(name, jobs=8, limit=None, progress=False):
     """Generate (once) a multi-file benchmark set under corpus/bench/<name>/
-    with a manifest.json; returns the list of file paths."""
+    with a manifest.json; returns the list of file paths.  limit: only the
+    first `limit` files of the set (the same files the full set has)."""
     import json
     from concurrent.futures import ThreadPoolExecutor
     d = os.path.join(BENCH, name)
     os.makedirs(d, exist_ok=True)
     plan = bench_set_plan(name)
+    if limit is not None:
+        plan = plan[:limit]
+    done = [0]
 
     def one(item):
         fname, _, _, args = item
@@ -287,6 +316,9 @@ def bench_set(name, jobs=8):
             tmp = path + ".tmp%d" % os.getpid()
             subprocess.run([MKCORPUS, "--out", tmp] + shlex.split(args), check=True)
             os.replace(tmp, path)
+        done[0] += 1
+        if progress and done[0] % 25 == 0:
+            print("[corpus] %s: %d/%d files" % (name, done[0], len(plan)), flush=True)
         return path
 
     # longest first so the pool finishes together

@@ -28,7 +28,7 @@ def test_exports_every_declared_symbol(cn):
     text = open(HDR).read()
     names = re.findall(r"^\s*(?:int|void|long long)\s+(cuopus_\w+)\s*\(", text, re.M)
     assert len(names) >= 14
-    lib = ctypes.CDLL(cn._find_lib())
+    lib = ctypes.CDLL(cn.library_path())
     for n in names:
         assert hasattr(lib, n), n
     for n in ["cuopus_probe", "cuopus_decode_f16k", "cuopus_decode_f16k_on", "cuopus_decode_f16k_multi",

@@ -112,3 +112,53 @@ def test_lossy_api_matches_oracle(cn):
     got = cn.decode_ogg_lossy(blob, lost)
     torch.cuda.synchronize()
     assert float(np.abs(got.cpu().numpy() - want).max()) <= RS_TOL
+
+
+@pytest.mark.parametrize("name", sorted(ol.GAIN_RECIPES))
+def test_output_gain_matches_libopus(cn, name):
+    """OpusHead output gain applied as libopus OPUS_SET_GAIN: the int16 and
+    f32 16 kHz outputs equal libopus opened with the same gain."""
+    import torch
+    blob, d, chains = _load(name)
+    assert d.gain_q8 != 0
+    data, off, ln = d.blob()
+    ref = ol.decode_chains(data, off, ln, chains, gain=d.gain_q8)
+    got = cn.decode_ogg_to_s16_tensor(blob).cpu().numpy()
+    assert np.array_equal(got, ref["pcm"])
+    out = cn.decode_ogg_to_cuda_tensor(blob)
+    torch.cuda.synchronize()
+    assert float(np.abs(out.cpu().numpy() - ol.resample(ref["pcm"])).max()) <= RS_TOL
+
+
+def test_output_gain_can_be_ignored(cn, monkeypatch):
+    """CUOPUS_OUTPUT_GAIN=0 decodes as the reference's path does (no gain)."""
+    blob, d, chains = _load("gain_up6")
+    data, off, ln = d.blob()
+    monkeypatch.setenv("CUOPUS_OUTPUT_GAIN", "0")
+    got = cn.decode_ogg_to_s16_tensor(blob).cpu().numpy()
+    assert np.array_equal(got, ol.decode_chains(data, off, ln, chains)["pcm"])
+
+
+def test_lost_before_dtx_alone_equals_batched(cn):
+    """A lost packet followed by a TOC-only (DTX) packet: the FEC decision
+    reads no byte past the next packet, so a file decodes the same alone and
+    merged with other files (whose bytes follow it in the merged buffer)."""
+    import torch
+    blob, d, chains = _load("silk_dtx")
+    lens = np.array([len(p) for p in d.packets])
+    dtx = np.nonzero(lens <= 1)[0]
+    assert dtx.size > 0
+    lost = np.zeros(len(d.packets), np.uint8)
+    lost[dtx[dtx > 0] - 1] = 1
+    lost[-1] = 1
+    other, _, _ = _load("celt_hi")
+    alone = cn.decode_ogg_lossy(blob, lost)
+    b = cn.Batch([blob, other], lost_masks=[lost, None], mode=cn.MODE_PERSISTENT)
+    b.run()
+    b.check()
+    torch.cuda.synchronize()
+    assert torch.equal(b.outputs[0], alone)
+    data, off, ln = d.blob()
+    ref = ol.decode_chains(data, off, ln, chains, lost=lost)
+    assert float(np.abs(alone.cpu().numpy() - ol.resample(ref["pcm"])).max()) <= RS_TOL
+    b.close()

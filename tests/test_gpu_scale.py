@@ -5,8 +5,15 @@ the float32 16 kHz outputs are within 1e-6 of the oracle's resampler.
 
 * silk_wb_16k_60s: the reference's CPU-runnable plain case (60 chains).
 * celt_fb_64k_2h: the headline workload (7200 chains, one launch).
-* uneven_250 (or its first 50 files, uneven_50) / hybrid_fec_200k (2% seeded
-  loss, FEC or concealment): run when their inputs are present in corpus/bench.
+* celt_fb_64k_2h through the resident batch exactly as bench.py times it
+  (persistent kernel, the default 4096 wavefronts, cost-model order).
+* uneven_1000 / uneven_250 / uneven_50 (whichever is present): every file.
+* hybrid_fec_200k (2% seeded loss, FEC or concealment): the full 200 files
+  when present; otherwise its first LOSS_SUBSET files (24,000 chains) are
+  generated on the spot by the corpus recipe and checked.
+
+The drop-in entry points (and the gate used here) run the persistent kernel
+with the default resident wavefronts, the shape bench.py measures.
 """
 import glob
 import os
@@ -71,6 +78,29 @@ def test_celt_fb_2h_all_chains(cn):
     assert nch == 7200
 
 
+def test_celt_fb_2h_benchmarked_batch(cn):
+    """The exact benchmarked shape: Batch(mode=persistent) with the library's
+    default slot count, several runs, every f32 output against the oracle."""
+    import torch
+    blob = _bench_file("celt_fb_64k_2h")
+    d, chains = ol.demux_chains(blob)
+    data, off, ln = d.blob()
+    ref = ol.decode_chains(data, off, ln, chains, threads=THREADS)
+    want = ol.resample_threads(ref["pcm"], THREADS)
+    b = cn.Batch([blob], mode=cn.MODE_PERSISTENT)
+    print("case batch persistent default slots, chains", b.chains, flush=True)
+    for rep in range(2):
+        b.outputs[0].fill_(float("nan"))
+        b.run()
+        assert b.check() == [want.size]
+        got = b.outputs[0].cpu().numpy()
+        md = float(np.abs(got - want).max())
+        print("  run %d: f16k max abs diff %.3g" % (rep, md), flush=True)
+        assert md <= 1e-6
+    b.close()
+    torch.cuda.synchronize()
+
+
 def _set_files(name):
     files = sorted(glob.glob(os.path.join(ol.BENCH, name, "f*.opus")))
     if not files:
@@ -79,16 +109,33 @@ def _set_files(name):
 
 
 def test_uneven_set_all_files(cn):
-    name = "uneven_250" if glob.glob(os.path.join(ol.BENCH, "uneven_250", "f*.opus")) else "uneven_50"
+    name = next((n for n in ("uneven_1000", "uneven_250", "uneven_50")
+                 if glob.glob(os.path.join(ol.BENCH, n, "f*.opus"))), "uneven_50")
     total = 0
     for f in _set_files(name):
         total += _check_stream(cn, open(f, "rb").read(), tag=os.path.basename(f))[0]
     print(name, "chains", total)
 
 
-def test_hybrid_fec_loss_set(cn):
+LOSS_SUBSET = 24
+
+
+@pytest.fixture(scope="module")
+def loss_files():
+    """The config-5 set: all 200 files when generated, else its first
+    LOSS_SUBSET files (identical to the full set's: same per-file seeds),
+    generated here by the corpus recipe."""
+    files = sorted(glob.glob(os.path.join(ol.BENCH, "hybrid_fec_200k", "f*.opus")))
+    if len(files) >= 200:
+        return files
+    if not os.path.exists(ol.MKCORPUS):
+        pytest.skip("corpus generator not built")
+    return ol.bench_set("hybrid_fec_200k", jobs=THREADS, limit=LOSS_SUBSET)
+
+
+def test_hybrid_fec_loss_set(cn, loss_files):
     total = lost_total = 0
-    for i, f in enumerate(_set_files("hybrid_fec_200k")):
+    for i, f in enumerate(loss_files):
         blob = open(f, "rb").read()
         d, _ = ol.demux_chains(blob)
         lost = ol.loss_mask(len(d.packets), 0.02, seed=0x0C0FFEE5 + i)

@@ -121,3 +121,53 @@ def test_submit_errors(cn):
         srv.submit(b"not an ogg stream")
     assert e.value.rc == 1
     srv.close()
+
+
+def test_submit_orders_after_callers_stream(cn):
+    """Requests submitted while the caller's stream still has queued work on
+    recycled memory: the caching allocator hands the freed block of a
+    tensor that a queued kernel still writes to the next request's output;
+    the server's launch waits for the caller's stream, so the request's
+    output is what it is alone."""
+    import torch
+    blobs = _blobs()
+    ref = {n: cn.decode_ogg_to_cuda_tensor(b) for n, b in blobs.items()}
+    srv = cn.Server(n_streams=4)
+    ids = {}
+    for rep in range(3):
+        for name, blob in blobs.items():
+            n16 = cn.probe(blob)
+            junk = torch.empty(n16, dtype=torch.float32, device="cuda")
+            for _ in range(4):
+                junk.fill_(float("nan"))  # queued work on this block, on the caller's stream
+            del junk                       # the next allocation of this size reuses the block
+            ids[srv.submit(blob)] = name
+    got = _drain(srv, len(ids))
+    for rid, (out, rc) in got.items():
+        assert rc == 0
+        assert torch.equal(out, ref[ids[rid]]), ids[rid]
+    srv.close()
+
+
+def test_held_memory_within_budget(cn):
+    """The server never holds more device memory than its budget (all launch
+    slots' buffers, decoder heaps included)."""
+    import torch
+    blobs = _blobs()
+    budget = 48 << 20
+    srv = cn.Server(n_streams=3, mem_budget=budget)
+    ids = {}
+    for name, blob in list(blobs.items()) * 3:
+        ids[srv.submit(blob)] = name
+    got = {}
+    t0 = time.time()
+    while len(got) < len(ids):
+        for rid, out, rc in srv.poll():
+            got[rid] = (out, rc)
+        assert srv.stats()["held_bytes"] <= budget
+        assert time.time() - t0 < 60.0, ("server did not finish", srv.stats())
+        time.sleep(0.0005)
+    for rid, (out, rc) in got.items():
+        assert rc == 0
+    srv.close()
+    torch.cuda.synchronize()

@@ -110,3 +110,32 @@ def test_range_decoder_schedule(hg):
         ol.lib().orc_ec_schedule(b.ctypes.data, b.size, a.ctypes.data)
         hg.hg_ec_schedule(b.ctypes.data, b.size, c.ctypes.data)
         assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("name", sorted(ol.GAIN_RECIPES))
+def test_hostgate_output_gain(hg, name):
+    """OpusHead output gain through the decoder's OPUS_SET_GAIN path: every
+    sample equal to libopus with the same gain (gain changes neither sample
+    counts nor final ranges)."""
+    hg.hg_decode_chain_gain.restype = ctypes.c_long
+    hg.hg_decode_chain_gain.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_long, P, P, ctypes.c_int]
+    blob = open(ol.small_corpus(name, ol.recipe(name)), "rb").read()
+    d, chains = ol.demux_chains(blob)
+    assert d.gain_q8 != 0
+    data, off, ln = d.blob()
+    ref = ol.decode_chains(data, off, ln, chains, gain=d.gain_q8)
+    plain = ol.decode_chains(data, off, ln, chains)
+    assert not np.array_equal(ref["pcm"], plain["pcm"])
+    pcm_all = []
+    for (first, npk, trim, keep) in chains:
+        cap = 5760 * (npk + 1)
+        pcm = np.zeros(cap, np.int16)
+        o = np.ascontiguousarray(off[first:first + npk])
+        n = np.ascontiguousarray(ln[first:first + npk])
+        r = hg.hg_decode_chain_gain(data.ctypes.data, o.ctypes.data, n.ctypes.data, None, npk, pcm.ctypes.data,
+                                    cap, None, None, d.gain_q8)
+        assert r >= 0
+        pcm_all.append(pcm[trim:trim + keep])
+    got = np.concatenate(pcm_all)
+    assert np.array_equal(got, ref["pcm"])
+    assert np.array_equal(ref["nsamp"], plain["nsamp"]) and np.array_equal(ref["rng"], plain["rng"])
