@@ -448,7 +448,70 @@ int bufs_from_workspace(const CallNeed &n, CallBufs *b, Err &e)
     return 0;
 }
 
+// ---- the allocation-free entry point's device memory
+// A file-scope device array, one copy per GPU, live until the process ends.
+// Host code never passes `scratch` itself to a kernel or a copy: its name is
+// a host-side stand-in, not a device address.  scratch_base() looks up each
+// device's address once (hipGetSymbolAddress with that device current) and
+// every use goes through that pointer.
+constexpr size_t kScratchBytes = 128ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
 constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+int scratch_base(uint8_t **out, Err &e)
+{
+    int dev = -1;
+    HIPCHK(e, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return e.set(CUOPUS_RC_ARG, "device %d out of range", dev);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t s = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (s != hipSuccess || !p)
+            return e.set(CUOPUS_RC_GPU, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(s));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (uint8_t *)g_scratch_ptr[dev];
+    return 0;
+}
+
+// Bytes a call needs in the static scratch (each region 256-byte aligned).
+size_t scratch_bytes(const CallNeed &n)
+{
+    auto a = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    return a(n.data) + 2 * a(4 * n.npk) + a(n.lost) + a(sizeof(ChainDesc) * n.nch) + a(4 * n.nch) +
+           a(sizeof(ChainResult) * n.nch) + a(2 * n.cat) + a(sizeof(RsFile) * n.nfiles) + a(256) +
+           a(sizeof(DecHeap) * n.nheaps);
+}
+
+int bufs_from_scratch(const CallNeed &n, CallBufs *b, Err &e)
+{
+    const size_t need = scratch_bytes(n);
+    if (need > kScratchBytes)
+        return e.set(CUOPUS_RC_ARG, "call needs %zu bytes of device scratch, the allocation-free path has %zu", need,
+                     kScratchBytes);
+    uint8_t *base;
+    int rc = scratch_base(&base, e);
+    if (rc) return rc;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        uint8_t *p = base + o;
+        o += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    b->data = take(n.data);
+    b->off = (uint32_t *)take(4 * n.npk);
+    b->len = (uint32_t *)take(4 * n.npk);
+    b->lost = n.lost ? take(n.lost) : nullptr;
+    b->chains = (ChainDesc *)take(sizeof(ChainDesc) * n.nch);
+    b->order = (uint32_t *)take(4 * n.nch);
+    b->res = (ChainResult *)take(sizeof(ChainResult) * n.nch);
+    b->cat = n.cat ? (int16_t *)take(2 * n.cat) : nullptr;
+    b->files = n.nfiles ? (RsFile *)take(sizeof(RsFile) * n.nfiles) : nullptr;
+    b->queue = (uint32_t *)take(256);
+    b->heap = (DecHeap *)take(sizeof(DecHeap) * n.nheaps);
+    return 0;
+}
 
 // Resident wavefronts of the persistent decoder when the caller does not
 // choose: CUOPUS_SLOTS, else the tuned value for this device, else every
@@ -468,6 +531,7 @@ struct RunOpts {
     uint32_t *pkt_rng = nullptr;
     bool persistent = true;              // persistent work-queue kernel (else one workgroup per chain)
     uint32_t n_slots = 0;                // persistent wavefronts (0: default_slots())
+    bool noalloc = false;                // device memory from the static scratch only
 };
 
 // Kernel selection of the drop-in entry points: the persistent work-queue
@@ -485,6 +549,7 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
 {
     const uint32_t nch = (uint32_t)pl.chains.size();
     const uint64_t npk = pl.off.size();
+    int rc;
     std::vector<RsFile> rf;
     uint64_t tiles = 0;
     const int rsv = resample_variant();
@@ -501,7 +566,7 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
             rf.push_back(r);
         }
     }
-    const uint32_t n_waves = std::max<uint32_t>(1, std::min<uint32_t>(o.n_slots ? o.n_slots : default_slots(), nch));
+    uint32_t n_waves = std::max<uint32_t>(1, std::min<uint32_t>(o.n_slots ? o.n_slots : default_slots(), nch));
     CallNeed need;
     need.data = pl.data.size();
     need.npk = npk;
@@ -511,7 +576,14 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     need.nfiles = rf.size();
     need.nheaps = o.persistent ? n_waves : nch;
     CallBufs b;
-    int rc = bufs_from_workspace(need, &b, e);
+    if (o.noalloc) {
+        // fewer resident wavefronts when their heaps would not fit (>= 64)
+        while (need.nheaps > 64 && scratch_bytes(need) > kScratchBytes) need.nheaps = std::max<size_t>(64, need.nheaps / 2);
+        n_waves = (uint32_t)need.nheaps;
+        rc = bufs_from_scratch(need, &b, e);
+    } else {
+        rc = bufs_from_workspace(need, &b, e);
+    }
     if (rc) return rc;
     int16_t *out = o.s16_out ? o.s16_out : b.cat;
 
@@ -624,7 +696,7 @@ extern "C" void cuopus_file_close(cuopus_file *f) { delete f; }
 namespace {
 
 int files_f16k(cuopus_file *const *fs, int nfiles, const uint8_t *lost, size_t mask_n, float *const *out_devs,
-               const size_t *out_caps, int64_t *out_lens, void *stream, Err &e)
+               const size_t *out_caps, int64_t *out_lens, void *stream, bool noalloc, Err &e)
 {
     last_ovf = 0;
     if (out_lens)
@@ -634,6 +706,7 @@ int files_f16k(cuopus_file *const *fs, int nfiles, const uint8_t *lost, size_t m
     Plan pl;
     RunOpts o;
     o.persistent = use_persistent();
+    o.noalloc = noalloc;
     pl.files.resize((size_t)nfiles);
     for (int f = 0; f < nfiles; f++) {
         if (!fs[f]) return e.set(CUOPUS_RC_ARG, "file %d is NULL", f);
@@ -658,7 +731,7 @@ int files_f16k(cuopus_file *const *fs, int nfiles, const uint8_t *lost, size_t m
 
 int files_f16k_from_ogg(const uint8_t *const *oggs, const size_t *ns, int nfiles, const uint8_t *lost,
                         size_t mask_n, float *const *out_devs, const size_t *out_caps, int64_t *out_lens,
-                        void *stream, Err &e)
+                        void *stream, bool noalloc, Err &e)
 {
     if (out_lens)
         for (int f = 0; f < nfiles; f++) out_lens[f] = 0;
@@ -673,7 +746,7 @@ int files_f16k_from_ogg(const uint8_t *const *oggs, const size_t *ns, int nfiles
         if (rc) return nfiles > 1 ? e.set(rc, "file %d: %s", f, sub) : e.set(rc, "%s", sub);
         fs[f] = &files[f];
     }
-    return files_f16k(fs.data(), nfiles, lost, mask_n, out_devs, out_caps, out_lens, stream, e);
+    return files_f16k(fs.data(), nfiles, lost, mask_n, out_devs, out_caps, out_lens, stream, noalloc, e);
 }
 
 }  // namespace
@@ -683,7 +756,7 @@ extern "C" int cuopus_file_decode_f16k_on(cuopus_file *f, float *out_dev, size_t
 {
     Err e(errbuf, errcap);
     if (!f) return e.set(CUOPUS_RC_ARG, "file is NULL");
-    return files_f16k(&f, 1, nullptr, 0, &out_dev, &out_cap, out_len, stream, e);
+    return files_f16k(&f, 1, nullptr, 0, &out_dev, &out_cap, out_len, stream, false, e);
 }
 
 extern "C" int cuopus_files_decode_f16k_multi(cuopus_file *const *fs, int nfiles, float *const *out_devs,
@@ -691,7 +764,7 @@ extern "C" int cuopus_files_decode_f16k_multi(cuopus_file *const *fs, int nfiles
                                               char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);
-    return files_f16k(fs, nfiles, nullptr, 0, out_devs, out_caps, out_lens, stream, e);
+    return files_f16k(fs, nfiles, nullptr, 0, out_devs, out_caps, out_lens, stream, false, e);
 }
 
 // =====================================================================================
@@ -699,8 +772,11 @@ extern "C" int cuopus_last_arena_overflow(void) { return last_ovf; }
 
 extern "C" long long cuopus_workspace_bytes(void) { return g_dev_bytes; }
 
+static void release_mel_tables();
+
 extern "C" void cuopus_release_workspaces(void)
 {
+    release_mel_tables();
     DevBuf *all[] = {&W.data, &W.off, &W.len, &W.lost, &W.chains, &W.order, &W.heap,
                      &W.cat, &W.res, &W.nsamp, &W.rng, &W.files, &W.queue};
     for (DevBuf *b : all) {
@@ -804,7 +880,7 @@ extern "C" int cuopus_decode_f16k_on(const uint8_t *ogg, size_t n, float *out_de
                                      void *stream, char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);
-    return files_f16k_from_ogg(&ogg, &n, 1, nullptr, 0, &out_dev, &out_cap, out_len, stream, e);
+    return files_f16k_from_ogg(&ogg, &n, 1, nullptr, 0, &out_dev, &out_cap, out_len, stream, false, e);
 }
 
 extern "C" int cuopus_decode_f16k(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap, int64_t *out_len,
@@ -813,13 +889,20 @@ extern "C" int cuopus_decode_f16k(const uint8_t *ogg, size_t n, float *out_dev, 
     return cuopus_decode_f16k_on(ogg, n, out_dev, out_cap, out_len, nullptr, errbuf, errcap);
 }
 
+extern "C" int cuopus_decode_f16k_noalloc_on(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap,
+                                             int64_t *out_len, void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    return files_f16k_from_ogg(&ogg, &n, 1, nullptr, 0, &out_dev, &out_cap, out_len, stream, true, e);
+}
+
 extern "C" int cuopus_decode_lossy_f16k_on(const uint8_t *ogg, size_t n, const uint8_t *lost, size_t mask_n,
                                            float *out_dev, size_t out_cap, int64_t *out_len, void *stream,
                                            char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);
     if (!lost) return e.set(CUOPUS_RC_ARG, "lost mask is NULL");
-    return files_f16k_from_ogg(&ogg, &n, 1, lost, mask_n, &out_dev, &out_cap, out_len, stream, e);
+    return files_f16k_from_ogg(&ogg, &n, 1, lost, mask_n, &out_dev, &out_cap, out_len, stream, false, e);
 }
 
 extern "C" int cuopus_decode_s16_on(const uint8_t *ogg, size_t n, int16_t *out_dev, size_t out_cap, int64_t *out_len,
@@ -850,7 +933,7 @@ extern "C" int cuopus_decode_f16k_multi(const uint8_t *const *oggs, const size_t
                                         void *stream, char *errbuf, size_t errcap)
 {
     Err e(errbuf, errcap);
-    return files_f16k_from_ogg(oggs, ns, nfiles, nullptr, 0, out_devs, out_caps, out_lens, stream, e);
+    return files_f16k_from_ogg(oggs, ns, nfiles, nullptr, 0, out_devs, out_caps, out_lens, stream, false, e);
 }
 
 extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, const uint32_t *len,
@@ -1022,6 +1105,19 @@ struct MelCache {
 MelCache g_mel[kMaxDevices];
 
 }  // namespace
+
+static void release_mel_tables()
+{
+    for (MelCache &m : g_mel) {
+        if (m.tab.p) {
+            hipFree(m.tab.p);
+            g_dev_bytes -= (long long)m.tab.cap;
+        }
+        m.tab.p = nullptr;
+        m.tab.cap = 0;
+        m.n_mels = 0;
+    }
+}
 
 extern "C" int cuopus_mel_filters(int n_mels, float *out, size_t cap)
 {
@@ -1858,5 +1954,7 @@ extern "C" int cuopus_warmup(char *errbuf, size_t errcap)
     launch_chains(1, 0, a);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipDeviceSynchronize());
-    return 0;
+    // the allocation-free path's device address on this device
+    uint8_t *base;
+    return scratch_base(&base, e);
 }

@@ -16,7 +16,7 @@ decoded.
 
 Additions: ``decode_ogg_to_s16_tensor`` (playable int16 48 kHz),
 ``decode_ogg_lossy`` (per-packet loss mask with in-band FEC),
-``log_mel``
+``decode_ogg_noalloc`` (the allocation-free entry point), ``log_mel``
 (log-mel features of a resident f32 16 kHz tensor), ``packet_count``,
 ``chain_table``, ``packets``, ``demux_stats``, the CUOP/CUC2 dump writers,
 ``gate_chains`` (per-packet sample counts and final ranges), ``Batch``
@@ -68,6 +68,7 @@ _SIGS = {
     "cuopus_probe_packets": (I32, [BUF, SZ, P_I64, P_I64], True),
     "cuopus_decode_f16k": (I32, [BUF, SZ, VP, SZ, P_I64], True),
     "cuopus_decode_f16k_on": (I32, [BUF, SZ, VP, SZ, P_I64, VP], True),
+    "cuopus_decode_f16k_noalloc_on": (I32, [BUF, SZ, VP, SZ, P_I64, VP], True),
     "cuopus_decode_s16_on": (I32, [BUF, SZ, VP, SZ, P_I64, VP], True),
     "cuopus_decode_lossy_f16k_on": (I32, [BUF, SZ, VP, SZ, VP, SZ, P_I64, VP], True),
     "cuopus_decode_f16k_multi": (I32, [_c.POINTER(BUF), _c.POINTER(SZ), I32, _c.POINTER(VP), _c.POINTER(SZ),
@@ -268,6 +269,21 @@ def release_workspaces():
 
 
 # ------------------------------------------------------------------ additions
+def decode_ogg_noalloc(path_or_bytes, out=None):
+    """cuopus_decode_f16k_noalloc_on: the same decode with every device
+    buffer in the library's static scratch (no device allocation by the
+    library).  `out` may be a preallocated float32 tensor of at least
+    probe() samples."""
+    data = _bytes_of(path_or_bytes)
+    n16 = probe(data)
+    if out is None:
+        out = _f32_cuda(n16)
+    wrote = I64(0)
+    _native()("cuopus_decode_f16k_noalloc_on", data, len(data), out.data_ptr() if n16 else None, out.numel(),
+              _c.byref(wrote), _cur_stream())
+    return out[:wrote.value]
+
+
 def packet_count(path_or_bytes):
     """(audio packets, chains) of a stream (host only)."""
     with _Parsed(path_or_bytes) as f:

@@ -252,6 +252,22 @@ int cuopus_server_poll(cuopus_server *srv, uint64_t *ids, int *rcs, int cap, int
 int cuopus_server_stats(const cuopus_server *srv, int64_t *s4);
 void cuopus_server_destroy(cuopus_server *srv);
 
+/* ---------------------------------------------------------------------------
+ * Allocation-free decode.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  As cuopus_decode_f16k_on (same
+ * kernels, same output), but every device buffer of the call (packets,
+ * tables, decoder heaps, int16 PCM) lives in one static device array of
+ * 128 MiB per GPU, reserved when the library loads and kept until the process
+ * ends; the call makes no hipMalloc/hipFree and no allocating library call.
+ * A stream whose buffers exceed it returns CUOPUS_RC_ARG with the byte count
+ * (about 10 minutes of 64 kbps audio fit: 2 B per playable 48 kHz sample plus
+ * the packets, 64 B per chain and 38 KB per resident wavefront, 64-4096 of
+ * them).  Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU (two host threads, or two streams without ordering) race. */
+int cuopus_decode_f16k_noalloc_on(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap, int64_t *out_len,
+                                  void *stream, char *errbuf, size_t errcap);
+
 #ifdef __cplusplus
 }
 #endif
