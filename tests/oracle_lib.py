@@ -21,14 +21,15 @@ REF = os.path.join(ROOT, "oracle", "_ref")
 MKCORPUS = os.path.join(REF, "mkcorpus")
 CACHE = os.path.join(ROOT, "corpus", "cache")
 
-_lib = None
+_libs = {}
 P = ctypes.c_void_p
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        path = os.path.join(REF, "liboracle.so")
+def lib(fast=False):
+    """The oracle library; fast=True: the same code over the -O3 intrinsics
+    build of libopus (the timed CPU baseline; bit-identity is a CPU test)."""
+    if fast not in _libs:
+        path = os.path.join(REF, "liboracle_fast.so" if fast else "liboracle.so")
         if not os.path.exists(path):
             raise OSError("oracle not built: %s (run __graft_entry__.build())" % path)
         L = ctypes.CDLL(path)
@@ -51,8 +52,8 @@ def lib():
         L.orc_packet_has_lbrr.argtypes = [P, ctypes.c_int]
         L.orc_packet_nb_samples.restype = ctypes.c_int
         L.orc_packet_nb_samples.argtypes = [P, ctypes.c_int]
-        _lib = L
-    return _lib
+        _libs[fast] = L
+    return _libs[fast]
 
 
 def small_corpus(name, args):
@@ -71,7 +72,7 @@ def demux_chains(blob, seconds=1.0, preroll_ms=250.0):
     return d, odemux.build_chains(d, seconds, preroll_ms)
 
 
-def decode_chains(data, off, ln, chains, lost=None, threads=1, caps=None, channels=1, gain=0):
+def decode_chains(data, off, ln, chains, lost=None, threads=1, caps=None, channels=1, gain=0, fast=False):
     """Decode every chain with a fresh fixed-point libopus decoder of
     `channels` output channels and OPUS_SET_GAIN(gain) (Q8 dB; an int for
     every chain or one per chain).
@@ -80,7 +81,7 @@ def decode_chains(data, off, ln, chains, lost=None, threads=1, caps=None, channe
     packet + 5760).  Returns dict: pcm (compacted keep windows, int16,
     interleaved for 2 channels), nsamp/rng per packet, ret per chain, full
     (list of per-chain full PCM arrays)."""
-    L = lib()
+    L = lib(fast)
     ch = np.asarray(chains, dtype=np.int64).reshape(-1, 4)
     n = ch.shape[0]
     first = np.ascontiguousarray(ch[:, 0], dtype=np.uint32)

@@ -156,3 +156,25 @@ def test_threaded_resampler_matches_serial():
         a = ol.resample(x)
         b = ol.resample_threads(x, 4, chunk=1000)
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["silk_wb_16k_60s", "silk_wb", "celt_fb", "hybrid_fec", "mixed", "celt_multi",
+                                  "silk_dtx"])
+def test_fast_cpu_build_bit_identical(name):
+    """The CPU baseline's libopus build (-O3, SSE4.1/AVX2 intrinsics) decodes
+    every chain bit-identically to the oracle build (intrinsics off), with and
+    without loss, so its timing is of the same computation."""
+    if not os.path.exists(os.path.join(ol.REF, "liboracle_fast.so")):
+        pytest.skip("fast CPU build not present")
+    if name in ol.BENCH_RECIPES:
+        blob = open(ol.bench_corpus(name), "rb").read()
+    else:
+        blob = open(ol.small_corpus(name, ol.recipe(name)), "rb").read()
+    d, chains = ol.demux_chains(blob)
+    data, off, ln = d.blob()
+    for lost in (None, ol.loss_mask(len(off), 0.05, seed=7)):
+        a = ol.decode_chains(data, off, ln, chains, lost=lost)
+        b = ol.decode_chains(data, off, ln, chains, lost=lost, fast=True)
+        assert np.array_equal(a["ret"], b["ret"])
+        assert np.array_equal(a["nsamp"], b["nsamp"]) and np.array_equal(a["rng"], b["rng"])
+        assert np.array_equal(a["pcm"], b["pcm"])
