@@ -44,6 +44,18 @@ def build_lib(arch="gfx950", out=None, defines=()):
     return out
 
 
+def build_cli(out=None):
+    """cuopus_amd/cuopus_decode: the command-line front-end (host code over
+    the C ABI, linked to libcuopus.so beside it)."""
+    out = out or os.path.join(PKG, "cuopus_decode")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "cli", "cuopus_decode.cpp"), "-L" + PKG, "-lcuopus",
+           "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_hostgate():
     src = os.path.join(ROOT, "tests", "hostgate", "hostgate.cpp")
     out = os.path.join(ROOT, "tests", "hostgate", "libhostgate.so")
@@ -67,4 +79,5 @@ def build_oracle(jobs=8):
 def build_all():
     build_oracle()
     build_lib()
+    build_cli()
     build_hostgate()
