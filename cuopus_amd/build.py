@@ -66,6 +66,25 @@ def build_hostgate():
     return out
 
 
+def build_fuzz_gate():
+    """tests/hostgate/fuzz_gate: the decoder source on the CPU under
+    AddressSanitizer + UndefinedBehaviorSanitizer (host code only; the
+    sanitizer flags go to the host compilation), linked to the oracle."""
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    if not os.path.exists(os.path.join(ref, "liboracle.so")):
+        return None
+    src = os.path.join(ROOT, "tests", "hostgate", "fuzz_gate.cpp")
+    out = os.path.join(ROOT, "tests", "hostgate", "fuzz_gate")
+    cmd = [HIPCC, "-x", "hip", "--offload-host-only", "-O1", "-g", "-std=c++17", "-Wno-unused-function",
+           "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer",
+           "-I" + os.path.join(ROOT, "tests", "hostgate"), src, "-L" + ref, "-loracle", "-Wl,-rpath," + ref,
+           "-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle(jobs=8):
     """Compile the oracle when the reference tree is present; otherwise keep
     whatever prebuilt oracle/_ref is there (the GPU box has no reference)."""
@@ -81,3 +100,4 @@ def build_all():
     build_lib()
     build_cli()
     build_hostgate()
+    build_fuzz_gate()

@@ -113,7 +113,16 @@ def decode_chains(data, off, ln, chains, lost=None, threads=1, caps=None, channe
     C = channels
     for i in range(n):
         r = int(ret[i])
-        seg = pcm[C * int(pcm_off[i]): C * (int(pcm_off[i]) + max(r, 0))]
+        if r < 0:
+            # a failed chain keeps what its packets produced before the error (a
+            # decoded packet always yields >= 120 samples; records stop there)
+            f0, np_ = int(first[i]), int(npk[i])
+            r = 0
+            for k in range(np_):
+                if nsamp[f0 + k] <= 0:
+                    break
+                r += int(nsamp[f0 + k])
+        seg = pcm[C * int(pcm_off[i]): C * (int(pcm_off[i]) + r)]
         full.append(seg)
         trim, keep = int(ch[i, 2]), int(ch[i, 3])
         kept.append(seg[C * trim: C * (trim + keep)])

@@ -1,0 +1,66 @@
+"""Malformed input: random, mis-framed, truncated and bit-flipped packets
+(tests/malformed.py, after libopus's test_opus_decode.c:295-367) decoded
+chain by chain, each chain from a fresh decoder, against the oracle.
+
+CPU: the decoder source compiled for the host under AddressSanitizer and
+UndefinedBehaviorSanitizer (tests/hostgate/fuzz_gate) -- no memory error,
+no undefined behaviour, and every return code, final range and sample up to
+a chain's first error equal to libopus.  The lane-parallel device versions
+of the SILK pulse/parameter/synthesis stages and the CELT codeword decode
+are not part of this build (they compile for the GPU only); the GPU test
+covers them.
+
+GPU: the same sets through cuopus_gate_chains: per-packet sample counts and
+final ranges, per-chain stop codes and the PCM each chain produced before
+its first error, all equal to the oracle.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import malformed as mf
+from tests import oracle_lib as ol
+
+FUZZ = os.path.join(ol.ROOT, "tests", "hostgate", "fuzz_gate")
+CASES = [(name, lossy) for name in sorted(mf.SETS) for lossy in (False, True)]
+
+
+def _set(name, lossy):
+    data, off, ln, chains = mf.SETS[name]()
+    lost = mf.loss_for(off) if lossy else None
+    return data, off, ln, lost, chains
+
+
+@pytest.mark.parametrize("name,lossy", CASES)
+def test_sanitized_host_gate_matches_libopus(name, lossy, tmp_path):
+    if not os.path.exists(FUZZ):
+        pytest.skip("sanitizer build of the host gate not present")
+    data, off, ln, lost, chains = _set(name, lossy)
+    path = tmp_path / "set.bin"
+    mf.write_set(str(path), data, off, ln, lost, chains)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run([FUZZ, str(path)], capture_output=True, text=True, timeout=600, env=env)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-4000:])
+    assert "ERROR: AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,lossy", CASES)
+def test_gpu_gate_matches_libopus(name, lossy):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from cuopus_amd import cuopus_native as cn
+    data, off, ln, lost, chains = _set(name, lossy)
+    chains, ref = mf.oracle_keep(data, off, ln, chains, lost)
+    print("case malformed", name, "lossy" if lossy else "", "chains", len(chains), flush=True)
+    pcm, nsamp, rng, stop, rc = cn.gate_chains(data, off, ln, chains, lost=lost)
+    assert rc in (0, 1)
+    want_stop = np.minimum(ref["ret"], 0)
+    assert np.array_equal(stop, want_stop), np.nonzero(stop != want_stop)[0][:10]
+    assert np.array_equal(nsamp, ref["nsamp"]), np.nonzero(nsamp != ref["nsamp"])[0][:10]
+    assert np.array_equal(rng, ref["rng"]), np.nonzero(rng != ref["rng"])[0][:10]
+    assert np.array_equal(pcm, ref["pcm"]), np.nonzero(pcm != ref["pcm"])[0][:10]
