@@ -79,6 +79,12 @@ WORKLOADS = {
                 "packets lost by a seeded mask per file, decoded from the next packet's LBRR when present, "
                 "concealed otherwise; all chains in one persistent kernel, f32 16 kHz out",
     },
+    "uneven_1000": {
+        "glob": "corpus/bench/uneven_1000/f*.opus",
+        "desc": "1000 mono speech files, durations log-uniform 5 s-30 min, mode per file from SILK-NB/WB, "
+                "hybrid-SWB/FB, CELT-FB, bitrate redrawn every 1 s in 12-64 kbps (mode switches, redundancy), "
+                "20 ms frames; all chains in one persistent kernel vs one launch per file",
+    },
     "uneven_250": {
         "glob": "corpus/bench/uneven_250/f*.opus",
         "desc": "250 mono speech files, durations log-uniform 5 s-30 min (18.1 h), mode per file from "
@@ -100,6 +106,8 @@ def _args():
     ap.add_argument("--graph", action="store_true", help="replay the launch sequence as a HIP graph")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner on this workload first and use its best launch shape")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="fit the work-order cost model's per-frame costs from per-chain times first")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-file comparison")
     ap.add_argument("--no-batching", action="store_true",
@@ -461,14 +469,18 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     modes = {"persistent": cn.MODE_PERSISTENT, "one": cn.MODE_ONE_LAUNCH, "plain": cn.MODE_PLAIN}
-    mode, slots, lpt = modes[a.mode], a.slots, True
+    mode, slots, lpt = modes[a.mode], a.slots, 3  # 3: the library's (tuned) default order
     tuned = None
     if a.tune:
         from cuopus_amd import autotune
         tuned = autotune.tune(blobs, verbose=True)
         mode, slots, lpt = tuned["mode"], tuned["n_slots"], tuned["lpt_order"]
         a.mode = {v: k for k, v in modes.items()}[mode]
-        _log("tuned: %s" % {k: tuned[k] for k in ("mode", "n_slots", "lpt_order", "ms")})
+        _log("tuned: %s" % {k: tuned[k] for k in ("mode", "n_slots", "order", "rs_variant", "ms")})
+    if a.calibrate:
+        from cuopus_amd import autotune
+        cal = autotune.calibrate(blobs, verbose=True)
+        _log("calibration: %s" % json.dumps(cal))
     if "loss" in w:
         # the seeded loss mask of each file, as the parity test draws it
         from tests import oracle_lib as ol
@@ -480,6 +492,15 @@ def main():
     _log("batch ready: %d files, %d chains, %.0f audio-s" % (len(blobs), b.chains, b.playable / 48000.0))
     wall, dec_ms, rs_ms = _time_batch(b, a.steps, a.warmup, torch)
     _log("%s: %.1f ms/step" % (a.mode, 1000.0 * wall / a.steps))
+    work = None
+    if a.mode != "plain":
+        import numpy as np
+        s0, s1 = b.chain_times()
+        ends = np.sort(s1)
+        work = {"span_ms": round(float(ends[-1] - s0.min()) / 1e5, 3),
+                "last_1pct_of_chains_ms": round(float(ends[-1] - ends[int(0.99 * (ends.size - 1))]) / 1e5, 3),
+                "chain_ms_mean": round(float((s1 - s0).mean()) / 1e5, 4),
+                "chain_ms_max": round(float((s1 - s0).max()) / 1e5, 4)}
     audio_s = b.playable / 48000.0
     ms = 1000.0 * wall / a.steps
     value = audio_s * a.steps / wall
@@ -554,6 +575,7 @@ def main():
         "config": {"workload": a.workload, "desc": w["desc"], "mode": a.mode, "graph": a.graph,
                    "n_slots": slots or "default", "tuned": tuned is not None,
                    "files": len(blobs), "audio_s_per_step": round(audio_s, 3)},
+        "work_distribution": work,
         "plain_value": plain,
         "batched_vs_plain": round(value / plain, 3) if plain else None,
         "roofline": roof,

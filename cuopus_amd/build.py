@@ -37,7 +37,7 @@ def build_lib(arch="gfx950", out=None, defines=()):
     tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + arch] + HIP_FLAGS + ["-D" + d for d in defines] + [
         "-shared", "-I" + CSRC,
-        os.path.join(CSRC, "libcuopus.hip"), os.path.join(CSRC, "host", "demux.cpp"),
+        os.path.join(CSRC, "libcuopus.hip"), os.path.join(CSRC, "stereo.hip"), os.path.join(CSRC, "host", "demux.cpp"),
         "-o", tmp]
     _run(cmd)
     os.replace(tmp, out)
@@ -57,13 +57,18 @@ def build_cli(out=None):
 
 
 def build_hostgate():
+    """tests/hostgate/libhostgate.so (mono build) and libhostgate_st.so (the
+    stereo-capable build, CUO_MAXCH=2)."""
     src = os.path.join(ROOT, "tests", "hostgate", "hostgate.cpp")
-    out = os.path.join(ROOT, "tests", "hostgate", "libhostgate.so")
-    cmd = [HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-fPIC", "-shared",
-           "-Wno-unused-function", src, "-o", out + ".tmp"]
-    _run(cmd)
-    os.replace(out + ".tmp", out)
-    return out
+    outs = []
+    for name, defs in (("libhostgate.so", []), ("libhostgate_st.so", ["-DCUO_MAXCH=2", "-DCUO_NS=cuo_st"])):
+        out = os.path.join(ROOT, "tests", "hostgate", name)
+        cmd = [HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-fPIC", "-shared",
+               "-Wno-unused-function"] + defs + [src, "-o", out + ".tmp"]
+        _run(cmd)
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 def build_fuzz_gate():

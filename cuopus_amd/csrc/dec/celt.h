@@ -17,7 +17,7 @@
 #include "entdec.h"
 #include "tables_gen.h"
 
-namespace cuo {
+namespace CUO_NS {
 
 constexpr int kDecBuf = 2048;
 constexpr int kOverlap = 120;
@@ -1912,4 +1912,4 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     return frame_size;
 }
 
-}  // namespace cuo
+}  // namespace CUO_NS

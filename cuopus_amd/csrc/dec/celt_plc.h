@@ -10,7 +10,7 @@
 #pragma once
 #include "celt.h"
 
-namespace cuo {
+namespace CUO_NS {
 
 constexpr int kPlcLagMax = 720;
 constexpr int kPlcLagMin = 100;
@@ -403,4 +403,4 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc)
     return 0;
 }
 
-}  // namespace cuo
+}  // namespace CUO_NS

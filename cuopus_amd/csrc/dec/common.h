@@ -20,7 +20,22 @@
 #define CUO_NOINLINE __host__ __device__ __attribute__((noinline))
 #define CUO_FI __host__ __device__ inline __attribute__((always_inline))
 
-namespace cuo {
+// The decoder compiles twice: a mono build (CUO_MAXCH 1, namespace cuo: the
+// drop-in path's kernels, four wavefronts per SIMD) and a stereo-capable
+// build (CUO_MAXCH 2, namespace cuo_st, stereo.hip: two coded channels, one
+// or two output channels, more LDS per wavefront).  Channel loops run to a
+// runtime count bounded by kMaxCh, so the mono build compiles them away.
+#ifndef CUO_MAXCH
+#define CUO_MAXCH 1
+#endif
+#ifndef CUO_NS
+#define CUO_NS cuo
+#endif
+
+namespace CUO_NS {
+
+constexpr int kMaxCh = CUO_MAXCH;
+static_assert(kMaxCh == 1 || kMaxCh == 2, "one or two channels");
 
 // Wave-uniform LDS pointer.  All decoder state, scratch and frame output of a
 // wavefront live in one LDS block (g_wave_lds, one 64-lane wavefront per
@@ -30,7 +45,7 @@ namespace cuo {
 // is the same in every lane: accesses become LDS instructions and the values
 // loaded are uniform (scalar registers, scalar ALU).  The host build
 // (same-source CPU gate) uses the pointer as it is.
-constexpr size_t kWaveLdsBytes = 10 * 1024;
+constexpr size_t kWaveLdsBytes = kMaxCh == 1 ? 10 * 1024 : 20 * 1024;
 __shared__ __attribute__((aligned(16))) char g_wave_lds[kWaveLdsBytes];
 #if defined(__HIP_DEVICE_COMPILE__)
 
@@ -631,4 +646,4 @@ CUO_FN int32_t silk_inverse32_varq(int32_t b32, int Qres)
     return 0;
 }
 
-}  // namespace cuo
+}  // namespace CUO_NS

@@ -11,7 +11,7 @@
 #pragma once
 #include "common.h"
 
-namespace cuo {
+namespace CUO_NS {
 
 struct RangeDec {
     const uint8_t *buf;
@@ -274,4 +274,4 @@ CUO_FI uint32_t ec_tell_frac(const RangeDec *d)
     return nbits - (uint32_t)l;
 }
 
-}  // namespace cuo
+}  // namespace CUO_NS

@@ -17,7 +17,7 @@
 #include "silk.h"
 #include "celt_plc.h"
 
-namespace cuo {
+namespace CUO_NS {
 
 enum { MODE_SILK_ONLY = 1000, MODE_HYBRID = 1001, MODE_CELT_ONLY = 1002 };
 enum { BW_NB = 1101, BW_MB = 1102, BW_WB = 1103, BW_SWB = 1104, BW_FB = 1105 };
@@ -28,6 +28,7 @@ struct OpusDecState {
     SilkDecoder silk;
     CeltState celt;
     SilkControl silk_ctl;
+    int32_t channels;         // output channels (1, or 2 in the stereo build)
     int32_t stream_channels;
     int32_t bandwidth;
     int32_t mode;
@@ -89,8 +90,9 @@ CUO_FN void dec_bind(OpusDecState *st, DecScratch *sc, DecHeap *h)
     sc->frame_out = h->frame_out;
 }
 
-CUO_FN void opus_dec_init(OpusDecState *st)
+CUO_FN void opus_dec_init(OpusDecState *st, int channels)
 {
+    st->channels = kMaxCh == 1 ? 1 : channels;
     silk_init_decoder(&st->silk);
     celt_init(&st->celt);
     st->silk_ctl.internalSampleRate = 0;
@@ -530,4 +532,4 @@ CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, Ou
     return decode_fec ? frame_size : nb_samples;
 }
 
-}  // namespace cuo
+}  // namespace CUO_NS

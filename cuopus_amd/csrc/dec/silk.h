@@ -14,7 +14,7 @@
 #include "entdec.h"
 #include "tables_gen.h"
 
-namespace cuo {
+namespace CUO_NS {
 
 constexpr int kSilkMaxFrame = 320;   // 20 ms at 16 kHz
 constexpr int kSilkMaxOrder = 16;
@@ -1905,4 +1905,4 @@ CUO_FN int silk_decode(SilkDecoder *dec, const SilkControl *ctl, int lostFlag, i
     return ret;
 }
 
-}  // namespace cuo
+}  // namespace CUO_NS

@@ -71,7 +71,7 @@ int main(int argc, char **argv)
     const CELTMode *m = &mode48000_960_120;
     fprintf(o, "// Generated by cuopus_amd/csrc/gen/gen_tables.c -- numeric constant tables of\n"
                "// the Opus decoder (RFC 6716; libopus 1.5.2 fixed-point values).  Do not edit.\n"
-               "#pragma once\n#include <stdint.h>\nnamespace cuo { namespace tab {\n");
+               "#pragma once\n#include <stdint.h>\nnamespace CUO_NS { namespace tab {\n");
 
     /* ---------------- CELT mode 48000/960/120 ---------------- */
     fprintf(o, "static constexpr int kNbEBands = %d;\n", m->nbEBands);
@@ -194,7 +194,7 @@ int main(int argc, char **argv)
     EMIT("int16_t", "kResUp2HQ0", silk_resampler_up2_hq_0, 3);
     EMIT("int16_t", "kResUp2HQ1", silk_resampler_up2_hq_1, 3);
 
-    fprintf(o, "}}  // namespace cuo::tab\n");
+    fprintf(o, "}}  // namespace CUO_NS::tab\n");
     fclose(o);
     return 0;
 }

@@ -16,7 +16,7 @@
 #pragma once
 #include "dec/opus_dec.h"
 
-namespace cuo {
+namespace CUO_NS {
 
 constexpr int kMaxPktSamples = 5760;
 // Resident wavefronts of the persistent decoder: four per SIMD (10 KB of
@@ -32,12 +32,15 @@ struct ChainDesc {
     uint32_t keep;    // samples kept
     uint64_t dst;     // offset of the keep window in the compacted output
     uint32_t rec_end; // per-packet records (gate): only packets below this index
-    int32_t gain;     // output gain, Q8 dB (OpusHead; libopus OPUS_SET_GAIN semantics)
+    int16_t gain;     // output gain, Q8 dB (OpusHead; libopus OPUS_SET_GAIN semantics)
+    uint8_t channels; // output channels of the chain's decoder (1, or 2 in the stereo build)
+    uint8_t pad;
 };
 
 struct ChainResult {
     int32_t stop;      // 0 or the first negative decoder return
     uint32_t samples;  // samples produced by the chain's packets
+    uint32_t t0, t1;   // constant 100 MHz clock (low 32 bits) when the chain started and ended
 };
 
 // Decoder memory of one wavefront.  The chain's state and per-call scratch
@@ -53,7 +56,9 @@ struct WaveState {
     DecScratch sc;
 };
 static_assert(sizeof(WaveState) <= kWaveLdsBytes, "wavefront decoder memory exceeds its LDS block");
-static_assert(kWaveLdsBytes * 16 <= 160 * 1024, "four wavefronts per SIMD need <= 10 KB of LDS each");
+// Resident wavefronts per SIMD: four for the mono build, two for the stereo one.
+constexpr int kWavesPerSimd = kMaxCh == 1 ? 4 : 2;
+static_assert(kWaveLdsBytes * 4 * kWavesPerSimd <= 160 * 1024, "LDS per wavefront exceeds the CU's share");
 
 struct DecodeArgs {
     const uint8_t *data;
@@ -77,14 +82,16 @@ struct DecodeArgs {
 struct ChainOut {
     int16_t *buf;
     int16_t *out;
-    uint64_t dst, lo, hi, pos;
+    uint64_t dst, lo, hi, pos;  // in sample frames (one sample per output channel)
     int lane;
+    int cc;                     // output channels, interleaved in buf and out
     __device__ void emit(int n)
     {
         CUO_PROF_SCOPE(kProfEmit);
+        const uint64_t C = kMaxCh == 1 ? 1 : (uint64_t)cc;
         uint64_t a = pos > lo ? pos : lo;
         uint64_t b = pos + (uint64_t)n < hi ? pos + (uint64_t)n : hi;
-        for (uint64_t i = a + (uint64_t)lane; i < b; i += 64) out[dst + (i - lo)] = buf[i - pos];
+        for (uint64_t e = C * a + (uint64_t)lane; e < C * b; e += 64) out[C * (dst - lo) + e] = buf[e - C * pos];
         pos += (uint64_t)n;
     }
 };
@@ -96,6 +103,7 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
                                               int lane)
 {
     prof_reset();
+    const uint32_t t_start = (uint32_t)__builtin_amdgcn_s_memrealtime();
     {
     CUO_PROF_SCOPE(kProfChain);
     const ChainDesc ch = a.chains[c];
@@ -107,7 +115,7 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
     OpusDecState *st = lds_uniform(&ws->st);
     DecScratch *sc = lds_uniform(&ws->sc);
     dec_bind(st, sc, heap);
-    opus_dec_init(st);
+    opus_dec_init(st, kMaxCh == 1 ? 1 : ch.channels);
     st->decode_gain = ch.gain;
     ChainOut out;
     out.buf = sc->frame_out;
@@ -117,6 +125,7 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
     out.hi = (uint64_t)ch.trim + ch.keep;
     out.pos = 0;
     out.lane = lane;
+    out.cc = st->channels;
     int32_t stop = 0;
     for (uint32_t k = 0; k < ch.npkts; k++) {
         const uint32_t p = ch.first + k;
@@ -144,9 +153,12 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
             pkt_rng[p] = st->rangeFinal;
         }
     }
+    const uint32_t t_end = (uint32_t)__builtin_amdgcn_s_memrealtime();
     if (lane == 0) {
         a.res[c].stop = stop;
         a.res[c].samples = (uint32_t)out.pos;
+        a.res[c].t0 = t_start;
+        a.res[c].t1 = t_end;
     }
     }
     prof_flush();
@@ -169,7 +181,7 @@ __device__ __noinline__ void plain_worker(const DecodeArgs *ap, uint32_t g, int 
     const uint32_t c = ap->order ? ap->order[g] : g;
     decode_one_chain(*ap, c, (WaveState *)g_wave_lds, ap->heap + g, lane);
 }
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_decode_chains(DecodeArgs)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void k_decode_chains(DecodeArgs)
 {
     plain_worker(launch_args(), blockIdx.x, (int)threadIdx.x);
 }
@@ -192,7 +204,7 @@ __device__ __noinline__ void persistent_worker(const DecodeArgs *ap, DecHeap *he
         decode_one_chain(*ap, c, ws, heap, lane);
     }
 }
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_decode_persistent(DecodeArgs)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void k_decode_persistent(DecodeArgs)
 {
     const DecodeArgs *ap = launch_args();
     persistent_worker(ap, ap->heap + blockIdx.x, (int)threadIdx.x);
@@ -309,4 +321,4 @@ __global__ __launch_bounds__(Threads) void k_resample_48to16(const int16_t *__re
     }
 }
 
-}  // namespace cuo
+}  // namespace CUO_NS

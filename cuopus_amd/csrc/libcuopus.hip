@@ -26,6 +26,7 @@
 #include "host/demux.h"
 #include "kernels.h"
 #include "logmel.h"
+#include "stereo_launch.h"
 
 using namespace cuo;
 using cuo_host::Chain;
@@ -113,6 +114,17 @@ void launch_persistent(uint32_t grid, hipStream_t st, const DecodeArgs &a)
     hipLaunchKernelGGL(k_decode_persistent, dim3(grid), dim3(64), 0, st, a);
 }
 
+// The decode launch of either kernel build (stereo: the stereo-capable one).
+void launch_decode(bool stereo, bool persistent, uint32_t grid, hipStream_t st, const DecodeArgs &a)
+{
+    if (stereo) cuo_stereo_launch(persistent ? 1 : 0, grid, st, &a);
+    else if (persistent) launch_persistent(grid, st, a);
+    else launch_chains(grid, st, a);
+}
+
+// HBM heap bytes per wavefront of the kernel build that decodes a launch.
+size_t heap_bytes(bool stereo) { return stereo ? cuo_stereo_heap_bytes() : sizeof(DecHeap); }
+
 // Kaiser I0 by its power series, in double.
 double bessel_i0(double x)
 {
@@ -147,11 +159,12 @@ RsTaps make_taps()
 // ---------------------------------------------------------------- tuned defaults
 // Per-device launch defaults written by the autotuner (cuopus_amd/autotune.py)
 // to cuopus_tuned.txt beside libcuopus.so, or the file CUOPUS_TUNED names.
-// One line per device: "<gcnArchName>/<CU count> slots=<n> rs=<variant>".
+// One line per device: "<gcnArchName>/<CU count> slots=<n> rs=<variant> order=<o>".
 // Read once per process and device; absent file or device: built-in defaults.
 struct TunedDefaults {
     uint32_t slots = 0;  // persistent wavefronts (0: kDefaultWaves)
     int rs = -1;         // resampler variant (-1: kRsDefaultVariant)
+    int order = -1;      // work order (-1: the cost model)
 };
 
 std::string lib_dir()
@@ -191,18 +204,25 @@ const TunedDefaults &tuned_defaults()
         if (s) cache[dev].slots = (uint32_t)strtoul(s + 6, nullptr, 10);
         const char *r = strstr(line, "rs=");
         if (r) cache[dev].rs = atoi(r + 3);
+        const char *o = strstr(line, "order=");
+        if (o) cache[dev].order = atoi(o + 6);
     }
     fclose(f);
     return cache[dev];
 }
 
-// Per-frame decode costs of the work-order model, by TOC (about 0.1 us of
-// one wavefront each; measured on MI355X with homogeneous streams).
-// CELT by frame size 2.5/5/10/20 ms, hybrid by 10/20 ms, SILK by 10/20/40/60 ms.
-constexpr uint32_t kCostCelt[4] = {95, 115, 150, 215};
-constexpr uint32_t kCostHybrid[2] = {250, 330};
-constexpr uint32_t kCostSilk[4] = {120, 180, 340, 500};
-constexpr uint32_t kCostLost = 100;
+// Per-frame decode costs of the work-order model, by TOC, in microseconds of
+// one wavefront on a loaded MI355X: fitted (non-negative least squares of
+// per-chain device times on the chains' frame counts, cuopus_amd/autotune.py
+// calibrate) on the 2 h CELT workload and the 50-file mixed set.  CELT by
+// frame size 2.5/5/10/20 ms (a 2.5 ms frame costs a quarter of a 20 ms one,
+// so twice as much per second of audio); hybrid 20 ms and SILK 20 ms fitted,
+// hybrid 10 ms and SILK 10/40/60 ms scaled from them; a lost packet
+// (concealment) about a frame.
+constexpr uint32_t kCostCelt[4] = {134, 227, 315, 539};
+constexpr uint32_t kCostHybrid[2] = {280, 461};
+constexpr uint32_t kCostSilk[4] = {130, 236, 470, 700};
+constexpr uint32_t kCostLost = 200;
 
 // Resampler launch: variant v picks outputs per thread (rs_variants table).
 int resample_variant()
@@ -239,9 +259,10 @@ struct FilePlan {
     Demuxed d;
     std::vector<Chain> chains;
     uint64_t chain0 = 0;     // first merged chain index
-    uint64_t keep = 0;       // playable samples
-    uint64_t cat_off = 0;    // offset of the file in the compacted output
+    uint64_t keep = 0;       // playable samples (per channel)
+    uint64_t cat_off = 0;    // offset of the file in the compacted output (int16 elements)
     int64_t out16 = 0;
+    int out_channels = 1;    // decoder output channels: 1 (mono render, the f32 path) or the stream's
 };
 
 // The decoder reads packet bytes as aligned 32-bit words and never depends
@@ -258,7 +279,8 @@ struct Plan {
     std::vector<uint64_t> cap;   // per chain: sum of TOC durations
     std::vector<uint32_t> cost;  // per chain: estimated decode cost (work order)
     uint64_t total_keep = 0;  // playable samples of all files
-    uint64_t cat_len = 0;     // length of the compacted PCM buffer (file starts padded to 8 samples)
+    uint64_t cat_len = 0;     // int16 elements of the compacted PCM buffer (file starts padded to 8)
+    bool stereo = false;      // some stream is stereo: the stereo-capable kernel build decodes the launch
 };
 
 // OpusHead output gain (RFC 7845 5.1, Q7.8 dB) is applied unless
@@ -275,7 +297,8 @@ int demux_file(const uint8_t *ogg, size_t n, FilePlan *fp, bool with_chains, Err
     if (!ogg || !n) return e.set(CUOPUS_RC_ARG, "empty input");
     if (!cuo_host::demux(ogg, n, &fp->d, &msg, cuo_host::demux_opts_from_env()))
         return e.set(CUOPUS_RC_SOFT, "%s", msg.c_str());
-    if (fp->d.channels != 1) return e.set(CUOPUS_RC_SOFT, "only mono supported (channels=%d)", fp->d.channels);
+    if (fp->d.channels != 1 && fp->d.channels != 2)
+        return e.set(CUOPUS_RC_SOFT, "only mono and stereo streams are supported (channels=%d)", fp->d.channels);
     if (with_chains) {
         double seg_s = env_f("CUOPUS_SEG_S", 1.0);
         double preroll_ms = env_f("CUOPUS_PREROLL_MS", 250.0);
@@ -287,11 +310,8 @@ int demux_file(const uint8_t *ogg, size_t n, FilePlan *fp, bool with_chains, Err
     return 0;
 }
 
-// Decode cost of one packet in units of about 0.1 us of one wavefront,
-// from its TOC: per-frame costs by mode and frame duration, measured on
-// MI355X with homogeneous streams (CELT frames cost far more per second of
-// audio at 2.5 ms than at 20 ms; SILK and hybrid frames cost more than CELT
-// frames of the same length).
+// Decode cost of one packet (microseconds of one wavefront) from its TOC:
+// the per-frame cost of its mode and frame duration times its frame count.
 uint32_t packet_cost(const uint8_t *p, uint32_t len)
 {
     if (len == 0) return kCostLost;
@@ -360,16 +380,19 @@ int merge(Plan &pl, Err &e)
             cd.npkts = c.npkts;
             cd.trim = c.trim;
             cd.keep = c.keep;
-            cd.dst = cat + fk;
+            cd.dst = cat / (uint64_t)f.out_channels + fk;  // in sample frames (cat is a multiple of 8)
             cd.rec_end = 0;  // no per-packet records outside the gate
-            cd.gain = gain_on ? gain : 0;
+            cd.gain = (int16_t)(gain_on ? gain : 0);
+            cd.channels = (uint8_t)f.out_channels;
+            cd.pad = 0;
             pl.cap[cb + i] = s;
             pl.cost[cb + i] = (uint32_t)std::min<uint64_t>(cost, 0xFFFFFFFFu);
             fk += c.keep;
         }
         if (fk != f.keep) return e.set(CUOPUS_RC_SOFT, "internal: keep mismatch");
         // each file's PCM starts 16-byte aligned (the resampler's wide loads)
-        cat += (fk + 7) & ~7ull;
+        cat += (fk * (uint64_t)f.out_channels + 7) & ~7ull;
+        pl.stereo = pl.stereo || f.d.channels == 2;
         keep_sum += fk;
         db += f.d.data.size();
         pb += f.d.off.size();
@@ -421,6 +444,7 @@ struct CallBufs {
 
 struct CallNeed {
     size_t data = 0, npk = 0, lost = 0, nch = 0, cat = 0, nfiles = 0, nheaps = 0;
+    size_t heap = sizeof(DecHeap);  // bytes of one wavefront's heap (the kernel build's)
 };
 
 int bufs_from_workspace(const CallNeed &n, CallBufs *b, Err &e)
@@ -429,7 +453,7 @@ int bufs_from_workspace(const CallNeed &n, CallBufs *b, Err &e)
     if ((rc = grow(W.data, n.data, e)) || (rc = grow(W.off, 4 * n.npk, e)) || (rc = grow(W.len, 4 * n.npk, e)) ||
         (rc = grow(W.chains, sizeof(ChainDesc) * n.nch, e)) || (rc = grow(W.order, 4 * n.nch, e)) ||
         (rc = grow(W.res, sizeof(ChainResult) * n.nch, e)) || (rc = grow(W.queue, 256, e)) ||
-        (rc = grow(W.heap, sizeof(DecHeap) * n.nheaps, e)))
+        (rc = grow(W.heap, n.heap * n.nheaps, e)))
         return rc;
     if (n.lost && (rc = grow(W.lost, n.lost, e))) return rc;
     if (n.cat && (rc = grow(W.cat, 2 * n.cat, e))) return rc;
@@ -481,7 +505,7 @@ size_t scratch_bytes(const CallNeed &n)
     auto a = [](size_t v) { return (v + 255) & ~(size_t)255; };
     return a(n.data) + 2 * a(4 * n.npk) + a(n.lost) + a(sizeof(ChainDesc) * n.nch) + a(4 * n.nch) +
            a(sizeof(ChainResult) * n.nch) + a(2 * n.cat) + a(sizeof(RsFile) * n.nfiles) + a(256) +
-           a(sizeof(DecHeap) * n.nheaps);
+           a(n.heap * n.nheaps);
 }
 
 int bufs_from_scratch(const CallNeed &n, CallBufs *b, Err &e)
@@ -509,13 +533,20 @@ int bufs_from_scratch(const CallNeed &n, CallBufs *b, Err &e)
     b->cat = n.cat ? (int16_t *)take(2 * n.cat) : nullptr;
     b->files = n.nfiles ? (RsFile *)take(sizeof(RsFile) * n.nfiles) : nullptr;
     b->queue = (uint32_t *)take(256);
-    b->heap = (DecHeap *)take(sizeof(DecHeap) * n.nheaps);
+    b->heap = (DecHeap *)take(n.heap * n.nheaps);
     return 0;
 }
 
 // Resident wavefronts of the persistent decoder when the caller does not
 // choose: CUOPUS_SLOTS, else the tuned value for this device, else every
 // resident slot of the chip.
+// Work order when the caller does not choose: the tuned one, else the cost model.
+int default_order()
+{
+    const int o = tuned_defaults().order;
+    return o >= 0 && o <= 2 ? o : 1;
+}
+
 uint32_t default_slots()
 {
     unsigned s = env_u("CUOPUS_SLOTS", 0);
@@ -575,6 +606,7 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     need.cat = o.s16_out ? 0 : (pl.cat_len ? pl.cat_len : 1);
     need.nfiles = rf.size();
     need.nheaps = o.persistent ? n_waves : nch;
+    need.heap = heap_bytes(pl.stereo);
     CallBufs b;
     if (o.noalloc) {
         // fewer resident wavefronts when their heaps would not fit (>= 64)
@@ -588,7 +620,7 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     int16_t *out = o.s16_out ? o.s16_out : b.cat;
 
     std::vector<uint32_t> order;
-    work_order(pl, 1, &order);
+    work_order(pl, default_order(), &order);
     HIPCHK(e, hipMemcpyAsync(b.data, pl.data.data(), pl.data.size(), hipMemcpyHostToDevice, st));
     if (npk) {
         HIPCHK(e, hipMemcpyAsync(b.off, pl.off.data(), 4 * npk, hipMemcpyHostToDevice, st));
@@ -619,9 +651,9 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
         // call's contents; no memset node if a caller captures the sequence)
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b.queue);
         HIPCHK(e, hipGetLastError());
-        launch_persistent(n_waves, st, a);
+        launch_decode(pl.stereo, true, n_waves, st, a);
     } else {
-        launch_chains(nch, st, a);
+        launch_decode(pl.stereo, false, nch, st, a);
     }
     HIPCHK(e, hipGetLastError());
     // one resampler launch over every file's tiles
@@ -915,10 +947,12 @@ extern "C" int cuopus_decode_s16_on(const uint8_t *ogg, size_t n, int16_t *out_d
     pl.files.resize(1);
     int rc = demux_file(ogg, n, &pl.files[0], true, e);
     if (rc) return rc;
-    const uint64_t keep = pl.files[0].keep;
-    if (keep > out_cap)
-        return e.set(CUOPUS_RC_ARG, "output capacity %zu < required %llu samples", out_cap, (unsigned long long)keep);
-    if (keep > 0 && !out_dev) return e.set(CUOPUS_RC_ARG, "out_dev is NULL");
+    FilePlan &fp = pl.files[0];
+    fp.out_channels = fp.d.channels;  // the stream's own channels, interleaved
+    const uint64_t keep = fp.keep, elems = keep * (uint64_t)fp.out_channels;
+    if (elems > out_cap)
+        return e.set(CUOPUS_RC_ARG, "output capacity %zu < required %llu samples", out_cap, (unsigned long long)elems);
+    if (elems > 0 && !out_dev) return e.set(CUOPUS_RC_ARG, "out_dev is NULL");
     RunOpts o;
     o.s16_out = out_dev;
     o.persistent = use_persistent();
@@ -936,13 +970,13 @@ extern "C" int cuopus_decode_f16k_multi(const uint8_t *const *oggs, const size_t
     return files_f16k_from_ogg(oggs, ns, nfiles, nullptr, 0, out_devs, out_caps, out_lens, stream, false, e);
 }
 
-extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, const uint32_t *len,
-                                  uint32_t n_packets, const uint8_t *lost, const uint32_t *first,
-                                  const uint32_t *npkts, const uint32_t *trim, const uint32_t *keep,
-                                  uint32_t n_chains, int16_t *pcm_out, int32_t *pkt_nsamp, uint32_t *pkt_rng,
-                                  int32_t *chain_stop, void *stream, char *errbuf, size_t errcap)
+namespace {
+int gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, const uint32_t *len, uint32_t n_packets,
+                const uint8_t *lost, const uint32_t *first, const uint32_t *npkts, const uint32_t *trim,
+                const uint32_t *keep, uint32_t n_chains, int channels, bool stereo_build, int16_t *pcm_out,
+                int32_t *pkt_nsamp, uint32_t *pkt_rng, int32_t *chain_stop, void *stream, Err &e)
 {
-    Err e(errbuf, errcap);
+    if (channels != 1 && channels != 2) return e.set(CUOPUS_RC_ARG, "channels must be 1 or 2");
     hipStream_t st = (hipStream_t)stream;
     int rc = refuse_capture(st, e);
     if (rc) return rc;
@@ -972,6 +1006,8 @@ extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint
         cd.dst = cat;
         cd.rec_end = c + 1 < n_chains ? first[c + 1] : 0xFFFFFFFFu;
         cd.gain = 0;
+        cd.channels = (uint8_t)channels;
+        cd.pad = 0;
         uint64_t s = 0, cost = 0;
         for (uint32_t k = 0; k < npkts[c]; k++) {
             const uint32_t p = first[c] + k;
@@ -983,7 +1019,8 @@ extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint
         cat += keep[c];
     }
     pl.total_keep = cat;
-    pl.cat_len = cat;
+    pl.cat_len = cat * (uint64_t)channels;
+    pl.stereo = stereo_build || channels == 2;
     if (n_chains == 0) return 0;
     if ((rc = grow(W.nsamp, 4ull * n_packets, e))) return rc;
     if ((rc = grow(W.rng, 4ull * n_packets, e))) return rc;
@@ -1003,9 +1040,33 @@ extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint
     for (uint32_t c = 0; c < n_chains && c < res.size(); c++) chain_stop[c] = res[c].stop;
     HIPCHK(e, hipMemcpyAsync(pkt_nsamp, W.nsamp.p, 4ull * n_packets, hipMemcpyDeviceToHost, st));
     HIPCHK(e, hipMemcpyAsync(pkt_rng, W.rng.p, 4ull * n_packets, hipMemcpyDeviceToHost, st));
-    HIPCHK(e, hipMemcpyAsync(pcm_out, W.cat.p, 2 * (cat ? cat : 0), hipMemcpyDeviceToHost, st));
+    HIPCHK(e, hipMemcpyAsync(pcm_out, W.cat.p, 2 * pl.cat_len, hipMemcpyDeviceToHost, st));
     HIPCHK(e, hipStreamSynchronize(st));
     return rc2;
+}
+}  // namespace
+
+extern "C" int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, const uint32_t *len,
+                                  uint32_t n_packets, const uint8_t *lost, const uint32_t *first,
+                                  const uint32_t *npkts, const uint32_t *trim, const uint32_t *keep,
+                                  uint32_t n_chains, int16_t *pcm_out, int32_t *pkt_nsamp, uint32_t *pkt_rng,
+                                  int32_t *chain_stop, void *stream, char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    return gate_chains(data, nbytes, off, len, n_packets, lost, first, npkts, trim, keep, n_chains, 1, false,
+                       pcm_out, pkt_nsamp, pkt_rng, chain_stop, stream, e);
+}
+
+extern "C" int cuopus_gate_chains_ch(const uint8_t *data, size_t nbytes, const uint32_t *off, const uint32_t *len,
+                                     uint32_t n_packets, const uint8_t *lost, const uint32_t *first,
+                                     const uint32_t *npkts, const uint32_t *trim, const uint32_t *keep,
+                                     uint32_t n_chains, int channels, int stereo_build, int16_t *pcm_out,
+                                     int32_t *pkt_nsamp, uint32_t *pkt_rng, int32_t *chain_stop, void *stream,
+                                     char *errbuf, size_t errcap)
+{
+    Err e(errbuf, errcap);
+    return gate_chains(data, nbytes, off, len, n_packets, lost, first, npkts, trim, keep, n_chains, channels,
+                       stereo_build != 0, pcm_out, pkt_nsamp, pkt_rng, chain_stop, stream, e);
 }
 
 extern "C" int cuopus_resample_s16_to_f16k(const int16_t *in_dev, int64_t n48, float *out_dev, size_t out_cap,
@@ -1130,6 +1191,14 @@ extern "C" int cuopus_mel_filters(int n_mels, float *out, size_t cap)
     return 0;
 }
 
+extern "C" int cuopus_tuned_defaults(int *slots, int *rs_variant, int *order)
+{
+    if (slots) *slots = (int)default_slots();
+    if (rs_variant) *rs_variant = resample_variant();
+    if (order) *order = default_order();
+    return 0;
+}
+
 extern "C" int cuopus_log_mel(const float *in_dev, int64_t n16, float *out_dev, size_t out_cap, int64_t *frames,
                               int n_mels, void *stream, char *errbuf, size_t errcap)
 {
@@ -1231,7 +1300,7 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
             if (nc) {
                 a.order = (const uint32_t *)b->order.p + c0;
                 a.n_chains = nc;
-                launch_chains(nc, st, a);
+                launch_decode(pl.stereo, false, nc, st, a);
                 HIPCHK(e, hipGetLastError());
             }
             const int r = b->file_rs[f];
@@ -1254,11 +1323,11 @@ int batch_enqueue(cuopus_batch *b, hipStream_t st, Err &e)
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, (uint32_t *)b->queue.p);
         HIPCHK(e, hipGetLastError());
         a.n_chains = nch;
-        launch_persistent(b->n_slots, st, a);
+        launch_decode(pl.stereo, true, b->n_slots, st, a);
         HIPCHK(e, hipGetLastError());
     } else {
         a.n_chains = nch;
-        launch_chains(nch, st, a);
+        launch_decode(pl.stereo, false, nch, st, a);
         HIPCHK(e, hipGetLastError());
     }
     if (prof) HIPCHK(e, hipEventRecord(b->ev[1], st));
@@ -1326,7 +1395,7 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
         b->file_chain0[f] = (uint32_t)pl.files[f].chain0;
         b->file_nchain[f] = (uint32_t)pl.files[f].chains.size();
     }
-    work_order(pl, b->opts.mode != 0 ? b->opts.lpt_order : 0, &order);
+    work_order(pl, b->opts.mode == 0 ? 0 : (b->opts.lpt_order == 3 ? default_order() : b->opts.lpt_order), &order);
     const uint32_t nch = (uint32_t)pl.chains.size();
     // persistent grid: n_slots wavefronts (default: the chip's resident
     // capacity at the kernel's LDS footprint), never more than the chains
@@ -1371,7 +1440,7 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
         (rc = grow(b->len, 4 * npk, e)) || (rc = grow(b->chains, sizeof(ChainDesc) * nch, e)) ||
         (rc = grow(b->order, 4ull * nch, e)) || (rc = grow(b->res, sizeof(ChainResult) * nch, e)) ||
         (rc = grow(b->cat, 2 * (pl.cat_len ? pl.cat_len : 1), e)) || (rc = grow(b->queue, 256, e)) ||
-        (rc = grow(b->heap, sizeof(DecHeap) * (size_t)b->n_heaps, e)) ||
+        (rc = grow(b->heap, heap_bytes(pl.stereo) * (size_t)b->n_heaps, e)) ||
         (rc = grow(b->files, sizeof(RsFile) * (rsf_upload.size() ? rsf_upload.size() : 1), e)) ||
         (!pl.lost.empty() && (rc = grow(b->lost, pl.lost.size(), e)))) {
         batch_free(b);
@@ -1469,6 +1538,22 @@ extern "C" int cuopus_batch_info(const cuopus_batch *b, int64_t *c)
     return 0;
 }
 
+extern "C" int cuopus_batch_chain_times(cuopus_batch *b, void *stream, uint32_t *t0, uint32_t *t1, int64_t cap)
+{
+    if (!b || !b->ran || cap < (int64_t)b->pl.chains.size()) return CUOPUS_RC_ARG;
+    const size_t nch = b->pl.chains.size();
+    std::vector<ChainResult> res(nch);
+    hipStream_t st = (hipStream_t)stream;
+    if (hipMemcpyAsync(res.data(), b->res.p, sizeof(ChainResult) * nch, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return CUOPUS_RC_GPU;
+    for (size_t c = 0; c < nch; c++) {
+        if (t0) t0[c] = res[c].t0;
+        if (t1) t1[c] = res[c].t1;
+    }
+    return 0;
+}
+
 extern "C" int cuopus_batch_kernel_ms(const cuopus_batch *b, float *decode_ms, float *resample_ms)
 {
     if (!b || !b->opts.profile || b->opts.use_graph) return CUOPUS_RC_ARG;
@@ -1528,6 +1613,7 @@ struct SrvReq {
 struct SrvNeed {
     uint64_t data = kPktPad, npk = 0, nch = 0, cat = 0, files = 0, lost = 0;
     uint32_t waves = 0;
+    bool stereo = false;
     void add(const SrvReq *r)
     {
         data += r->fp.d.data.size();
@@ -1536,6 +1622,7 @@ struct SrvNeed {
         cat += (r->fp.keep + 7) & ~7ull;
         files += r->out16 > 0;
         lost = lost || !r->lost.empty();
+        stereo = stereo || r->fp.d.channels == 2;
     }
 };
 
@@ -1614,7 +1701,7 @@ uint64_t srv_extra(const cuopus_server *srv, const SrvLaunch &L, const SrvNeed &
     const uint64_t waves = std::max<uint64_t>(1, std::min<uint64_t>(srv->waves, n.nch));
     const uint64_t need[11] = {n.data, 4 * n.npk, 4 * n.npk, n.lost ? n.npk : 0, sizeof(ChainDesc) * n.nch, 4 * n.nch,
                                2 * std::max<uint64_t>(1, n.cat), sizeof(ChainResult) * n.nch,
-                               sizeof(RsFile) * std::max<uint64_t>(1, n.files), 256, sizeof(DecHeap) * waves};
+                               sizeof(RsFile) * std::max<uint64_t>(1, n.files), 256, heap_bytes(n.stereo) * waves};
     const auto have = L.bufs();
     uint64_t extra = 0;
     for (int i = 0; i < 11; i++)
@@ -1645,7 +1732,7 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
         }
     }
     std::vector<uint32_t> order;
-    work_order(pl, 1, &order);
+    work_order(pl, default_order(), &order);
     std::vector<RsFile> rsf;
     uint64_t tiles = 0;
     for (size_t f = 0; f < pl.files.size(); f++) {
@@ -1666,7 +1753,7 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
         (rc = grow(L.order, 4ull * nch, e)) || (rc = grow(L.res, sizeof(ChainResult) * nch, e)) ||
         (rc = grow(L.cat, 2 * (pl.cat_len ? pl.cat_len : 1), e)) || (rc = grow(L.queue, 256, e)) ||
         (rc = grow(L.files, sizeof(RsFile) * (rsf.empty() ? 1 : rsf.size()), e)) ||
-        (rc = grow(L.heap, sizeof(DecHeap) * (size_t)waves, e)) ||
+        (rc = grow(L.heap, heap_bytes(pl.stereo) * (size_t)waves, e)) ||
         (any_lost && (rc = grow(L.lost, npk, e))))
         return rc;
     // stage everything in one pinned block, then async copies on the launch stream
@@ -1714,7 +1801,7 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
     a.queue = (uint32_t *)L.queue.p;
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, a.queue);
     HIPCHK(e, hipGetLastError());
-    launch_persistent(waves, st, a);
+    launch_decode(pl.stereo, true, waves, st, a);
     HIPCHK(e, hipGetLastError());
     if (tiles) {
         launch_resample(tiles, st, (const int16_t *)L.cat.p, (const RsFile *)L.files.p, (int)rsf.size(), srv->taps,

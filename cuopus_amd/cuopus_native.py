@@ -79,9 +79,12 @@ _SIGS = {
                                        True),
     "cuopus_file_close": (None, [VP], False),
     "cuopus_gate_chains": (I32, [VP, SZ, VP, VP, U32, VP, VP, VP, VP, VP, U32, VP, VP, VP, VP, VP], True),
+    "cuopus_gate_chains_ch": (I32, [VP, SZ, VP, VP, U32, VP, VP, VP, VP, VP, U32, I32, I32, VP, VP, VP, VP, VP],
+                              True),
     "cuopus_resample_s16_to_f16k": (I32, [VP, I64, VP, SZ, P_I64, VP], True),
     "cuopus_log_mel": (I32, [VP, I64, VP, SZ, P_I64, I32, VP], True),
     "cuopus_mel_filters": (I32, [I32, VP, SZ], False),
+    "cuopus_tuned_defaults": (I32, [_c.POINTER(I32), _c.POINTER(I32), _c.POINTER(I32)], False),
     "cuopus_dump_chains": (I32, [BUF, SZ, VP, SZ, P_I64], True),
     "cuopus_dump_packets": (I32, [BUF, SZ, VP, SZ, VP, SZ, P_I64, P_I64], True),
     "cuopus_demux_stats": (I32, [BUF, SZ, VP], True),
@@ -101,6 +104,7 @@ _SIGS = {
     "cuopus_batch_check": (I32, [VP, VP, P_I64], True),
     "cuopus_batch_info": (I32, [VP, P_I64], False),
     "cuopus_batch_kernel_ms": (I32, [VP, _c.POINTER(_c.c_float), _c.POINTER(_c.c_float)], False),
+    "cuopus_batch_chain_times": (I32, [VP, VP, VP, VP, I64], False),
     "cuopus_batch_destroy": (None, [VP], False),
 }
 
@@ -365,14 +369,15 @@ def decode_ogg_lossy(path_or_bytes, lost_mask):
 
 def decode_ogg_to_s16_tensor(path_or_bytes):
     """Playable int16 48 kHz PCM (preroll and pre-skip removed) as a device
-    tensor."""
+    tensor: shape (n,) for a mono stream, (n, 2) interleaved L/R for stereo."""
     import torch
     with _Parsed(path_or_bytes) as f:
-        out = torch.empty(max(1, f.play48), dtype=torch.int16, device="cuda")
+        out = torch.empty(max(1, f.play48 * f.channels), dtype=torch.int16, device="cuda")
         wrote = I64(0)
         _native()("cuopus_decode_s16_on", f.data, len(f.data), out.data_ptr(), out.numel(), _c.byref(wrote),
                   _cur_stream())
-    return out[:wrote.value]
+        n, ch = wrote.value, f.channels
+    return out[:n] if ch == 1 else out[:n * ch].view(n, ch)
 
 
 def resample_s16(pcm48):
@@ -409,6 +414,14 @@ def log_mel(f16k, n_mels=80):
     return out
 
 
+def tuned_defaults():
+    """(slots, resampler variant, work order) the library uses on the current
+    device when a caller does not choose."""
+    v = [I32(0), I32(0), I32(0)]
+    _native().raw("cuopus_tuned_defaults")(*[_c.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
 def mel_filters(n_mels=80):
     """(n_mels, 257) float32 filter weights that log_mel applies (host only)."""
     out = np.zeros((n_mels, MEL_N_FFT // 2 + 1), np.float32)
@@ -418,27 +431,30 @@ def mel_filters(n_mels=80):
     return out
 
 
-def gate_chains(data, off, length, chains, lost=None):
+def gate_chains(data, off, length, chains, lost=None, channels=1, stereo_build=False):
     """Per-packet gate on the GPU.
 
     data: uint8 packet bytes; off/length: uint32 per packet; chains: int array
-    (n, 4) of (first, npkts, trim, keep).  Returns (pcm int16 sum(keep),
-    nsamp int32 per packet, rng uint32 per packet, stop int32 per chain, rc)
-    where rc 1 means some chain stopped early (see stop).
+    (n, 4) of (first, npkts, trim, keep).  channels: decoder output channels
+    (2: interleaved stereo); stereo_build: decode with the stereo-capable
+    kernel build (implied by channels=2).  Returns (pcm int16 sum(keep) x
+    channels, nsamp int32 per packet, rng uint32 per packet, stop int32 per
+    chain, rc) where rc 1 means some chain stopped early (see stop).
     """
     as_u32 = lambda a: np.ascontiguousarray(a, dtype=np.uint32)  # noqa: E731
     data = np.ascontiguousarray(data, dtype=np.uint8)
     off, length = as_u32(off), as_u32(length)
     table = as_u32(np.asarray(chains, dtype=np.int64).reshape(-1, 4))
     cols = [np.ascontiguousarray(table[:, i]) for i in range(4)]
-    pcm = np.zeros(int(cols[3].sum(dtype=np.uint64)), np.int16)
+    pcm = np.zeros(int(cols[3].sum(dtype=np.uint64)) * channels, np.int16)
     nsamp = np.zeros(off.size, np.int32)
     rng = np.zeros(off.size, np.uint32)
     stop = np.zeros(len(table), np.int32)
     mask = None if lost is None else np.ascontiguousarray(lost, dtype=np.uint8)
-    rc = _native()("cuopus_gate_chains", data.ctypes.data, data.size, off.ctypes.data, length.ctypes.data, off.size,
-                   None if mask is None else mask.ctypes.data, *[c.ctypes.data for c in cols], len(table),
-                   pcm.ctypes.data, nsamp.ctypes.data, rng.ctypes.data, stop.ctypes.data, _cur_stream(), ok=(0, 1))
+    rc = _native()("cuopus_gate_chains_ch", data.ctypes.data, data.size, off.ctypes.data, length.ctypes.data,
+                   off.size, None if mask is None else mask.ctypes.data, *[c.ctypes.data for c in cols], len(table),
+                   channels, int(bool(stereo_build)), pcm.ctypes.data, nsamp.ctypes.data, rng.ctypes.data,
+                   stop.ctypes.data, _cur_stream(), ok=(0, 1))
     return pcm, nsamp, rng, stop, rc
 
 
@@ -469,7 +485,8 @@ class Batch:
     MODE_ONE_LAUNCH (every chain in one decode launch) or MODE_PERSISTENT
     (n_slots wavefronts pulling chains from a device work queue).
     lpt_order: 0 file order, 1 (True) costliest chains first by the per-packet
-    cost model, 2 longest first by samples and packets.  rs_variant: -1 the
+    cost model, 2 longest first by samples and packets, 3 the library default
+    (tuned).  rs_variant: -1 the
     library's default resampler shape, else a variant index.
     Outputs are float32 16 kHz device tensors, one per file (``outputs``).
     """
@@ -519,6 +536,21 @@ class Batch:
         if rc:
             raise CuopusError(rc, "batch was not created with profile=True (or uses a graph)")
         return d.value, r.value
+
+    def chain_times(self, stream=None):
+        """(start, end) int64 arrays per chain (file order) of the last run, in
+        10 ns ticks of the device's constant clock, start of the earliest
+        chain = 0."""
+        t0 = np.zeros(self.chains, np.uint32)
+        t1 = np.zeros(self.chains, np.uint32)
+        rc = _native().raw("cuopus_batch_chain_times")(self._h, _cur_stream() if stream is None else stream,
+                                                       t0.ctypes.data, t1.ctypes.data, self.chains)
+        if rc:
+            raise CuopusError(rc, "chain times need a batch that has run")
+        base = t0.min() if t0.size else 0
+        s = (t0.astype(np.int64) - base) & 0xFFFFFFFF
+        e = (t1.astype(np.int64) - base) & 0xFFFFFFFF
+        return s, e
 
     def close(self):
         if self._h:

@@ -81,9 +81,13 @@ void cuopus_release_workspaces(void);
  * Additions on the same path (no counterpart symbol in the reference).
  * ------------------------------------------------------------------------- */
 
-/* Playable int16 48 kHz PCM (preroll and pre-skip removed) into out_dev;
- * *out_len = playable samples.  The reference produces this buffer internally
- * (its cuopus_compact output, cuopus_kernels.cuh:420-433). */
+/* Playable int16 48 kHz PCM (preroll and pre-skip removed) into out_dev, in
+ * the stream's channels (stereo interleaved L R); out_cap in int16 elements,
+ * *out_len = playable samples per channel.  The reference produces the mono
+ * buffer internally (its cuopus_compact output, cuopus_kernels.cuh:420-433).
+ * The f32 16 kHz entry points decode a stereo stream to one channel the way
+ * libopus opened with one channel does (SILK mid channel, CELT bands summed
+ * before synthesis). */
 int cuopus_decode_s16_on(const uint8_t *ogg, size_t n, int16_t *out_dev, size_t out_cap, int64_t *out_len,
                          void *stream, char *errbuf, size_t errcap);
 
@@ -132,6 +136,16 @@ int cuopus_gate_chains(const uint8_t *data, size_t nbytes, const uint32_t *off, 
                        const uint32_t *trim, const uint32_t *keep, uint32_t n_chains, int16_t *pcm_out,
                        int32_t *pkt_nsamp, uint32_t *pkt_rng, int32_t *chain_stop, void *stream, char *errbuf,
                        size_t errcap);
+
+/* cuopus_gate_chains with an output channel count (1 or 2; pcm_out holds
+ * sum(keep) x channels interleaved samples) and the kernel build: stereo_build
+ * != 0 (or channels == 2) decodes with the stereo-capable build, which also
+ * decodes stereo packets for a one-channel output as libopus does. */
+int cuopus_gate_chains_ch(const uint8_t *data, size_t nbytes, const uint32_t *off, const uint32_t *len,
+                          uint32_t n_packets, const uint8_t *lost, const uint32_t *first, const uint32_t *npkts,
+                          const uint32_t *trim, const uint32_t *keep, uint32_t n_chains, int channels,
+                          int stereo_build, int16_t *pcm_out, int32_t *pkt_nsamp, uint32_t *pkt_rng,
+                          int32_t *chain_stop, void *stream, char *errbuf, size_t errcap);
 
 /* Parsed files: demux + chain table once (host only), then decode.  info6,
  * when non-NULL: [0] f32 16 kHz output length, [1] playable 48 kHz samples,
@@ -189,7 +203,8 @@ typedef struct {
     int lpt_order;   /* work order (modes 1-2): 0 file order, 1 costliest
                         chains first by the per-packet cost model (TOC mode
                         and frame duration), 2 longest chains first by
-                        samples + 960 x packets */
+                        samples + 960 x packets, 3 the library default
+                        (tuned, else 1) */
     int use_graph;   /* 1: capture the launch sequence into a HIP graph on
                         the first run and replay it afterwards */
     int profile;     /* 1: time the decode and resample kernels with events */
@@ -210,6 +225,17 @@ int cuopus_batch_check(cuopus_batch *b, void *stream, int64_t *out_lens, char *e
  * [6] packets read by the chains (preroll packets counted once per chain),
  * [7] packet bytes read by the chains */
 int cuopus_batch_info(const cuopus_batch *b, int64_t *counts8);
+/* The launch defaults in effect on the current device (no counterpart in
+ * the reference): persistent wavefronts, resampler variant, work order --
+ * CUOPUS_SLOTS / CUOPUS_RS, else this device's line of cuopus_tuned.txt
+ * (beside the library, or the file CUOPUS_TUNED names), else built-in. */
+int cuopus_tuned_defaults(int *slots, int *rs_variant, int *order);
+
+/* Per-chain start / end times of the last run (constant 100 MHz device clock,
+ * low 32 bits), indexed by chain in file order; cap >= chain count.  For
+ * calibrating the work-order cost model and measuring the persistent
+ * kernel's tail.  Synchronises the stream. */
+int cuopus_batch_chain_times(cuopus_batch *b, void *stream, uint32_t *t0, uint32_t *t1, int64_t cap);
 /* Kernel times of the last profiled run in milliseconds (decode, resample). */
 int cuopus_batch_kernel_ms(const cuopus_batch *b, float *decode_ms, float *resample_ms);
 void cuopus_batch_destroy(cuopus_batch *b);

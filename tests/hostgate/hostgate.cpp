@@ -9,25 +9,28 @@
 #include <string.h>
 #include "../../cuopus_amd/csrc/dec/opus_dec.h"
 
-using namespace cuo;
+using namespace CUO_NS;
 
 // Frame sink: appends each decoded frame (decoded into sc->frame_out) to the
 // caller's PCM buffer.
 struct HostOut {
     int16_t *buf;
     int16_t *dst;
-    long pos;
+    long pos;  // sample frames
+    int cc;    // output channels, interleaved
     void emit(int n)
     {
-        memcpy(dst + pos, buf, sizeof(int16_t) * (size_t)n);
+        memcpy(dst + pos * cc, buf, sizeof(int16_t) * (size_t)n * cc);
         pos += n;
     }
 };
 
-// One chain from a fresh decoder with OPUS_SET_GAIN(gain) (Q8 dB).
-extern "C" long hg_decode_chain_gain(const uint8_t *data, const uint32_t *off, const uint32_t *len,
-                                     const uint8_t *lost, int npk, int16_t *pcm, long cap,
-                                     int32_t *nsamp, uint32_t *rng, int gain)
+// One chain from a fresh decoder of `channels` output channels (1 or 2; 2
+// needs the stereo build) with OPUS_SET_GAIN(gain) (Q8 dB); pcm interleaved,
+// cap in sample frames.
+extern "C" long hg_decode_chain_cfg(const uint8_t *data, const uint32_t *off, const uint32_t *len,
+                                    const uint8_t *lost, int npk, int16_t *pcm, long cap,
+                                    int32_t *nsamp, uint32_t *rng, int channels, int gain)
 {
     OpusDecState *st = (OpusDecState *)malloc(sizeof(OpusDecState));
     DecScratch *sc = (DecScratch *)malloc(sizeof(DecScratch));
@@ -36,9 +39,9 @@ extern "C" long hg_decode_chain_gain(const uint8_t *data, const uint32_t *off, c
     memset(st, 0x5A, sizeof(OpusDecState));
     memset(heap, 0x3C, sizeof(DecHeap));
     dec_bind(st, sc, heap);
-    opus_dec_init(st);
+    opus_dec_init(st, channels);
     st->decode_gain = gain;
-    HostOut out{sc->frame_out, pcm, 0};
+    HostOut out{sc->frame_out, pcm, 0, st->channels};
     long total = 0;
     long ret = 0;
     for (int k = 0; k < npk; k++) {
@@ -68,6 +71,13 @@ extern "C" long hg_decode_chain_gain(const uint8_t *data, const uint32_t *off, c
     return ret < 0 ? ret : total;
 }
 
+extern "C" long hg_decode_chain_gain(const uint8_t *data, const uint32_t *off, const uint32_t *len,
+                                     const uint8_t *lost, int npk, int16_t *pcm, long cap,
+                                     int32_t *nsamp, uint32_t *rng, int gain)
+{
+    return hg_decode_chain_cfg(data, off, len, lost, npk, pcm, cap, nsamp, rng, 1, gain);
+}
+
 extern "C" long hg_decode_chain(const uint8_t *data, const uint32_t *off, const uint32_t *len,
                                 const uint8_t *lost, int npk, int16_t *pcm, long cap,
                                 int32_t *nsamp, uint32_t *rng)
@@ -89,6 +99,8 @@ extern "C" void hg_ec_schedule(const uint8_t *buf, uint32_t n, uint32_t *out)
     out[k++] = d.rng;
     out[k++] = (uint32_t)d.error;
 }
+
+extern "C" int hg_max_channels() { return kMaxCh; }
 
 extern "C" int hg_sizes(int *out)
 {
@@ -117,8 +129,8 @@ extern "C" void hg_decode_chains_reuse(const uint8_t *data, const uint32_t *off,
     long pos = 0;
     for (int c = 0; c < nch; c++) {
         dec_bind(st, sc, heap);
-        opus_dec_init(st);
-        HostOut out{sc->frame_out, pcm + pos, 0};
+        opus_dec_init(st, 1);
+        HostOut out{sc->frame_out, pcm + pos, 0, 1};
         long r = 0;
         for (uint32_t k = 0; k < npkts[c]; k++) {
             uint32_t p = first[c] + k;

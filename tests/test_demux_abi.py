@@ -71,10 +71,10 @@ def test_error_paths(cn):
     assert _rc(cn, b"not an ogg stream") == 1       # no pages
     assert _rc(cn, blob[:60]) == 1                  # truncated: fewer than 2 packets
     assert cn.probe(blob[:200]) == 0                # headers only: nothing playable
-    # stereo OpusHead -> soft failure
+    # more than two channels (a multistream mapping) -> soft failure
     i = blob.find(b"OpusHead")
     st = bytearray(blob)
-    st[i + 9] = 2
+    st[i + 9] = 3
     assert _rc(cn, bytes(st)) == 1
     # corrupt a mid-stream granule -> cross-check failure
     pos = [m.start() for m in re.finditer(b"OggS", blob)]

@@ -273,6 +273,6 @@ def test_error_paths_unchanged(cn, monkeypatch):
     assert _rc(cn, b"not an ogg stream") == 1
     assert _rc(cn, blob[:60]) == 1
     st = bytearray(blob)
-    st[blob.find(b"OpusHead") + 9] = 2
+    st[blob.find(b"OpusHead") + 9] = 3  # three channels: a multistream mapping, not supported
     assert _rc(cn, bytes(st)) == 1
     assert re.search(rb"OpusHead", blob)
