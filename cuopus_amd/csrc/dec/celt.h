@@ -1,4 +1,5 @@
-// celt.h -- CELT fixed-point decoder (mono, 48 kHz, 960-sample mode).
+// celt.h -- CELT fixed-point decoder (48 kHz, 960-sample mode; one coded
+// channel, or two in the stereo build: kMaxCh == 2).
 //
 // Restates libopus 1.5.2's celt_decode_with_ec (vendor/celt/celt_decoder.c:
 // 970-1369) and its closure: coarse/fine energy (quant_bands.c:428-543,
@@ -28,14 +29,20 @@ constexpr int kCombMinPeriod = 15;
 
 enum { SPREAD_NONE = 0, SPREAD_LIGHT = 1, SPREAD_NORMAL = 2, SPREAD_AGGRESSIVE = 3 };
 
+// Decode history of one output channel: kDecBuf + kOverlap samples.
+constexpr int kMemStride = kDecBuf + kOverlap;
+
 struct CeltState {
-    // decode history, kDecBuf + kOverlap samples, owned by the caller (HBM on
-    // the device: it is only touched by lane-parallel loops and concealment)
+    // decode history, kMemStride samples per output channel (channel c at
+    // c * kMemStride), owned by the caller (HBM on the device: it is only
+    // touched by lane-parallel loops and concealment)
     int32_t *mem;
     struct CeltPlcScratch *plc;  // pitch-concealment buffers, caller-owned
-    int32_t *freq;               // synthesis spectrum (960), caller-owned: only lane-parallel loops touch it
+    int32_t *freq;               // synthesis spectra (kMaxCh x 960), caller-owned: only lane-parallel loops touch them
     int32_t *etmp;               // prefilter_and_fold output (kOverlap), caller-owned
     int32_t start, end;
+    int32_t channels;            // output channels (CC), fixed at init
+    int32_t stream_channels;     // coded channels of the frame (C), set by the packet layer
     // ---- cleared by reset (OPUS_RESET_STATE) from here on
     uint32_t rng;
     int32_t error;
@@ -46,8 +53,8 @@ struct CeltState {
     int32_t pf_gain, pf_gain_old;   // opus_val16 values
     int32_t pf_tapset, pf_tapset_old;
     int32_t prefilter_and_fold;
-    int32_t preemph_mem;
-    int16_t lpc[kLpcOrder];
+    int32_t preemph_mem[kMaxCh];
+    int16_t lpc[kMaxCh * kLpcOrder];
     int16_t oldBandE[2 * kNbBands];
     int16_t oldLogE[2 * kNbBands];
     int16_t oldLogE2[2 * kNbBands];
@@ -66,29 +73,35 @@ struct CeltPlcScratch {
 };
 
 struct CeltScratch {
-    // frame decode and noise-based concealment
-    int16_t X[960];
-    int16_t norm[8 * 78];
+    // frame decode and noise-based concealment (channel c of X at c * 960;
+    // norm holds the folding source of each coded channel, 8 * 78 apart)
+    int16_t X[kMaxCh * 960];
+    int16_t norm[kMaxCh * 8 * 78];
     int16_t tmp[176];
     int32_t iy[176];
     int32_t pulses[kNbBands], fine_quant[kNbBands], fine_priority[kNbBands];
     int32_t cap[kNbBands], offsets[kNbBands], tf_res[kNbBands];
     int32_t bits1[kNbBands], bits2[kNbBands], thresh[kNbBands], trim_offset[kNbBands];
-    uint8_t collapse[kNbBands];
+    uint8_t collapse[kMaxCh * kNbBands];
     RangeDec dec;  // decoder of a frame opened here (redundancy frames)
     // explicit stack of the band splitter (quant_partition), 6 frames
     alignas(8) unsigned char qp_stack[6 * 96];
 };
+
+// Output (CC) and coded (C) channel counts; constant 1 in the mono build.
+CUO_FN int celt_cc(const CeltState *st) { return kMaxCh == 1 ? 1 : uniform(st->channels); }
+CUO_FN int celt_c(const CeltState *st) { return kMaxCh == 1 ? 1 : uniform(st->stream_channels); }
 
 // ------------------------------------------------------------------ reset
 CUO_FN void celt_reset(CeltState *st)
 {
     st->rng = 0; st->error = 0; st->last_pitch_index = 0; st->loss_duration = 0;
     st->pf_period = st->pf_period_old = 0; st->pf_gain = st->pf_gain_old = 0;
-    st->pf_tapset = st->pf_tapset_old = 0; st->prefilter_and_fold = 0; st->preemph_mem = 0;
+    st->pf_tapset = st->pf_tapset_old = 0; st->prefilter_and_fold = 0;
+    for (int c = 0; c < kMaxCh; c++) st->preemph_mem[c] = 0;
     int32_t *mem = st->mem;
-    CUO_PFOR(i, 0, kDecBuf + kOverlap) mem[i] = 0;
-    for (int i = 0; i < kLpcOrder; i++) st->lpc[i] = 0;
+    CUO_PFOR(i, 0, celt_cc(st) * kMemStride) mem[i] = 0;
+    for (int i = 0; i < kMaxCh * kLpcOrder; i++) st->lpc[i] = 0;
     for (int i = 0; i < 2 * kNbBands; i++) {
         st->oldBandE[i] = 0;
         st->oldLogE[i] = st->oldLogE2[i] = -(28 << kDbShift);
@@ -97,10 +110,11 @@ CUO_FN void celt_reset(CeltState *st)
     st->skip_plc = 1;
 }
 
-CUO_FN void celt_init(CeltState *st)
+CUO_FN void celt_init(CeltState *st, int channels)
 {
     st->start = 0;
     st->end = kNbBands;
+    st->channels = st->stream_channels = kMaxCh == 1 ? 1 : channels;
     celt_reset(st);
 }
 
@@ -223,57 +237,73 @@ CUO_FI int laplace_decode(RangeDec *d, uint32_t fs, int decay)
     return val;
 }
 
-CUO_FI void unquant_coarse_energy(int start, int end, int16_t *oldE, int intra, RangeDec *d, int LM)
+// Energies of coded channel c live at oldE[c * kNbBands + i]; each band's
+// channels are coded one after the other (quant_bands.c:428-543).
+CUO_FI void unquant_coarse_energy(int start, int end, int16_t *oldE, int intra, RangeDec *d, int LM, int C)
 {
+    C = kMaxCh == 1 ? 1 : C;
     const int32_t *prob = kEProb32.v + (LM * 2 + intra) * 42;
-    int32_t prev = 0;
+    int32_t prev0 = 0, prev1 = 0;  // predictor state of each channel (kept in registers)
     int16_t coef, beta;
     if (intra) { coef = 0; beta = tab::kBetaIntra; }
     else { beta = (int16_t)ld_s16(tab::kBetaCoef, LM); coef = (int16_t)ld_s16(tab::kPredCoef, LM); }
     int32_t budget = (int32_t)uniform(d->storage) * 8;
     for (int i = start; i < end; i++) {
-        int qi;
-        int32_t tell = ec_tell(d);
-        if (budget - tell >= 15) {
-            int pi = 2 * imin(i, 20);
-            qi = laplace_decode(d, (uint32_t)prob[pi] << 7, prob[pi + 1] << 6);
-        } else if (budget - tell >= 2) {
-            qi = ec_icdf(d, tab::kSmallEnergyIcdf, 2);
-            qi = (qi >> 1) ^ -(qi & 1);
-        } else if (budget - tell >= 1) {
-            qi = -ec_bit_logp(d, 1);
-        } else {
-            qi = -1;
+        for (int c = 0; c < C; c++) {
+            int qi;
+            int32_t tell = ec_tell(d);
+            if (budget - tell >= 15) {
+                int pi = 2 * imin(i, 20);
+                qi = laplace_decode(d, (uint32_t)prob[pi] << 7, prob[pi + 1] << 6);
+            } else if (budget - tell >= 2) {
+                qi = ec_icdf(d, tab::kSmallEnergyIcdf, 2);
+                qi = (qi >> 1) ^ -(qi & 1);
+            } else if (budget - tell >= 1) {
+                qi = -ec_bit_logp(d, 1);
+            } else {
+                qi = -1;
+            }
+            int32_t q = shl32(qi, kDbShift);
+            int16_t &e = oldE[c * kNbBands + i];
+            e = max16(-(9 << kDbShift), e);
+            const int32_t prev = c ? prev1 : prev0;
+            int32_t tmp = pshr32(mul16_16(coef, e), 8) + prev + shl32(q, 7);
+            tmp = max32(-(28 << (kDbShift + 7)), tmp);
+            e = (int16_t)pshr32(tmp, 7);
+            const int32_t next = prev + shl32(q, 7) - mul16_16(beta, pshr32(q, 8));
+            if (c) prev1 = next; else prev0 = next;
         }
-        int32_t q = shl32(qi, kDbShift);
-        oldE[i] = max16(-(9 << kDbShift), oldE[i]);
-        int32_t tmp = pshr32(mul16_16(coef, oldE[i]), 8) + prev + shl32(q, 7);
-        tmp = max32(-(28 << (kDbShift + 7)), tmp);
-        oldE[i] = (int16_t)pshr32(tmp, 7);
-        prev = prev + shl32(q, 7) - mul16_16(beta, pshr32(q, 8));
     }
 }
 
-CUO_FI void unquant_fine_energy(int start, int end, int16_t *oldE, const int32_t *fine_quant, RangeDec *d)
+CUO_FI void unquant_fine_energy(int start, int end, int16_t *oldE, const int32_t *fine_quant, RangeDec *d, int C)
 {
+    C = kMaxCh == 1 ? 1 : C;
     for (int i = start; i < end; i++) {
         if (fine_quant[i] <= 0) continue;
-        int q2 = (int)ec_bits(d, fine_quant[i]);
-        int16_t offset = (int16_t)(((shl32(q2, kDbShift) + (1 << (kDbShift - 1))) >> fine_quant[i]) - (1 << (kDbShift - 1)));
-        oldE[i] = (int16_t)(oldE[i] + offset);
+        for (int c = 0; c < C; c++) {
+            int q2 = (int)ec_bits(d, fine_quant[i]);
+            int16_t offset =
+                (int16_t)(((shl32(q2, kDbShift) + (1 << (kDbShift - 1))) >> fine_quant[i]) - (1 << (kDbShift - 1)));
+            oldE[c * kNbBands + i] = (int16_t)(oldE[c * kNbBands + i] + offset);
+        }
     }
 }
 
 CUO_FI void unquant_energy_finalise(int start, int end, int16_t *oldE, const int32_t *fine_quant,
-                                    const int32_t *fine_priority, int bits_left, RangeDec *d)
+                                    const int32_t *fine_priority, int bits_left, RangeDec *d, int C)
 {
+    C = kMaxCh == 1 ? 1 : C;
     for (int prio = 0; prio < 2; prio++) {
-        for (int i = start; i < end && bits_left >= 1; i++) {
+        for (int i = start; i < end && bits_left >= C; i++) {
             if (fine_quant[i] >= 8 || fine_priority[i] != prio) continue;
-            int q2 = (int)ec_bits(d, 1);
-            int16_t offset = (int16_t)((int16_t)(shl16(q2, kDbShift) - (1 << (kDbShift - 1))) >> (fine_quant[i] + 1));
-            oldE[i] = (int16_t)(oldE[i] + offset);
-            bits_left--;
+            for (int c = 0; c < C; c++) {
+                int q2 = (int)ec_bits(d, 1);
+                int16_t offset =
+                    (int16_t)((int16_t)(shl16(q2, kDbShift) - (1 << (kDbShift - 1))) >> (fine_quant[i] + 1));
+                oldE[c * kNbBands + i] = (int16_t)(oldE[c * kNbBands + i] + offset);
+                bits_left--;
+            }
         }
     }
 }
@@ -303,18 +333,33 @@ CUO_FI void tf_decode(int start, int end, int isTransient, int32_t *tf_res, int 
     for (int i = start; i < end; i++) tf_res[i] = tsel[4 * isTransient + 2 * tf_select + tf_res[i]];
 }
 
-// clt_compute_allocation + interp_bits2pulses, mono decoder (rate.c:245-646)
+// clt_compute_allocation + interp_bits2pulses, decoder (rate.c:245-646).
+// For two coded channels it also decodes the intensity band and the dual
+// stereo flag.
 CUO_FI int compute_allocation(int start, int end, const int32_t *offsets, const int32_t *cap, int alloc_trim,
                               int32_t total, int32_t *balance_out, int32_t *bits, int32_t *ebits,
-                              int32_t *fine_priority, int LM, RangeDec *d, CeltScratch *sc)
+                              int32_t *fine_priority, int LM, RangeDec *d, CeltScratch *sc, int Cin,
+                              int *intensity, int *dual_stereo)
 {
     const int32_t *eB = kEB32.v;
     int32_t *bits1 = sc->bits1, *bits2 = sc->bits2, *thresh = sc->thresh, *trim_offset = sc->trim_offset;
-    const int C = 1;
+    const int C = kMaxCh == 1 ? 1 : Cin;
+    const int stereo = C > 1;
     total = max32(total, 0);
     int skip_start = start;
     int skip_rsv = total >= 1 << kBitRes ? 1 << kBitRes : 0;
     total -= skip_rsv;
+    int intensity_rsv = 0, dual_stereo_rsv = 0;
+    if (C == 2) {
+        intensity_rsv = ld_u8(tab::kLog2FracTable, end - start);
+        if (intensity_rsv > total) {
+            intensity_rsv = 0;
+        } else {
+            total -= intensity_rsv;
+            dual_stereo_rsv = total >= 1 << kBitRes ? 1 << kBitRes : 0;
+            total -= dual_stereo_rsv;
+        }
+    }
     CUO_PFOR(j, start, end) {
         thresh[j] = imax(C << kBitRes, (3 * (eB[j + 1] - eB[j]) << LM << kBitRes) >> 4);
         int t = C * (eB[j + 1] - eB[j]) * (alloc_trim - 5 - LM) * (end - j - 1) * (1 << (LM + kBitRes)) >> 6;
@@ -412,11 +457,18 @@ CUO_FI int compute_allocation(int start, int end, const int32_t *offsets, const 
             psum += 1 << kBitRes;
             band_bits -= 1 << kBitRes;
         }
-        psum -= bits[j];
+        psum -= bits[j] + intensity_rsv;
+        if (intensity_rsv > 0) intensity_rsv = ld_u8(tab::kLog2FracTable, j - start);
+        psum += intensity_rsv;
         if (band_bits >= alloc_floor) { psum += alloc_floor; bits[j] = alloc_floor; }
         else bits[j] = 0;
     }
-    // mono: no intensity / dual stereo symbols
+    *intensity = intensity_rsv > 0 ? start + (int)ec_uint(d, codedBands + 1 - start) : 0;
+    if (*intensity <= start) {
+        total += dual_stereo_rsv;
+        dual_stereo_rsv = 0;
+    }
+    *dual_stereo = dual_stereo_rsv > 0 ? ec_bit_logp(d, 1) : 0;
     int32_t left = total - psum;
     int32_t percoeff = left / (eB[codedBands] - eB[start]);
     left -= (eB[codedBands] - eB[start]) * percoeff;
@@ -438,7 +490,7 @@ CUO_FI int compute_allocation(int start, int end, const int32_t *offsets, const 
         if (N > 1) {
             excess = max32(bit - cap[j], 0);
             bits[j] = bit - excess;
-            int den = C * N;
+            int den = C * N + (C == 2 && N > 2 && !*dual_stereo && j < *intensity ? 1 : 0);
             int NClogN = den * (kLogN32.v[j] + logM);
             int offset = (NClogN >> 1) - den * 21;
             if (N == 2) offset += den << kBitRes >> 2;
@@ -446,7 +498,7 @@ CUO_FI int compute_allocation(int start, int end, const int32_t *offsets, const 
             else if (bits[j] + offset < den * 3 << kBitRes) offset += NClogN >> 3;
             ebits[j] = imax(0, bits[j] + offset + (den << (kBitRes - 1)));
             ebits[j] = (ebits[j] / den) >> kBitRes;
-            if (C * ebits[j] > (bits[j] >> kBitRes)) ebits[j] = bits[j] >> kBitRes;
+            if (C * ebits[j] > (bits[j] >> kBitRes)) ebits[j] = bits[j] >> stereo >> kBitRes;
             ebits[j] = imin(ebits[j], 8);
             fine_priority[j] = ebits[j] * (den << kBitRes) >= bits[j] + offset;
             bits[j] -= C * ebits[j] << kBitRes;
@@ -457,7 +509,7 @@ CUO_FI int compute_allocation(int start, int end, const int32_t *offsets, const 
             fine_priority[j] = 1;
         }
         if (excess > 0) {
-            int extra_fine = imin(excess >> kBitRes, 8 - ebits[j]);
+            int extra_fine = imin(excess >> (stereo + kBitRes), 8 - ebits[j]);
             ebits[j] += extra_fine;
             int extra_bits = extra_fine * C << kBitRes;
             fine_priority[j] = extra_bits >= excess - balance;
@@ -467,7 +519,7 @@ CUO_FI int compute_allocation(int start, int end, const int32_t *offsets, const 
     }
     *balance_out = balance;
     for (; j < end; j++) {
-        ebits[j] = bits[j] >> kBitRes;
+        ebits[j] = bits[j] >> stereo >> kBitRes;
         bits[j] = 0;
         fine_priority[j] = ebits[j] < 1;
     }
@@ -886,13 +938,16 @@ struct BandCtx {
     int32_t remaining_bits;
     uint32_t seed;
     int32_t *iy;
+    int intensity;    // first intensity-stereo band (two coded channels)
+    int disable_inv;  // phase inversion off: a stereo stream decoded to one channel
 };
 
 static constexpr int16_t kExp2Table8[8] = {16384, 17866, 19483, 21247, 23170, 25267, 27554, 30048};
 
-CUO_FN int compute_qn(int N, int b, int offset, int pulse_cap)
+CUO_FN int compute_qn(int N, int b, int offset, int pulse_cap, int stereo)
 {
     int N2 = 2 * N - 1;
+    if (stereo && N == 2) N2--;
     int qb = (b + N2 * offset) / N2;
     qb = imin(b - pulse_cap - (4 << kBitRes), qb);
     qb = imin(8 << kBitRes, qb);
@@ -910,7 +965,7 @@ CUO_FI SplitOut compute_theta_mono(BandCtx *ctx, int N, int *b, int B, int B0, i
     int i = ctx->band;
     int pulse_cap = kLogN32.v[i] + LM * (1 << kBitRes);
     int offset = (pulse_cap >> 1) - 4;
-    int qn = compute_qn(N, *b, offset, pulse_cap);
+    int qn = compute_qn(N, *b, offset, pulse_cap, 0);
     int32_t tell = (int32_t)ec_tell_frac(ctx->ec);
     int itheta = 0;
     if (qn != 1) {
@@ -1186,27 +1241,185 @@ CUO_FI unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_
     return cm;
 }
 
-// quant_all_bands, decoder, mono (bands.c:1398-1672)
-CUO_FI void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, const int32_t *pulses,
-                            int shortBlocks, int spread, const int32_t *tf_res, int32_t total_bits,
-                            int32_t balance, RangeDec *ec, int LM, int codedBands, uint32_t *seed,
-                            CeltScratch *sc)
+// ------------------------------------------------------------------ stereo bands
+// The top split of a band coded as mid/side (compute_theta with stereo = 1,
+// bands.c:647-903): a stepped pdf for N > 2, a uniform one for N == 2, no
+// angle above the intensity band (then an optional phase-inversion flag).
+struct StereoSplit { int imid, iside, delta, itheta, qalloc, inv; };
+
+CUO_FI StereoSplit compute_theta_stereo(BandCtx *ctx, int N, int *b, int B, int LM, int *fill)
+{
+    StereoSplit o;
+    const int i = ctx->band;
+    const int pulse_cap = kLogN32.v[i] + LM * (1 << kBitRes);
+    const int offset = (pulse_cap >> 1) - (N == 2 ? 16 : 4);  // QTHETA_OFFSET_TWOPHASE, QTHETA_OFFSET
+    int qn = compute_qn(N, *b, offset, pulse_cap, 1);
+    if (i >= ctx->intensity) qn = 1;
+    const int32_t tell = (int32_t)ec_tell_frac(ctx->ec);
+    int itheta = 0, inv = 0;
+    if (qn != 1) {
+        if (N > 2) {
+            // probability 3 up to itheta = qn / 2, then 1
+            const int p0 = 3, x0 = qn / 2, ft = p0 * (x0 + 1) + x0;
+            const int fs = (int)ec_decode(ctx->ec, (uint32_t)ft);
+            const int x = fs < (x0 + 1) * p0 ? fs / p0 : x0 + 1 + (fs - (x0 + 1) * p0);
+            const int fl = x <= x0 ? p0 * x : (x - 1 - x0) + (x0 + 1) * p0;
+            const int fh = x <= x0 ? p0 * (x + 1) : (x - x0) + (x0 + 1) * p0;
+            ec_update(ctx->ec, (uint32_t)fl, (uint32_t)fh, (uint32_t)ft);
+            itheta = x;
+        } else {
+            itheta = (int)ec_uint(ctx->ec, (uint32_t)(qn + 1));
+        }
+        itheta = (int)((int32_t)itheta * 16384 / qn);
+    } else {
+        if (*b > 2 << kBitRes && ctx->remaining_bits > 2 << kBitRes) inv = ec_bit_logp(ctx->ec, 2);
+        if (ctx->disable_inv) inv = 0;
+    }
+    const int qalloc = (int32_t)ec_tell_frac(ctx->ec) - tell;
+    *b -= qalloc;
+    if (itheta == 0) {
+        o.imid = 32767; o.iside = 0; *fill &= (1 << B) - 1; o.delta = -16384;
+    } else if (itheta == 16384) {
+        o.imid = 0; o.iside = 32767; *fill &= ((1 << B) - 1) << B; o.delta = 16384;
+    } else {
+        o.imid = bitexact_cos((int16_t)itheta);
+        o.iside = bitexact_cos((int16_t)(16384 - itheta));
+        o.delta = frac_mul16((N - 1) << 7, bitexact_log2tan(o.iside, o.imid));
+    }
+    o.itheta = itheta;
+    o.qalloc = qalloc;
+    o.inv = inv;
+    return o;
+}
+
+// stereo_merge (bands.c:426-475): mid/side back to left/right with each
+// side renormalised; X is the mid (unit norm), Y the side already scaled.
+CUO_FN void stereo_merge(int16_t *X, int16_t *Y, int16_t mid, int N)
+{
+    int32_t xp = 0, side = 0;
+    CUO_PFOR(j, 0, N) {
+        xp = mac16_16(xp, Y[j], X[j]);
+        side = mac16_16(side, Y[j], Y[j]);
+    }
+    xp = wave_sum(xp);
+    side = wave_sum(side);
+    xp = mul16_32_q15(mid, xp);
+    const int16_t mid2 = (int16_t)(mid >> 1);
+    const int32_t El = mul16_16(mid2, mid2) + side - 2 * xp;
+    const int32_t Er = mul16_16(mid2, mid2) + side + 2 * xp;
+    if (Er < 161061 || El < 161061) {  // QCONST32(6e-4f, 28)
+        CUO_PFOR(j, 0, N) Y[j] = X[j];
+        return;
+    }
+    int kl = celt_ilog2(El) >> 1, kr = celt_ilog2(Er) >> 1;
+    const int16_t lgain = celt_rsqrt_norm(vshr32(El, (kl - 7) << 1));
+    const int16_t rgain = celt_rsqrt_norm(vshr32(Er, (kr - 7) << 1));
+    kl = imax(kl, 7);
+    kr = imax(kr, 7);
+    CUO_PFOR(j, 0, N) {
+        const int16_t l = (int16_t)mul16_16_p15(mid, X[j]);
+        const int16_t r = Y[j];
+        X[j] = (int16_t)pshr32(mul16_16(lgain, sub16(l, r)), kl + 1);
+        Y[j] = (int16_t)pshr32(mul16_16(rgain, add16(l, r)), kr + 1);
+    }
+}
+
+CUO_FI unsigned quant_band(BandCtx *ctx, int16_t *X, int N, int b, int B, int16_t *lowband, int LM,
+                           int16_t *lowband_out, int16_t gain, int16_t *lowband_scratch, int fill, int16_t *tmp);
+
+// quant_band_stereo, decoder (bands.c:905-937 for N == 1, 1235-1382)
+CUO_FN unsigned quant_band_stereo(BandCtx *ctx, int16_t *X, int16_t *Y, int N, int b, int B, int16_t *lowband,
+                                  int LM, int16_t *lowband_out, int16_t *lowband_scratch, int fill, int16_t *tmp)
+{
+    if (N == 1) {
+        for (int c = 0; c < 2; c++) {
+            int sign = 0;
+            if (ctx->remaining_bits >= 1 << kBitRes) {
+                sign = (int)ec_bits(ctx->ec, 1);
+                ctx->remaining_bits -= 1 << kBitRes;
+            }
+            (c ? Y : X)[0] = sign ? -16384 : 16384;
+        }
+        if (lowband_out) lowband_out[0] = (int16_t)(X[0] >> 4);
+        return 1;
+    }
+    const int orig_fill = fill;
+    StereoSplit sp;
+    {
+        CUO_PROF_SCOPE(kProfTheta);
+        sp = compute_theta_stereo(ctx, N, &b, B, LM, &fill);
+    }
+    const int16_t mid = (int16_t)sp.imid, side = (int16_t)sp.iside;
+    unsigned cm;
+    if (N == 2) {
+        // the side is orthogonal to the mid: one sign bit codes it
+        const int sbits = sp.itheta != 0 && sp.itheta != 16384 ? 1 << kBitRes : 0;
+        const int mbits = b - sbits;
+        const int c = sp.itheta > 8192;
+        ctx->remaining_bits -= sp.qalloc + sbits;
+        int16_t *x2 = c ? Y : X, *y2 = c ? X : Y;
+        int sign = 0;
+        if (sbits) sign = (int)ec_bits(ctx->ec, 1);
+        sign = 1 - 2 * sign;
+        cm = quant_band(ctx, x2, N, mbits, B, lowband, LM, lowband_out, kQ15One, lowband_scratch, orig_fill, tmp);
+        const int16_t a0 = x2[0], a1 = x2[1];
+        y2[0] = (int16_t)(-sign * a1);
+        y2[1] = (int16_t)(sign * a0);
+        const int16_t x0 = (int16_t)mul16_16_q15(mid, X[0]), x1 = (int16_t)mul16_16_q15(mid, X[1]);
+        const int16_t s0 = (int16_t)mul16_16_q15(side, Y[0]), s1 = (int16_t)mul16_16_q15(side, Y[1]);
+        X[0] = (int16_t)sub16(x0, s0);
+        Y[0] = add16(x0, s0);
+        X[1] = (int16_t)sub16(x1, s1);
+        Y[1] = add16(x1, s1);
+    } else {
+        int mbits = imax(0, imin(b, (b - sp.delta) / 2));
+        int sbits = b - mbits;
+        ctx->remaining_bits -= sp.qalloc;
+        int32_t rebalance = ctx->remaining_bits;
+        if (mbits >= sbits) {
+            // the mid keeps unit gain: it is the folding source of later bands
+            cm = quant_band(ctx, X, N, mbits, B, lowband, LM, lowband_out, kQ15One, lowband_scratch, fill, tmp);
+            rebalance = mbits - (rebalance - ctx->remaining_bits);
+            if (rebalance > 3 << kBitRes && sp.itheta != 0) sbits += rebalance - (3 << kBitRes);
+            cm |= quant_band(ctx, Y, N, sbits, B, nullptr, LM, nullptr, side, nullptr, fill >> B, tmp);
+        } else {
+            cm = quant_band(ctx, Y, N, sbits, B, nullptr, LM, nullptr, side, nullptr, fill >> B, tmp);
+            rebalance = sbits - (rebalance - ctx->remaining_bits);
+            if (rebalance > 3 << kBitRes && sp.itheta != 16384) mbits += rebalance - (3 << kBitRes);
+            cm |= quant_band(ctx, X, N, mbits, B, lowband, LM, lowband_out, kQ15One, lowband_scratch, fill, tmp);
+        }
+        stereo_merge(X, Y, mid, N);
+    }
+    if (sp.inv) CUO_PFOR(j, 0, N) Y[j] = (int16_t)-Y[j];
+    return cm;
+}
+
+// quant_all_bands, decoder (bands.c:1398-1672).  Y_ is the second coded
+// channel's spectrum (nullptr for one channel).
+CUO_FI void quant_all_bands(int start, int end, int16_t *X_, int16_t *Y_, uint8_t *collapse, const int32_t *pulses,
+                            int shortBlocks, int spread, int dual_stereo, int intensity, const int32_t *tf_res,
+                            int32_t total_bits, int32_t balance, RangeDec *ec, int LM, int codedBands,
+                            uint32_t *seed, int disable_inv, CeltScratch *sc)
 {
     const int32_t *eB = kEB32.v;
+    const int C = kMaxCh == 1 || Y_ == nullptr ? 1 : 2;
     const int M = 1 << LM;
     const int B = shortBlocks ? M : 1;
     const int norm_offset = M * eB[start];
     int16_t *norm = sc->norm;
+    int16_t *norm2 = norm + M * eB[kNbBands - 1] - norm_offset;
     int16_t *lowband_scratch = X_ + M * eB[kNbBands - 1];
     int lowband_offset = 0, update_lowband = 1;
     BandCtx ctx;
     ctx.ec = ec; ctx.spread = spread; ctx.seed = *seed; ctx.iy = sc->iy; ctx.qp_stack = sc->qp_stack;
+    ctx.intensity = intensity; ctx.disable_inv = disable_inv;
     for (int i = start; i < end; i++) {
         int effective_lowband = -1;
         ctx.band = i;
         int last = i == end - 1;
         CUO_PROF_SCOPE(kProfBandsPrologue);
         int16_t *X = X_ + M * eB[i];
+        int16_t *Y = C == 2 ? Y_ + M * eB[i] : nullptr;
         int N = M * eB[i + 1] - M * eB[i];
         int32_t tell = (int32_t)ec_tell_frac(ec);
         if (i != start) balance -= tell;
@@ -1225,41 +1438,68 @@ CUO_FI void quant_all_bands(int start, int end, int16_t *X_, uint8_t *collapse, 
         if ((M * eB[i] - N >= M * eB[start] || i == start + 1) && (update_lowband || lowband_offset == 0))
             lowband_offset = i;
         if (i == start + 1) {
+            // special_hybrid_folding: the source ends where the target starts
             int n1 = M * (eB[start + 1] - eB[start]);
             int n2 = M * (eB[start + 2] - eB[start + 1]);
-            CUO_PFOR(j, 0, n2 - n1) norm[n1 + j] = norm[2 * n1 - n2 + j];  // source ends where target starts
+            CUO_PFOR(j, 0, n2 - n1) norm[n1 + j] = norm[2 * n1 - n2 + j];
+            if (dual_stereo) CUO_PFOR(j, 0, n2 - n1) norm2[n1 + j] = norm2[2 * n1 - n2 + j];
         }
         int tf_change = tf_res[i];
         ctx.tf_change = tf_change;
         if (last) lowband_scratch = nullptr;
-        unsigned x_cm;
+        unsigned x_cm, y_cm;
         if (lowband_offset != 0 && (spread != SPREAD_AGGRESSIVE || B > 1 || tf_change < 0)) {
             effective_lowband = imax(0, M * eB[lowband_offset] - norm_offset - N);
             int fold_start = lowband_offset;
             while (M * eB[--fold_start] > effective_lowband + norm_offset) {}
             int fold_end = lowband_offset - 1;
             while (++fold_end < i && M * eB[fold_end] < effective_lowband + norm_offset + N) {}
-            x_cm = 0;
+            x_cm = y_cm = 0;
             int fold_i = fold_start;
-            do { x_cm |= collapse[fold_i]; } while (++fold_i < fold_end);
+            do {
+                x_cm |= collapse[fold_i * C];
+                y_cm |= collapse[fold_i * C + C - 1];
+            } while (++fold_i < fold_end);
         } else {
-            x_cm = (1u << B) - 1;
+            x_cm = y_cm = (1u << B) - 1;
         }
-        x_cm = quant_band(&ctx, X, N, b, B, effective_lowband != -1 ? norm + effective_lowband : nullptr, LM,
-                          last ? nullptr : norm + M * eB[i] - norm_offset, kQ15One, lowband_scratch, (int)x_cm,
-                          sc->tmp);
-        collapse[i] = (uint8_t)x_cm;
+        if (dual_stereo && i == intensity) {
+            // from the intensity band on, both channels fold from their average
+            dual_stereo = 0;
+            CUO_PFOR(j, 0, M * eB[i] - norm_offset) norm[j] = (int16_t)(((int32_t)norm[j] + norm2[j]) >> 1);
+        }
+        int16_t *lowband = effective_lowband != -1 ? norm + effective_lowband : nullptr;
+        int16_t *lowband_out = last ? nullptr : norm + M * eB[i] - norm_offset;
+        if (dual_stereo) {
+            x_cm = quant_band(&ctx, X, N, b / 2, B, lowband, LM, lowband_out, kQ15One, lowband_scratch, (int)x_cm,
+                              sc->tmp);
+            y_cm = quant_band(&ctx, Y, N, b / 2, B, effective_lowband != -1 ? norm2 + effective_lowband : nullptr,
+                              LM, last ? nullptr : norm2 + M * eB[i] - norm_offset, kQ15One, lowband_scratch,
+                              (int)y_cm, sc->tmp);
+        } else {
+            if (C == 2)
+                x_cm = quant_band_stereo(&ctx, X, Y, N, b, B, lowband, LM, lowband_out, lowband_scratch,
+                                         (int)(x_cm | y_cm), sc->tmp);
+            else
+                x_cm = quant_band(&ctx, X, N, b, B, lowband, LM, lowband_out, kQ15One, lowband_scratch,
+                                  (int)(x_cm | y_cm), sc->tmp);
+            y_cm = x_cm;
+        }
+        collapse[i * C] = (uint8_t)x_cm;
+        collapse[i * C + C - 1] = (uint8_t)y_cm;
         balance += pulses[i] + tell;
         update_lowband = b > (N << kBitRes);
     }
     *seed = ctx.seed;
 }
 
-// anti_collapse, mono (bands.c:268-370)
-CUO_FN void anti_collapse(int16_t *X_, const uint8_t *collapse, int LM, int start, int end, const int16_t *logE,
-                          const int16_t *prev1logE, const int16_t *prev2logE, const int32_t *pulses, uint32_t seed)
+// anti_collapse (bands.c:268-370); channel c's spectrum at X_ + c * size
+CUO_FN void anti_collapse(int16_t *X_, const uint8_t *collapse, int LM, int C, int size, int start, int end,
+                          const int16_t *logE, const int16_t *prev1logE, const int16_t *prev2logE,
+                          const int32_t *pulses, uint32_t seed)
 {
     const int32_t *eB = kEB32.v;
+    C = kMaxCh == 1 ? 1 : C;
     for (int i = start; i < end; i++) {
         int N0 = eB[i + 1] - eB[i];
         int depth = ((1 + pulses[i]) / N0) >> LM;
@@ -1269,32 +1509,38 @@ CUO_FN void anti_collapse(int16_t *X_, const uint8_t *collapse, int LM, int star
         int shift = celt_ilog2(t) >> 1;
         t = shl32(t, (7 - shift) << 1);
         int16_t sqrt_1 = celt_rsqrt_norm(t);
-        int16_t prev1 = max16(prev1logE[i], prev1logE[kNbBands + i]);
-        int16_t prev2 = max16(prev2logE[i], prev2logE[kNbBands + i]);
-        int32_t Ediff = (int32_t)logE[i] - (int32_t)min16(prev1, prev2);
-        Ediff = max32(0, Ediff);
-        int16_t r;
-        if (Ediff < 16384) {
-            int32_t r32 = celt_exp2((int16_t)-(int16_t)Ediff) >> 1;
-            r = (int16_t)(2 * min32(16383, r32));
-        } else {
-            r = 0;
-        }
-        if (LM == 3) r = (int16_t)mul16_16_q14(23170, min32(23169, r));
-        r = (int16_t)(min16(thresh, r) >> 1);
-        r = (int16_t)(mul16_16_q15(sqrt_1, r) >> shift);
-        int16_t *X = X_ + (eB[i] << LM);
-        int renormalize = 0;
-        for (int k = 0; k < 1 << LM; k++) {
-            if (!(collapse[i] & 1 << k)) {
-                for (int j = 0; j < N0; j++) {
-                    seed = lcg_rand(seed);
-                    X[(j << LM) + k] = (seed & 0x8000) ? r : (int16_t)-r;
-                }
-                renormalize = 1;
+        for (int c = 0; c < C; c++) {
+            int16_t prev1 = prev1logE[c * kNbBands + i];
+            int16_t prev2 = prev2logE[c * kNbBands + i];
+            if (C == 1) {
+                prev1 = max16(prev1, prev1logE[kNbBands + i]);
+                prev2 = max16(prev2, prev2logE[kNbBands + i]);
             }
+            int32_t Ediff = (int32_t)logE[c * kNbBands + i] - (int32_t)min16(prev1, prev2);
+            Ediff = max32(0, Ediff);
+            int16_t r;
+            if (Ediff < 16384) {
+                int32_t r32 = celt_exp2((int16_t)-(int16_t)Ediff) >> 1;
+                r = (int16_t)(2 * min32(16383, r32));
+            } else {
+                r = 0;
+            }
+            if (LM == 3) r = (int16_t)mul16_16_q14(23170, min32(23169, r));
+            r = (int16_t)(min16(thresh, r) >> 1);
+            r = (int16_t)(mul16_16_q15(sqrt_1, r) >> shift);
+            int16_t *X = X_ + c * size + (eB[i] << LM);
+            int renormalize = 0;
+            for (int k = 0; k < 1 << LM; k++) {
+                if (!(collapse[i * C + c] & 1 << k)) {
+                    for (int j = 0; j < N0; j++) {
+                        seed = lcg_rand(seed);
+                        X[(j << LM) + k] = (seed & 0x8000) ? r : (int16_t)-r;
+                    }
+                    renormalize = 1;
+                }
+            }
+            if (renormalize) renormalise_vector(X, N0 << LM, kQ15One);
         }
-        if (renormalize) renormalise_vector(X, N0 << LM, kQ15One);
     }
 }
 
@@ -1550,10 +1796,17 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
 // FFT (480 complex values).
 static_assert(offsetof(CeltScratch, iy) + sizeof(CeltScratch::iy) - offsetof(CeltScratch, X) >= 480 * 8,
               "band scratch too small for the synthesis FFT");
+// celt_synthesis (celt_decoder.c:382-458).  C coded channels (spectra at X
+// and X + N) to CC output channels (histories kMemStride apart from out_syn):
+// a mono stream is synthesised into both outputs, a stereo stream to one
+// output is the average of the two denormalised spectra.  Every spectrum is
+// denormalised before the first inverse MDCT, whose FFT reuses X's storage.
 CUO_FI void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *oldBandE, int start, int effEnd,
-                           int isTransient, int LM, int silence, int32_t *freq, CeltScratch *sc)
+                           int C, int CC, int isTransient, int LM, int silence, int32_t *freq, CeltScratch *sc)
 {
     CUO_PROF_SCOPE(kProfCeltSynth);
+    C = kMaxCh == 1 ? 1 : C;
+    CC = kMaxCh == 1 ? 1 : CC;
     X = lds_uniform(X);
     oldBandE = lds_uniform(oldBandE);
     Cpx *fbuf = lds_uniform((Cpx *)sc->X);
@@ -1562,8 +1815,19 @@ CUO_FI void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *ol
     if (isTransient) { B = M; NB = 120; shift = 3; }
     else { B = 1; NB = 120 << LM; shift = 3 - LM; }
     denormalise_bands(X, freq, oldBandE, start, effEnd, M, silence);
-    for (int b = 0; b < B; b++) mdct_backward(&freq[b], out_syn + NB * b, shift, B, fbuf);
-    CUO_PFOR(i, 0, N) out_syn[i] = saturate(out_syn[i], kSigSat);
+    if (C == 2) {
+        int32_t *freq2 = freq + 960;
+        denormalise_bands(X + N, freq2, oldBandE + kNbBands, start, effEnd, M, silence);
+        if (CC == 1) CUO_PFOR(i, 0, N) freq[i] = (freq[i] >> 1) + (freq2[i] >> 1);
+    }
+    for (int c = 0; c < CC; c++) {
+        const int32_t *fc = C == 2 ? freq + 960 * c : freq;
+        for (int b = 0; b < B; b++) mdct_backward(&fc[b], out_syn + kMemStride * c + NB * b, shift, B, fbuf);
+    }
+    for (int c = 0; c < CC; c++) {
+        int32_t *o = out_syn + kMemStride * c;
+        CUO_PFOR(i, 0, N) o[i] = saturate(o[i], kSigSat);
+    }
 }
 
 // ------------------------------------------------------------------ post-filter
@@ -1628,11 +1892,13 @@ CUO_FN void comb_filter(int32_t *y, int32_t *x, int T0, int T1, int N, int16_t g
     comb_filter_const(y + overlap, x + overlap, T1, N - overlap, g10, g11, g12, blk1);
 }
 
-// deemphasis, mono, no downsampling (celt_decoder.c:279-377)
+// deemphasis of one channel, no downsampling (celt_decoder.c:279-377); the
+// output is written every `stride` samples (interleaved channels).
 // The history (x) and the frame output live in HBM: the one-pole recurrence
 // runs register-blocked, lanes loading and storing 64 samples at a time.
-CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int accum)
+CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int accum, int stride)
 {
+    stride = kMaxCh == 1 ? 1 : stride;
     CUO_PROF_SCOPE(kProfCeltDeemph);
     const int16_t coef0 = tab::kPreemph[0];
     int32_t m = uniform(*mem);
@@ -1649,21 +1915,26 @@ CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int 
         });
         if (l < nb) {
             int16_t y = sig2word16(t);
-            pcm[j0 + l] = accum ? sat16(pcm[j0 + l] + y) : y;
+            int16_t *p = pcm + (j0 + l) * stride;
+            *p = accum ? sat16(*p + y) : y;
         }
     }
     *mem = m;
 }
 
 // ------------------------------------------------------------------ loss concealment
+// prefilter_and_fold (celt_decoder.c:515-549), each output channel
 CUO_FN void prefilter_and_fold(CeltState *st, int N, int32_t *etmp)
 {
-    int32_t *mem = st->mem;
-    comb_filter(etmp, mem + kDecBuf - N, st->pf_period_old, st->pf_period, kOverlap, (int16_t)-st->pf_gain_old,
-                (int16_t)-st->pf_gain, st->pf_tapset_old, st->pf_tapset, nullptr, 0);
-    CUO_PFOR(i, 0, kOverlap / 2)
-        mem[kDecBuf - N + i] = mul16_32_q15(tab::kWindow[i], etmp[kOverlap - 1 - i])
-                             + mul16_32_q15(tab::kWindow[kOverlap - i - 1], etmp[i]);
+    const int CC = celt_cc(st);
+    for (int c = 0; c < CC; c++) {
+        int32_t *mem = st->mem + kMemStride * c;
+        comb_filter(etmp, mem + kDecBuf - N, st->pf_period_old, st->pf_period, kOverlap, (int16_t)-st->pf_gain_old,
+                    (int16_t)-st->pf_gain, st->pf_tapset_old, st->pf_tapset, nullptr, 0);
+        CUO_PFOR(i, 0, kOverlap / 2)
+            mem[kDecBuf - N + i] = mul16_32_q15(tab::kWindow[i], etmp[kOverlap - 1 - i])
+                                 + mul16_32_q15(tab::kWindow[kOverlap - i - 1], etmp[i]);
+    }
 }
 
 // Pitch-based concealment (CELT-only frames after a loss); defined in celt_plc.h.
@@ -1674,6 +1945,7 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc);
 CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
 {
     CUO_PROF_SCOPE(kProfCeltLost);
+    const int CC = celt_cc(st);
     int32_t *mem = st->mem;
     int32_t *out_syn = mem + kDecBuf - N;
     int loss_duration = st->loss_duration;
@@ -1682,24 +1954,29 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
     if (noise_based) {
         int end = st->end;
         int effEnd = imax(start, imin(end, kNbBands));
-        shift_history(mem, N);
+        for (int c = 0; c < CC; c++) shift_history(mem + kMemStride * c, N);
         if (st->prefilter_and_fold) prefilter_and_fold(st, N, st->etmp);
         int16_t decay = loss_duration == 0 ? 1536 : 512;
-        for (int i = start; i < end; i++)
-            st->oldBandE[i] = max16(st->backgroundLogE[i], (int16_t)(st->oldBandE[i] - decay));
-        uint32_t seed = st->rng;
-        int16_t *X = sc->X;
-        for (int i = start; i < effEnd; i++) {
-            int boffs = kEB32.v[i] << LM;
-            int blen = (kEB32.v[i + 1] - kEB32.v[i]) << LM;
-            for (int j = 0; j < blen; j++) {
-                seed = lcg_rand(seed);
-                X[boffs + j] = (int16_t)((int32_t)seed >> 20);
+        for (int c = 0; c < CC; c++)
+            for (int i = start; i < end; i++) {
+                const int k = c * kNbBands + i;
+                st->oldBandE[k] = max16(st->backgroundLogE[k], (int16_t)(st->oldBandE[k] - decay));
             }
-            renormalise_vector(X + boffs, blen, kQ15One);
+        uint32_t seed = st->rng;
+        for (int c = 0; c < CC; c++) {
+            int16_t *X = sc->X + N * c;
+            for (int i = start; i < effEnd; i++) {
+                int boffs = kEB32.v[i] << LM;
+                int blen = (kEB32.v[i + 1] - kEB32.v[i]) << LM;
+                for (int j = 0; j < blen; j++) {
+                    seed = lcg_rand(seed);
+                    X[boffs + j] = (int16_t)((int32_t)seed >> 20);
+                }
+                renormalise_vector(X + boffs, blen, kQ15One);
+            }
         }
         st->rng = seed;
-        celt_synthesis(X, out_syn, st->oldBandE, start, effEnd, 0, LM, 0, st->freq, sc);
+        celt_synthesis(sc->X, out_syn, st->oldBandE, start, effEnd, CC, CC, 0, LM, 0, st->freq, sc);
         st->prefilter_and_fold = 0;
         st->skip_plc = 1;
     } else {
@@ -1734,11 +2011,14 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     int effEnd = end > kNbBands ? kNbBands : end;
     int16_t *oldBandE = st->oldBandE, *oldLogE = st->oldLogE, *oldLogE2 = st->oldLogE2;
 
+    const int CC = celt_cc(st);
     if (data == nullptr || len <= 1) {
         if (celt_decode_lost(st, N, LM, sc) < 0) return -3;
-        deemphasis(out_syn, pcm, N, &st->preemph_mem, accum);
+        for (int c = 0; c < CC; c++)
+            deemphasis(out_syn + kMemStride * c, pcm + c, N, &st->preemph_mem[c], accum, CC);
         return frame_size;
     }
+    const int C = celt_c(st);
     if (st->loss_duration == 0) st->skip_plc = 0;
     if (dec == nullptr) { ec_init(&sc->dec, data, (uint32_t)len); dec = &sc->dec; }
     // The frame decodes on a register copy of the range decoder (every
@@ -1748,7 +2028,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     RangeDec *const dec_home = dec;
     RangeDec dec_regs = ec_regs(dec_home);
     dec = &dec_regs;
-    CUO_PFOR(i, 0, kNbBands) oldBandE[i] = max16(oldBandE[i], oldBandE[kNbBands + i]);
+    if (C == 1) CUO_PFOR(i, 0, kNbBands) oldBandE[i] = max16(oldBandE[i], oldBandE[kNbBands + i]);
 
     int32_t total_bits = len * 8;
     int32_t tell = ec_tell(dec);
@@ -1803,7 +2083,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     }
     {
         CUO_PROF_SCOPE(kProfCeltEnergy);
-        unquant_coarse_energy(start, end, oldBandE, intra_ener, dec, LM);
+        unquant_coarse_energy(start, end, oldBandE, intra_ener, dec, LM, C);
     }
     int32_t *tf_res = sc->tf_res;
     tf_decode(start, end, isTransient, tf_res, LM, dec);
@@ -1813,13 +2093,13 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     int32_t *cap = sc->cap, *offsets = sc->offsets;
     CUO_PFOR(i, 0, kNbBands) {
         int Nb = (kEB32.v[i + 1] - kEB32.v[i]) << LM;
-        cap[i] = (kCacheCaps32.v[kNbBands * (2 * LM + 1 - 1) + i] + 64) * 1 * Nb >> 2;
+        cap[i] = (kCacheCaps32.v[kNbBands * (2 * LM + C - 1) + i] + 64) * C * Nb >> 2;
     }
     int dynalloc_logp = 6;
     total_bits <<= kBitRes;
     tell = (int32_t)ec_tell_frac(dec);
     for (int i = start; i < end; i++) {
-        int width = (kEB32.v[i + 1] - kEB32.v[i]) << LM;
+        int width = C * (kEB32.v[i + 1] - kEB32.v[i]) << LM;
         int quanta = imin(width << kBitRes, imax(6 << kBitRes, width));
         int loop_logp = dynalloc_logp;
         int boost = 0;
@@ -1840,38 +2120,40 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     bits -= anti_collapse_rsv;
     int32_t balance;
     int32_t *pulses = sc->pulses, *fine_quant = sc->fine_quant, *fine_priority = sc->fine_priority;
-    int codedBands;
+    int codedBands, intensity = 0, dual_stereo = 0;
     {
         CUO_PROF_SCOPE(kProfCeltAlloc);
         codedBands = compute_allocation(start, end, offsets, cap, alloc_trim, bits, &balance, pulses, fine_quant,
-                                        fine_priority, LM, dec, sc);
+                                        fine_priority, LM, dec, sc, C, &intensity, &dual_stereo);
     }
-    unquant_fine_energy(start, end, oldBandE, fine_quant, dec);
-    shift_history(mem, N);
+    unquant_fine_energy(start, end, oldBandE, fine_quant, dec, C);
+    for (int c = 0; c < CC; c++) shift_history(mem + kMemStride * c, N);
     int16_t *X = sc->X;
     {
         CUO_PROF_SCOPE(kProfCeltBands);
-        quant_all_bands(start, end, X, sc->collapse, pulses, shortBlocks, spread_decision, tf_res,
-                        len * (8 << kBitRes) - anti_collapse_rsv, balance, dec, LM, codedBands, &st->rng, sc);
+        quant_all_bands(start, end, X, C == 2 ? X + N : nullptr, sc->collapse, pulses, shortBlocks, spread_decision,
+                        dual_stereo, intensity, tf_res, len * (8 << kBitRes) - anti_collapse_rsv, balance, dec, LM,
+                        codedBands, &st->rng, CC == 1, sc);
     }
     int anti_collapse_on = 0;
     if (anti_collapse_rsv > 0) anti_collapse_on = (int)ec_bits(dec, 1);
-    unquant_energy_finalise(start, end, oldBandE, fine_quant, fine_priority, len * 8 - ec_tell(dec), dec);
+    unquant_energy_finalise(start, end, oldBandE, fine_quant, fine_priority, len * 8 - ec_tell(dec), dec, C);
     if (anti_collapse_on)
-        anti_collapse(X, sc->collapse, LM, start, end, oldBandE, oldLogE, oldLogE2, pulses, st->rng);
+        anti_collapse(X, sc->collapse, LM, C, N, start, end, oldBandE, oldLogE, oldLogE2, pulses, st->rng);
     if (silence)
-        for (int i = 0; i < kNbBands; i++) oldBandE[i] = -(28 << kDbShift);
+        for (int i = 0; i < C * kNbBands; i++) oldBandE[i] = -(28 << kDbShift);
     if (st->prefilter_and_fold) prefilter_and_fold(st, N, st->etmp);
-    celt_synthesis(X, out_syn, oldBandE, start, effEnd, isTransient, LM, silence, st->freq, sc);
+    celt_synthesis(X, out_syn, oldBandE, start, effEnd, C, CC, isTransient, LM, silence, st->freq, sc);
 
     st->pf_period = imax(st->pf_period, kCombMinPeriod);
     st->pf_period_old = imax(st->pf_period_old, kCombMinPeriod);
-    {
+    for (int c = 0; c < CC; c++) {
         CUO_PROF_SCOPE(kProfCeltComb);
-        comb_filter(out_syn, out_syn, st->pf_period_old, st->pf_period, 120, (int16_t)st->pf_gain_old,
+        int32_t *os = out_syn + kMemStride * c;
+        comb_filter(os, os, st->pf_period_old, st->pf_period, 120, (int16_t)st->pf_gain_old,
                     (int16_t)st->pf_gain, st->pf_tapset_old, st->pf_tapset, tab::kWindow, kOverlap);
         if (LM != 0)
-            comb_filter(out_syn + 120, out_syn + 120, st->pf_period, postfilter_pitch, N - 120,
+            comb_filter(os + 120, os + 120, st->pf_period, postfilter_pitch, N - 120,
                         (int16_t)st->pf_gain, postfilter_gain, st->pf_tapset, postfilter_tapset, tab::kWindow,
                         kOverlap);
     }
@@ -1886,7 +2168,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
         st->pf_gain_old = st->pf_gain;
         st->pf_tapset_old = st->pf_tapset;
     }
-    CUO_PFOR(i, 0, kNbBands) oldBandE[kNbBands + i] = oldBandE[i];
+    if (C == 1) CUO_PFOR(i, 0, kNbBands) oldBandE[kNbBands + i] = oldBandE[i];
     if (!isTransient) {
         CUO_PFOR(i, 0, 2 * kNbBands) { oldLogE2[i] = oldLogE[i]; oldLogE[i] = oldBandE[i]; }
     } else {
@@ -1904,7 +2186,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     }
     st->rng = dec->rng;
     *dec_home = dec_regs;
-    deemphasis(out_syn, pcm, N, &st->preemph_mem, accum);
+    for (int c = 0; c < CC; c++) deemphasis(out_syn + kMemStride * c, pcm + c, N, &st->preemph_mem[c], accum, CC);
     st->loss_duration = 0;
     st->prefilter_and_fold = 0;
     if (ec_tell(dec) > 8 * len) return -3;  // OPUS_INTERNAL_ERROR

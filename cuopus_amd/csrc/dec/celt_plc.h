@@ -216,16 +216,25 @@ CUO_FN void find_best_pitch(const int32_t *xcorr, const int16_t *y, int len, int
     }
 }
 
-// pitch_downsample, mono (pitch.c:140-213)
-CUO_FN void pitch_downsample(const int32_t *x, int16_t *x_lp, int len, int16_t *xx)
+// pitch_downsample (pitch.c:140-213); C channels, x1 the second (or null)
+CUO_FN void pitch_downsample(const int32_t *x, const int32_t *x1, int C, int16_t *x_lp, int len, int16_t *xx)
 {
+    C = kMaxCh == 1 ? 1 : C;
     int32_t maxabs = maxabs32(x, len);
+    if (C == 2) maxabs = max32(maxabs, maxabs32(x1, len));
     if (maxabs < 1) maxabs = 1;
     int shift = celt_ilog2(maxabs) - 10;
     if (shift < 0) shift = 0;
+    if (C == 2) shift++;
     for (int i = 1; i < len >> 1; i++)
         x_lp[i] = (int16_t)((x[2 * i - 1] >> (shift + 2)) + (x[2 * i + 1] >> (shift + 2)) + (x[2 * i] >> (shift + 1)));
     x_lp[0] = (int16_t)((x[1] >> (shift + 2)) + (x[0] >> (shift + 1)));
+    if (C == 2) {
+        for (int i = 1; i < len >> 1; i++)
+            x_lp[i] = (int16_t)(x_lp[i] + (x1[2 * i - 1] >> (shift + 2)) + (x1[2 * i + 1] >> (shift + 2))
+                                + (x1[2 * i] >> (shift + 1)));
+        x_lp[0] = (int16_t)(x_lp[0] + (x1[1] >> (shift + 2)) + (x1[0] >> (shift + 1)));
+    }
     int32_t ac[5];
     celt_autocorr(x_lp, ac, nullptr, 0, 4, len >> 1, xx);
     ac[0] += ac[0] >> 13;
@@ -306,16 +315,18 @@ CUO_FN int pitch_search(const int16_t *x_lp, const int16_t *y, int len, int max_
     return 2 * best_pitch[0] - offset;
 }
 
-// celt_decode_lost, periodic branch (celt_decoder.c:700-955), mono.
+// celt_decode_lost, periodic branch (celt_decoder.c:700-955): one pitch
+// search over all output channels, then each channel extrapolated with its
+// own LPC filter.
 CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc)
 {
-    int32_t *buf = st->mem;
+    const int CC = celt_cc(st);
     const int16_t *window = tab::kWindow;
     int16_t fade = kQ15One;
     int pitch_index;
     if (st->loss_duration == 0) {
         int16_t *lp = st->plc->lp_pitch;
-        pitch_downsample(buf, lp, kDecBuf, st->plc->plc_a);
+        pitch_downsample(st->mem, st->mem + kMemStride, CC, lp, kDecBuf, st->plc->plc_a);
         pitch_index = pitch_search(lp + (kPlcLagMax >> 1), lp, kDecBuf - kPlcLagMax, kPlcLagMax - kPlcLagMin,
                                    st->plc->plc_a, st->plc->plc_b, st->plc->xcorr);
         pitch_index = kPlcLagMax - pitch_index;
@@ -325,79 +336,85 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc)
         fade = 26214;  // QCONST16(.8f, 15)
     }
     const int exc_length = imin(2 * pitch_index, kMaxPeriod);
-    int16_t *exc = st->plc->exc + kLpcOrder;
-    for (int i = 0; i < kMaxPeriod + kLpcOrder; i++)
-        exc[i - kLpcOrder] = sround16(buf[kDecBuf - kMaxPeriod - kLpcOrder + i], kSigShift);
-    if (st->loss_duration == 0) {
-        int32_t ac[kLpcOrder + 1];
-        celt_autocorr(exc, ac, window, kOverlap, kLpcOrder, kMaxPeriod, st->plc->plc_a);
-        ac[0] += ac[0] >> 13;
-        for (int i = 1; i <= kLpcOrder; i++) ac[i] -= mul16_32_q15(2 * i * i, ac[i]);
-        celt_lpc(st->lpc, ac, kLpcOrder);
-        while (1) {
-            int16_t tmp = kQ15One;
-            int32_t sum = 1 << kSigShift;
-            for (int i = 0; i < kLpcOrder; i++) sum += st->lpc[i] < 0 ? -st->lpc[i] : st->lpc[i];
-            if (sum < 65535) break;
-            for (int i = 0; i < kLpcOrder; i++) {
-                tmp = (int16_t)mul16_16_q15(32440, tmp);
-                st->lpc[i] = (int16_t)mul16_16_q15(st->lpc[i], tmp);
+    for (int c = 0; c < CC; c++) {
+        int32_t *buf = st->mem + kMemStride * c;
+        int16_t *lpc = st->lpc + kLpcOrder * c;
+        int16_t *exc = st->plc->exc + kLpcOrder;
+        for (int i = 0; i < kMaxPeriod + kLpcOrder; i++)
+            exc[i - kLpcOrder] = sround16(buf[kDecBuf - kMaxPeriod - kLpcOrder + i], kSigShift);
+        if (st->loss_duration == 0) {
+            int32_t ac[kLpcOrder + 1];
+            celt_autocorr(exc, ac, window, kOverlap, kLpcOrder, kMaxPeriod, st->plc->plc_a);
+            ac[0] += ac[0] >> 13;
+            for (int i = 1; i <= kLpcOrder; i++) ac[i] -= mul16_32_q15(2 * i * i, ac[i]);
+            celt_lpc(lpc, ac, kLpcOrder);
+            while (1) {
+                int16_t tmp = kQ15One;
+                int32_t sum = 1 << kSigShift;
+                for (int i = 0; i < kLpcOrder; i++) sum += lpc[i] < 0 ? -lpc[i] : lpc[i];
+                if (sum < 65535) break;
+                for (int i = 0; i < kLpcOrder; i++) {
+                    tmp = (int16_t)mul16_16_q15(32440, tmp);
+                    lpc[i] = (int16_t)mul16_16_q15(lpc[i], tmp);
+                }
             }
         }
-    }
-    celt_fir(exc + kMaxPeriod - exc_length, st->lpc, st->plc->fir_tmp, exc_length, kLpcOrder);
-    for (int i = 0; i < exc_length; i++) exc[kMaxPeriod - exc_length + i] = st->plc->fir_tmp[i];
-    int16_t decay;
-    {
-        int32_t E1 = 1, E2 = 1;
-        int shift = imax(0, 2 * celt_zlog2(maxabs16(&exc[kMaxPeriod - exc_length], exc_length)) - 20);
-        int decay_length = exc_length >> 1;
-        for (int i = 0; i < decay_length; i++) {
-            int16_t e = exc[kMaxPeriod - decay_length + i];
-            E1 += mul16_16(e, e) >> shift;
-            e = exc[kMaxPeriod - 2 * decay_length + i];
-            E2 += mul16_16(e, e) >> shift;
-        }
-        E1 = min32(E1, E2);
-        decay = (int16_t)celt_sqrt(frac_div32(E1 >> 1, E2));
-    }
-    for (int i = 0; i < kDecBuf - N; i++) buf[i] = buf[i + N];
-    const int extrapolation_offset = kMaxPeriod - pitch_index;
-    const int extrapolation_len = N + kOverlap;
-    int16_t attenuation = (int16_t)mul16_16_q15(fade, decay);
-    int32_t S1 = 0;
-    for (int i = 0, j = 0; i < extrapolation_len; i++, j++) {
-        if (j >= pitch_index) {
-            j -= pitch_index;
-            attenuation = (int16_t)mul16_16_q15(attenuation, decay);
-        }
-        buf[kDecBuf - N + i] = shl32((int32_t)(int16_t)mul16_16_q15(attenuation, exc[extrapolation_offset + j]),
-                                     kSigShift);
-        int16_t tmp = sround16(buf[kDecBuf - kMaxPeriod - N + extrapolation_offset + j], kSigShift);
-        S1 += mul16_16(tmp, tmp) >> 10;
-    }
-    {
-        int16_t lpc_mem[kLpcOrder];
-        for (int i = 0; i < kLpcOrder; i++) lpc_mem[i] = sround16(buf[kDecBuf - N - 1 - i], kSigShift);
-        celt_iir(buf + kDecBuf - N, st->lpc, buf + kDecBuf - N, extrapolation_len, kLpcOrder, lpc_mem, st->plc->plc_b);
-        for (int i = 0; i < extrapolation_len; i++) buf[kDecBuf - N + i] = saturate(buf[kDecBuf - N + i], kSigSat);
-    }
-    {
-        int32_t S2 = 0;
-        for (int i = 0; i < extrapolation_len; i++) {
-            int16_t tmp = sround16(buf[kDecBuf - N + i], kSigShift);
-            S2 += mul16_16(tmp, tmp) >> 10;
-        }
-        if (!(S1 > (S2 >> 2))) {
-            for (int i = 0; i < extrapolation_len; i++) buf[kDecBuf - N + i] = 0;
-        } else if (S1 < S2) {
-            int16_t ratio = (int16_t)celt_sqrt(frac_div32((S1 >> 1) + 1, S2 + 1));
-            for (int i = 0; i < kOverlap; i++) {
-                int16_t tmp_g = (int16_t)(kQ15One - mul16_16_q15(window[i], kQ15One - ratio));
-                buf[kDecBuf - N + i] = mul16_32_q15(tmp_g, buf[kDecBuf - N + i]);
+        celt_fir(exc + kMaxPeriod - exc_length, lpc, st->plc->fir_tmp, exc_length, kLpcOrder);
+        for (int i = 0; i < exc_length; i++) exc[kMaxPeriod - exc_length + i] = st->plc->fir_tmp[i];
+        int16_t decay;
+        {
+            int32_t E1 = 1, E2 = 1;
+            int shift = imax(0, 2 * celt_zlog2(maxabs16(&exc[kMaxPeriod - exc_length], exc_length)) - 20);
+            int decay_length = exc_length >> 1;
+            for (int i = 0; i < decay_length; i++) {
+                int16_t e = exc[kMaxPeriod - decay_length + i];
+                E1 += mul16_16(e, e) >> shift;
+                e = exc[kMaxPeriod - 2 * decay_length + i];
+                E2 += mul16_16(e, e) >> shift;
             }
-            for (int i = kOverlap; i < extrapolation_len; i++)
-                buf[kDecBuf - N + i] = mul16_32_q15(ratio, buf[kDecBuf - N + i]);
+            E1 = min32(E1, E2);
+            decay = (int16_t)celt_sqrt(frac_div32(E1 >> 1, E2));
+        }
+        for (int i = 0; i < kDecBuf - N; i++) buf[i] = buf[i + N];
+        const int extrapolation_offset = kMaxPeriod - pitch_index;
+        const int extrapolation_len = N + kOverlap;
+        int16_t attenuation = (int16_t)mul16_16_q15(fade, decay);
+        int32_t S1 = 0;
+        for (int i = 0, j = 0; i < extrapolation_len; i++, j++) {
+            if (j >= pitch_index) {
+                j -= pitch_index;
+                attenuation = (int16_t)mul16_16_q15(attenuation, decay);
+            }
+            buf[kDecBuf - N + i] =
+                shl32((int32_t)(int16_t)mul16_16_q15(attenuation, exc[extrapolation_offset + j]), kSigShift);
+            int16_t tmp = sround16(buf[kDecBuf - kMaxPeriod - N + extrapolation_offset + j], kSigShift);
+            S1 += mul16_16(tmp, tmp) >> 10;
+        }
+        {
+            int16_t lpc_mem[kLpcOrder];
+            for (int i = 0; i < kLpcOrder; i++) lpc_mem[i] = sround16(buf[kDecBuf - N - 1 - i], kSigShift);
+            celt_iir(buf + kDecBuf - N, lpc, buf + kDecBuf - N, extrapolation_len, kLpcOrder, lpc_mem,
+                     st->plc->plc_b);
+            for (int i = 0; i < extrapolation_len; i++)
+                buf[kDecBuf - N + i] = saturate(buf[kDecBuf - N + i], kSigSat);
+        }
+        {
+            int32_t S2 = 0;
+            for (int i = 0; i < extrapolation_len; i++) {
+                int16_t tmp = sround16(buf[kDecBuf - N + i], kSigShift);
+                S2 += mul16_16(tmp, tmp) >> 10;
+            }
+            if (!(S1 > (S2 >> 2))) {
+                for (int i = 0; i < extrapolation_len; i++) buf[kDecBuf - N + i] = 0;
+            } else if (S1 < S2) {
+                int16_t ratio = (int16_t)celt_sqrt(frac_div32((S1 >> 1) + 1, S2 + 1));
+                for (int i = 0; i < kOverlap; i++) {
+                    int16_t tmp_g = (int16_t)(kQ15One - mul16_16_q15(window[i], kQ15One - ratio));
+                    buf[kDecBuf - N + i] = mul16_32_q15(tmp_g, buf[kDecBuf - N + i]);
+                }
+                for (int i = kOverlap; i < extrapolation_len; i++)
+                    buf[kDecBuf - N + i] = mul16_32_q15(ratio, buf[kDecBuf - N + i]);
+            }
         }
     }
     return 0;

@@ -562,6 +562,7 @@ CUO_FN int32_t silk_mla(int32_t a, int32_t b, int32_t c) { return (int32_t)((uin
 CUO_FN int32_t silk_rshift_round(int32_t a, int s) { return s == 1 ? (a >> 1) + (a & 1) : ((a >> (s - 1)) + 1) >> 1; }
 CUO_FN int64_t silk_rshift_round64(int64_t a, int s) { return s == 1 ? (a >> 1) + (a & 1) : ((a >> (s - 1)) + 1) >> 1; }
 CUO_FN int32_t silk_lshift(int32_t a, int s) { return (int32_t)((uint32_t)a << s); }
+CUO_FN int32_t silk_add_lshift32(int32_t a, int32_t b, int s) { return (int32_t)((uint32_t)a + ((uint32_t)b << s)); }
 // silk_ADD_SAT32: the wrapped sum unless both operands share a sign the sum
 // lacks, then the limit of that sign (written without branches).
 CUO_FN int32_t silk_add_sat32(int32_t a, int32_t b)

@@ -1,7 +1,9 @@
 // opus_dec.h -- Opus packet / frame layer over the SILK and CELT decoders.
 //
-// Restates libopus 1.5.2 (vendor/src/opus_decoder.c) for a mono 48 kHz
-// fixed-point decoder: opus_decoder_init (:129-174), smooth_fade (:205-220),
+// Restates libopus 1.5.2 (vendor/src/opus_decoder.c) for a 48 kHz
+// fixed-point decoder of one output channel, or of one or two in the stereo
+// build (kMaxCh == 2; PCM interleaved): opus_decoder_init (:129-174),
+// smooth_fade (:205-220),
 // opus_decode_frame (:237-669: mode dispatch, celt_accum on the SILK PCM,
 // SILK/hybrid/CELT transitions with the 5 ms PLC crossfade, CELT redundancy in
 // both directions, rangeFinal = rng ^ redundant_rng), opus_decode_native
@@ -55,25 +57,28 @@ struct DecScratch {
     int16_t *pcm_silk;
     int16_t *pcm_transition;
     int16_t *redundant_audio;
+    int16_t *silk_rs_tmp;  // one channel's 48 kHz SILK output before interleaving (two output channels)
     RangeDec dec;  // the frame's range decoder (shared by SILK and CELT)
     int16_t frame_sizes[48];  // packet_parse output
     // One frame of output, caller-owned (HBM on the device): a frame decode
-    // writes at most st->frame_size <= 2880 samples (60 ms).
+    // writes at most st->frame_size <= 2880 samples (60 ms) per channel.
     int16_t *frame_out;
 };
 
-// Memory of a decoder that lives outside the state/scratch block.
+// Memory of a decoder that lives outside the state/scratch block (sizes per
+// output channel times kMaxCh).
 struct DecHeap {
-    int32_t celt_mem[2048 + 120];
-    int16_t frame_out[kMaxFrameOut];
+    int32_t celt_mem[kMaxCh * kMemStride];
+    int16_t frame_out[kMaxCh * kMaxFrameOut];
     CeltPlcScratch plc;
-    int16_t pcm_silk[480];
-    int16_t pcm_transition[240];
-    int16_t redundant_audio[240];
-    int32_t silk_cng_exc[kSilkMaxFrame];
-    int32_t celt_freq[960];
+    int16_t pcm_silk[kMaxCh * 480];
+    int16_t pcm_transition[kMaxCh * 240];
+    int16_t redundant_audio[kMaxCh * 240];
+    int32_t silk_cng_exc[kMaxCh * kSilkMaxFrame];
+    int32_t celt_freq[kMaxCh * 960];
     int32_t celt_etmp[kOverlap];
     int32_t silk_cng_sig[kSilkMaxFrame + kSilkMaxOrder];
+    int16_t silk_rs_tmp[kMaxCh == 1 ? 2 : 960];
 };
 
 CUO_FN void dec_bind(OpusDecState *st, DecScratch *sc, DecHeap *h)
@@ -87,17 +92,23 @@ CUO_FN void dec_bind(OpusDecState *st, DecScratch *sc, DecHeap *h)
     sc->pcm_silk = h->pcm_silk;
     sc->pcm_transition = h->pcm_transition;
     sc->redundant_audio = h->redundant_audio;
+    sc->silk_rs_tmp = h->silk_rs_tmp;
     sc->frame_out = h->frame_out;
 }
+
+// Output channels of the decoder; constant 1 in the mono build.
+CUO_FN int dec_cc(const OpusDecState *st) { return kMaxCh == 1 ? 1 : uniform(st->channels); }
 
 CUO_FN void opus_dec_init(OpusDecState *st, int channels)
 {
     st->channels = kMaxCh == 1 ? 1 : channels;
     silk_init_decoder(&st->silk);
-    celt_init(&st->celt);
+    celt_init(&st->celt, st->channels);
     st->silk_ctl.internalSampleRate = 0;
     st->silk_ctl.payloadSize_ms = 0;
-    st->stream_channels = 1;
+    st->silk_ctl.nChannelsInternal = 0;
+    st->silk_ctl.nChannelsAPI = st->channels;
+    st->stream_channels = st->channels;
     st->bandwidth = 0;
     st->mode = 0;
     st->prev_mode = 0;
@@ -227,7 +238,7 @@ CUO_FN int packet_parse(const uint8_t *data, int len, int16_t *size, int *payloa
     return count;
 }
 
-// opus_packet_has_lbrr (opus_decoder.c:1139-1162), mono
+// opus_packet_has_lbrr (opus_decoder.c:1139-1162)
 CUO_FN int packet_has_lbrr(const uint8_t *p, int len)
 {
     if (toc_mode(p[0]) == MODE_CELT_ONLY) return 0;
@@ -250,11 +261,15 @@ CUO_FN int packet_has_lbrr(const uint8_t *p, int len)
 // log2(10) / (20 * 256) in Q25: Q8 dB -> Q10 log2 (QCONST16(6.48814081e-4f, 25))
 constexpr int32_t kGainLog2Q25 = 21771;
 
-CUO_FN void smooth_fade(const int16_t *in1, const int16_t *in2, int16_t *out, int overlap)
+// smooth_fade (opus_decoder.c:205-220) over `overlap` samples of `channels`
+// interleaved channels
+CUO_FN void smooth_fade(const int16_t *in1, const int16_t *in2, int16_t *out, int overlap, int channels)
 {
-    CUO_PFOR(i, 0, overlap) {
+    channels = kMaxCh == 1 ? 1 : channels;
+    CUO_PFOR(e, 0, overlap * channels) {
+        const int i = channels == 1 ? e : e >> 1;
         int16_t w = (int16_t)mul16_16_q15(tab::kWindow[i], tab::kWindow[i]);
-        out[i] = (int16_t)((mul16_16(w, in2[i]) + mul16_16(kQ15One - w, in1[i])) >> 15);
+        out[e] = (int16_t)((mul16_16(w, in2[e]) + mul16_16(kQ15One - w, in1[e])) >> 15);
     }
 }
 
@@ -282,6 +297,7 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
     const int F20 = 960, F10 = 480, F5 = 240, F2_5 = 120;
     st = lds_uniform(st);
     sc = lds_uniform(sc);
+    const int CC = dec_cc(st);  // output channels, interleaved in pcm
     RangeDec &dec = sc->dec;
     if (frame_size < F2_5) return OPUS_BUFFER_TOO_SMALL;
     frame_size = imin(frame_size, 48000 / 25 * 3);
@@ -304,7 +320,7 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
         mode = st->prev_redundancy ? MODE_CELT_ONLY : st->prev_mode;
         bandwidth = 0;
         if (mode == 0) {
-            CUO_PFOR(i, 0, audiosize) pcm[i] = 0;
+            CUO_PFOR(i, 0, audiosize * CC) pcm[i] = 0;
             return audiosize;
         }
         if constexpr (Kind == kFrameConceal) {
@@ -313,7 +329,7 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
                 do {
                     int ret = frame_decode<kFrameConcealLeaf>(st, nullptr, 0, pcm, imin(audiosize, F20), 0, sc);
                     if (ret < 0) return ret;
-                    pcm += ret;
+                    pcm += ret * CC;
                     audiosize -= ret;
                 } while (audiosize > 0);
                 return frame_size;
@@ -346,6 +362,7 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
         if (st->prev_mode == MODE_CELT_ONLY) silk_reset_decoder(&st->silk);
         st->silk_ctl.payloadSize_ms = imax(10, 1000 * audiosize / 48000);
         if (data != nullptr) {
+            st->silk_ctl.nChannelsInternal = st->stream_channels;
             if (mode == MODE_SILK_ONLY) {
                 if (bandwidth == BW_NB) st->silk_ctl.internalSampleRate = 8000;
                 else if (bandwidth == BW_MB) st->silk_ctl.internalSampleRate = 12000;
@@ -360,16 +377,16 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
             int32_t silk_frame_size;
             int first_frame = decoded == 0;
             int silk_ret = silk_decode(&st->silk, &st->silk_ctl, lost_flag, first_frame, &dec, pcm_ptr,
-                                       &silk_frame_size, &sc->silk);
+                                       &silk_frame_size, &sc->silk, sc->silk_rs_tmp);
             if (silk_ret) {
                 if (lost_flag) {
                     silk_frame_size = frame_size;
-                    CUO_PFOR(i, 0, frame_size) pcm_ptr[i] = 0;
+                    CUO_PFOR(i, 0, frame_size * CC) pcm_ptr[i] = 0;
                 } else {
                     return OPUS_INTERNAL_ERROR;
                 }
             }
-            pcm_ptr += silk_frame_size;
+            pcm_ptr += silk_frame_size * CC;
             decoded += silk_frame_size;
         } while (decoded < frame_size);
     }
@@ -408,6 +425,7 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
         }
         st->celt.end = endband;
     }
+    st->celt.stream_channels = st->stream_channels;  // CELT_SET_CHANNELS
     int16_t *redundant_audio = sc->redundant_audio;
     uint32_t redundant_rng = 0;
     if (HasData && redundancy && celt_to_silk) {
@@ -424,37 +442,38 @@ CUO_FN int frame_decode(OpusDecState *st, const uint8_t *data, int len, int16_t 
                                data != nullptr ? &dec : nullptr, celt_accum, &sc->celt);
     } else {
         alignas(4) static constexpr uint8_t silence[4] = {0xFF, 0xFF, 0, 0};  // a 2-byte frame
-        if (!celt_accum) CUO_PFOR(i, 0, frame_size) pcm[i] = 0;
+        if (!celt_accum) CUO_PFOR(i, 0, frame_size * CC) pcm[i] = 0;
         if (st->prev_mode == MODE_HYBRID && !(redundancy && celt_to_silk && st->prev_redundancy)) {
             st->celt.start = 0;
             celt_decode(&st->celt, silence, 2, pcm, F2_5, nullptr, celt_accum, &sc->celt);
         }
     }
-    if (mode != MODE_CELT_ONLY && !celt_accum) CUO_PFOR(i, 0, frame_size) pcm[i] = sat16(pcm[i] + pcm_silk[i]);
+    if (mode != MODE_CELT_ONLY && !celt_accum) CUO_PFOR(i, 0, frame_size * CC) pcm[i] = sat16(pcm[i] + pcm_silk[i]);
     if (HasData && redundancy && !celt_to_silk) {
         celt_reset(&st->celt);
         st->celt.start = 0;
         celt_decode(&st->celt, data + len, redundancy_bytes, redundant_audio, F5, nullptr, 0, &sc->celt);
         redundant_rng = st->celt.rng;
-        smooth_fade(pcm + frame_size - F2_5, redundant_audio + F2_5, pcm + frame_size - F2_5, F2_5);
+        smooth_fade(pcm + CC * (frame_size - F2_5), redundant_audio + CC * F2_5, pcm + CC * (frame_size - F2_5),
+                    F2_5, CC);
     }
     if (HasData && redundancy && celt_to_silk && (st->prev_mode != MODE_SILK_ONLY || st->prev_redundancy)) {
-        CUO_PFOR(i, 0, F2_5) pcm[i] = redundant_audio[i];
-        smooth_fade(redundant_audio + F2_5, pcm + F2_5, pcm + F2_5, F2_5);
+        CUO_PFOR(i, 0, F2_5 * CC) pcm[i] = redundant_audio[i];
+        smooth_fade(redundant_audio + CC * F2_5, pcm + CC * F2_5, pcm + CC * F2_5, F2_5, CC);
     }
     if (HasData && transition) {
         if (audiosize >= F5) {
-            CUO_PFOR(i, 0, F2_5) pcm[i] = pcm_transition[i];
-            smooth_fade(pcm_transition + F2_5, pcm + F2_5, pcm + F2_5, F2_5);
+            CUO_PFOR(i, 0, F2_5 * CC) pcm[i] = pcm_transition[i];
+            smooth_fade(pcm_transition + CC * F2_5, pcm + CC * F2_5, pcm + CC * F2_5, F2_5, CC);
         } else {
-            smooth_fade(pcm_transition, pcm, pcm, F2_5);
+            smooth_fade(pcm_transition, pcm, pcm, F2_5, CC);
         }
     }
     if (st->decode_gain) {
         // OPUS_SET_GAIN (opus_decoder.c:639-648): gain = 2^(decode_gain / (20 log10(2) 256)), applied
         // as a Q16 multiply with saturation
         const int32_t gain = celt_exp2((int16_t)mul16_16_p15(kGainLog2Q25, st->decode_gain));
-        CUO_PFOR(i, 0, frame_size) pcm[i] = (int16_t)saturate(mul16_32_p16(pcm[i], gain), 32767);
+        CUO_PFOR(i, 0, frame_size * CC) pcm[i] = (int16_t)saturate(mul16_32_p16(pcm[i], gain), 32767);
     }
     st->rangeFinal = len <= 1 ? 0 : (HasData ? dec.rng ^ redundant_rng : 0);
     st->prev_mode = mode;
@@ -500,7 +519,8 @@ CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, Ou
     int offset;
     int count = packet_parse(data, len, size, &offset);
     if (count < 0) return count;
-    if (packet_stream_channels != 1) return OPUS_UNIMPLEMENTED;  // mono streams only
+    // the mono build keeps one coded channel: stereo packets go to the stereo build
+    if (kMaxCh == 1 && packet_stream_channels != 1) return OPUS_UNIMPLEMENTED;
     data += offset;
     int nframes = count;
     if (decode_fec) {

@@ -88,12 +88,15 @@ struct SilkChannel {
     SilkPlcState plc;
 };
 
+// silk_decoder (vendor/silk/structs.h): the mid (and side) channel states
+// and the stereo unmixing state.
 struct SilkDecoder {
-    SilkChannel ch;
-    int16_t sMid[2];
+    SilkChannel ch[kMaxCh];
+    int16_t sMid[2], sSide[2];     // stereo_dec_state: last two samples of mid and side
+    int32_t pred_prev_Q13[2];      // stereo_dec_state: previous mid/side predictors
     int32_t nChannelsAPI, nChannelsInternal, prev_decode_only_middle;
     int32_t *cng_sig;  // caller-owned comfort-noise buffer (kept by init and reset)
-    int32_t *cng_exc;  // caller-owned CNG excitation history (kSilkMaxFrame), part of the channel state
+    int32_t *cng_exc;  // caller-owned CNG excitation histories (kMaxCh x kSilkMaxFrame), part of the channel states
 };
 
 struct SilkCtrl {
@@ -111,7 +114,7 @@ struct SilkScratch {
     int32_t sLTP_Q15[2 * kSilkMaxFrame];
     int32_t res_Q14[80];
     int32_t sLPC_Q14[80 + kSilkMaxOrder];
-    int16_t frame_out[kSilkMaxFrame + 2];     // internal-rate frame + 2-sample history
+    int16_t frame_out[kMaxCh][kSilkMaxFrame + 2];  // internal-rate frame + 2-sample history, per channel
     int16_t exc_buf[160];
     int16_t rs_buf[2 * 160 + 8];
     int32_t *cng_sig;  // comfort-noise synthesis buffer (kSilkMaxFrame + kSilkMaxOrder), caller-owned
@@ -629,30 +632,36 @@ CUO_FN void silk_reset_channel(SilkChannel *ch)
 
 // The CNG excitation history is channel state kept outside the channel
 // struct: a channel reset clears it too.
-CUO_FN void silk_reset_cng_exc(SilkDecoder *d)
+CUO_FN void silk_reset_channel_of(SilkDecoder *d, int n)
 {
-    int32_t *e = d->cng_exc;
+    silk_reset_channel(&d->ch[n]);
+    int32_t *e = d->cng_exc + kSilkMaxFrame * n;
     CUO_PFOR(i, 0, kSilkMaxFrame) e[i] = 0;
 }
 
-CUO_FN void silk_init_decoder(SilkDecoder *d)
+CUO_FN void silk_reset_stereo(SilkDecoder *d)
 {
-    silk_reset_channel(&d->ch);
-    silk_reset_cng_exc(d);
     d->sMid[0] = d->sMid[1] = 0;
-    d->nChannelsAPI = 0;
-    d->nChannelsInternal = 0;
+    d->sSide[0] = d->sSide[1] = 0;
+    d->pred_prev_Q13[0] = d->pred_prev_Q13[1] = 0;
     d->prev_decode_only_middle = 0;
 }
 
-// silk_ResetDecoder (dec_API.c:77-92): resets channel state and the stereo
+// silk_InitDecoder (dec_API.c:59-75), on a zeroed decoder
+CUO_FN void silk_init_decoder(SilkDecoder *d)
+{
+    for (int n = 0; n < kMaxCh; n++) silk_reset_channel_of(d, n);
+    silk_reset_stereo(d);
+    d->nChannelsAPI = 0;
+    d->nChannelsInternal = 0;
+}
+
+// silk_ResetDecoder (dec_API.c:77-92): resets channel states and the stereo
 // memory but keeps nChannelsAPI / nChannelsInternal.
 CUO_FN void silk_reset_decoder(SilkDecoder *d)
 {
-    silk_reset_channel(&d->ch);
-    silk_reset_cng_exc(d);
-    d->sMid[0] = d->sMid[1] = 0;
-    d->prev_decode_only_middle = 0;
+    for (int n = 0; n < kMaxCh; n++) silk_reset_channel_of(d, n);
+    silk_reset_stereo(d);
 }
 
 CUO_FN void silk_set_fs(SilkChannel *ch, int fs_kHz, int32_t fs_API_Hz)
@@ -1823,85 +1832,242 @@ CUO_FN void silk_decode_frame(SilkChannel *ch, RangeDec *d, int16_t *pOut, int32
 struct SilkControl {
     int32_t internalSampleRate;
     int32_t payloadSize_ms;
+    int32_t nChannelsInternal;  // coded channels of the packet (1, or 2 in the stereo build)
+    int32_t nChannelsAPI;       // output channels
 };
 
-// silk_Decode (dec_API.c:132-452), mono in / mono out at 48 kHz.
-// Returns 0 or a negative SILK error.
+// silk_stereo_decode_pred (stereo_decode_pred.c:34-67): mid/side predictors
+CUO_FN void silk_stereo_decode_pred(RangeDec *d, int32_t *pred0, int32_t *pred1)
+{
+    int n = ec_icdf(d, tab::kStereoPredJointIcdf, 8);
+    int ix02 = n / 5, ix12 = n - 5 * ix02;
+    int ix00 = ec_icdf(d, tab::kUniform3Icdf, 8);
+    int ix01 = ec_icdf(d, tab::kUniform5Icdf, 8);
+    int ix10 = ec_icdf(d, tab::kUniform3Icdf, 8);
+    int ix11 = ec_icdf(d, tab::kUniform5Icdf, 8);
+    int32_t p[2];
+    const int ix[2][3] = {{ix00, ix01, ix02}, {ix10, ix11, ix12}};
+    for (int k = 0; k < 2; k++) {
+        const int i0 = ix[k][0] + 3 * ix[k][2];
+        const int32_t low_Q13 = ld_s16(tab::kStereoPredQuant_Q13, i0);
+        const int32_t step_Q13 = silk_smulwb(ld_s16(tab::kStereoPredQuant_Q13, i0 + 1) - low_Q13,
+                                             silk_fix(0.5 / 5, 16));  // 0.5 / STEREO_QUANT_SUB_STEPS
+        p[k] = silk_smlabb(low_Q13, step_Q13, 2 * ix[k][1] + 1);
+    }
+    *pred0 = p[0] - p[1];
+    *pred1 = p[1];
+}
+
+// silk_stereo_MS_to_LR (stereo_MS_to_LR.c:34-85): x1 / x2 hold two samples
+// of history then the frame; the side gets the interpolated prediction from
+// the mid, then both become left / right.  Lane-parallel: every output
+// depends only on inputs of the same sample and its two neighbours.
+CUO_FN void silk_stereo_ms_to_lr(SilkDecoder *dec, int16_t *x1, int16_t *x2, int32_t pred0_Q13, int32_t pred1_Q13,
+                                 int fs_kHz, int frame_length)
+{
+    x1[0] = dec->sMid[0]; x1[1] = dec->sMid[1];
+    x2[0] = dec->sSide[0]; x2[1] = dec->sSide[1];
+    const int16_t m0 = x1[frame_length], m1 = x1[frame_length + 1];
+    const int16_t s0 = x2[frame_length], s1 = x2[frame_length + 1];
+    dec->sMid[0] = m0; dec->sMid[1] = m1;
+    dec->sSide[0] = s0; dec->sSide[1] = s1;
+    const int32_t p0 = dec->pred_prev_Q13[0], p1 = dec->pred_prev_Q13[1];
+    const int interp = 8 * fs_kHz;  // STEREO_INTERP_LEN_MS
+    const int32_t denom_Q16 = (1 << 16) / interp;
+    const int32_t delta0_Q13 = silk_rshift_round(silk_smulbb(pred0_Q13 - p0, denom_Q16), 16);
+    const int32_t delta1_Q13 = silk_rshift_round(silk_smulbb(pred1_Q13 - p1, denom_Q16), 16);
+    // side[n + 1] from mid[n .. n + 2] and side[n + 1]: computed into
+    // registers first, stored after, so no lane reads an updated side sample
+    for (int n0 = 0; n0 < frame_length; n0 += kLanes) {
+        const int n = n0 + lane_id();
+        int16_t out = 0;
+        if (n < frame_length) {
+            int32_t q0, q1;
+            if (n < interp) { q0 = p0 + (n + 1) * delta0_Q13; q1 = p1 + (n + 1) * delta1_Q13; }
+            else { q0 = pred0_Q13; q1 = pred1_Q13; }
+            int32_t sum = silk_lshift(silk_add_lshift32(x1[n] + (int32_t)x1[n + 2], x1[n + 1], 1), 9);
+            sum = silk_smlawb(silk_lshift((int32_t)x2[n + 1], 8), sum, q0);
+            sum = silk_smlawb(sum, silk_lshift((int32_t)x1[n + 1], 11), q1);
+            out = (int16_t)silk_sat16(silk_rshift_round(sum, 8));
+        }
+        if (n < frame_length) x2[n + 1] = out;
+    }
+    dec->pred_prev_Q13[0] = pred0_Q13;
+    dec->pred_prev_Q13[1] = pred1_Q13;
+    CUO_PFOR(n, 0, frame_length) {
+        const int32_t sum = x1[n + 1] + (int32_t)x2[n + 1];
+        const int32_t diff = x1[n + 1] - (int32_t)x2[n + 1];
+        x1[n + 1] = (int16_t)silk_sat16(sum);
+        x2[n + 1] = (int16_t)silk_sat16(diff);
+    }
+}
+
+// silk_Decode (dec_API.c:132-452) at 48 kHz: one or two coded channels (mid /
+// side) to one or two output channels (interleaved).  A stereo stream to one
+// output channel renders the mid channel.  Returns 0 or a negative SILK
+// error.
 CUO_FN int silk_decode(SilkDecoder *dec, const SilkControl *ctl, int lostFlag, int newPacketFlag, RangeDec *d,
-                       int16_t *samplesOut, int32_t *nSamplesOut, SilkScratch *sc)
+                       int16_t *samplesOut, int32_t *nSamplesOut, SilkScratch *sc, int16_t *rs_tmp)
 {
     CUO_PROF_SCOPE(kProfSilk);
     dec = lds_uniform(dec);
     ctl = lds_uniform(ctl);
     d = lds_uniform(d);
     sc = lds_uniform(sc);
-    SilkChannel *ch = &dec->ch;
+    const int nInt = kMaxCh == 1 ? 1 : uniform(ctl->nChannelsInternal);
+    const int nAPI = kMaxCh == 1 ? 1 : uniform(ctl->nChannelsAPI);
+    SilkChannel *ch0 = &dec->ch[0];
     int ret = 0;
     sc->cng_sig = dec->cng_sig;
-    sc->cng_exc = dec->cng_exc;
-    if (newPacketFlag) ch->nFramesDecoded = 0;
-    if (1 > dec->nChannelsInternal) {
-        // growing to 1 internal channel only touches channel_state[1] in
-        // libopus, which a mono stream never uses
-    }
-    if (ch->nFramesDecoded == 0) {
-        switch (ctl->payloadSize_ms) {
-        case 0: case 10: ch->nFramesPerPacket = 1; ch->nb_subfr = 2; break;
-        case 20: ch->nFramesPerPacket = 1; ch->nb_subfr = 4; break;
-        case 40: ch->nFramesPerPacket = 2; ch->nb_subfr = 4; break;
-        case 60: ch->nFramesPerPacket = 3; ch->nb_subfr = 4; break;
-        default: return -203;  // SILK_DEC_INVALID_FRAME_SIZE
+    if (newPacketFlag)
+        for (int n = 0; n < nInt; n++) dec->ch[n].nFramesDecoded = 0;
+    if (kMaxCh == 2 && nInt > dec->nChannelsInternal) silk_reset_channel_of(dec, kMaxCh - 1);
+    const int stereo_to_mono = kMaxCh == 2 && nInt == 1 && dec->nChannelsInternal == 2 &&
+                               ctl->internalSampleRate == 1000 * ch0->fs_kHz;
+    if (ch0->nFramesDecoded == 0) {
+        for (int n = 0; n < nInt; n++) {
+            SilkChannel *ch = &dec->ch[n];
+            switch (ctl->payloadSize_ms) {
+            case 0: case 10: ch->nFramesPerPacket = 1; ch->nb_subfr = 2; break;
+            case 20: ch->nFramesPerPacket = 1; ch->nb_subfr = 4; break;
+            case 40: ch->nFramesPerPacket = 2; ch->nb_subfr = 4; break;
+            case 60: ch->nFramesPerPacket = 3; ch->nb_subfr = 4; break;
+            default: return -203;  // SILK_DEC_INVALID_FRAME_SIZE
+            }
+            int fs_kHz_dec = (ctl->internalSampleRate >> 10) + 1;
+            if (fs_kHz_dec != 8 && fs_kHz_dec != 12 && fs_kHz_dec != 16) return -200;
+            silk_set_fs(ch, fs_kHz_dec, 48000);
         }
-        int fs_kHz_dec = (ctl->internalSampleRate >> 10) + 1;
-        if (fs_kHz_dec != 8 && fs_kHz_dec != 12 && fs_kHz_dec != 16) return -200;
-        silk_set_fs(ch, fs_kHz_dec, 48000);
     }
-    dec->nChannelsAPI = 1;
-    dec->nChannelsInternal = 1;
-    if (lostFlag != FLAG_LOST && ch->nFramesDecoded == 0) {
-        for (int i = 0; i < ch->nFramesPerPacket; i++) ch->VAD_flags[i] = ec_bit_logp(d, 1);
-        ch->LBRR_flag = ec_bit_logp(d, 1);
-        for (int i = 0; i < 3; i++) ch->LBRR_flags[i] = 0;
-        if (ch->LBRR_flag) {
-            if (ch->nFramesPerPacket == 1) {
-                ch->LBRR_flags[0] = 1;
-            } else {
-                const uint8_t *icdf = ch->nFramesPerPacket == 2 ? tab::kLbrrFlags2Icdf : tab::kLbrrFlags3Icdf;
-                int sym = ec_icdf(d, icdf, 8) + 1;
-                for (int i = 0; i < ch->nFramesPerPacket; i++) ch->LBRR_flags[i] = (sym >> i) & 1;
+    if (kMaxCh == 2 && nAPI == 2 && nInt == 2 && (dec->nChannelsAPI == 1 || dec->nChannelsInternal == 1)) {
+        dec->pred_prev_Q13[0] = dec->pred_prev_Q13[1] = 0;
+        dec->sSide[0] = dec->sSide[1] = 0;
+        dec->ch[kMaxCh - 1].rs = ch0->rs;
+    }
+    dec->nChannelsAPI = nAPI;
+    dec->nChannelsInternal = nInt;
+    int decode_only_middle = 0;
+    if (lostFlag != FLAG_LOST && ch0->nFramesDecoded == 0) {
+        for (int n = 0; n < nInt; n++) {
+            SilkChannel *ch = &dec->ch[n];
+            for (int i = 0; i < ch->nFramesPerPacket; i++) ch->VAD_flags[i] = ec_bit_logp(d, 1);
+            ch->LBRR_flag = ec_bit_logp(d, 1);
+        }
+        for (int n = 0; n < nInt; n++) {
+            SilkChannel *ch = &dec->ch[n];
+            for (int i = 0; i < 3; i++) ch->LBRR_flags[i] = 0;
+            if (ch->LBRR_flag) {
+                if (ch->nFramesPerPacket == 1) {
+                    ch->LBRR_flags[0] = 1;
+                } else {
+                    const uint8_t *icdf = ch->nFramesPerPacket == 2 ? tab::kLbrrFlags2Icdf : tab::kLbrrFlags3Icdf;
+                    int sym = ec_icdf(d, icdf, 8) + 1;
+                    for (int i = 0; i < ch->nFramesPerPacket; i++) ch->LBRR_flags[i] = (sym >> i) & 1;
+                }
             }
         }
         if (lostFlag == FLAG_NORMAL) {
             // skip the LBRR payload of every frame in this packet
-            for (int i = 0; i < ch->nFramesPerPacket; i++) {
-                if (ch->LBRR_flags[i]) {
-                    int cc = (i > 0 && ch->LBRR_flags[i - 1]) ? CODE_COND : CODE_INDEP;
-                    silk_decode_indices(ch, d, i, 1, cc);
-                    silk_decode_pulses(d, sc->pulses, ch->indices.signalType, ch->indices.quantOffsetType,
-                                       ch->frame_length);
+            for (int i = 0; i < ch0->nFramesPerPacket; i++) {
+                for (int n = 0; n < nInt; n++) {
+                    SilkChannel *ch = &dec->ch[n];
+                    if (ch->LBRR_flags[i]) {
+                        if (nInt == 2 && n == 0) {
+                            int32_t q0, q1;
+                            silk_stereo_decode_pred(d, &q0, &q1);
+                            if (dec->ch[kMaxCh - 1].LBRR_flags[i] == 0)
+                                decode_only_middle = ec_icdf(d, tab::kStereoOnlyCodeMidIcdf, 8);
+                        }
+                        int cc = (i > 0 && ch->LBRR_flags[i - 1]) ? CODE_COND : CODE_INDEP;
+                        silk_decode_indices(ch, d, i, 1, cc);
+                        silk_decode_pulses(d, sc->pulses, ch->indices.signalType, ch->indices.quantOffsetType,
+                                           ch->frame_length);
+                    }
                 }
             }
         }
     }
-    int16_t *tmp = sc->frame_out;   // [0..1] previous samples, [2..] this frame
-    int32_t nSamplesOutDec;
-    {
-        int FrameIndex = ch->nFramesDecoded;
-        int condCoding;
-        if (FrameIndex <= 0) condCoding = CODE_INDEP;
-        else if (lostFlag == FLAG_LBRR) condCoding = ch->LBRR_flags[FrameIndex - 1] ? CODE_COND : CODE_INDEP;
-        else condCoding = CODE_COND;
-        silk_decode_frame(ch, d, &tmp[2], &nSamplesOutDec, lostFlag, condCoding, sc);
+    int32_t pred0 = 0, pred1 = 0;
+    if (nInt == 2) {
+        SilkChannel *ch1 = &dec->ch[kMaxCh - 1];
+        const int fi = ch0->nFramesDecoded;
+        if (lostFlag == FLAG_NORMAL || (lostFlag == FLAG_LBRR && ch0->LBRR_flags[fi] == 1)) {
+            silk_stereo_decode_pred(d, &pred0, &pred1);
+            if ((lostFlag == FLAG_NORMAL && ch1->VAD_flags[fi] == 0) ||
+                (lostFlag == FLAG_LBRR && ch1->LBRR_flags[fi] == 0))
+                decode_only_middle = ec_icdf(d, tab::kStereoOnlyCodeMidIcdf, 8);
+            else
+                decode_only_middle = 0;
+        } else {
+            pred0 = dec->pred_prev_Q13[0];
+            pred1 = dec->pred_prev_Q13[1];
+        }
+        if (decode_only_middle == 0 && dec->prev_decode_only_middle == 1) {
+            // the side resumes after mid-only frames: from a clean history
+            for (int i = 0; i < kSilkMaxFrame + 2 * 80; i++) ch1->outBuf[i] = 0;
+            for (int i = 0; i < kSilkMaxOrder; i++) ch1->sLPC_Q14_buf[i] = 0;
+            ch1->lagPrev = 100;
+            ch1->LastGainIndex = 10;
+            ch1->prevSignalType = SIG_NO_VOICE;
+            ch1->first_frame_after_reset = 1;
+        }
+    }
+    int has_side;
+    if (lostFlag == FLAG_NORMAL)
+        has_side = !decode_only_middle;
+    else
+        has_side = !dec->prev_decode_only_middle ||
+                   (nInt == 2 && lostFlag == FLAG_LBRR &&
+                    dec->ch[kMaxCh - 1].LBRR_flags[dec->ch[kMaxCh - 1].nFramesDecoded] == 1);
+    int32_t nSamplesOutDec = 0;
+    for (int n = 0; n < nInt; n++) {
+        SilkChannel *ch = &dec->ch[n];
+        int16_t *fo = sc->frame_out[n];   // [0..1] previous samples, [2..] this frame
+        if (n == 0 || has_side) {
+            const int FrameIndex = ch0->nFramesDecoded - n;
+            int condCoding;
+            if (FrameIndex <= 0) condCoding = CODE_INDEP;
+            else if (lostFlag == FLAG_LBRR) condCoding = ch->LBRR_flags[FrameIndex - 1] ? CODE_COND : CODE_INDEP;
+            else if (n > 0 && dec->prev_decode_only_middle) condCoding = CODE_INDEP_NO_LTP_SCALING;
+            else condCoding = CODE_COND;
+            sc->cng_exc = dec->cng_exc + kSilkMaxFrame * n;
+            silk_decode_frame(ch, d, &fo[2], &nSamplesOutDec, lostFlag, condCoding, sc);
+        } else {
+            CUO_PFOR(i, 0, nSamplesOutDec) fo[2 + i] = 0;
+        }
         ch->nFramesDecoded++;
     }
-    tmp[0] = dec->sMid[0];
-    tmp[1] = dec->sMid[1];
-    dec->sMid[0] = tmp[nSamplesOutDec];
-    dec->sMid[1] = tmp[nSamplesOutDec + 1];
-    *nSamplesOut = nSamplesOutDec * 48000 / silk_smulbb(ch->fs_kHz, 1000);
-    silk_resample(&ch->rs, samplesOut, &tmp[1], nSamplesOutDec, sc->rs_buf);
-    if (lostFlag == FLAG_LOST) ch->LastGainIndex = 10;
-    else dec->prev_decode_only_middle = 0;
+    int16_t *tmp0 = sc->frame_out[0];
+    if (kMaxCh == 2 && nAPI == 2 && nInt == 2) {
+        silk_stereo_ms_to_lr(dec, tmp0, sc->frame_out[kMaxCh - 1], pred0, pred1, ch0->fs_kHz, nSamplesOutDec);
+    } else {
+        tmp0[0] = dec->sMid[0];
+        tmp0[1] = dec->sMid[1];
+        dec->sMid[0] = tmp0[nSamplesOutDec];
+        dec->sMid[1] = tmp0[nSamplesOutDec + 1];
+    }
+    const int32_t nOut = nSamplesOutDec * 48000 / silk_smulbb(ch0->fs_kHz, 1000);
+    *nSamplesOut = nOut;
+    if (kMaxCh == 1 || nAPI == 1) {
+        silk_resample(&ch0->rs, samplesOut, &tmp0[1], nSamplesOutDec, sc->rs_buf);
+    } else {
+        // two output channels: each resampled on its own, then interleaved
+        for (int n = 0; n < 2; n++) {
+            const bool own = n < nInt;
+            if (!own && !stereo_to_mono) {
+                CUO_PFOR(i, 0, nOut) samplesOut[2 * i + 1] = samplesOut[2 * i];
+                break;
+            }
+            const int16_t *in = own ? &sc->frame_out[n][1] : &tmp0[1];
+            silk_resample(&dec->ch[n < kMaxCh ? n : 0].rs, rs_tmp, in, nSamplesOutDec, sc->rs_buf);
+            CUO_PFOR(i, 0, nOut) samplesOut[2 * i + n] = rs_tmp[i];
+        }
+    }
+    if (lostFlag == FLAG_LOST) {
+        for (int n = 0; n < nInt; n++) dec->ch[n].LastGainIndex = 10;
+    } else {
+        dec->prev_decode_only_middle = decode_only_middle;
+    }
     return ret;
 }
 

@@ -271,6 +271,7 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
             }
             d->sample_off[k] = cum;
             cum += pkt_samples48(d->data.data() + d->off[k], d->len[k]);
+            if (p.len > 0 && (d->data[d->off[k]] & 0x4)) d->stereo_packets++;
         }
         lk.npkts = (uint32_t)k - lk.first;
         lk.toc_total = cum;

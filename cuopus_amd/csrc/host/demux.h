@@ -77,8 +77,13 @@ struct Demuxed {
     int64_t pcm_samples_48k = 0;    // playable samples, all links
     uint32_t n_pages = 0;
     uint32_t granule_checked = 0, granule_match = 0;
+    uint32_t stereo_packets = 0;    // packets whose TOC codes two channels
     DemuxStats stats;
 };
+
+// Whether decoding needs the stereo-capable decoder build: a stereo stream,
+// or a mono stream that holds stereo packets (decoded to one channel).
+inline bool needs_stereo_build(const Demuxed &d) { return d.channels == 2 || d.stereo_packets > 0; }
 
 struct Chain {
     uint32_t first, npkts, trim, keep;

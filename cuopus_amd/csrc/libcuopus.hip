@@ -392,7 +392,7 @@ int merge(Plan &pl, Err &e)
         if (fk != f.keep) return e.set(CUOPUS_RC_SOFT, "internal: keep mismatch");
         // each file's PCM starts 16-byte aligned (the resampler's wide loads)
         cat += (fk * (uint64_t)f.out_channels + 7) & ~7ull;
-        pl.stereo = pl.stereo || f.d.channels == 2;
+        pl.stereo = pl.stereo || cuo_host::needs_stereo_build(f.d);
         keep_sum += fk;
         db += f.d.data.size();
         pb += f.d.off.size();
@@ -1622,7 +1622,7 @@ struct SrvNeed {
         cat += (r->fp.keep + 7) & ~7ull;
         files += r->out16 > 0;
         lost = lost || !r->lost.empty();
-        stereo = stereo || r->fp.d.channels == 2;
+        stereo = stereo || cuo_host::needs_stereo_build(r->fp.d);
     }
 };
 

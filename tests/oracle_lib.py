@@ -216,6 +216,29 @@ GAIN_RECIPES = {
 }
 
 
+# Stereo streams (OpusHead channels 2): CELT at high rate (dual stereo) and low rate (intensity
+# stereo, phase inversion, some mono packets), SILK mid/side with mid-only frames, SILK at 10 kbps
+# (coded as mono packets throughout), hybrid, mode switching with mono packets inside the stereo
+# stream, SILK switching between stereo and mono packets, FEC + loss, and the output gain
+ST_RECIPES = {
+    "st_celt": "--seed 50 --dur 6 --signal music --app lowdelay --bw fb --bitrate 96000 "
+               "--frameset 120,240,480,960 --channels 2",
+    "st_celt_lo": "--seed 51 --dur 6 --signal music --app lowdelay --bw fb --bitrate 24000 "
+                  "--frameset 120,240,480,960 --channels 2",
+    "st_silk": "--seed 52 --dur 6 --signal speech --app voip --bw wb --force silk --bitrate 24000 --channels 2",
+    "st_silk_lo": "--seed 53 --dur 6 --signal speech --app voip --bw nb --force silk --bitrate 10000 --channels 2",
+    "st_hybrid": "--seed 54 --dur 6 --signal speech --app voip --bw fb --force hybrid --bitrate 40000 --channels 2",
+    "st_mixed": "--seed 55 --dur 10 --signal mixed --app audio --bw fb --brange 12000 96000 --channels 2 "
+                "--monoflip 0.3",
+    "st_fec": "--seed 56 --dur 8 --signal speech --app voip --bw fb --force hybrid --bitrate 32000 --fec 1 "
+              "--loss 5 --channels 2",
+    "st_silk_flip": "--seed 57 --dur 8 --signal speech --app voip --bw wb --force silk --bitrate 20000 "
+                    "--channels 2 --monoflip 0.3",
+    "st_gain": "--seed 58 --dur 6 --signal mixed --app audio --bw fb --brange 16000 80000 --channels 2 "
+               "--gain -900",
+}
+
+
 # More seeds of the mode-switching mixes (SILK/hybrid/CELT transitions, redundancy, FEC): decoded with
 # seeded loss by one CPU-gate test and one GPU test each
 SEED_SWEEP = {
@@ -227,7 +250,7 @@ SEED_SWEEP = {
 
 
 def recipe(name):
-    for table in (RECIPES, SEED_SWEEP, GAIN_RECIPES):
+    for table in (RECIPES, SEED_SWEEP, GAIN_RECIPES, ST_RECIPES):
         if name in table:
             return table[name]
     raise KeyError(name)
