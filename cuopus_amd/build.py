@@ -72,22 +72,26 @@ def build_hostgate():
 
 
 def build_fuzz_gate():
-    """tests/hostgate/fuzz_gate: the decoder source on the CPU under
-    AddressSanitizer + UndefinedBehaviorSanitizer (host code only; the
-    sanitizer flags go to the host compilation), linked to the oracle."""
+    """tests/hostgate/fuzz_gate and fuzz_gate_st (the stereo build): the
+    decoder source on the CPU under AddressSanitizer +
+    UndefinedBehaviorSanitizer (host code only; the sanitizer flags go to the
+    host compilation), linked to the oracle."""
     ref = os.path.join(ROOT, "oracle", "_ref")
     if not os.path.exists(os.path.join(ref, "liboracle.so")):
         return None
     src = os.path.join(ROOT, "tests", "hostgate", "fuzz_gate.cpp")
-    out = os.path.join(ROOT, "tests", "hostgate", "fuzz_gate")
-    cmd = [HIPCC, "-x", "hip", "--offload-host-only", "-O1", "-g", "-std=c++17", "-Wno-unused-function",
-           "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
-           "-Xarch_host", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer",
-           "-I" + os.path.join(ROOT, "tests", "hostgate"), src, "-L" + ref, "-loracle", "-Wl,-rpath," + ref,
-           "-o", out + ".tmp"]
-    _run(cmd)
-    os.replace(out + ".tmp", out)
-    return out
+    outs = []
+    for name, defs in (("fuzz_gate", []), ("fuzz_gate_st", ["-DCUO_MAXCH=2", "-DCUO_NS=cuo_st"])):
+        out = os.path.join(ROOT, "tests", "hostgate", name)
+        cmd = [HIPCC, "-x", "hip", "--offload-host-only", "-O1", "-g", "-std=c++17", "-Wno-unused-function",
+               "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+               "-Xarch_host", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"] + defs + [
+               "-I" + os.path.join(ROOT, "tests", "hostgate"), src, "-L" + ref, "-loracle", "-Wl,-rpath," + ref,
+               "-o", out + ".tmp"]
+        _run(cmd)
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 def build_oracle(jobs=8):

@@ -6,8 +6,10 @@
 // compiled for the host (the same-source gate) and with the oracle (libopus
 // 1.5.2 fixed point, liboracle.so, not instrumented), and compares each
 // packet's return value and final range and every decoded sample up to the
-// first error of the chain.  Prints one line per mismatching chain; exit
-// status 0 when everything matches, 1 on mismatches, 2 on bad input.
+// first error of the chain.  Optional second argument: output channels (1
+// or 2; 2 needs the stereo build, fuzz_gate_st).  Prints one line per
+// mismatching chain; exit status 0 when everything matches, 1 on
+// mismatches, 2 on bad input.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -23,7 +25,9 @@ static bool rd(FILE *f, void *p, size_t n) { return n == 0 || fread(p, 1, n, f) 
 
 int main(int argc, char **argv)
 {
-    if (argc != 2) { fprintf(stderr, "usage: %s set.bin\n", argv[0]); return 2; }
+    if (argc != 2 && argc != 3) { fprintf(stderr, "usage: %s set.bin [channels]\n", argv[0]); return 2; }
+    const int C = argc == 3 ? atoi(argv[2]) : 1;
+    if (C < 1 || C > kMaxCh) { fprintf(stderr, "channels must be 1..%d\n", kMaxCh); return 2; }
     FILE *f = fopen(argv[1], "rb");
     if (!f) { perror(argv[1]); return 2; }
     char magic[4];
@@ -42,14 +46,14 @@ int main(int argc, char **argv)
     for (uint32_t c = 0; c < nch; c++) {
         const uint32_t n = npkts[c], f0 = first[c];
         const long cap = 5760L * (n + 1);
-        std::vector<int16_t> pa(cap), pb(cap);
+        std::vector<int16_t> pa(cap * C), pb(cap * C);
         std::vector<int32_t> na(n, 0), nb(n, 0);
         std::vector<uint32_t> ra(n, 0), rb(n, 0);
         const uint8_t *lp = has_lost ? lost.data() + f0 : nullptr;
-        long a = hg_decode_chain(data.data(), off.data() + f0, len.data() + f0, lp, (int)n, pa.data(), cap, na.data(),
-                                 ra.data());
+        long a = hg_decode_chain_cfg(data.data(), off.data() + f0, len.data() + f0, lp, (int)n, pa.data(), cap,
+                                     na.data(), ra.data(), C, 0);
         long b = orc_decode_chain_cfg(data.data(), off.data() + f0, len.data() + f0, lp, (int)n, pb.data(), cap,
-                                      nb.data(), rb.data(), 1, 0);
+                                      nb.data(), rb.data(), C, 0);
         long done = 0;
         uint32_t k = 0;
         for (; k < n && nb[k] > 0; k++) done += nb[k];
@@ -57,7 +61,8 @@ int main(int argc, char **argv)
         samples += done;
         bool same = (a < 0) == (b < 0) && (a < 0 ? a == b : a == b) && memcmp(na.data(), nb.data(), 4 * n) == 0 &&
                     memcmp(ra.data(), rb.data(), 4 * n) == 0 &&
-                    memcmp(pa.data(), pb.data(), sizeof(int16_t) * (size_t)done) == 0 && (uint32_t)done >= keep[c];
+                    memcmp(pa.data(), pb.data(), sizeof(int16_t) * (size_t)(done * C)) == 0 &&
+                    (uint32_t)done >= keep[c];
         if (!same) {
             if (bad < 20)
                 printf("chain %u: gate %ld oracle %ld (packets ok %u of %u)\n", c, a, b, k, n);

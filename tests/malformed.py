@@ -26,8 +26,10 @@ import numpy as np
 
 from tests import oracle_lib as ol
 
-# Whether damaged packets may have the TOC's stereo flag set: a stereo packet
-# is decoded by libopus's mono decoder (SILK mid channel, CELT downmix).
+# The mono sets keep the TOC's stereo flag clear (the mono kernel build
+# decodes one coded channel); the stereo sets below set it freely and are
+# decoded by the stereo build to one output channel (libopus's mono decoder:
+# SILK mid channel, CELT downmix) and to two.
 STEREO_PACKETS = False
 
 REAL = ["silk_wb", "silk_nb", "celt_fb", "hybrid_fec", "mixed", "celt_multi", "silk_40ms", "hybrid_10ms",
@@ -67,40 +69,49 @@ def _debruijn(k):
     return seq + seq[:1]
 
 
-def random_toc(seed=1):
+def random_toc(seed=1, stereo=False):
+    """Every TOC configuration (mono, or stereo when `stereo`), code 0."""
     rs = np.random.default_rng(seed)
     packets, sizes = [], []
     for cfg in range(32):
         lens = list(range(1 + int(rs.integers(0, 7)), 1276, 37)) + [1275]
         for n in lens:
-            packets.append(bytes([cfg << 3]) + rs.integers(0, 256, n - 1, dtype=np.uint8).tobytes())
+            packets.append(bytes([cfg << 3 | (4 if stereo else 0)]) +
+                           rs.integers(0, 256, n - 1, dtype=np.uint8).tobytes())
         sizes.append(len(lens))
     return _pack(packets, sizes)
 
 
-def mode_pairs(seed=2, plen=None, chain=48):
+def mode_pairs(seed=2, plen=None, chain=48, stereo=False):
+    """De Bruijn order over the 32 configurations, or over the 64
+    configuration x channel-count symbols when `stereo`."""
     rs = np.random.default_rng(seed)
     plen = plen or int(rs.integers(3, 21)) * 8 + 3
-    seq = _debruijn(32)
-    packets = [bytes([cfg << 3]) + (rs.integers(0, 256, plen, dtype=np.uint8) |
-                                    rs.integers(0, 256, plen, dtype=np.uint8)).astype(np.uint8).tobytes()
-               for cfg in seq]
+    seq = _debruijn(64 if stereo else 32)
+    packets = [bytes([(sym >> 1) << 3 | (sym & 1) << 2 if stereo else sym << 3]) +
+               (rs.integers(0, 256, plen, dtype=np.uint8) |
+                rs.integers(0, 256, plen, dtype=np.uint8)).astype(np.uint8).tobytes()
+               for sym in seq]
     sizes = [chain] * (len(packets) // chain) + ([len(packets) % chain] if len(packets) % chain else [])
     return _pack(packets, sizes)
 
 
-def framing(seed=3, n=1536, chain=1):
+def framing(seed=3, n=1536, chain=1, stereo=False):
     rs = np.random.default_rng(seed)
     packets = []
     for _ in range(n):
-        cfg = int(rs.integers(0, 32))
+        cfg = int(rs.integers(0, 32)) << 1 | (int(rs.integers(0, 2)) if stereo else 0)
         code = int(rs.integers(1, 4))
         length = int(rs.integers(1, 400))
         body = rs.integers(0, 256, length, dtype=np.uint8)
         if code == 3 and length > 1 and rs.random() < 0.7:
             body[0] = (body[0] & 0xC0) | int(rs.integers(1, 8))  # plausible frame counts
-        packets.append(bytes([(cfg << 3) | code]) + body.tobytes())
+        packets.append(bytes([(cfg >> 1) << 3 | (cfg & 1) << 2 | code]) + body.tobytes())
     return _pack(packets, [chain] * (n // chain))  # one packet per chain: each from a fresh decoder
+
+
+REAL_ST = ["st_celt", "st_celt_lo", "st_silk", "st_hybrid", "st_mixed", "st_fec", "st_silk_flip", "mixed",
+           "hybrid_fec"]
 
 
 def _real(names=REAL):
@@ -112,10 +123,12 @@ def _real(names=REAL):
     return out
 
 
-def damaged(kind, seed=4, chain=40, per_file=6):
+def damaged(kind, seed=4, chain=40, per_file=6, stereo=False):
+    """Damaged real packets: mono corpora with the stereo flag kept clear, or
+    (stereo) stereo and mono corpora with every bit open to flipping."""
     rs = np.random.default_rng(seed + (0 if kind == "truncated" else 100))
     packets, sizes = [], []
-    for pk in _real():
+    for pk in _real(REAL_ST if stereo else REAL):
         for _ in range(per_file):
             s = int(rs.integers(0, max(1, len(pk) - chain)))
             seg = [bytearray(p) for p in pk[s:s + chain]]
@@ -127,7 +140,7 @@ def damaged(kind, seed=4, chain=40, per_file=6):
                         for _b in range(int(rs.integers(1, 9))):
                             bit = int(rs.integers(0, 8 * len(p)))
                             p[bit >> 3] ^= 1 << (bit & 7)
-                        if not STEREO_PACKETS:
+                        if not (stereo or STEREO_PACKETS):
                             p[0] &= 0xFB  # keep the TOC's stereo flag clear (see STEREO_PACKETS)
             packets += [bytes(p) for p in seg]
             sizes.append(len(seg))
@@ -142,19 +155,28 @@ SETS = {
     "bitflip": lambda: damaged("bitflip"),
 }
 
+# Sets with stereo packets, for the stereo build
+STEREO_SETS = {
+    "st_random_toc": lambda: random_toc(seed=11, stereo=True),
+    "st_mode_pairs": lambda: mode_pairs(seed=12, stereo=True),
+    "st_framing": lambda: framing(seed=13, n=1024, stereo=True),
+    "st_truncated": lambda: damaged("truncated", seed=14, stereo=True),
+    "st_bitflip": lambda: damaged("bitflip", seed=15, stereo=True),
+}
+
 
 def loss_for(off, seed=5, p=0.1):
     return ol.loss_mask(len(off), p, seed)
 
 
-def oracle_keep(data, off, ln, chains, lost=None):
+def oracle_keep(data, off, ln, chains, lost=None, channels=1):
     """Decode with the oracle; fill each chain's keep with the samples its
     packets produced before the first error (the part both decoders emit),
     and return (chains, oracle result)."""
-    ref = ol.decode_chains(data, off, ln, chains, lost=lost)
+    ref = ol.decode_chains(data, off, ln, chains, lost=lost, channels=channels)
     ch = np.array(chains, np.int64).copy()
-    ch[:, 3] = [len(seg) for seg in ref["full"]]
-    return ch, ol.decode_chains(data, off, ln, ch, lost=lost)
+    ch[:, 3] = [len(seg) // channels for seg in ref["full"]]
+    return ch, ol.decode_chains(data, off, ln, ch, lost=lost, channels=channels)
 
 
 def write_set(path, data, off, ln, lost, chains):
