@@ -85,6 +85,12 @@ WORKLOADS = {
                 "hybrid-SWB/FB, CELT-FB, bitrate redrawn every 1 s in 12-64 kbps (mode switches, redundancy), "
                 "20 ms frames; all chains in one persistent kernel vs one launch per file",
     },
+    "st_celt_fb_64k_1h": {
+        "files": ["corpus/bench/st_celt_fb_64k_1h.opus"],
+        "desc": "CELT-only FB stereo 64 kbps VBR (mid/side, intensity and dual stereo), frame size per 1 s from "
+                "{2.5,5,10,20} ms, one 1 h file, 1 s chains + 250 ms preroll, one launch of the stereo kernel "
+                "build, f32 16 kHz out (the one-channel render)",
+    },
     "uneven_250": {
         "glob": "corpus/bench/uneven_250/f*.opus",
         "desc": "250 mono speech files, durations log-uniform 5 s-30 min (18.1 h), mode per file from "
@@ -120,6 +126,8 @@ def _args():
                     help="playable audio-seconds of the one-thread CPU sample")
     ap.add_argument("--no-logmel", action="store_true", help="skip the log-mel front-end timing")
     ap.add_argument("--no-full-calls", action="store_true", help="skip the full drop-in call timings")
+    ap.add_argument("--no-stereo", action="store_true",
+                    help="skip the stereo timings (when the stereo benchmark file is present)")
     ap.add_argument("--serve-qps", default="50",
                     help="comma-separated request rates for the serving run ('' to skip)")
     ap.add_argument("--serve-seconds", type=float, default=8.0, help="arrival window per rate")
@@ -338,6 +346,35 @@ def _full_calls(cn, torch, blobs, multi_name="uneven_50"):
     return out
 
 
+def _stereo(cn, torch, steps, name="st_celt_fb_64k_1h"):
+    """Stereo stream through the stereo kernel build: the resident batch
+    (persistent, f32 16 kHz one-channel render) timed like the headline, and
+    the drop-in calls for the interleaved int16 48 kHz output and the f32
+    16 kHz output, median of 5."""
+    path = os.path.join(ROOT, WORKLOADS[name]["files"][0])
+    if not os.path.exists(path):
+        return None
+    blob = open(path, "rb").read()
+    b = cn.Batch([blob], mode=cn.MODE_PERSISTENT, profile=True)
+    wall, dec_ms, _ = _time_batch(b, steps, 1, torch)
+    audio_s = b.playable / 48000.0
+    out = {"workload": name, "chains": b.chains, "audio_s": round(audio_s, 1),
+           "batch_ms_per_step": round(1000.0 * wall / steps, 3),
+           "batch_value": round(audio_s * steps / wall, 1), "unit": "audio-s/s",
+           "decode_kernel_ms": round(sum(dec_ms) / len(dec_ms), 3) if dec_ms else None}
+    b.close()
+    del b
+    med, best = _median_ms(lambda: cn.decode_ogg_to_s16_tensor(blob), torch)
+    out["s16_call"] = {"ms_median": round(med, 2), "ms_best": round(best, 2),
+                       "value": round(audio_s / (med / 1000.0), 1), "unit": "audio-s/s", "channels": 2}
+    med, best = _median_ms(lambda: cn.decode_ogg_to_cuda_tensor(blob), torch)
+    out["f16k_call"] = {"ms_median": round(med, 2), "ms_best": round(best, 2),
+                        "value": round(audio_s / (med / 1000.0), 1), "unit": "audio-s/s", "channels": 1}
+    _log("stereo %s: batch %.1f ms/step, s16 call %.1f ms, f16k call %.1f ms"
+         % (name, out["batch_ms_per_step"], out["s16_call"]["ms_median"], out["f16k_call"]["ms_median"]))
+    return out
+
+
 def _logmel(cn, torch, x, n_mels=80):
     """The log-mel front-end on the workload's f32 16 kHz output: median of 5
     launches timed with events on the current stream; HBM roofline with 4 B
@@ -547,6 +584,10 @@ def main():
     if not a.no_batching and not a.workload.startswith("uneven"):
         batching = _batching(cn, torch, max(1, min(a.steps, 3)))
 
+    stereo = None
+    if not a.no_stereo and not a.workload.startswith("st_"):
+        stereo = _stereo(cn, torch, max(1, min(a.steps, 3)))
+
     serve = None
     if a.serve_qps:
         serve = _serve(cn, torch, [float(q) for q in a.serve_qps.split(",")], a.serve_seconds)
@@ -584,6 +625,7 @@ def main():
         "full_calls": full,
         "cpu_baseline": cpu,
         "batching": batching,
+        "stereo": stereo,
         "serve": serve,
     }
     print(json.dumps(line), flush=True)

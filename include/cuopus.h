@@ -287,8 +287,9 @@ void cuopus_server_destroy(cuopus_server *srv);
  * 128 MiB per GPU, reserved when the library loads and kept until the process
  * ends; the call makes no hipMalloc/hipFree and no allocating library call.
  * A stream whose buffers exceed it returns CUOPUS_RC_ARG with the byte count
- * (about 10 minutes of 64 kbps audio fit: 2 B per playable 48 kHz sample plus
- * the packets, 64 B per chain and 38 KB per resident wavefront, 64-4096 of
+ * (about 10 minutes of 64 kbps mono audio fit: 2 B per playable 48 kHz sample
+ * and channel plus the packets, 64 B per chain and 38 KB per resident
+ * wavefront, 60 KB for a stream that needs the stereo build, 64-4096 of
  * them).  Every call on a GPU shares that GPU's array: calls that overlap on
  * one GPU (two host threads, or two streams without ordering) race. */
 int cuopus_decode_f16k_noalloc_on(const uint8_t *ogg, size_t n, float *out_dev, size_t out_cap, int64_t *out_len,

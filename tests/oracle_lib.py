@@ -261,6 +261,9 @@ BENCH = os.path.join(ROOT, "corpus", "bench")
 BENCH_RECIPES = {
     "celt_fb_64k_2h": "--seed 0x0C0FFEE2 --dur 7200 --signal music --app lowdelay --bw fb --vbr 1 --bitrate 64000 "
                       "--frameset 120,240,480,960",
+    # a stereo stream for the stereo kernel build's timings: CELT FB 64 kbps, 1 h (3600 chains)
+    "st_celt_fb_64k_1h": "--seed 0x0C0FFEE6 --dur 3600 --signal music --app lowdelay --bw fb --vbr 1 "
+                         "--bitrate 64000 --frameset 120,240,480,960 --channels 2",
     # the reference's CPU-runnable plain case: 60 s SILK-only WB speech, 16 kbps CBR, 20 ms
     "silk_wb_16k_60s": "--seed 0x0C0FFEE1 --dur 60 --signal speech --app voip --bw wb --force silk --vbr 0 "
                        "--bitrate 16000 --frame 960",

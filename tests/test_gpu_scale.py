@@ -7,6 +7,9 @@ the float32 16 kHz outputs are within 1e-6 of the oracle's resampler.
 * celt_fb_64k_2h: the headline workload (7200 chains, one launch).
 * celt_fb_64k_2h through the resident batch exactly as bench.py times it
   (persistent kernel, the default 4096 wavefronts, cost-model order).
+* st_celt_fb_64k_1h (stereo, 3600 chains, the stereo kernel build): every
+  chain at two output channels through the gate, the interleaved int16
+  entry point, and the f32 16 kHz one-channel render.
 * uneven_1000 / uneven_250 / uneven_50 (whichever is present): every file.
 * hybrid_fec_200k (2% seeded loss, FEC or concealment): the full 200 files
   when present; otherwise its first LOSS_SUBSET files (24,000 chains) are
@@ -99,6 +102,32 @@ def test_celt_fb_2h_benchmarked_batch(cn):
         assert md <= 1e-6
     b.close()
     torch.cuda.synchronize()
+
+
+def test_stereo_1h_all_chains(cn):
+    import torch
+    blob = _bench_file("st_celt_fb_64k_1h")
+    d, chains = ol.demux_chains(blob)
+    data, off, ln = d.blob()
+    assert d.channels == 2
+    print("case scale st_celt_fb_64k_1h packets", len(off), "chains", len(chains), flush=True)
+    pcm, nsamp, rng, stop, rc = cn.gate_chains(data, off, ln, chains, channels=2)
+    ref = ol.decode_chains(data, off, ln, chains, threads=THREADS, channels=2)
+    assert rc == 0 and not stop.any() and (ref["ret"] >= 0).all()
+    assert np.array_equal(nsamp, ref["nsamp"]) and np.array_equal(rng, ref["rng"])
+    diff = np.nonzero(pcm != ref["pcm"])[0]
+    assert diff.size == 0, ("pcm mismatch at", diff[:10])
+    s16 = cn.decode_ogg_to_s16_tensor(blob).cpu().numpy()
+    assert np.array_equal(s16.reshape(-1), ref["pcm"])
+    mono = ol.decode_chains(data, off, ln, chains, threads=THREADS, channels=1)
+    want = ol.resample_threads(mono["pcm"], THREADS)
+    out = cn.decode_ogg_to_cuda_tensor(blob)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert got.shape == want.shape
+    md = float(np.abs(got - want).max())
+    print("  f16k (one-channel render) max abs diff %.3g over %d outputs" % (md, got.size), flush=True)
+    assert md <= 1e-6
 
 
 def _set_files(name):
