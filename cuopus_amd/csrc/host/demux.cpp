@@ -56,24 +56,28 @@ DemuxOpts demux_opts_from_env()
 }
 
 namespace {
+// A lacing run of one page: consecutive segments up to and including the
+// first one shorter than 255 bytes (a packet ends there), or the page's
+// trailing run of 255-byte segments (the packet continues on the next page).
 struct Span {
     uint64_t off;
     uint32_t len;
-    bool done;  // ends in a lacing value < 255 (packet completes here)
+    bool done;
 };
 
 struct Page {
-    uint64_t off;        // byte offset of the capture pattern
     int64_t granule;
     uint32_t serial, seq;
     uint8_t flags;       // 1 continued, 2 BOS, 4 EOS
-    uint32_t span0, nspan;
 };
 
-// A packet assembled from spans of consecutive pages of one stream: its
-// pieces are parts[part0, part0 + nparts) of the link.
+// A packet assembled from spans of consecutive pages of one stream: its first
+// piece inline, further pieces (a packet crossing pages) in the link's
+// `more` list at [more0, more0 + nparts - 1).
 struct Packet {
-    uint32_t part0 = 0, nparts = 0;
+    uint64_t off0 = 0;
+    uint32_t len0 = 0;
+    uint32_t more0 = 0, nparts = 0;
     uint32_t len = 0;
     uint32_t page = 0;  // page on which it completes
 };
@@ -84,18 +88,11 @@ struct LinkBuild {
     bool have_seq = false;
     uint32_t last_seq = 0;
     std::vector<Packet> packets;  // header packets included
-    std::vector<std::pair<uint64_t, uint32_t>> parts;  // (file offset, length)
+    std::vector<std::pair<uint64_t, uint32_t>> more;  // (file offset, length) of continuation pieces
     Packet partial;
     bool has_partial = false;
     std::vector<uint32_t> pages;  // page indices of this link
 };
-
-bool starts_opus_head(const uint8_t *buf, const std::vector<Span> &spans, const Page &pg)
-{
-    if ((pg.flags & 1) || pg.nspan == 0) return false;
-    const Span &s = spans[pg.span0];
-    return s.len >= 19 && memcmp(buf + s.off, "OpusHead", 8) == 0;
-}
 }  // namespace
 
 bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const DemuxOpts &opts)
@@ -104,13 +101,18 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
     if (!buf || n == 0) { *err = "empty stream"; return false; }
     DemuxStats &st = d->stats;
 
-    // ---- pages
-    std::vector<Span> spans;
+    // ---- pages, and the packets of the Opus links, in one pass
     std::vector<Page> pages;
+    pages.reserve(n / 4000 + 16);
+    std::vector<LinkBuild> links;
+    Span spans[255];
+    size_t done_spans = 0;  // spans ending a packet on any accepted page (error messages only)
     size_t pos = 0;
     static const uint8_t kCapture[4] = {'O', 'g', 'g', 'S'};
     while (pos < n) {
-        const uint8_t *hit = std::search(buf + pos, buf + n, kCapture, kCapture + 4);
+        const uint8_t *hit = pos + 4 <= n && memcmp(buf + pos, kCapture, 4) == 0
+                                 ? buf + pos
+                                 : std::search(buf + pos, buf + n, kCapture, kCapture + 4);
         if (hit == buf + n) { st.skipped_bytes += n - pos; break; }
         size_t idx = (size_t)(hit - buf);
         st.skipped_bytes += idx - pos;
@@ -123,7 +125,6 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
         for (int i = 0; i < nsegs; i++) body_len += seg[i];
         if (body + body_len > n) { st.skipped_bytes += n - idx; break; }
         Page pg;
-        pg.off = idx;
         memcpy(&pg.granule, buf + idx + 6, 8);
         memcpy(&pg.serial, buf + idx + 14, 4);
         memcpy(&pg.seq, buf + idx + 18, 4);
@@ -146,38 +147,38 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
                 continue;
             }
         }
-        pg.span0 = (uint32_t)spans.size();
-        uint64_t run_start = body;
-        uint32_t run_len = 0;
-        int last = 0;
-        for (int i = 0; i < nsegs; i++) {
-            run_len += seg[i];
-            last = seg[i];
-            if (seg[i] < 255) {
-                spans.push_back({run_start, run_len, true});
-                run_start += run_len;
-                run_len = 0;
+        // this page's spans
+        int nspan = 0;
+        {
+            uint64_t run_start = body;
+            uint32_t run_len = 0;
+            int last = 0;
+            for (int i = 0; i < nsegs; i++) {
+                run_len += seg[i];
+                last = seg[i];
+                if (seg[i] < 255) {
+                    spans[nspan++] = {run_start, run_len, true};
+                    run_start += run_len;
+                    run_len = 0;
+                }
             }
+            if (run_len > 0 || (nsegs > 0 && last == 255)) spans[nspan++] = {run_start, run_len, false};
         }
-        if (run_len > 0 || (nsegs > 0 && last == 255)) spans.push_back({run_start, run_len, false});
-        pg.nspan = (uint32_t)spans.size() - pg.span0;
+        const uint32_t pi = (uint32_t)pages.size();
         pages.push_back(pg);
         pos = body + body_len;
-    }
-    if (pages.empty()) { *err = "no OGG pages found"; return false; }
+        for (int i = 0; i < nspan; i++) done_spans += spans[i].done;
 
-    // ---- logical streams and packets
-    std::vector<LinkBuild> links;
-    for (uint32_t pi = 0; pi < (uint32_t)pages.size(); pi++) {
-        const Page &pg = pages[pi];
-        const bool head = starts_opus_head(buf, spans, pg);
+        // ---- link of this page (first Opus stream, a new chained link, or foreign)
+        const bool head = !(pg.flags & 1) && nspan > 0 && spans[0].len >= 19 &&
+                          memcmp(buf + spans[0].off, "OpusHead", 8) == 0;
         LinkBuild *L = links.empty() ? nullptr : &links.back();
         if (head && (!L || ((pg.flags & 2) && (L->eos || L->packets.size() > 2)))) {
-            // first Opus stream, or a new link of a chained stream
             if (L && L->has_partial) st.dropped_packets++;
             links.emplace_back();
             L = &links.back();
             L->serial = pg.serial;
+            if (links.size() == 1) L->packets.reserve(n / 32 + 16);  // about one packet per 32 bytes
         } else if (!L || pg.serial != L->serial) {
             st.foreign_pages++;
             continue;
@@ -186,23 +187,29 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
         st.pages++;
         if (L->has_partial && (!(pg.flags & 1) || (L->have_seq && pg.seq != L->last_seq + 1))) {
             L->has_partial = false;  // its continuation is missing
-            L->parts.resize(L->partial.part0);
+            if (L->partial.nparts > 1) L->more.resize(L->partial.more0);
             L->partial = Packet();
             st.dropped_packets++;
         }
         L->have_seq = true;
         L->last_seq = pg.seq;
-        uint32_t s = pg.span0, se = pg.span0 + pg.nspan;
-        if ((pg.flags & 1) && !L->has_partial && s < se) {
+        int s = 0;
+        if ((pg.flags & 1) && !L->has_partial && s < nspan) {
             // continuation of a packet whose start is missing
-            while (s < se && !spans[s].done) s++;
-            if (s < se) { s++; st.dropped_packets++; }
+            while (s < nspan && !spans[s].done) s++;
+            if (s < nspan) { s++; st.dropped_packets++; }
         }
-        for (; s < se; s++) {
+        for (; s < nspan; s++) {
             Packet &p = L->partial;
-            if (!L->has_partial) p.part0 = (uint32_t)L->parts.size();
-            L->parts.push_back({spans[s].off, spans[s].len});
-            p.nparts++;
+            if (!L->has_partial) {
+                p.off0 = spans[s].off;
+                p.len0 = spans[s].len;
+                p.nparts = 1;
+            } else {
+                if (p.nparts == 1) p.more0 = (uint32_t)L->more.size();
+                L->more.push_back({spans[s].off, spans[s].len});
+                p.nparts++;
+            }
             p.len += spans[s].len;
             if (spans[s].done) {
                 p.page = pi;
@@ -215,14 +222,13 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
         }
         if (pg.flags & 4) L->eos = true;
     }
+    if (pages.empty()) { *err = "no OGG pages found"; return false; }
     for (auto &L : links)
         if (L.has_partial) st.dropped_packets++;  // trailing partial packet
     d->n_pages = (uint32_t)pages.size();
     if (links.empty()) {
         // mirror the reference's messages for streams without an Opus head
-        size_t np = 0;
-        for (const Span &s : spans) np += s.done;
-        *err = np < 2 ? "not an OGG Opus stream (fewer than 2 packets)" : "not an OGG Opus stream (no OpusHead)";
+        *err = done_spans < 2 ? "not an OGG Opus stream (fewer than 2 packets)" : "not an OGG Opus stream (no OpusHead)";
         return false;
     }
     if (links[0].packets.size() < 2) { *err = "not an OGG Opus stream (fewer than 2 packets)"; return false; }
@@ -240,7 +246,6 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
     d->off.resize(n_audio);
     d->len.resize(n_audio);
     d->sample_off.resize(n_audio);
-    std::vector<uint32_t> pkt_page(n_audio);
     uint64_t w = 0;
     size_t k = 0;
     for (size_t li = 0; li < links.size(); li++) {
@@ -248,54 +253,56 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
         Link lk;
         lk.serial = LB.serial;
         const Packet &h = LB.packets[0];
-        if (LB.packets.size() < 2 || h.nparts == 0 || LB.parts[h.part0].second < 19) {
+        if (LB.packets.size() < 2 || h.nparts == 0 || h.len0 < 19) {
             // a link cut short before its headers completed carries no audio
             if (li == 0) { *err = "not an OGG Opus stream (no OpusHead)"; return false; }
             st.dropped_packets += (uint32_t)LB.packets.size();
             continue;
         }
-        const uint8_t *hp = buf + LB.parts[h.part0].first;
+        const uint8_t *hp = buf + h.off0;
         lk.channels = hp[9];
         lk.preskip = hp[10] | (hp[11] << 8);
         lk.gain_q8 = (int16_t)(uint16_t)(hp[16] | (hp[17] << 8));
         lk.first = (uint32_t)k;
+        // packets, their TOC timeline and, per page, the end (link timeline) of
+        // the last packet completing on it: granule cross-check below
+        std::vector<std::pair<uint32_t, int64_t>> page_end;  // (page, end) in page order
+        int64_t page_begin_final = 0;
         int64_t cum = 0;
+        int last_dur = 0;
         for (size_t i = 2; i < LB.packets.size(); i++, k++) {
             const Packet &p = LB.packets[i];
+            uint8_t *dst = d->data.data() + w;
             d->off[k] = (uint32_t)w;
             d->len[k] = p.len;
-            pkt_page[k] = p.page;
-            for (uint32_t j = p.part0; j < p.part0 + p.nparts; j++) {
-                memcpy(d->data.data() + w, buf + LB.parts[j].first, LB.parts[j].second);
-                w += LB.parts[j].second;
+            memcpy(dst, buf + p.off0, p.len0);
+            w += p.len0;
+            for (uint32_t j = 0; j + 1 < p.nparts; j++) {
+                const auto &piece = LB.more[p.more0 + j];
+                memcpy(d->data.data() + w, buf + piece.first, piece.second);
+                w += piece.second;
             }
+            const int dur = pkt_samples48(dst, p.len);
             d->sample_off[k] = cum;
-            cum += pkt_samples48(d->data.data() + d->off[k], d->len[k]);
-            if (p.len > 0 && (d->data[d->off[k]] & 0x4)) d->stereo_packets++;
+            if (page_end.empty() || page_end.back().first != p.page) {
+                page_begin_final = cum;
+                page_end.push_back({p.page, cum + dur});
+            } else {
+                page_end.back().second = cum + dur;
+            }
+            cum += dur;
+            last_dur = dur;
+            if (p.len > 0 && (dst[0] & 0x4)) d->stereo_packets++;
         }
         lk.npkts = (uint32_t)k - lk.first;
         lk.toc_total = cum;
+        if (!lk.npkts) last_dur = 0;
 
         // ---- granule cross-check over this link's pages
-        // page -> end sample (link timeline) of the last packet completing on it
-        std::vector<std::pair<uint32_t, int64_t>> page_end;  // (page, end) in page order
-        int64_t page_begin_final = 0;
-        for (uint32_t q = lk.first; q < lk.first + lk.npkts; q++) {
-            int64_t end = d->sample_off[q] + pkt_samples48(d->data.data() + d->off[q], d->len[q]);
-            if (page_end.empty() || page_end.back().first != pkt_page[q]) {
-                page_begin_final = d->sample_off[q];
-                page_end.push_back({pkt_page[q], end});
-            } else {
-                page_end.back().second = end;
-            }
-        }
         const int final_page = page_end.empty() ? -1 : (int)page_end.back().first;
-        const int last_dur = lk.npkts ? pkt_samples48(d->data.data() + d->off[lk.first + lk.npkts - 1],
-                                                      d->len[lk.first + lk.npkts - 1])
-                                      : 0;
         int64_t last_granule = 0;
-        for (uint32_t pi : LB.pages)
-            if (pages[pi].granule != -1 && pages[pi].granule > last_granule) last_granule = pages[pi].granule;
+        for (uint32_t pgi : LB.pages)
+            if (pages[pgi].granule != -1 && pages[pgi].granule > last_granule) last_granule = pages[pgi].granule;
         lk.last_granule = last_granule;
         uint32_t checked = 0, match = 0;
         int64_t base = 0;  // lenient: granule minus TOC time, re-based at each mismatch
@@ -357,9 +364,8 @@ static bool link_chains(const Demuxed &d, const Link &lk, int64_t step, int64_t 
     const size_t n_pk = lk.npkts;
     if (n_pk == 0) { *err = "no audio packets"; return false; }
     const int64_t *so = d.sample_off.data() + lk.first;
-    std::vector<int64_t> cum(n_pk + 1, 0);
-    for (size_t k = 0; k < n_pk; k++)
-        cum[k + 1] = cum[k] + pkt_samples48(d.data.data() + d.off[lk.first + k], d.len[lk.first + k]);
+    // cum(k): TOC time at the start of packet k (the link's total past the last)
+    auto cum = [&](int64_t k) -> int64_t { return k < (int64_t)n_pk ? so[k] : lk.toc_total; };
     int64_t kept = 0;
     for (int64_t k = 0; k * step < total_play; k++) {
         int64_t play_start = k * step;
@@ -370,7 +376,7 @@ static bool link_chains(const Demuxed &d, const Link &lk, int64_t step, int64_t 
         // first packet whose span reaches past the preroll start
         int64_t first = (int64_t)(std::upper_bound(so, so + n_pk, start_target) - so) - 1;
         if (first < 0) first = 0;
-        if (cum[first + 1] <= start_target) first++;
+        if (cum(first + 1) <= start_target) first++;
         // last packet starting before the window end
         int64_t last = (int64_t)(std::lower_bound(so, so + n_pk, toc_end) - so) - 1;
         if (last < first) last = first;
@@ -378,7 +384,7 @@ static bool link_chains(const Demuxed &d, const Link &lk, int64_t step, int64_t 
         int64_t trim = toc_start - so[first];
         int64_t keep = play_end - play_start;
         if (trim < 0) { *err = "negative trim"; return false; }
-        int64_t capn = cum[last + 1] - cum[first];
+        int64_t capn = cum(last + 1) - cum(first);
         if (capn < trim + keep) {
             *err = "chain " + std::to_string(out->size()) + ": capacity " + std::to_string(capn) + " < trim+keep";
             return false;

@@ -31,6 +31,8 @@
 #pragma once
 #include <stdint.h>
 #include <string>
+#include <memory>
+#include <utility>
 #include <vector>
 
 namespace cuo_host {
@@ -65,10 +67,28 @@ struct DemuxStats {
     uint64_t skipped_bytes = 0;     // bytes outside accepted pages (garbage, bad pages)
 };
 
+// std::vector storage whose resize leaves new elements uninitialised: the
+// demuxer writes every element of its large tables once, so zero-filling
+// them first would only add a pass over memory.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U>
+    void construct(U *p) noexcept { ::new ((void *)p) U; }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) { ::new ((void *)p) U(std::forward<A>(a)...); }
+};
+template <class T>
+using RawVec = std::vector<T, NoInitAlloc<T>>;
+
 struct Demuxed {
-    std::vector<uint8_t> data;      // audio packets, contiguous
-    std::vector<uint32_t> off, len; // per audio packet
-    std::vector<int64_t> sample_off;  // per packet, from the start of its link
+    RawVec<uint8_t> data;           // audio packets, contiguous
+    RawVec<uint32_t> off, len;      // per audio packet
+    RawVec<int64_t> sample_off;     // per packet, from the start of its link
     std::vector<Link> links;
     int64_t toc_total = 0;
     int channels = 0;

@@ -17,10 +17,12 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 #include <array>
 #include <deque>
+#include <memory>
 
 #include "../../include/cuopus.h"
 #include "host/demux.h"
@@ -256,7 +258,7 @@ void launch_resample(uint64_t tiles, hipStream_t st, const int16_t *in, const Rs
 
 // ---------------------------------------------------------------- planning
 struct FilePlan {
-    Demuxed d;
+    std::shared_ptr<const Demuxed> d;  // shared by every plan that decodes this file (never copied)
     std::vector<Chain> chains;
     uint64_t chain0 = 0;     // first merged chain index
     uint64_t keep = 0;       // playable samples (per channel)
@@ -281,6 +283,13 @@ struct Plan {
     uint64_t total_keep = 0;  // playable samples of all files
     uint64_t cat_len = 0;     // int16 elements of the compacted PCM buffer (file starts padded to 8)
     bool stereo = false;      // some stream is stereo: the stereo-capable kernel build decodes the launch
+    // Merged with borrow = true: data and len stay empty and each file's own
+    // packet bytes and lengths are uploaded in place (then kPktPad zeros);
+    // off is built only when there are several files (shifted offsets).
+    bool borrowed = false;
+    uint64_t npk_total = 0, nbytes_total = 0;
+    uint64_t n_packets() const { return borrowed ? npk_total : off.size(); }
+    uint64_t data_bytes() const { return borrowed ? nbytes_total + kPktPad : data.size(); }
 };
 
 // OpusHead output gain (RFC 7845 5.1, Q7.8 dB) is applied unless
@@ -295,18 +304,20 @@ int demux_file(const uint8_t *ogg, size_t n, FilePlan *fp, bool with_chains, Err
 {
     std::string msg;
     if (!ogg || !n) return e.set(CUOPUS_RC_ARG, "empty input");
-    if (!cuo_host::demux(ogg, n, &fp->d, &msg, cuo_host::demux_opts_from_env()))
+    auto dm = std::make_shared<Demuxed>();
+    if (!cuo_host::demux(ogg, n, dm.get(), &msg, cuo_host::demux_opts_from_env()))
         return e.set(CUOPUS_RC_SOFT, "%s", msg.c_str());
-    if (fp->d.channels != 1 && fp->d.channels != 2)
-        return e.set(CUOPUS_RC_SOFT, "only mono and stereo streams are supported (channels=%d)", fp->d.channels);
+    fp->d = dm;
+    if (fp->d->channels != 1 && fp->d->channels != 2)
+        return e.set(CUOPUS_RC_SOFT, "only mono and stereo streams are supported (channels=%d)", fp->d->channels);
     if (with_chains) {
         double seg_s = env_f("CUOPUS_SEG_S", 1.0);
         double preroll_ms = env_f("CUOPUS_PREROLL_MS", 250.0);
-        if (!cuo_host::build_chains(fp->d, seg_s, preroll_ms, &fp->chains, &msg))
+        if (!cuo_host::build_chains(*fp->d, seg_s, preroll_ms, &fp->chains, &msg))
             return e.set(CUOPUS_RC_SOFT, "%s", msg.c_str());
     }
-    fp->keep = fp->d.pcm_samples_48k > 0 ? (uint64_t)fp->d.pcm_samples_48k : 0;
-    fp->out16 = cuo_host::out16_len(fp->d.pcm_samples_48k);
+    fp->keep = fp->d->pcm_samples_48k > 0 ? (uint64_t)fp->d->pcm_samples_48k : 0;
+    fp->out16 = cuo_host::out16_len(fp->d->pcm_samples_48k);
     return 0;
 }
 
@@ -331,49 +342,60 @@ uint32_t packet_cost(const uint8_t *p, uint32_t len)
 }
 
 // Merge the demuxed files into one packet table and one chain table.
-int merge(Plan &pl, Err &e)
+int merge(Plan &pl, Err &e, bool borrow = false)
 {
     uint64_t nbytes = 0, npk = 0, nch = 0;
     for (auto &f : pl.files) {
-        nbytes += f.d.data.size();
-        npk += f.d.off.size();
+        nbytes += f.d->data.size();
+        npk += f.d->off.size();
         nch += f.chains.size();
     }
     if (nbytes + kPktPad >= (1ull << 32)) return e.set(CUOPUS_RC_ARG, "merged packet data exceeds 4 GiB; split the call");
-    pl.data.assign(nbytes + kPktPad, 0);
-    pl.off.resize(npk);
-    pl.len.resize(npk);
+    pl.borrowed = borrow;
+    pl.npk_total = npk;
+    pl.nbytes_total = nbytes;
+    if (!borrow) {
+        pl.data.assign(nbytes + kPktPad, 0);
+        pl.off.resize(npk);
+        pl.len.resize(npk);
+    } else if (pl.files.size() > 1) {
+        pl.off.resize(npk);
+    }
     pl.chains.resize(nch);
     pl.cap.resize(nch);
     pl.cost.resize(nch);
     const bool gain_on = apply_output_gain();
     uint64_t db = 0, pb = 0, cb = 0, cat = 0, keep_sum = 0;
     for (auto &f : pl.files) {
-        if (!f.d.data.empty()) memcpy(pl.data.data() + db, f.d.data.data(), f.d.data.size());
-        for (size_t k = 0; k < f.d.off.size(); k++) {
-            pl.off[pb + k] = f.d.off[k] + (uint32_t)db;
-            pl.len[pb + k] = f.d.len[k];
+        if (!borrow) {
+            if (!f.d->data.empty()) memcpy(pl.data.data() + db, f.d->data.data(), f.d->data.size());
+            for (size_t k = 0; k < f.d->off.size(); k++) {
+                pl.off[pb + k] = f.d->off[k] + (uint32_t)db;
+                pl.len[pb + k] = f.d->len[k];
+            }
+        } else if (pl.files.size() > 1) {
+            for (size_t k = 0; k < f.d->off.size(); k++) pl.off[pb + k] = f.d->off[k] + (uint32_t)db;
         }
         f.chain0 = cb;
         f.cat_off = cat;
         uint64_t fk = 0;
         for (size_t i = 0; i < f.chains.size(); i++) {
             const Chain &c = f.chains[i];
-            if ((uint64_t)c.first + c.npkts > f.d.off.size())
+            if ((uint64_t)c.first + c.npkts > f.d->off.size())
                 return e.set(CUOPUS_RC_SOFT, "chain %zu out of packet range", i);
             uint64_t s = 0, cost = 0;
             for (uint32_t k = 0; k < c.npkts; k++) {
                 uint32_t p = c.first + k;
-                const uint8_t *pk = f.d.data.data() + f.d.off[p];
-                s += (uint64_t)cuo_host::pkt_samples48(pk, f.d.len[p]);
-                cost += packet_cost(pk, f.d.len[p]);
+                const uint8_t *pk = f.d->data.data() + f.d->off[p];
+                s += (uint64_t)cuo_host::pkt_samples48(pk, f.d->len[p]);
+                cost += packet_cost(pk, f.d->len[p]);
             }
             if (s < (uint64_t)c.trim + c.keep)
                 return e.set(CUOPUS_RC_SOFT, "chain %zu capacity %llu < trim %u + keep %u", i,
                              (unsigned long long)s, c.trim, c.keep);
             // chains never cross a link: the link holding the first packet gives the gain
             int gain = 0;
-            for (const cuo_host::Link &lk : f.d.links)
+            for (const cuo_host::Link &lk : f.d->links)
                 if (c.first >= lk.first && c.first < lk.first + lk.npkts) gain = lk.gain_q8;
             ChainDesc &cd = pl.chains[cb + i];
             cd.first = c.first + (uint32_t)pb;
@@ -392,10 +414,10 @@ int merge(Plan &pl, Err &e)
         if (fk != f.keep) return e.set(CUOPUS_RC_SOFT, "internal: keep mismatch");
         // each file's PCM starts 16-byte aligned (the resampler's wide loads)
         cat += (fk * (uint64_t)f.out_channels + 7) & ~7ull;
-        pl.stereo = pl.stereo || cuo_host::needs_stereo_build(f.d);
+        pl.stereo = pl.stereo || cuo_host::needs_stereo_build(*f.d);
         keep_sum += fk;
-        db += f.d.data.size();
-        pb += f.d.off.size();
+        db += f.d->data.size();
+        pb += f.d->off.size();
         cb += f.chains.size();
     }
     pl.total_keep = keep_sum;
@@ -565,6 +587,22 @@ struct RunOpts {
     bool noalloc = false;                // device memory from the static scratch only
 };
 
+// Host-phase timing of a drop-in call, printed to stderr when CUOPUS_TRACE
+// is set (demux, plan, upload, device work, validation).
+struct HostTrace {
+    bool on;
+    std::chrono::steady_clock::time_point last;
+    HostTrace() : on(getenv("CUOPUS_TRACE") != nullptr), last(std::chrono::steady_clock::now()) {}
+    void mark(const char *phase)
+    {
+        if (!on) return;
+        const auto t = std::chrono::steady_clock::now();
+        fprintf(stderr, "[cuopus] %-16s %9.3f ms\n", phase,
+                std::chrono::duration<double, std::milli>(t - last).count());
+        last = t;
+    }
+};
+
 // Kernel selection of the drop-in entry points: the persistent work-queue
 // kernel with min(chains, slots) wavefronts (its heap memory is O(slots));
 // CUOPUS_KERNEL=plain forces one workgroup per chain.
@@ -579,7 +617,7 @@ bool use_persistent()
 int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &results, Err &e)
 {
     const uint32_t nch = (uint32_t)pl.chains.size();
-    const uint64_t npk = pl.off.size();
+    const uint64_t npk = pl.n_packets();
     int rc;
     std::vector<RsFile> rf;
     uint64_t tiles = 0;
@@ -599,7 +637,7 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     }
     uint32_t n_waves = std::max<uint32_t>(1, std::min<uint32_t>(o.n_slots ? o.n_slots : default_slots(), nch));
     CallNeed need;
-    need.data = pl.data.size();
+    need.data = pl.data_bytes();
     need.npk = npk;
     need.lost = pl.lost.empty() ? 0 : pl.lost.size();
     need.nch = nch;
@@ -618,18 +656,40 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     }
     if (rc) return rc;
     int16_t *out = o.s16_out ? o.s16_out : b.cat;
+    HostTrace tr;
 
     std::vector<uint32_t> order;
     work_order(pl, default_order(), &order);
-    HIPCHK(e, hipMemcpyAsync(b.data, pl.data.data(), pl.data.size(), hipMemcpyHostToDevice, st));
-    if (npk) {
-        HIPCHK(e, hipMemcpyAsync(b.off, pl.off.data(), 4 * npk, hipMemcpyHostToDevice, st));
-        HIPCHK(e, hipMemcpyAsync(b.len, pl.len.data(), 4 * npk, hipMemcpyHostToDevice, st));
+    tr.mark("order");
+    if (pl.borrowed) {
+        static const uint8_t kZeros[kPktPad] = {};
+        uint64_t db = 0, pb = 0;
+        for (const FilePlan &f : pl.files) {
+            const Demuxed &dm = *f.d;
+            const uint64_t np = dm.off.size();
+            if (dm.data.size()) HIPCHK(e, hipMemcpyAsync(b.data + db, dm.data.data(), dm.data.size(), hipMemcpyHostToDevice, st));
+            if (np) {
+                HIPCHK(e, hipMemcpyAsync(b.len + pb, dm.len.data(), 4 * np, hipMemcpyHostToDevice, st));
+                if (pl.files.size() == 1)
+                    HIPCHK(e, hipMemcpyAsync(b.off, dm.off.data(), 4 * np, hipMemcpyHostToDevice, st));
+            }
+            db += dm.data.size();
+            pb += np;
+        }
+        HIPCHK(e, hipMemcpyAsync(b.data + db, kZeros, kPktPad, hipMemcpyHostToDevice, st));
+        if (pl.files.size() > 1 && npk) HIPCHK(e, hipMemcpyAsync(b.off, pl.off.data(), 4 * npk, hipMemcpyHostToDevice, st));
+    } else {
+        HIPCHK(e, hipMemcpyAsync(b.data, pl.data.data(), pl.data.size(), hipMemcpyHostToDevice, st));
+        if (npk) {
+            HIPCHK(e, hipMemcpyAsync(b.off, pl.off.data(), 4 * npk, hipMemcpyHostToDevice, st));
+            HIPCHK(e, hipMemcpyAsync(b.len, pl.len.data(), 4 * npk, hipMemcpyHostToDevice, st));
+        }
     }
     HIPCHK(e, hipMemcpyAsync(b.chains, pl.chains.data(), sizeof(ChainDesc) * nch, hipMemcpyHostToDevice, st));
     HIPCHK(e, hipMemcpyAsync(b.order, order.data(), 4ull * nch, hipMemcpyHostToDevice, st));
     if (b.lost) HIPCHK(e, hipMemcpyAsync(b.lost, pl.lost.data(), pl.lost.size(), hipMemcpyHostToDevice, st));
     if (!rf.empty()) HIPCHK(e, hipMemcpyAsync(b.files, rf.data(), sizeof(RsFile) * rf.size(), hipMemcpyHostToDevice, st));
+    tr.mark("upload issued");
 
     DecodeArgs a;
     memset(&a, 0, sizeof(a));
@@ -664,13 +724,16 @@ int run_plan(Plan &pl, hipStream_t st, RunOpts &o, std::vector<ChainResult> &res
     }
     results.resize(nch);
     HIPCHK(e, hipMemcpyAsync(results.data(), b.res, sizeof(ChainResult) * nch, hipMemcpyDeviceToHost, st));
+    tr.mark("launches issued");
     HIPCHK(e, hipStreamSynchronize(st));
+    tr.mark("device done");
     for (uint32_t c = 0; c < nch; c++) {
         if (results[c].stop != 0) return e.set(CUOPUS_RC_SOFT, "chain %u: device decode error %d", c, results[c].stop);
         if (results[c].samples != pl.cap[c])
             return e.set(CUOPUS_RC_SOFT, "chain %u: decoded %u samples, expected %llu", c, results[c].samples,
                          (unsigned long long)pl.cap[c]);
     }
+    tr.mark("validated");
     return 0;
 }
 
@@ -679,10 +742,13 @@ int decode_plan(Plan &pl, const uint8_t *lost, size_t mask_n, RunOpts &o, hipStr
 {
     int rc = refuse_capture(st, e);
     if (rc) return rc;
-    if ((rc = merge(pl, e))) return rc;
+    HostTrace tr;
+    if ((rc = merge(pl, e, true))) return rc;
+    tr.mark("merge");
     if (lost) {
-        if (mask_n != pl.off.size())
-            return e.set(CUOPUS_RC_ARG, "loss mask has %zu entries, stream has %zu packets", mask_n, pl.off.size());
+        if (mask_n != pl.n_packets())
+            return e.set(CUOPUS_RC_ARG, "loss mask has %zu entries, stream has %llu packets", mask_n,
+                         (unsigned long long)pl.n_packets());
         pl.lost.assign(lost, lost + mask_n);
     }
     if (pl.chains.empty()) return 0;
@@ -708,16 +774,18 @@ extern "C" int cuopus_file_open(const uint8_t *ogg, size_t n, cuopus_file **out,
     Err e(errbuf, errcap);
     if (!out) return e.set(CUOPUS_RC_ARG, "out is NULL");
     *out = nullptr;
+    HostTrace tr;
     cuopus_file *f = new cuopus_file();
     int rc = demux_file(ogg, n, &f->fp, true, e);
     if (rc) { delete f; return rc; }
+    tr.mark("demux + chains");
     if (info6) {
         info6[0] = f->fp.out16;
         info6[1] = (int64_t)f->fp.keep;
-        info6[2] = f->fp.d.channels;
-        info6[3] = (int64_t)f->fp.d.off.size();
+        info6[2] = f->fp.d->channels;
+        info6[3] = (int64_t)f->fp.d->off.size();
         info6[4] = (int64_t)f->fp.chains.size();
-        info6[5] = f->fp.d.links.empty() ? 0 : f->fp.d.links[0].gain_q8;
+        info6[5] = f->fp.d->links.empty() ? 0 : f->fp.d->links[0].gain_q8;
     }
     *out = f;
     return 0;
@@ -735,6 +803,7 @@ int files_f16k(cuopus_file *const *fs, int nfiles, const uint8_t *lost, size_t m
         for (int f = 0; f < nfiles; f++) out_lens[f] = 0;
     if (nfiles <= 0) return 0;
     if (!fs || !out_devs || !out_caps) return e.set(CUOPUS_RC_ARG, "NULL argument array");
+    HostTrace tr;
     Plan pl;
     RunOpts o;
     o.persistent = use_persistent();
@@ -754,6 +823,7 @@ int files_f16k(cuopus_file *const *fs, int nfiles, const uint8_t *lost, size_t m
         pl.files[f] = fp;
         o.f16_out.push_back(out_devs[f]);
     }
+    tr.mark("plan files");
     int rc = decode_plan(pl, lost, mask_n, o, (hipStream_t)stream, e);
     if (rc) return rc;
     if (out_lens)
@@ -839,7 +909,7 @@ extern "C" int cuopus_probe_packets(const uint8_t *ogg, size_t n, int64_t *n_pac
     FilePlan fp;
     int rc = demux_file(ogg, n, &fp, true, e);
     if (rc) return rc;
-    if (n_packets) *n_packets = (int64_t)fp.d.off.size();
+    if (n_packets) *n_packets = (int64_t)fp.d->off.size();
     if (n_chains) *n_chains = (int64_t)fp.chains.size();
     return 0;
 }
@@ -872,17 +942,17 @@ extern "C" int cuopus_dump_packets(const uint8_t *ogg, size_t n, uint8_t *data, 
     FilePlan fp;
     int rc = demux_file(ogg, n, &fp, false, e);
     if (rc) return rc;
-    const size_t np = fp.d.off.size();
-    const size_t nb = np ? (size_t)fp.d.off[np - 1] + fp.d.len[np - 1] : 0;
+    const size_t np = fp.d->off.size();
+    const size_t nb = np ? (size_t)fp.d->off[np - 1] + fp.d->len[np - 1] : 0;
     if (n_packets) *n_packets = (int64_t)np;
     if (n_bytes) *n_bytes = (int64_t)nb;
     if (lens) {
         if (cap_packets < np) return e.set(CUOPUS_RC_ARG, "packet capacity %zu < %zu", cap_packets, np);
-        memcpy(lens, fp.d.len.data(), np * sizeof(uint32_t));
+        memcpy(lens, fp.d->len.data(), np * sizeof(uint32_t));
     }
     if (data) {
         if (data_cap < nb) return e.set(CUOPUS_RC_ARG, "byte capacity %zu < %zu", data_cap, nb);
-        memcpy(data, fp.d.data.data(), nb);
+        memcpy(data, fp.d->data.data(), nb);
     }
     return 0;
 }
@@ -894,17 +964,17 @@ extern "C" int cuopus_demux_stats(const uint8_t *ogg, size_t n, int64_t *s10, ch
     FilePlan fp;
     int rc = demux_file(ogg, n, &fp, false, e);
     if (rc) return rc;
-    const cuo_host::DemuxStats &st = fp.d.stats;
+    const cuo_host::DemuxStats &st = fp.d->stats;
     s10[0] = st.pages;
-    s10[1] = (int64_t)fp.d.links.size();
+    s10[1] = (int64_t)fp.d->links.size();
     s10[2] = st.crc_bad;
     s10[3] = st.foreign_pages;
     s10[4] = st.dropped_packets;
     s10[5] = st.granule_warnings;
-    s10[6] = fp.d.granule_checked;
+    s10[6] = fp.d->granule_checked;
     s10[7] = (int64_t)st.skipped_bytes;
-    s10[8] = (int64_t)fp.d.off.size();
-    s10[9] = fp.d.pcm_samples_48k;
+    s10[8] = (int64_t)fp.d->off.size();
+    s10[9] = fp.d->pcm_samples_48k;
     return 0;
 }
 
@@ -948,7 +1018,7 @@ extern "C" int cuopus_decode_s16_on(const uint8_t *ogg, size_t n, int16_t *out_d
     int rc = demux_file(ogg, n, &pl.files[0], true, e);
     if (rc) return rc;
     FilePlan &fp = pl.files[0];
-    fp.out_channels = fp.d.channels;  // the stream's own channels, interleaved
+    fp.out_channels = fp.d->channels;  // the stream's own channels, interleaved
     const uint64_t keep = fp.keep, elems = keep * (uint64_t)fp.out_channels;
     if (elems > out_cap)
         return e.set(CUOPUS_RC_ARG, "output capacity %zu < required %llu samples", out_cap, (unsigned long long)elems);
@@ -1376,7 +1446,7 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
         pl.lost.assign(pl.off.size(), 0);
         uint64_t pb = 0;
         for (int f = 0; f < nfiles && !rc; f++) {
-            size_t np = pl.files[f].d.off.size();
+            size_t np = pl.files[f].d->off.size();
             if (lost_masks[f]) {
                 if (!mask_ns || mask_ns[f] != np)
                     rc = e.set(CUOPUS_RC_ARG, "file %d: loss mask size != %zu packets", f, np);
@@ -1526,7 +1596,7 @@ extern "C" int cuopus_batch_info(const cuopus_batch *b, int64_t *c)
     if (!b || !c) return CUOPUS_RC_ARG;
     int64_t dec = 0, out16 = 0, bytes = 0;
     for (uint64_t v : b->pl.cap) dec += (int64_t)v;
-    for (auto &f : b->pl.files) { out16 += f.out16; bytes += (int64_t)f.d.data.size(); }
+    for (auto &f : b->pl.files) { out16 += f.out16; bytes += (int64_t)f.d->data.size(); }
     c[0] = (int64_t)b->pl.chains.size();
     c[1] = (int64_t)b->pl.off.size();
     c[2] = (int64_t)b->pl.total_keep;
@@ -1616,13 +1686,13 @@ struct SrvNeed {
     bool stereo = false;
     void add(const SrvReq *r)
     {
-        data += r->fp.d.data.size();
-        npk += r->fp.d.off.size();
+        data += r->fp.d->data.size();
+        npk += r->fp.d->off.size();
         nch += r->fp.chains.size();
         cat += (r->fp.keep + 7) & ~7ull;
         files += r->out16 > 0;
         lost = lost || !r->lost.empty();
-        stereo = stereo || cuo_host::needs_stereo_build(r->fp.d);
+        stereo = stereo || cuo_host::needs_stereo_build(*r->fp.d);
     }
 };
 
@@ -1728,7 +1798,7 @@ int srv_launch(cuopus_server *srv, SrvLaunch &L, Err &e)
         uint64_t pb = 0;
         for (size_t f = 0; f < L.reqs.size(); f++) {
             if (!L.reqs[f]->lost.empty()) memcpy(pl.lost.data() + pb, L.reqs[f]->lost.data(), L.reqs[f]->lost.size());
-            pb += pl.files[f].d.off.size();
+            pb += pl.files[f].d->off.size();
         }
     }
     std::vector<uint32_t> order;
@@ -1877,7 +1947,7 @@ extern "C" int cuopus_server_submit(cuopus_server *srv, const uint8_t *ogg, size
     SrvReq *r = new SrvReq();
     int rc = demux_file(ogg, n, &r->fp, true, e);
     if (!rc && lost) {
-        if (mask_n != r->fp.d.off.size()) rc = e.set(CUOPUS_RC_ARG, "loss mask size != %zu packets", r->fp.d.off.size());
+        if (mask_n != r->fp.d->off.size()) rc = e.set(CUOPUS_RC_ARG, "loss mask size != %zu packets", r->fp.d->off.size());
         else r->lost.assign(lost, lost + mask_n);
     }
     if (!rc && r->fp.out16 > 0 && (!out_dev || (uint64_t)r->fp.out16 > out_cap))
