@@ -45,7 +45,7 @@ static_assert(kMaxCh == 1 || kMaxCh == 2, "one or two channels");
 // is the same in every lane: accesses become LDS instructions and the values
 // loaded are uniform (scalar registers, scalar ALU).  The host build
 // (same-source CPU gate) uses the pointer as it is.
-constexpr size_t kWaveLdsBytes = kMaxCh == 1 ? 10 * 1024 : 20 * 1024;
+constexpr size_t kWaveLdsBytes = kMaxCh == 1 ? 10 * 1024 : 13 * 1024;
 __shared__ __attribute__((aligned(16))) char g_wave_lds[kWaveLdsBytes];
 #if defined(__HIP_DEVICE_COMPILE__)
 

@@ -79,6 +79,7 @@ struct DecHeap {
     int32_t celt_etmp[kOverlap];
     int32_t silk_cng_sig[kSilkMaxFrame + kSilkMaxOrder];
     int16_t silk_rs_tmp[kMaxCh == 1 ? 2 : 960];
+    int32_t silk_exc[kMaxCh == 1 ? 1 : kMaxCh * kSilkMaxFrame];  // excitation buffers (stereo build)
 };
 
 CUO_FN void dec_bind(OpusDecState *st, DecScratch *sc, DecHeap *h)
@@ -89,6 +90,8 @@ CUO_FN void dec_bind(OpusDecState *st, DecScratch *sc, DecHeap *h)
     st->celt.etmp = h->celt_etmp;
     st->silk.cng_sig = h->silk_cng_sig;
     st->silk.cng_exc = h->silk_cng_exc;
+    if (kMaxCh == 2)
+        for (int n = 0; n < kMaxCh; n++) st->silk.ch[n].exc_Q14.bind(h->silk_exc + kSilkMaxFrame * n);
     sc->pcm_silk = h->pcm_silk;
     sc->pcm_transition = h->pcm_transition;
     sc->redundant_audio = h->redundant_audio;

@@ -11,6 +11,7 @@
 // resampler_private_IIR_FIR.c, resampler_private_up2_HQ.c), including the LBRR
 // (in-band FEC) frame path (dec_API.c:252-289, decode_frame.c:70-72).
 #pragma once
+#include <type_traits>
 #include "entdec.h"
 #include "tables_gen.h"
 
@@ -63,11 +64,33 @@ struct SilkCngState {
     int32_t fs_kHz;
 };
 
+// A channel's excitation buffer (kSilkMaxFrame int32).  The mono build keeps
+// it in the channel state (LDS); the stereo build keeps it in the caller's
+// heap (HBM), which keeps two channel states inside an LDS block that allows
+// three wavefronts per SIMD.  Both index and convert like the array.
+struct ExcInState {
+    int32_t v[kSilkMaxFrame];
+    CUO_FN int32_t &operator[](int i) { return v[i]; }
+    CUO_FN operator int32_t *() { return v; }
+    CUO_FN operator const int32_t *() const { return v; }
+    CUO_FN int32_t *heap() const { return nullptr; }
+    CUO_FN void bind(int32_t *) {}
+};
+struct ExcInHeap {
+    int32_t *v;
+    CUO_FN int32_t &operator[](int i) { return v[i]; }
+    CUO_FN operator int32_t *() { return v; }
+    CUO_FN operator const int32_t *() const { return v; }
+    CUO_FN int32_t *heap() const { return v; }
+    CUO_FN void bind(int32_t *p) { v = p; }
+};
+using SilkExc = std::conditional_t<kMaxCh == 1, ExcInState, ExcInHeap>;
+
 // silk_decoder_state for one channel, fields in libopus order from the reset
 // point (prev_gain_Q16) on (vendor/silk/structs.h).
 struct SilkChannel {
     int32_t prev_gain_Q16;
-    int32_t exc_Q14[kSilkMaxFrame];
+    SilkExc exc_Q14;
     int32_t sLPC_Q14_buf[kSilkMaxOrder];
     int16_t outBuf[kSilkMaxFrame + 2 * 80];
     int32_t lagPrev;
@@ -486,8 +509,7 @@ CUO_FN void silk_sum_sqr_shift(int32_t *energy, int32_t *shift, const int16_t *x
 // ------------------------------------------------------------------ resampler (decoder: 8/12/16 -> 48 kHz)
 CUO_FN void silk_resampler_init(SilkResampler *S, int32_t Fs_in, int32_t Fs_out)
 {
-    int16_t *p16 = (int16_t *)S;
-    for (unsigned i = 0; i < sizeof(SilkResampler) / 2; i++) p16[i] = 0;
+    __builtin_memset(S, 0, sizeof(SilkResampler));  // memset: ordered before the field stores below
     const int in_id = Fs_in == 8000 ? 0 : (Fs_in == 12000 ? 1 : 2);
     static constexpr int8_t kDelayTo48[3] = {0, 4, 7};  // delay_matrix_dec[.][48 kHz]
     S->inputDelay = kDelayTo48[in_id];
@@ -622,8 +644,15 @@ CUO_FN void silk_plc_reset(SilkChannel *ch)
 
 CUO_FN void silk_reset_channel(SilkChannel *ch)
 {
-    int32_t *p = (int32_t *)ch;
-    for (unsigned i = 0; i < sizeof(SilkChannel) / 4; i++) p[i] = 0;
+    int32_t *exc = ch->exc_Q14.heap();  // a heap excitation buffer survives the wipe, cleared
+    // memset, not a loop of int32 stores: those could be ordered after the
+    // field stores below, which type-based alias analysis lets the compiler
+    // assume they do not overlap
+    __builtin_memset(ch, 0, sizeof(SilkChannel));
+    if (exc) {
+        ch->exc_Q14.bind(exc);
+        CUO_PFOR(i, 0, kSilkMaxFrame) exc[i] = 0;
+    }
     ch->first_frame_after_reset = 1;
     ch->prev_gain_Q16 = 65536;
     silk_cng_reset(ch);

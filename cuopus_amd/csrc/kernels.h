@@ -56,8 +56,9 @@ struct WaveState {
     DecScratch sc;
 };
 static_assert(sizeof(WaveState) <= kWaveLdsBytes, "wavefront decoder memory exceeds its LDS block");
-// Resident wavefronts per SIMD: four for the mono build, two for the stereo one.
-constexpr int kWavesPerSimd = kMaxCh == 1 ? 4 : 2;
+// Resident wavefronts per SIMD: four for the mono build (10 KB of LDS, <= 128
+// VGPRs), three for the stereo one (13 KB, <= 168 VGPRs).
+constexpr int kWavesPerSimd = kMaxCh == 1 ? 4 : 3;
 static_assert(kWaveLdsBytes * 4 * kWavesPerSimd <= 160 * 1024, "LDS per wavefront exceeds the CU's share");
 
 struct DecodeArgs {

@@ -3,10 +3,11 @@
 // The decoder headers compiled with two coded channels and one or two output
 // channels (CUO_MAXCH 2) in namespace cuo_st: the same kernels as the mono
 // build (one wavefront per chain, persistent or one workgroup per chain), with
-// a 20 KB LDS block and a larger HBM heap per wavefront, two wavefronts per
-// SIMD.  The host runtime (libcuopus.hip) launches them for streams whose
-// OpusHead says two channels; their argument block has the mono build's
-// layout (pointers and counts only).
+// a 13 KB LDS block (the SILK excitation buffers move to the heap) and a
+// larger HBM heap per wavefront, three wavefronts per SIMD.  The host runtime
+// (libcuopus.hip) launches them for launches holding a stream whose OpusHead
+// says two channels or that has stereo packets; their argument block has the
+// mono build's layout (pointers and counts only).
 #define CUO_MAXCH 2
 #define CUO_NS cuo_st
 #include "kernels.h"

@@ -19,6 +19,7 @@ render, resampled) and a launch mixing mono and stereo files.
 """
 import ctypes
 import os
+import time
 
 import numpy as np
 import pytest
@@ -219,3 +220,59 @@ def test_gpu_mixed_mono_stereo_launch(cn):
         got = m.cpu().numpy()
         assert got.shape == want.shape, name
         assert float(np.abs(got - want).max()) <= RS_TOL, name
+
+
+def _oracle_render(name):
+    _, d, chains = _load(name)
+    data, off, ln = d.blob()
+    return ol.resample(ol.decode_chains(data, off, ln, chains, channels=1, gain=d.gain_q8)["pcm"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_gpu_batch_with_stereo_files(cn, mode):
+    """Resident batches holding stereo and mono files (the stereo kernel
+    build decodes the launch): every output equals the oracle's one-channel
+    render, in every batch mode, on repeated runs."""
+    import torch
+    names = ["st_celt", "silk_wb", "st_silk_flip", "st_fec"]
+    blobs = [_load(n)[0] for n in names]
+    wants = [_oracle_render(n) for n in names]
+    print("case stereo batch mode", mode, flush=True)
+    b = cn.Batch(blobs, mode=mode, n_slots=5 if mode == 2 else 0)
+    for rep in range(2):
+        for o in b.outputs:
+            o.fill_(float("nan"))
+        b.run()
+        b.check()
+        torch.cuda.synchronize()
+        for name, o, want in zip(names, b.outputs, wants):
+            got = o.cpu().numpy()
+            assert got.shape == want.shape, name
+            assert float(np.abs(got - want).max()) <= RS_TOL, (name, rep)
+    b.close()
+
+
+@pytest.mark.gpu
+def test_gpu_server_with_stereo_requests(cn):
+    """Stereo and mono requests through the continuous-batching server equal
+    their single-file decodes."""
+    import torch
+    names = ["st_hybrid", "celt_fb", "st_gain", "st_mixed"]
+    blobs = [_load(n)[0] for n in names]
+    singles = [cn.decode_ogg_to_cuda_tensor(b) for b in blobs]
+    srv = cn.Server(n_streams=2)
+    ids = {srv.submit(b): i for i, b in enumerate(blobs)}
+    got = {}
+    for _ in range(20000):
+        for rid, out, rc in srv.poll():
+            assert rc == 0
+            got[rid] = out
+        if len(got) == len(ids):
+            break
+        time.sleep(0.001)
+    torch.cuda.synchronize()
+    srv.close()
+    assert len(got) == len(ids)
+    for rid, i in ids.items():
+        assert torch.equal(got[rid], singles[i]), names[i]

@@ -82,3 +82,19 @@ def test_noalloc_refuses_oversize_stream(env):
                                            torch.cuda.current_stream().cuda_stream, err, 512)
     assert rc == 3, err.value
     assert b"scratch" in err.value
+
+
+def test_noalloc_stereo_stream(env):
+    """A stereo stream (the stereo kernel build, its larger heaps from the
+    same static scratch) gives the allocating entry point's output."""
+    torch, cn, lib = env
+    blob = open(ol.small_corpus("st_mixed", ol.recipe("st_mixed")), "rb").read()
+    cap = cn.probe(blob)
+    rc, ref, msg = _call(torch, lib, "cuopus_decode_f16k_on", blob, cap)
+    assert rc == 0, msg
+    lib.cuopus_release_workspaces()
+    print("case noalloc stereo", len(blob), "bytes", flush=True)
+    rc, out, msg = _call(torch, lib, "cuopus_decode_f16k_noalloc_on", blob, cap)
+    assert rc == 0, msg
+    assert torch.equal(out, ref)
+    assert lib.cuopus_workspace_bytes() == 0
