@@ -1210,7 +1210,7 @@ bool mel_tables(int n_mels, MelTables *t)
         const double lo = edge[m], mid = edge[m + 1], hi = edge[m + 2];
         const double enorm = 2.0 / (hi - lo);
         int first = -1, count = 0;
-        t->woff[m] = nw;
+        t->filt.woff[m] = nw;
         for (int k = 0; k < kMelBins; k++) {
             const double f = k * sr / kMelFft;
             const double w = std::max(0.0, std::min((f - lo) / (mid - lo), (hi - f) / (hi - mid)));
@@ -1220,11 +1220,11 @@ bool mel_tables(int n_mels, MelTables *t)
             }
             if (first < 0) first = k;
             if (nw >= kMelMaxWeights) return false;
-            t->weight[nw++] = (float)(w * enorm);
+            t->filt.weight[nw++] = (float)(w * enorm);
             count++;
         }
-        t->first[m] = first < 0 ? 0 : first;
-        t->count[m] = count;
+        t->filt.first[m] = first < 0 ? 0 : first;
+        t->filt.count[m] = count;
     }
     return true;
 }
@@ -1257,7 +1257,7 @@ extern "C" int cuopus_mel_filters(int n_mels, float *out, size_t cap)
     if (!mel_tables(n_mels, &t)) return CUOPUS_RC_ARG;
     for (size_t i = 0; i < (size_t)n_mels * kMelBins; i++) out[i] = 0.f;
     for (int m = 0; m < n_mels; m++)
-        for (int i = 0; i < t.count[m]; i++) out[(size_t)m * kMelBins + t.first[m] + i] = t.weight[t.woff[m] + i];
+        for (int i = 0; i < t.filt.count[m]; i++) out[(size_t)m * kMelBins + t.filt.first[m] + i] = t.filt.weight[t.filt.woff[m] + i];
     return 0;
 }
 
@@ -1294,7 +1294,7 @@ extern "C" int cuopus_log_mel(const float *in_dev, int64_t n16, float *out_dev, 
         HIPCHK(e, hipMemcpy(mc.tab.p, &t, sizeof(MelTables), hipMemcpyHostToDevice));
         mc.n_mels = n_mels;
     }
-    const uint64_t blocks = ((uint64_t)nf + kMelWaves - 1) / kMelWaves;
+    const uint64_t blocks = ((uint64_t)nf + kMelFramesPerBlock - 1) / kMelFramesPerBlock;
     hipLaunchKernelGGL(k_log_mel, dim3((unsigned)blocks), dim3(64 * kMelWaves), 0, st, in_dev, nf,
                        (const MelTables *)mc.tab.p, n_mels, out_dev);
     HIPCHK(e, hipGetLastError());
