@@ -31,12 +31,19 @@ enum { SPREAD_NONE = 0, SPREAD_LIGHT = 1, SPREAD_NORMAL = 2, SPREAD_AGGRESSIVE =
 
 // Decode history of one output channel: kDecBuf + kOverlap samples.
 constexpr int kMemStride = kDecBuf + kOverlap;
+// Each channel's history window slides forward through a buffer of kMemCap
+// samples (advance_history); the kMemSlack samples beyond one window let it
+// move on for several frames before the window is copied back to the start.
+constexpr int kMemSlack = 3840;
+constexpr int kMemCap = kMemStride + kMemSlack;
 
 struct CeltState {
-    // decode history, kMemStride samples per output channel (channel c at
-    // c * kMemStride), owned by the caller (HBM on the device: it is only
-    // touched by lane-parallel loops and concealment)
+    // decode history: the current window of kMemStride samples per output
+    // channel (channel c at c * kMemCap), inside a caller-owned buffer (HBM
+    // on the device: it is only touched by lane-parallel loops and
+    // concealment)
     int32_t *mem;
+    int32_t *mem_base;           // start of the history buffer (kMemCap samples per channel)
     struct CeltPlcScratch *plc;  // pitch-concealment buffers, caller-owned
     int32_t *freq;               // synthesis spectra (kMaxCh x 960), caller-owned: only lane-parallel loops touch them
     int32_t *etmp;               // prefilter_and_fold output (kOverlap), caller-owned
@@ -99,8 +106,9 @@ CUO_FN void celt_reset(CeltState *st)
     st->pf_period = st->pf_period_old = 0; st->pf_gain = st->pf_gain_old = 0;
     st->pf_tapset = st->pf_tapset_old = 0; st->prefilter_and_fold = 0;
     for (int c = 0; c < kMaxCh; c++) st->preemph_mem[c] = 0;
+    st->mem = st->mem_base;
     int32_t *mem = st->mem;
-    CUO_PFOR(i, 0, celt_cc(st) * kMemStride) mem[i] = 0;
+    for (int c = 0; c < celt_cc(st); c++) CUO_PFOR(i, 0, kMemStride) mem[kMemCap * c + i] = 0;
     for (int i = 0; i < kMaxCh * kLpcOrder; i++) st->lpc[i] = 0;
     for (int i = 0; i < 2 * kNbBands; i++) {
         st->oldBandE[i] = 0;
@@ -140,12 +148,30 @@ alignas(16) static constexpr I32Table<336> kEProb32 = widen(tab::kEProbModel);
 alignas(16) static constexpr I32Table<32> kTfSelect32 = widen(tab::kTfSelect);
 
 // ------------------------------------------------------------------ small helpers
-// mem[0 .. kDecBuf - N + kOverlap) = mem[N ..): the decode history moves down
-// by one frame.  Lanes copy 64-element blocks in increasing order; with
-// N >= 120 no block reads what an earlier block wrote.
-CUO_FN void shift_history(int32_t *mem, int N)
+// The history moves N samples on (OPUS_MOVE of the decode memory by one
+// frame, celt_decoder.c:661-662): the window pointer advances; when the
+// buffer has no room left, the kMemStride - N samples the next window keeps
+// are copied to its start.  Lanes copy 64-element blocks in increasing order
+// and the source lies N >= 120 samples above the destination, so no block
+// reads what an earlier block wrote.  Only the window's last N samples
+// differ from what a shift leaves there (stale values either way): the
+// synthesis or the concealment overwrites each of them before anything reads
+// it, so the decoded output is the same.
+CUO_FN int32_t *advance_history(CeltState *st, int N)
 {
-    CUO_PFOR(i, 0, kDecBuf - N + kOverlap) mem[i] = mem[i + N];
+    int32_t *mem = st->mem;
+    int32_t *base = st->mem_base;
+    if ((int)(mem - base) + N + kMemStride <= kMemCap) {
+        st->mem = mem + N;
+        return mem + N;
+    }
+    for (int c = 0; c < celt_cc(st); c++) {
+        int32_t *dst = base + kMemCap * c;
+        const int32_t *src = mem + kMemCap * c + N;
+        CUO_PFOR(i, 0, kMemStride - N) dst[i] = src[i];
+    }
+    st->mem = base;
+    return base;
 }
 
 CUO_FN int32_t inner_prod16(const int16_t *x, const int16_t *y, int n)
@@ -1797,7 +1823,7 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
 static_assert(offsetof(CeltScratch, iy) + sizeof(CeltScratch::iy) - offsetof(CeltScratch, X) >= 480 * 8,
               "band scratch too small for the synthesis FFT");
 // celt_synthesis (celt_decoder.c:382-458).  C coded channels (spectra at X
-// and X + N) to CC output channels (histories kMemStride apart from out_syn):
+// and X + N) to CC output channels (histories kMemCap apart from out_syn):
 // a mono stream is synthesised into both outputs, a stereo stream to one
 // output is the average of the two denormalised spectra.  Every spectrum is
 // denormalised before the first inverse MDCT, whose FFT reuses X's storage.
@@ -1822,10 +1848,10 @@ CUO_FI void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *ol
     }
     for (int c = 0; c < CC; c++) {
         const int32_t *fc = C == 2 ? freq + 960 * c : freq;
-        for (int b = 0; b < B; b++) mdct_backward(&fc[b], out_syn + kMemStride * c + NB * b, shift, B, fbuf);
+        for (int b = 0; b < B; b++) mdct_backward(&fc[b], out_syn + kMemCap * c + NB * b, shift, B, fbuf);
     }
     for (int c = 0; c < CC; c++) {
-        int32_t *o = out_syn + kMemStride * c;
+        int32_t *o = out_syn + kMemCap * c;
         CUO_PFOR(i, 0, N) o[i] = saturate(o[i], kSigSat);
     }
 }
@@ -1928,7 +1954,7 @@ CUO_FN void prefilter_and_fold(CeltState *st, int N, int32_t *etmp)
 {
     const int CC = celt_cc(st);
     for (int c = 0; c < CC; c++) {
-        int32_t *mem = st->mem + kMemStride * c;
+        int32_t *mem = st->mem + kMemCap * c;
         comb_filter(etmp, mem + kDecBuf - N, st->pf_period_old, st->pf_period, kOverlap, (int16_t)-st->pf_gain_old,
                     (int16_t)-st->pf_gain, st->pf_tapset_old, st->pf_tapset, nullptr, 0);
         CUO_PFOR(i, 0, kOverlap / 2)
@@ -1954,7 +1980,8 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
     if (noise_based) {
         int end = st->end;
         int effEnd = imax(start, imin(end, kNbBands));
-        for (int c = 0; c < CC; c++) shift_history(mem + kMemStride * c, N);
+        mem = advance_history(st, N);
+        out_syn = mem + kDecBuf - N;
         if (st->prefilter_and_fold) prefilter_and_fold(st, N, st->etmp);
         int16_t decay = loss_duration == 0 ? 1536 : 512;
         for (int c = 0; c < CC; c++)
@@ -2014,8 +2041,9 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     const int CC = celt_cc(st);
     if (data == nullptr || len <= 1) {
         if (celt_decode_lost(st, N, LM, sc) < 0) return -3;
+        out_syn = st->mem + kDecBuf - N;
         for (int c = 0; c < CC; c++)
-            deemphasis(out_syn + kMemStride * c, pcm + c, N, &st->preemph_mem[c], accum, CC);
+            deemphasis(out_syn + kMemCap * c, pcm + c, N, &st->preemph_mem[c], accum, CC);
         return frame_size;
     }
     const int C = celt_c(st);
@@ -2127,7 +2155,8 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
                                         fine_priority, LM, dec, sc, C, &intensity, &dual_stereo);
     }
     unquant_fine_energy(start, end, oldBandE, fine_quant, dec, C);
-    for (int c = 0; c < CC; c++) shift_history(mem + kMemStride * c, N);
+    mem = advance_history(st, N);
+    out_syn = mem + kDecBuf - N;
     int16_t *X = sc->X;
     {
         CUO_PROF_SCOPE(kProfCeltBands);
@@ -2149,7 +2178,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     st->pf_period_old = imax(st->pf_period_old, kCombMinPeriod);
     for (int c = 0; c < CC; c++) {
         CUO_PROF_SCOPE(kProfCeltComb);
-        int32_t *os = out_syn + kMemStride * c;
+        int32_t *os = out_syn + kMemCap * c;
         comb_filter(os, os, st->pf_period_old, st->pf_period, 120, (int16_t)st->pf_gain_old,
                     (int16_t)st->pf_gain, st->pf_tapset_old, st->pf_tapset, tab::kWindow, kOverlap);
         if (LM != 0)
@@ -2186,7 +2215,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     }
     st->rng = dec->rng;
     *dec_home = dec_regs;
-    for (int c = 0; c < CC; c++) deemphasis(out_syn + kMemStride * c, pcm + c, N, &st->preemph_mem[c], accum, CC);
+    for (int c = 0; c < CC; c++) deemphasis(out_syn + kMemCap * c, pcm + c, N, &st->preemph_mem[c], accum, CC);
     st->loss_duration = 0;
     st->prefilter_and_fold = 0;
     if (ec_tell(dec) > 8 * len) return -3;  // OPUS_INTERNAL_ERROR

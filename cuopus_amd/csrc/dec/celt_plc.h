@@ -326,7 +326,7 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc)
     int pitch_index;
     if (st->loss_duration == 0) {
         int16_t *lp = st->plc->lp_pitch;
-        pitch_downsample(st->mem, st->mem + kMemStride, CC, lp, kDecBuf, st->plc->plc_a);
+        pitch_downsample(st->mem, st->mem + kMemCap, CC, lp, kDecBuf, st->plc->plc_a);
         pitch_index = pitch_search(lp + (kPlcLagMax >> 1), lp, kDecBuf - kPlcLagMax, kPlcLagMax - kPlcLagMin,
                                    st->plc->plc_a, st->plc->plc_b, st->plc->xcorr);
         pitch_index = kPlcLagMax - pitch_index;
@@ -337,7 +337,7 @@ CUO_FN int celt_plc_periodic(CeltState *st, int N, CeltScratch *sc)
     }
     const int exc_length = imin(2 * pitch_index, kMaxPeriod);
     for (int c = 0; c < CC; c++) {
-        int32_t *buf = st->mem + kMemStride * c;
+        int32_t *buf = st->mem + kMemCap * c;
         int16_t *lpc = st->lpc + kLpcOrder * c;
         int16_t *exc = st->plc->exc + kLpcOrder;
         for (int i = 0; i < kMaxPeriod + kLpcOrder; i++)

@@ -68,7 +68,7 @@ struct DecScratch {
 // Memory of a decoder that lives outside the state/scratch block (sizes per
 // output channel times kMaxCh).
 struct DecHeap {
-    int32_t celt_mem[kMaxCh * kMemStride];
+    int32_t celt_mem[kMaxCh * kMemCap];
     int16_t frame_out[kMaxCh * kMaxFrameOut];
     CeltPlcScratch plc;
     int16_t pcm_silk[kMaxCh * 480];
@@ -84,7 +84,7 @@ struct DecHeap {
 
 CUO_FN void dec_bind(OpusDecState *st, DecScratch *sc, DecHeap *h)
 {
-    st->celt.mem = h->celt_mem;
+    st->celt.mem = st->celt.mem_base = h->celt_mem;
     st->celt.plc = &h->plc;
     st->celt.freq = h->celt_freq;
     st->celt.etmp = h->celt_etmp;
