@@ -1762,8 +1762,10 @@ CUO_FN void fft_impl(int k, Cpx *fout)
 // overlap-added time signal.  The N/4-point FFT runs in fbuf (LDS, N/4
 // complex values) rather than in place in the history buffer: the
 // pre-rotation writes fbuf, the butterflies work on LDS, the post-rotation
-// reads fbuf and writes the history.
-CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride, Cpx *fbuf)
+// reads fbuf and writes the history.  Outputs below sat_n are stored
+// saturated to +-kSigSat (the synthesis' saturation pass folded into the
+// stores: with one block per frame nothing reads them in between).
+CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride, Cpx *fbuf, int sat_n)
 {
     int N = tab::kMdctN;
     const int16_t *trig = tab::kMdctTrig;
@@ -1795,14 +1797,16 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
             int32_t yi = sub32_ovf(smul(re, t1), smul(im, t0));
             re = fb1[1];
             im = fb1[0];
-            yp0[0] = yr;
-            yp1[1] = yi;
+            // TDAC rereads the outputs below kOverlap: those are saturated there
+            const int o0 = (kOverlap >> 1) + 2 * i, o1 = (kOverlap >> 1) + N2 - 2 - 2 * i;
+            yp0[0] = o0 >= kOverlap && o0 < sat_n ? saturate(yr, kSigSat) : yr;
+            yp1[1] = o1 + 1 >= kOverlap && o1 + 1 < sat_n ? saturate(yi, kSigSat) : yi;
             t0 = trig[N4 - i - 1];
             t1 = trig[N2 - i - 1];
             yr = add32_ovf(smul(re, t0), smul(im, t1));
             yi = sub32_ovf(smul(re, t1), smul(im, t0));
-            yp1[0] = yr;
-            yp0[1] = yi;
+            yp1[0] = o1 >= kOverlap && o1 < sat_n ? saturate(yr, kSigSat) : yr;
+            yp0[1] = o0 + 1 >= kOverlap && o0 + 1 < sat_n ? saturate(yi, kSigSat) : yi;
         }
     }
     {
@@ -1811,8 +1815,10 @@ CUO_FN void mdct_backward(const int32_t *in, int32_t *out, int shift, int stride
         CUO_PFOR(i, 0, kOverlap / 2) {
             int32_t x1 = out[kOverlap - 1 - i], x2 = out[i];
             int16_t w1 = win[i], w2 = win[kOverlap - 1 - i];
-            out[i] = sub32_ovf(mul16_32_q15(w2, x2), mul16_32_q15(w1, x1));
-            out[kOverlap - 1 - i] = add32_ovf(mul16_32_q15(w1, x2), mul16_32_q15(w2, x1));
+            const int32_t a = sub32_ovf(mul16_32_q15(w2, x2), mul16_32_q15(w1, x1));
+            const int32_t b = add32_ovf(mul16_32_q15(w1, x2), mul16_32_q15(w2, x1));
+            out[i] = sat_n > 0 ? saturate(a, kSigSat) : a;
+            out[kOverlap - 1 - i] = sat_n > 0 ? saturate(b, kSigSat) : b;
         }
     }
 }
@@ -1848,11 +1854,16 @@ CUO_FI void celt_synthesis(const int16_t *X, int32_t *out_syn, const int16_t *ol
     }
     for (int c = 0; c < CC; c++) {
         const int32_t *fc = C == 2 ? freq + 960 * c : freq;
-        for (int b = 0; b < B; b++) mdct_backward(&fc[b], out_syn + kMemCap * c + NB * b, shift, B, fbuf);
+        // one block: saturated as stored; several: each block's TDAC reads
+        // the previous block's unsaturated output, so saturate afterwards
+        for (int b = 0; b < B; b++)
+            mdct_backward(&fc[b], out_syn + kMemCap * c + NB * b, shift, B, fbuf, B == 1 ? N : 0);
     }
-    for (int c = 0; c < CC; c++) {
-        int32_t *o = out_syn + kMemCap * c;
-        CUO_PFOR(i, 0, N) o[i] = saturate(o[i], kSigSat);
+    if (B > 1) {
+        for (int c = 0; c < CC; c++) {
+            int32_t *o = out_syn + kMemCap * c;
+            CUO_PFOR(i, 0, N) o[i] = saturate(o[i], kSigSat);
+        }
     }
 }
 
