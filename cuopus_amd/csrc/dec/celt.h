@@ -626,9 +626,11 @@ CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
 
 #if defined(__HIP_DEVICE_COMPILE__)
 // Lane-parallel PVQ codeword decode (device).  Lane j holds U(n, j) for the
-// current dimension n: the first row is one vector gather from the U table
-// (U(j, n) = U(n, j); the row of min(j, n) holds it), each later row follows
-// from the one above (pvq_row_down).  The table walks of cwrs.c:463-537 --
+// current dimension n: each row is one vector gather from the U table
+// (U(j, n) = U(n, j); the row of min(j, n) holds it), issued one dimension
+// ahead of its use since the rows do not depend on the decoded values (a
+// gather takes fewer instructions than deriving the row from the one above
+// with a wave prefix sum).  The table walks of cwrs.c:463-537 --
 // "largest k' below a bound with U(n, k') <= i" -- become one ballot over the
 // lanes: U(n, .) is non-decreasing, so the lanes that pass form a prefix and
 // the highest passing lane is the walk's result.  Needs k + 1 < 64; lanes
@@ -643,14 +645,6 @@ __device__ __forceinline__ uint32_t pvq_row_lanes(int n, int row_lane, int row_n
 }
 __device__ __forceinline__ int pvq_row_off(int r) { return tab::kPvqURow[r < 14 ? r : 14]; }
 
-// Next row down from the current one, without memory: the recurrence
-// U(n,j) = U(n-1,j) + U(n,j-1) + U(n-1,j-1) gives
-// U(n-1,j) + U(n-1,j-1) = U(n,j) - U(n,j-1), and with U(n-1,0) = 0
-// (n-1 >= 1) that telescopes to U(n-1,j) = (-1)^j (2 P(j) - F(j)), where
-// F(t) = (-1)^t U(n,t) and P is the inclusive prefix sum of F over the
-// lanes.  The arithmetic wraps mod 2^32, so
-// every lane whose true value fits 32 bits -- all lanes up to the highest one
-// a decode step reads -- is exact, and a lane only depends on lanes below it.
 // Inclusive prefix sum over all 64 lanes (wrapping): a 16-lane DPP scan per
 // row plus the totals of the rows below, read from lanes 15, 31 and 47.
 __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
@@ -662,13 +656,6 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x)
     const uint32_t t2 = (uint32_t)lane_get((int32_t)v, LaneK<47>());
     const uint32_t below = l < 16 ? 0u : (l < 32 ? t0 : (l < 48 ? t0 + t1 : t0 + t1 + t2));
     return v + below;
-}
-__device__ __forceinline__ uint32_t pvq_row_down(uint32_t row)
-{
-    const int l = lane_id();
-    const uint32_t f = (l & 1) ? 0u - row : row;
-    const uint32_t sv = 2u * wave_incl_sum(f) - f;
-    return (l & 1) ? 0u - sv : sv;
 }
 __device__ __forceinline__ int highest_lane(uint64_t m) { return m ? 63 - __builtin_clzll(m) : 0; }
 
@@ -693,7 +680,7 @@ __device__ __forceinline__ int pvq_zero_run(int n, int k, uint32_t &i, int32_t *
     const uint32_t b = tab::kPvqU[ib];
     const uint32_t incl = wave_incl_sum(a);
     const uint32_t im = i - (incl - a);
-    const uint64_t fail = __ballot(valid && !(a <= im && im < b));
+    const uint64_t fail = lanes_s<kCmpSgt>(d, lim) & (lanes_u<kCmpUgt>(a, im) | lanes_u<kCmpUge>(im, b));
     const int run = fail ? __builtin_ctzll(fail) : imin(n - lim, 64);
     i -= (uint32_t)lane_get((int32_t)incl, run - 1);
     if (m < run) y[m] = 0;
@@ -715,7 +702,8 @@ __device__ __noinline__ int32_t cwrsi_lanes(int n, int k, uint32_t i, int32_t *y
     const int l = lane_id();
     int32_t yy = 0;
     while (n > 2) {
-        const uint32_t next = pvq_row_down(row);
+        // the next dimension's row, consumed one step later
+        const uint32_t next = n > 3 ? pvq_row_lanes(n - 1, row_lane, lane_get(row_lane, imin(n - 1, 14))) : 0u;
         uint32_t p;
         int s, k0;
         int16_t val;
@@ -726,7 +714,7 @@ __device__ __noinline__ int32_t cwrsi_lanes(int n, int k, uint32_t i, int32_t *y
             k0 = k;
             const uint32_t q = (uint32_t)lane_get((int32_t)row, n);
             const int bound = q > i ? n - 1 : k;  // largest k' <= bound with U(n, k') <= i
-            k = highest_lane(__ballot(l <= bound && row <= i));
+            k = highest_lane(lanes_s<kCmpSle>(l, bound) & lanes_u<kCmpUle>(row, i));
             p = (uint32_t)lane_get((int32_t)row, k);
             i -= p;
             val = (int16_t)((k0 - k + s) ^ s);
@@ -749,7 +737,7 @@ __device__ __noinline__ int32_t cwrsi_lanes(int n, int k, uint32_t i, int32_t *y
                 s = -(i >= q);
                 i -= q & (uint32_t)s;
                 k0 = k;
-                k = highest_lane(__ballot(l < k && row <= i));
+                k = highest_lane(lanes_s<kCmpSlt>(l, k) & lanes_u<kCmpUle>(row, i));
                 p = (uint32_t)lane_get((int32_t)row, k);
                 i -= p;
                 val = (int16_t)((k0 - k + s) ^ s);

@@ -148,6 +148,16 @@ template <int K>
 using LaneK = std::integral_constant<int, K>;
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ __forceinline__ int32_t lane_get(int32_t v, int k) { return __builtin_amdgcn_readlane(v, k); }
+// Mask of the lanes where p holds.
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+// Mask of the lanes where `a Pred b` holds, straight from one v_cmp (a ballot
+// of a combined condition first turns the compare masks into a per-lane 0/1
+// and compares that again); combine masks with & and |.
+enum { kCmpEq = 32, kCmpNe, kCmpUgt, kCmpUge, kCmpUlt, kCmpUle, kCmpSgt, kCmpSge, kCmpSlt, kCmpSle };
+template <int Pred>
+__device__ __forceinline__ uint64_t lanes_u(uint32_t a, uint32_t b) { return __builtin_amdgcn_uicmp(a, b, Pred); }
+template <int Pred>
+__device__ __forceinline__ uint64_t lanes_s(int32_t a, int32_t b) { return __builtin_amdgcn_sicmp(a, b, Pred); }
 __device__ __forceinline__ int32_t lane_set(int32_t v, int k, int32_t x) { return lane_id() == k ? x : v; }
 template <int K>
 __device__ __forceinline__ int32_t lane_get(int32_t v, LaneK<K>)

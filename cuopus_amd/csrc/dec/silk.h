@@ -946,7 +946,7 @@ __device__ inline void silk_decode_pulses_lanes(RangeDec *dp, int16_t *pulses, i
             const uint32_t icdf0 = ld_u8(icdf_ptr, imin(p & 0x1F, 6));
             int16_t *q = &pulses[i * 16];
             const int32_t qv = lane < 16 ? (int32_t)q[lane] : 0;
-            uint64_t nz = __ballot(qv > 0);
+            uint64_t nz = lanes_s<kCmpSgt>(qv, 0);
             uint32_t neg = 0;
             while (nz) {
                 const int j = __builtin_ctzll(nz);
@@ -1064,7 +1064,7 @@ __device__ inline int32_t inv_pred_gain_lanes(int32_t aQ12, int d)
             bad = t64 > INT32_MAX || t64 < INT32_MIN;
             nv = (int32_t)t64;
         }
-        if (__ballot(bad)) return 0;
+        if (wave_ballot(bad)) return 0;
         A = nv;
     }
     const int32_t A0 = lane_get(A, 0);
@@ -1089,7 +1089,7 @@ __device__ inline int32_t nlsf_stabilize_lanes(int32_t v, const int16_t *dmin, i
         const int32_t diff = lane <= L ? base - ((lane == 0 ? 0 : pv) + dm) : INT32_MAX;
         const int32_t min_diff = wave_min(diff);
         if (min_diff >= 0) return v;
-        const int I = __builtin_ctzll(__ballot(diff == min_diff));
+        const int I = __builtin_ctzll(lanes_s<kCmpEq>(diff, min_diff));
         if (I == 0) {
             v = lane == 0 ? dmin0 : v;
         } else if (I == L) {
@@ -1191,7 +1191,7 @@ __device__ inline void nlsf2a_lanes(int16_t *a_Q12, int32_t nlsf)
     for (it = 0; it < 10; it++) {
         const int32_t absv = lane < D ? (a > 0 ? a : -a) : 0;
         int32_t maxabs = wave_max(absv);
-        const int idx = __builtin_ctzll(__ballot(lane < D && absv == maxabs));
+        const int idx = __builtin_ctzll(lanes_s<kCmpSlt>(lane, D) & lanes_s<kCmpEq>(absv, maxabs));
         maxabs = silk_rshift_round(maxabs, 17 - 12);
         if (maxabs > 32767) {
             maxabs = min32(maxabs, 163838);
