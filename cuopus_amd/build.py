@@ -6,6 +6,9 @@ CPU oracle.
                        (tests only) -> tests/hostgate/libhostgate.so.
 * ``build_oracle()``   make -C oracle (libopus 1.5.2 from the reference tree,
                        tests / CPU baseline only) -> oracle/_ref/.
+* ``build_twins()``    the library with the serial twins of the lane-parallel
+                       stages on the GPU (tests only) ->
+                       tests/hostgate/libcuopus_twins.so.
 """
 import os
 import shutil
@@ -42,6 +45,14 @@ def build_lib(arch="gfx950", out=None, defines=()):
     _run(cmd)
     os.replace(tmp, out)
     return out
+
+
+def build_twins():
+    """tests/hostgate/libcuopus_twins.so: the library with CUO_SERIAL_TWINS,
+    i.e. the serial twins of the wavefront-parallel decoder stages (the code
+    the host gate and its sanitizer build run) compiled for the GPU.  Test
+    infrastructure: tests/test_gpu_twins.py compares it with the product."""
+    return build_lib(out=os.path.join(ROOT, "tests", "hostgate", "libcuopus_twins.so"), defines=("CUO_SERIAL_TWINS",))
 
 
 def build_cli(out=None):
@@ -110,3 +121,4 @@ def build_all():
     build_cli()
     build_hostgate()
     build_fuzz_gate()
+    build_twins()

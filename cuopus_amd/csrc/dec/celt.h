@@ -624,7 +624,7 @@ CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
     return yy;
 }
 
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CUO_LANE_PATHS
 // Lane-parallel PVQ codeword decode (device).  Lane j holds U(n, j) for the
 // current dimension n: each row is one vector gather from the U table
 // (U(j, n) = U(n, j); the row of min(j, n) holds it), issued one dimension
@@ -864,7 +864,7 @@ CUO_FI unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDe
     CUO_PROF_SCOPE(kProfPvq);
     uint32_t idx;
     int32_t Ryy;
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CUO_LANE_PATHS
     N = uniform(N);
     K = uniform(K);
     if (K + 1 < 64 && N > 2) {

@@ -16,6 +16,18 @@
 #include <stdint.h>
 #include <type_traits>
 
+// The wavefront-parallel versions of the SILK pulse, parameter and
+// synthesis stages and of the CELT codeword decode exist on the device only;
+// their serial twins are what the host build runs (and what the sanitizer
+// build checks).  A device build with CUO_SERIAL_TWINS runs the serial twins
+// on the GPU instead (test builds only: it lets a test compare every lane
+// version with its twin on the same inputs).
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(CUO_SERIAL_TWINS)
+#define CUO_LANE_PATHS 1
+#else
+#define CUO_LANE_PATHS 0
+#endif
+
 #define CUO_FN __host__ __device__ inline
 #define CUO_NOINLINE __host__ __device__ __attribute__((noinline))
 #define CUO_FI __host__ __device__ inline __attribute__((always_inline))

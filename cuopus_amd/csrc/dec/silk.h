@@ -811,7 +811,7 @@ CUO_FN void silk_shell_decoder(int16_t *p0, RangeDec *d, int pulses4)
     silk_shell_split(&p0[14], &p0[15], d, p1[7], t0);
 }
 
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CUO_LANE_PATHS
 // ec_icdf over a lane-held table starting at byte off: the same search as
 // ec_icdf_t, its words read with v_readlane instead of scalar loads.
 __device__ __forceinline__ int ec_icdf_lt(RangeDec *d, int32_t tabv, int off, int ftb)
@@ -962,7 +962,7 @@ __device__ inline void silk_decode_pulses_lanes(RangeDec *dp, int16_t *pulses, i
 
 CUO_FN void silk_decode_pulses(RangeDec *d, int16_t *pulses, int signalType, int quantOffsetType, int frame_length)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CUO_LANE_PATHS
     silk_decode_pulses_lanes(d, pulses, signalType, quantOffsetType, frame_length);
     return;
 #endif
@@ -1015,7 +1015,7 @@ CUO_FN void silk_decode_pulses(RangeDec *d, int16_t *pulses, int signalType, int
     }
 }
 
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CUO_LANE_PATHS
 // ---- parameter decoding on the wavefront.  Coefficient vectors (NLSFs,
 // polynomial and predictor coefficients) live one element per lane; the
 // serial recurrences of NLSF2A.c, LPC_fit.c, LPC_inv_pred_gain.c and
@@ -1309,7 +1309,7 @@ CUO_FN int32_t lpc_pred_q10(const int32_t *s, const int16_t *A, int order)
     return p;
 }
 
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CUO_LANE_PATHS
 // Excitation (decode_core.c:73-88) across the lanes.  The seed recurrence
 // s' = rand(s) + pulse = a*s + c + pulse (mod 2^32) is an affine map per
 // sample: each lane composes the maps of its block of consecutive samples, an
@@ -1464,7 +1464,7 @@ __device__ inline void silk_decode_core_lanes(SilkChannel *ch, SilkCtrl *ctrl, i
 // silk_decode_core (decode_core.c:38-237)
 CUO_FN void silk_decode_core(SilkChannel *ch, SilkCtrl *ctrl, int16_t *xq, const int16_t *pulses, SilkScratch *sc)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CUO_LANE_PATHS
     silk_decode_core_lanes(ch, ctrl, xq, pulses, sc);
     return;
 #endif
@@ -1833,7 +1833,7 @@ CUO_FN void silk_decode_frame(SilkChannel *ch, RangeDec *d, int16_t *pOut, int32
         int16_t *pulses = sc->pulses;
         silk_decode_indices(ch, d, ch->nFramesDecoded, lostFlag, condCoding);
         silk_decode_pulses(d, pulses, ch->indices.signalType, ch->indices.quantOffsetType, ch->frame_length);
-#if defined(__HIP_DEVICE_COMPILE__)
+#if CUO_LANE_PATHS
         silk_decode_parameters_lanes(ch, ctrl, condCoding, sc->nlsf_tmp);
 #else
         silk_decode_parameters(ch, ctrl, condCoding);

@@ -431,7 +431,7 @@ def mel_filters(n_mels=80):
     return out
 
 
-def gate_chains(data, off, length, chains, lost=None, channels=1, stereo_build=False):
+def gate_chains(data, off, length, chains, lost=None, channels=1, stereo_build=False, lib=None):
     """Per-packet gate on the GPU.
 
     data: uint8 packet bytes; off/length: uint32 per packet; chains: int array
@@ -439,7 +439,8 @@ def gate_chains(data, off, length, chains, lost=None, channels=1, stereo_build=F
     (2: interleaved stereo); stereo_build: decode with the stereo-capable
     kernel build (implied by channels=2).  Returns (pcm int16 sum(keep) x
     channels, nsamp int32 per packet, rng uint32 per packet, stop int32 per
-    chain, rc) where rc 1 means some chain stopped early (see stop).
+    chain, rc) where rc 1 means some chain stopped early (see stop).  lib: a
+    ``_Native`` of another build of the library (tests), default the loaded one.
     """
     as_u32 = lambda a: np.ascontiguousarray(a, dtype=np.uint32)  # noqa: E731
     data = np.ascontiguousarray(data, dtype=np.uint8)
@@ -451,7 +452,7 @@ def gate_chains(data, off, length, chains, lost=None, channels=1, stereo_build=F
     rng = np.zeros(off.size, np.uint32)
     stop = np.zeros(len(table), np.int32)
     mask = None if lost is None else np.ascontiguousarray(lost, dtype=np.uint8)
-    rc = _native()("cuopus_gate_chains_ch", data.ctypes.data, data.size, off.ctypes.data, length.ctypes.data,
+    rc = (lib or _native())("cuopus_gate_chains_ch", data.ctypes.data, data.size, off.ctypes.data, length.ctypes.data,
                    off.size, None if mask is None else mask.ctypes.data, *[c.ctypes.data for c in cols], len(table),
                    channels, int(bool(stereo_build)), pcm.ctypes.data, nsamp.ctypes.data, rng.ctypes.data,
                    stop.ctypes.data, _cur_stream(), ok=(0, 1))

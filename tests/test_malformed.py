@@ -7,8 +7,9 @@ UndefinedBehaviorSanitizer (tests/hostgate/fuzz_gate) -- no memory error,
 no undefined behaviour, and every return code, final range and sample up to
 a chain's first error equal to libopus.  The lane-parallel device versions
 of the SILK pulse/parameter/synthesis stages and the CELT codeword decode
-are not part of this build (they compile for the GPU only); the GPU test
-covers them.
+are not part of this build (they compile for the GPU only): the GPU tests
+here cover them, and tests/test_gpu_twins.py compares them with the serial
+twins this build checks, on these sets and every corpus recipe.
 
 GPU: the same sets through cuopus_gate_chains: per-packet sample counts and
 final ranges, per-chain stop codes and the PCM each chain produced before
