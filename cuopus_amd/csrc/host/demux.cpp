@@ -52,7 +52,18 @@ DemuxOpts demux_opts_from_env()
     o.check_crc = c && *c && strcmp(c, "0") != 0;
     const char *g = getenv("CUOPUS_GRANULE");
     o.strict = !(g && (!strcmp(g, "warn") || !strcmp(g, "lenient")));
+    const char *t = getenv("CUOPUS_HOST_THREADS");
+    if (t && *t) o.threads = atoi(t);
+    const char *m = getenv("CUOPUS_DEMUX_PARALLEL_MIN");
+    if (m && *m) o.parallel_min_bytes = (size_t)strtoull(m, nullptr, 10);
     return o;
+}
+
+int host_threads(const DemuxOpts &opts)
+{
+    if (opts.threads > 0) return opts.threads;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return hw == 0 ? 1 : (int)std::min(16u, hw);
 }
 
 namespace {
@@ -74,12 +85,14 @@ struct Page {
 // A packet assembled from spans of consecutive pages of one stream: its first
 // piece inline, further pieces (a packet crossing pages) in the link's
 // `more` list at [more0, more0 + nparts - 1).
+// (No member initialisers: the parallel scan sizes its packet table without
+// a zero-filling pass; Packet() / Packet{} still zero every field.)
 struct Packet {
-    uint64_t off0 = 0;
-    uint32_t len0 = 0;
-    uint32_t more0 = 0, nparts = 0;
-    uint32_t len = 0;
-    uint32_t page = 0;  // page on which it completes
+    uint64_t off0;
+    uint32_t len0;
+    uint32_t more0, nparts;
+    uint32_t len;
+    uint32_t page;  // page on which it completes
 };
 
 struct LinkBuild {
@@ -87,26 +100,25 @@ struct LinkBuild {
     bool eos = false;
     bool have_seq = false;
     uint32_t last_seq = 0;
-    std::vector<Packet> packets;  // header packets included
+    RawVec<Packet> packets;  // header packets included
     std::vector<std::pair<uint64_t, uint32_t>> more;  // (file offset, length) of continuation pieces
-    Packet partial;
+    Packet partial{};
     bool has_partial = false;
     std::vector<uint32_t> pages;  // page indices of this link
 };
 }  // namespace
 
-bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const DemuxOpts &opts)
+namespace {
+// Pass 1 over the whole stream, one page at a time: pages, links and the
+// packets of each link (the general path: any framing, foreign streams,
+// chained links, missing pages, CRC policy).  False with *err set on a strict
+// error.
+bool scan_sequential(const uint8_t *buf, size_t n, const DemuxOpts &opts, std::vector<Page> &pages,
+                     std::vector<LinkBuild> &links, DemuxStats &st, size_t &done_spans, std::string *err)
 {
-    *d = Demuxed();
-    if (!buf || n == 0) { *err = "empty stream"; return false; }
-    DemuxStats &st = d->stats;
-
     // ---- pages, and the packets of the Opus links, in one pass
-    std::vector<Page> pages;
     pages.reserve(n / 4000 + 16);
-    std::vector<LinkBuild> links;
     Span spans[255];
-    size_t done_spans = 0;  // spans ending a packet on any accepted page (error messages only)
     size_t pos = 0;
     static const uint8_t kCapture[4] = {'O', 'g', 'g', 'S'};
     while (pos < n) {
@@ -222,6 +234,191 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
         }
         if (pg.flags & 4) L->eos = true;
     }
+    return true;
+}
+
+// Pass 1 for a well-formed single-stream file, in parallel page ranges.
+// Eligible streams (checked first, sequentially and cheaply): pages back to
+// back from byte 0, one serial number, the first page a BOS page opening with
+// OpusHead, no other BOS page, EOS only on the last page, consecutive
+// sequence numbers, every page's continuation flag equal to whether the
+// previous page ended inside a packet, the last page ending on a packet
+// boundary, and (with check_crc) every CRC and version byte good.  Then the
+// sequential scan would take every page into one link and drop nothing, and
+// each packet is the spans from its start to its first done span: thread t
+// builds the packets starting on its pages (following the last one onto the
+// next pages), written at their final index.  Returns false (nothing
+// written) when the stream is not eligible.
+bool scan_parallel(const uint8_t *buf, size_t n, const DemuxOpts &opts, int nthreads, std::vector<Page> &pages,
+                   std::vector<LinkBuild> &links, DemuxStats &st, size_t &done_spans)
+{
+    struct PageAt {
+        uint64_t body;
+        uint32_t body_len;
+        uint32_t starts;  // packets starting on the page
+        uint32_t done;    // spans ending a packet
+        int nsegs;
+    };
+    std::vector<PageAt> at;
+    at.reserve(n / 4000 + 16);
+    pages.clear();
+    pages.reserve(n / 4000 + 16);
+    size_t pos = 0;
+    bool prev_partial = false;
+    while (pos < n) {
+        if (pos + 27 > n || memcmp(buf + pos, "OggS", 4) != 0) return false;
+        const int nsegs = buf[pos + 26];
+        if (nsegs == 0) return false;
+        const uint8_t *seg = buf + pos + 27;
+        const size_t body = pos + 27 + (size_t)nsegs;
+        if (body > n) return false;
+        uint32_t body_len = 0, done = 0;
+        for (int i = 0; i < nsegs; i++) {
+            body_len += seg[i];
+            done += seg[i] < 255;
+        }
+        if (body + body_len > n) return false;
+        Page pg;
+        memcpy(&pg.granule, buf + pos + 6, 8);
+        memcpy(&pg.serial, buf + pos + 14, 4);
+        memcpy(&pg.seq, buf + pos + 18, 4);
+        pg.flags = buf[pos + 5];
+        const bool partial = seg[nsegs - 1] == 255;
+        if (pages.empty()) {
+            if ((pg.flags & 3) != 2) return false;
+            if (seg[0] == 255 || seg[0] < 19 || memcmp(buf + body, "OpusHead", 8) != 0) return false;
+        } else {
+            const Page &q = pages.back();
+            if ((pg.flags & 2) || (q.flags & 4) || pg.serial != q.serial || pg.seq != q.seq + 1 ||
+                ((pg.flags & 1) != 0) != prev_partial)
+                return false;
+        }
+        const uint32_t spans = done + (partial ? 1 : 0);
+        at.push_back({body, body_len, spans - (prev_partial ? 1u : 0u), done, nsegs});
+        pages.push_back(pg);
+        prev_partial = partial;
+        pos = body + body_len;
+    }
+    if (pages.empty() || prev_partial) return false;
+    const size_t np = pages.size();
+    // page ranges of about equal bytes; packet index of each range's first packet
+    std::vector<size_t> cut(1, 0);
+    for (size_t p = 0, acc = 0; p < np; p++) {
+        acc += at[p].body_len;
+        if (acc * (size_t)nthreads >= n * cut.size() && cut.size() < (size_t)nthreads && p + 1 < np)
+            cut.push_back(p + 1);
+    }
+    cut.push_back(np);
+    const int nr = (int)cut.size() - 1;
+    std::vector<size_t> first_pkt(nr + 1, 0);
+    for (int r = 0; r < nr; r++) {
+        size_t c = 0;
+        for (size_t p = cut[r]; p < cut[r + 1]; p++) c += at[p].starts;
+        first_pkt[r + 1] = first_pkt[r] + c;
+    }
+    if (first_pkt[nr] < 2) return false;
+    std::vector<uint8_t> crc_ok(opts.check_crc ? np : 0, 1);
+    links.clear();
+    links.emplace_back();
+    LinkBuild &L = links.back();
+    L.serial = pages[0].serial;
+    L.eos = (pages.back().flags & 4) != 0;
+    L.have_seq = true;
+    L.last_seq = pages.back().seq;
+    L.packets.resize(first_pkt[nr]);
+    L.pages.resize(np);
+    std::vector<std::vector<std::pair<uint64_t, uint32_t>>> more(nr);
+    parallel_for(nr, [&](int r) {
+        if (opts.check_crc) {
+            for (size_t p = cut[r]; p < cut[r + 1]; p++) {
+                const uint64_t hdr = at[p].body - 27 - (uint64_t)at[p].nsegs;
+                uint32_t want;
+                memcpy(&want, buf + hdr + 22, 4);
+                static const uint8_t kZero[4] = {0, 0, 0, 0};
+                uint32_t c = ogg_crc(buf + hdr, 22);
+                c = ogg_crc(kZero, 4, c);
+                c = ogg_crc(buf + hdr + 26, 1 + (size_t)at[p].nsegs + at[p].body_len, c);
+                crc_ok[p] = c == want && buf[hdr + 4] == 0;
+            }
+        }
+        for (size_t p = cut[r]; p < cut[r + 1]; p++) L.pages[p] = (uint32_t)p;
+        Packet *out = L.packets.data() + first_pkt[r];
+        std::vector<std::pair<uint64_t, uint32_t>> &mv = more[r];
+        Packet cur;
+        bool open = false;   // a packet of this range is being assembled
+        bool skip = (pages[cut[r]].flags & 1) != 0;  // spans of a packet owned by the previous range
+        for (size_t p = cut[r]; p < np; p++) {
+            if (p >= cut[r + 1] && !open) break;
+            const uint8_t *seg = buf + at[p].body - (uint64_t)at[p].nsegs;
+            uint64_t run_start = at[p].body;
+            uint32_t run_len = 0;
+            for (int i = 0; i < at[p].nsegs; i++) {
+                run_len += seg[i];
+                const bool end = seg[i] < 255;
+                if (!end && i + 1 < at[p].nsegs) continue;
+                // a span: [run_start, run_start + run_len), ending a packet when `end`
+                if (skip) {
+                    if (end) skip = false;
+                } else if (!open) {
+                    if (p >= cut[r + 1]) break;
+                    cur = Packet();
+                    cur.off0 = run_start;
+                    cur.len0 = run_len;
+                    cur.nparts = 1;
+                    cur.len = run_len;
+                    open = true;
+                } else {
+                    if (cur.nparts == 1) cur.more0 = (uint32_t)mv.size();
+                    mv.push_back({run_start, run_len});
+                    cur.nparts++;
+                    cur.len += run_len;
+                }
+                if (end && open) {
+                    cur.page = (uint32_t)p;
+                    *out++ = cur;
+                    open = false;
+                }
+                run_start += run_len;
+                run_len = 0;
+            }
+        }
+    });
+    for (uint8_t ok : crc_ok)
+        if (!ok) return false;
+    // the continuation pieces of all ranges in one list (more0 re-based)
+    size_t nmore = 0;
+    for (int r = 0; r < nr; r++) nmore += more[r].size();
+    L.more.reserve(nmore);
+    for (int r = 0; r < nr; r++) {
+        const uint32_t base = (uint32_t)L.more.size();
+        if (base)
+            for (size_t k = first_pkt[r]; k < first_pkt[r + 1]; k++)
+                if (L.packets[k].nparts > 1) L.packets[k].more0 += base;
+        L.more.insert(L.more.end(), more[r].begin(), more[r].end());
+    }
+    st.pages = (uint32_t)np;
+    done_spans = 0;
+    for (const PageAt &a : at) done_spans += a.done;
+    return true;
+}
+}  // namespace
+
+bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const DemuxOpts &opts)
+{
+    *d = Demuxed();
+    if (!buf || n == 0) { *err = "empty stream"; return false; }
+    DemuxStats &st = d->stats;
+    const int nthreads = n >= opts.parallel_min_bytes ? host_threads(opts) : 1;
+    std::vector<Page> pages;
+    std::vector<LinkBuild> links;
+    size_t done_spans = 0;  // spans ending a packet on any accepted page (error messages only)
+    if (nthreads <= 1 || !scan_parallel(buf, n, opts, nthreads, pages, links, st, done_spans)) {
+        st = DemuxStats();
+        pages.clear();
+        links.clear();
+        done_spans = 0;
+        if (!scan_sequential(buf, n, opts, pages, links, st, done_spans, err)) return false;
+    }
     if (pages.empty()) { *err = "no OGG pages found"; return false; }
     for (auto &L : links)
         if (L.has_partial) st.dropped_packets++;  // trailing partial packet
@@ -234,13 +431,28 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
     if (links[0].packets.size() < 2) { *err = "not an OGG Opus stream (fewer than 2 packets)"; return false; }
 
     // ---- packet table
+    // audio packets run from index 2 of each link; a link's packets are cut
+    // into nr ranges for the parallel passes below
+    auto nranges = [&](size_t na) { return (int)std::min<size_t>((size_t)nthreads, std::max<size_t>(1, na / 4096)); };
+    // range_bytes[li][r + 1]: packet bytes of link li's ranges up to r (inclusive)
     uint64_t audio_bytes = 0;
     size_t n_audio = 0;
-    for (auto &L : links)
-        for (size_t i = 2; i < L.packets.size(); i++) {
-            audio_bytes += L.packets[i].len;
-            n_audio++;
-        }
+    std::vector<std::vector<uint64_t>> range_bytes(links.size());
+    for (size_t li = 0; li < links.size(); li++) {
+        const LinkBuild &L = links[li];
+        const size_t na = L.packets.size() > 2 ? L.packets.size() - 2 : 0;
+        const int nr = nranges(na);
+        std::vector<uint64_t> &rb = range_bytes[li];
+        rb.assign(nr + 1, 0);
+        parallel_for(nr, [&](int r) {
+            uint64_t b = 0;
+            for (size_t i = na * (size_t)r / nr; i < na * (size_t)(r + 1) / nr; i++) b += L.packets[i + 2].len;
+            rb[r + 1] = b;
+        });
+        for (int r = 0; r < nr; r++) rb[r + 1] += rb[r];
+        audio_bytes += rb[nr];
+        n_audio += na;
+    }
     if (audio_bytes >= (1ull << 32)) { *err = "stream too large (>= 4 GiB of packets)"; return false; }
     d->data.resize(audio_bytes ? audio_bytes : 1);
     d->off.resize(n_audio);
@@ -264,42 +476,85 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
         lk.preskip = hp[10] | (hp[11] << 8);
         lk.gain_q8 = (int16_t)(uint16_t)(hp[16] | (hp[17] << 8));
         lk.first = (uint32_t)k;
-        // packets, their TOC timeline and, per page, the end (link timeline) of
-        // the last packet completing on it: granule cross-check below
-        std::vector<std::pair<uint32_t, int64_t>> page_end;  // (page, end) in page order
-        int64_t page_begin_final = 0;
-        int64_t cum = 0;
+        // packets and their TOC timeline, in parallel packet ranges: each range
+        // copies its packets' bytes to their place (byte offsets from the
+        // ranges' byte totals above), forms its timeline from 0 and notes per
+        // page the timeline at the start of the first and the end of the last
+        // packet completing on it; the ranges' timelines are then shifted by
+        // the durations before them.  Those page spans drive the granule
+        // cross-check below.
+        const size_t na = LB.packets.size() - 2;
+        struct PageSpan { uint32_t page; int64_t begin, end; };
+        std::vector<PageSpan> page_end;  // in page order
         int last_dur = 0;
-        for (size_t i = 2; i < LB.packets.size(); i++, k++) {
-            const Packet &p = LB.packets[i];
-            uint8_t *dst = d->data.data() + w;
-            d->off[k] = (uint32_t)w;
-            d->len[k] = p.len;
-            memcpy(dst, buf + p.off0, p.len0);
-            w += p.len0;
-            for (uint32_t j = 0; j + 1 < p.nparts; j++) {
-                const auto &piece = LB.more[p.more0 + j];
-                memcpy(d->data.data() + w, buf + piece.first, piece.second);
-                w += piece.second;
+        int64_t cum = 0;
+        {
+            const int nr = nranges(na);
+            auto lo = [&](int r) { return na * (size_t)r / (size_t)nr; };
+            const std::vector<uint64_t> &rb = range_bytes[li];
+            std::vector<int64_t> base(nr + 1, 0);
+            std::vector<uint32_t> stereo(nr, 0);
+            std::vector<std::vector<PageSpan>> part(nr);
+            uint8_t *const data = d->data.data();
+            uint32_t *const offs = d->off.data() + k;
+            uint32_t *const lens = d->len.data() + k;
+            int64_t *const so = d->sample_off.data() + k;
+            const Packet *const pk = LB.packets.data() + 2;
+            const std::pair<uint64_t, uint32_t> *const more = LB.more.data();
+            parallel_for(nr, [&](int r) {
+                uint64_t ww = w + rb[r];
+                uint32_t sp = 0;
+                int64_t c = 0;
+                std::vector<PageSpan> &pe = part[r];
+                for (size_t i = lo(r); i < lo(r + 1); i++) {
+                    const Packet &p = pk[i];
+                    uint8_t *dst = data + ww;
+                    offs[i] = (uint32_t)ww;
+                    lens[i] = p.len;
+                    memcpy(dst, buf + p.off0, p.len0);
+                    ww += p.len0;
+                    for (uint32_t j = 0; j + 1 < p.nparts; j++) {
+                        memcpy(data + ww, buf + more[p.more0 + j].first, more[p.more0 + j].second);
+                        ww += more[p.more0 + j].second;
+                    }
+                    const int dur = pkt_samples48(dst, p.len);
+                    so[i] = c;
+                    if (pe.empty() || pe.back().page != p.page) pe.push_back({p.page, c, c + dur});
+                    else pe.back().end = c + dur;
+                    c += dur;
+                    if (p.len > 0 && (dst[0] & 0x4)) sp++;
+                }
+                base[r + 1] = c;
+                stereo[r] = sp;
+            });
+            for (int r = 0; r < nr; r++) {
+                base[r + 1] += base[r];
+                d->stereo_packets += stereo[r];
             }
-            const int dur = pkt_samples48(dst, p.len);
-            d->sample_off[k] = cum;
-            if (page_end.empty() || page_end.back().first != p.page) {
-                page_begin_final = cum;
-                page_end.push_back({p.page, cum + dur});
-            } else {
-                page_end.back().second = cum + dur;
-            }
-            cum += dur;
-            last_dur = dur;
-            if (p.len > 0 && (dst[0] & 0x4)) d->stereo_packets++;
+            if (nr > 1)
+                parallel_for(nr, [&](int r) {
+                    if (r == 0) return;
+                    for (size_t i = lo(r); i < lo(r + 1); i++) so[i] += base[r];
+                });
+            for (int r = 0; r < nr; r++)
+                for (PageSpan x : part[r]) {
+                    x.begin += base[r];
+                    x.end += base[r];
+                    if (!page_end.empty() && page_end.back().page == x.page) page_end.back().end = x.end;
+                    else page_end.push_back(x);
+                }
+            cum = base[nr];
+            if (na) last_dur = (int)(cum - so[na - 1]);
+            w += rb[nr];
+            k += na;
         }
+        const int64_t page_begin_final = page_end.empty() ? 0 : page_end.back().begin;
         lk.npkts = (uint32_t)k - lk.first;
         lk.toc_total = cum;
         if (!lk.npkts) last_dur = 0;
 
         // ---- granule cross-check over this link's pages
-        const int final_page = page_end.empty() ? -1 : (int)page_end.back().first;
+        const int final_page = page_end.empty() ? -1 : (int)page_end.back().page;
         int64_t last_granule = 0;
         for (uint32_t pgi : LB.pages)
             if (pages[pgi].granule != -1 && pages[pgi].granule > last_granule) last_granule = pages[pgi].granule;
@@ -307,22 +562,22 @@ bool demux(const uint8_t *buf, size_t n, Demuxed *d, std::string *err, const Dem
         uint32_t checked = 0, match = 0;
         int64_t base = 0;  // lenient: granule minus TOC time, re-based at each mismatch
         int64_t end_trim = 0;
-        for (auto &pe : page_end) {
-            int64_t g = pages[pe.first].granule;
+        for (const PageSpan &pe : page_end) {
+            int64_t g = pages[pe.page].granule;
             if (g == -1) continue;
             checked++;
-            const bool is_final = (int)pe.first == final_page;
-            if (g == pe.second) match++;
-            else if (is_final && pe.second - g >= 0 && pe.second - g < last_dur) match++;
+            const bool is_final = (int)pe.page == final_page;
+            if (g == pe.end) match++;
+            else if (is_final && pe.end - g >= 0 && pe.end - g < last_dur) match++;
             if (opts.strict) continue;
-            int64_t want = pe.second + base;
+            int64_t want = pe.end + base;
             if (g == want) continue;
-            if (is_final && g < want && want - g <= pe.second - page_begin_final) {
+            if (is_final && g < want && want - g <= pe.end - page_begin_final) {
                 end_trim = want - g;
                 continue;
             }
             st.granule_warnings++;
-            if (!is_final) base = g - pe.second;
+            if (!is_final) base = g - pe.end;
         }
         d->granule_checked += checked;
         d->granule_match += match;

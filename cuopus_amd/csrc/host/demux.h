@@ -32,6 +32,7 @@
 #include <stdint.h>
 #include <string>
 #include <memory>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -40,10 +41,25 @@ namespace cuo_host {
 struct DemuxOpts {
     bool check_crc = false;   // CUOPUS_CHECK_CRC=1
     bool strict = true;       // CUOPUS_GRANULE=strict (default) | warn
+    // Host threads for one stream (CUOPUS_HOST_THREADS; 0: the machine's
+    // hardware threads, at most 16) and the stream size from which they are
+    // used (CUOPUS_DEMUX_PARALLEL_MIN, bytes).  The result does not depend on
+    // either.
+    int threads = 0;
+    size_t parallel_min_bytes = 4u << 20;
 };
 
-// Options from the environment (CUOPUS_CHECK_CRC, CUOPUS_GRANULE).
+// Options from the environment (CUOPUS_CHECK_CRC, CUOPUS_GRANULE,
+// CUOPUS_HOST_THREADS, CUOPUS_DEMUX_PARALLEL_MIN).
 DemuxOpts demux_opts_from_env();
+
+// Host threads the demuxer and the library's host loops use: opts.threads if
+// set, else the machine's hardware threads, at most 16.
+int host_threads(const DemuxOpts &opts);
+
+// Run f(t) for t = 0 .. n-1 on n threads (t = 0 on the caller's thread).
+template <class F>
+void parallel_for(int n, F &&f);
 
 // One logical Opus stream of a (possibly chained) physical stream.
 struct Link {
@@ -127,6 +143,20 @@ inline int64_t out16_len(int64_t play48)
 {
     int64_t v = (play48 - 48) / 3;
     return v < 0 ? 0 : v;
+}
+
+template <class F>
+void parallel_for(int n, F &&f)
+{
+    if (n <= 1) {
+        if (n == 1) f(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve((size_t)n - 1);
+    for (int t = 1; t < n; t++) th.emplace_back([&f, t] { f(t); });
+    f(0);
+    for (auto &x : th) x.join();
 }
 
 }  // namespace cuo_host

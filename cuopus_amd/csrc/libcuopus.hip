@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 #include <array>
+#include <atomic>
 #include <deque>
 #include <memory>
 
@@ -300,13 +301,21 @@ bool apply_output_gain()
     return !(v && v[0] == '0');
 }
 
+// Demuxes running at once in this process: each takes an equal share of the
+// host threads (files of a multi-file call are demuxed side by side).
+std::atomic<int> g_demuxing{0};
+
 int demux_file(const uint8_t *ogg, size_t n, FilePlan *fp, bool with_chains, Err &e)
 {
     std::string msg;
     if (!ogg || !n) return e.set(CUOPUS_RC_ARG, "empty input");
     auto dm = std::make_shared<Demuxed>();
-    if (!cuo_host::demux(ogg, n, dm.get(), &msg, cuo_host::demux_opts_from_env()))
-        return e.set(CUOPUS_RC_SOFT, "%s", msg.c_str());
+    cuo_host::DemuxOpts dopts = cuo_host::demux_opts_from_env();
+    const int active = ++g_demuxing;
+    dopts.threads = std::max(1, cuo_host::host_threads(dopts) / active);
+    const bool ok = cuo_host::demux(ogg, n, dm.get(), &msg, dopts);
+    --g_demuxing;
+    if (!ok) return e.set(CUOPUS_RC_SOFT, "%s", msg.c_str());
     fp->d = dm;
     if (fp->d->channels != 1 && fp->d->channels != 2)
         return e.set(CUOPUS_RC_SOFT, "only mono and stereo streams are supported (channels=%d)", fp->d->channels);
@@ -831,6 +840,25 @@ int files_f16k(cuopus_file *const *fs, int nfiles, const uint8_t *lost, size_t m
     return 0;
 }
 
+// Demux files [0, nfiles) into fps[f] on up to the host threads (files side
+// by side); rc of the first failing file in file order, its message in e.
+int demux_files(const uint8_t *const *oggs, const size_t *ns, int nfiles, FilePlan *const *fps, Err &e)
+{
+    std::vector<int> rcs((size_t)nfiles, 0);
+    std::vector<std::array<char, 256>> msgs((size_t)nfiles);
+    std::atomic<int> next{0};
+    const int nt = std::min(nfiles, cuo_host::host_threads(cuo_host::demux_opts_from_env()));
+    cuo_host::parallel_for(nt, [&](int) {
+        for (int f; (f = next++) < nfiles;) {
+            Err fe(msgs[f].data(), msgs[f].size());
+            rcs[f] = demux_file(oggs[f], ns[f], fps[f], true, fe);
+        }
+    });
+    for (int f = 0; f < nfiles; f++)
+        if (rcs[f]) return nfiles > 1 ? e.set(rcs[f], "file %d: %s", f, msgs[f].data()) : e.set(rcs[f], "%s", msgs[f].data());
+    return 0;
+}
+
 int files_f16k_from_ogg(const uint8_t *const *oggs, const size_t *ns, int nfiles, const uint8_t *lost,
                         size_t mask_n, float *const *out_devs, const size_t *out_caps, int64_t *out_lens,
                         void *stream, bool noalloc, Err &e)
@@ -841,13 +869,12 @@ int files_f16k_from_ogg(const uint8_t *const *oggs, const size_t *ns, int nfiles
     if (!oggs || !ns) return e.set(CUOPUS_RC_ARG, "NULL argument array");
     std::vector<cuopus_file> files((size_t)nfiles);
     std::vector<cuopus_file *> fs((size_t)nfiles);
+    std::vector<FilePlan *> fps((size_t)nfiles);
     for (int f = 0; f < nfiles; f++) {
-        char sub[256];
-        Err fe(sub, sizeof(sub));
-        int rc = demux_file(oggs[f], ns[f], &files[f].fp, true, fe);
-        if (rc) return nfiles > 1 ? e.set(rc, "file %d: %s", f, sub) : e.set(rc, "%s", sub);
         fs[f] = &files[f];
+        fps[f] = &files[f].fp;
     }
+    if (int rc = demux_files(oggs, ns, nfiles, fps.data(), e)) return rc;
     return files_f16k(fs.data(), nfiles, lost, mask_n, out_devs, out_caps, out_lens, stream, noalloc, e);
 }
 
@@ -1430,12 +1457,11 @@ extern "C" int cuopus_batch_create(const uint8_t *const *oggs, const size_t *ns,
     else memset(&b->opts, 0, sizeof(b->opts));
     Plan &pl = b->pl;
     pl.files.resize((size_t)nfiles);
-    int rc = 0;
+    std::vector<FilePlan *> fps((size_t)nfiles);
+    for (int f = 0; f < nfiles; f++) fps[f] = &pl.files[f];
+    int rc = demux_files(oggs, ns, nfiles, fps.data(), e);
     for (int f = 0; f < nfiles && !rc; f++) {
-        char sub[256];
-        Err fe(sub, sizeof(sub));
-        if ((rc = demux_file(oggs[f], ns[f], &pl.files[f], true, fe))) rc = e.set(rc, "file %d: %s", f, sub);
-        else if (out_devs && out_caps && (uint64_t)pl.files[f].out16 > out_caps[f])
+        if (out_devs && out_caps && (uint64_t)pl.files[f].out16 > out_caps[f])
             rc = e.set(CUOPUS_RC_ARG, "file %d: output capacity %zu < %lld", f, out_caps[f],
                        (long long)pl.files[f].out16);
         else if (out_devs && pl.files[f].out16 > 0 && !out_devs[f])

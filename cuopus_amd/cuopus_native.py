@@ -229,14 +229,39 @@ def decode_ogg_to_cuda_tensor(path_or_bytes):
     return out
 
 
+def _parse_all(items):
+    """_Parsed handles of items, demuxed side by side on a few threads (the C
+    call releases the interpreter lock); on an error the handles opened so
+    far are closed and the first failing item's error is raised."""
+    items = list(items)
+    if len(items) <= 1:
+        return [_Parsed(src) for src in items]
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(src):
+        try:
+            return _Parsed(src), None
+        except Exception as exc:  # noqa: BLE001 -- re-raised below in item order
+            return None, exc
+
+    with ThreadPoolExecutor(max_workers=min(len(items), os.cpu_count() or 1, 16)) as ex:
+        res = list(ex.map(one, items))
+    err = next((exc for _, exc in res if exc is not None), None)
+    if err is not None:
+        for f, _ in res:
+            if f is not None:
+                f.close()
+        raise err
+    return [f for f, _ in res]
+
+
 def decode_ogg_multi(items):
     """Decode several streams in one decode launch and one resample launch;
     returns one float32 16 kHz tensor per item, each identical to what
     decode_ogg_to_cuda_tensor gives for it."""
     parsed = []
     try:
-        for src in items:
-            parsed.append(_Parsed(src))
+        parsed = _parse_all(items)
         if not parsed:
             return []
         outs = [_f32_cuda(f.n16) for f in parsed]
