@@ -374,6 +374,7 @@ int merge(Plan &pl, Err &e, bool borrow = false)
     pl.cap.resize(nch);
     pl.cost.resize(nch);
     const bool gain_on = apply_output_gain();
+    const int threads = cuo_host::host_threads(cuo_host::demux_opts_from_env());
     uint64_t db = 0, pb = 0, cb = 0, cat = 0, keep_sum = 0;
     for (auto &f : pl.files) {
         if (!borrow) {
@@ -387,18 +388,35 @@ int merge(Plan &pl, Err &e, bool borrow = false)
         }
         f.chain0 = cb;
         f.cat_off = cat;
+        // each chain's TOC capacity and cost over its packets, in parallel
+        // chain ranges (chains overlap by their preroll, so this reads more
+        // packets than the file has)
+        const size_t nc = f.chains.size();
+        std::vector<uint64_t> cap_s(nc), cost_s(nc);
+        {
+            const int nr = (int)std::min<size_t>((size_t)threads, std::max<size_t>(1, nc / 256));
+            cuo_host::parallel_for(nr, [&](int r) {
+                for (size_t i = nc * (size_t)r / nr; i < nc * (size_t)(r + 1) / nr; i++) {
+                    const Chain &c = f.chains[i];
+                    uint64_t s = 0, cost = 0;
+                    if ((uint64_t)c.first + c.npkts <= f.d->off.size())
+                        for (uint32_t k = 0; k < c.npkts; k++) {
+                            const uint32_t p = c.first + k;
+                            const uint8_t *pk = f.d->data.data() + f.d->off[p];
+                            s += (uint64_t)cuo_host::pkt_samples48(pk, f.d->len[p]);
+                            cost += packet_cost(pk, f.d->len[p]);
+                        }
+                    cap_s[i] = s;
+                    cost_s[i] = cost;
+                }
+            });
+        }
         uint64_t fk = 0;
         for (size_t i = 0; i < f.chains.size(); i++) {
             const Chain &c = f.chains[i];
             if ((uint64_t)c.first + c.npkts > f.d->off.size())
                 return e.set(CUOPUS_RC_SOFT, "chain %zu out of packet range", i);
-            uint64_t s = 0, cost = 0;
-            for (uint32_t k = 0; k < c.npkts; k++) {
-                uint32_t p = c.first + k;
-                const uint8_t *pk = f.d->data.data() + f.d->off[p];
-                s += (uint64_t)cuo_host::pkt_samples48(pk, f.d->len[p]);
-                cost += packet_cost(pk, f.d->len[p]);
-            }
+            const uint64_t s = cap_s[i], cost = cost_s[i];
             if (s < (uint64_t)c.trim + c.keep)
                 return e.set(CUOPUS_RC_SOFT, "chain %zu capacity %llu < trim %u + keep %u", i,
                              (unsigned long long)s, c.trim, c.keep);
