@@ -182,6 +182,12 @@ CUO_FN const uint8_t *pitch_contour_icdf(const SilkChannel *ch)
     if (ch->fs_kHz == 8) return ch->nb_subfr == 4 ? tab::kPitchContourNBIcdf : tab::kPitchContour10msNBIcdf;
     return ch->nb_subfr == 4 ? tab::kPitchContourIcdf : tab::kPitchContour10msIcdf;
 }
+// entries of pitch_contour_icdf(ch)
+CUO_FN int pitch_contour_n(const SilkChannel *ch)
+{
+    if (ch->fs_kHz == 8) return ch->nb_subfr == 4 ? 11 : 3;
+    return ch->nb_subfr == 4 ? 34 : 12;
+}
 CUO_FN const uint8_t *pitch_lag_low_icdf(const SilkChannel *ch)
 {
     return ch->fs_kHz == 16 ? tab::kUniform8Icdf : (ch->fs_kHz == 12 ? tab::kUniform6Icdf : tab::kUniform4Icdf);
@@ -718,6 +724,29 @@ CUO_FN void silk_set_fs(SilkChannel *ch, int fs_kHz, int32_t fs_API_Hz)
 }
 
 // ------------------------------------------------------------------ bitstream
+// ec_icdf for a table of n entries (its last one 0).  On the device a table
+// of more than four entries is searched in one step: lane j reads entry j
+// (one vector load), forms r * icdf[j], and the symbol is the lowest lane with
+// val >= r * icdf[j] -- where the serial search stops.
+#if CUO_LANE_PATHS
+__device__ __forceinline__ int silk_icdf(RangeDec *d, const uint8_t *icdf, int n)
+{
+    if (n <= 4) return ec_icdf(d, icdf, 8);
+    const uint32_t s = uniform(d->rng), v = uniform(d->val);
+    const uint32_t r = s >> 8;
+    const int j = lane_id();
+    const uint32_t tj = r * (uint32_t)icdf[j < n ? j : n - 1];
+    const uint64_t m = lanes_u<kCmpUge>(v, tj) & lanes_s<kCmpSlt>(j, n);
+    const int ret = __builtin_ctzll(m);
+    const uint32_t sv = (uint32_t)lane_get((int32_t)tj, ret);
+    const uint32_t tv = ret ? (uint32_t)lane_get((int32_t)tj, ret - 1) : s;
+    ec_normalize_rv(d, tv - sv, v - sv);
+    return ret;
+}
+#else
+CUO_FN int silk_icdf(RangeDec *d, const uint8_t *icdf, int) { return ec_icdf(d, icdf, 8); }
+#endif
+
 CUO_FN void silk_decode_indices(SilkChannel *ch, RangeDec *dp, int FrameIndex, int decode_LBRR, int condCoding)
 {
     CUO_PROF_SCOPE(kProfSilkIndices);
@@ -732,31 +761,31 @@ CUO_FN void silk_decode_indices(SilkChannel *ch, RangeDec *dp, int FrameIndex, i
     ix.signalType = (int8_t)(Ix >> 1);
     ix.quantOffsetType = (int8_t)(Ix & 1);
     if (condCoding == CODE_COND) {
-        ix.GainsIndices[0] = (int8_t)ec_icdf(d, tab::kDeltaGainIcdf, 8);
+        ix.GainsIndices[0] = (int8_t)silk_icdf(d, tab::kDeltaGainIcdf, 41);
     } else {
-        ix.GainsIndices[0] = (int8_t)silk_lshift(ec_icdf(d, tab::kGainIcdf + 8 * ix.signalType, 8), 3);
-        ix.GainsIndices[0] = (int8_t)(ix.GainsIndices[0] + ec_icdf(d, tab::kUniform8Icdf, 8));
+        ix.GainsIndices[0] = (int8_t)silk_lshift(silk_icdf(d, tab::kGainIcdf + 8 * ix.signalType, 8), 3);
+        ix.GainsIndices[0] = (int8_t)(ix.GainsIndices[0] + silk_icdf(d, tab::kUniform8Icdf, 8));
     }
-    for (int i = 1; i < ch->nb_subfr; i++) ix.GainsIndices[i] = (int8_t)ec_icdf(d, tab::kDeltaGainIcdf, 8);
+    for (int i = 1; i < ch->nb_subfr; i++) ix.GainsIndices[i] = (int8_t)silk_icdf(d, tab::kDeltaGainIcdf, 41);
     NlsfCb cb = nlsf_cb(ch->fs_kHz == 16);
-    const int cb1 = ec_icdf(d, &cb.CB1_iCDF[(ix.signalType >> 1) * cb.nVectors], 8);
+    const int cb1 = silk_icdf(d, &cb.CB1_iCDF[(ix.signalType >> 1) * cb.nVectors], cb.nVectors);
     ix.NLSFIndices[0] = (int8_t)cb1;
     // entropy-table selector of coefficient i (silk_NLSF_unpack, NLSF_unpack.c:35-55)
     const uint8_t *sel = &cb.ec_sel[cb1 * cb.order / 2];
     for (int i = 0; i < cb.order; i++) {
         const int entry = (int)ld_u8(sel, i >> 1);
         const int ec_ix = silk_smulbb((entry >> ((i & 1) ? 5 : 1)) & 7, 2 * 4 + 1);
-        Ix = ec_icdf(d, &cb.ec_iCDF[ec_ix], 8);
-        if (Ix == 0) Ix -= ec_icdf(d, tab::kNlsfExtIcdf, 8);
-        else if (Ix == 2 * 4) Ix += ec_icdf(d, tab::kNlsfExtIcdf, 8);
+        Ix = silk_icdf(d, &cb.ec_iCDF[ec_ix], 2 * 4 + 1);
+        if (Ix == 0) Ix -= silk_icdf(d, tab::kNlsfExtIcdf, 7);
+        else if (Ix == 2 * 4) Ix += silk_icdf(d, tab::kNlsfExtIcdf, 7);
         ix.NLSFIndices[i + 1] = (int8_t)(Ix - 4);
     }
-    if (ch->nb_subfr == 4) ix.NLSFInterpCoef_Q2 = (int8_t)ec_icdf(d, tab::kNlsfInterpFactorIcdf, 8);
+    if (ch->nb_subfr == 4) ix.NLSFInterpCoef_Q2 = (int8_t)silk_icdf(d, tab::kNlsfInterpFactorIcdf, 5);
     else ix.NLSFInterpCoef_Q2 = 4;
     if (ix.signalType == SIG_VOICED) {
         int decode_absolute = 1;
         if (condCoding == CODE_COND && ch->ec_prevSignalType == SIG_VOICED) {
-            int delta = (int16_t)ec_icdf(d, tab::kPitchDeltaIcdf, 8);
+            int delta = (int16_t)silk_icdf(d, tab::kPitchDeltaIcdf, 21);
             if (delta > 0) {
                 delta = delta - 9;
                 ix.lagIndex = (int16_t)(ch->ec_prevLagIndex + delta);
@@ -764,13 +793,14 @@ CUO_FN void silk_decode_indices(SilkChannel *ch, RangeDec *dp, int FrameIndex, i
             }
         }
         if (decode_absolute) {
-            ix.lagIndex = (int16_t)(ec_icdf(d, tab::kPitchLagIcdf, 8) * (ch->fs_kHz >> 1));
-            ix.lagIndex = (int16_t)(ix.lagIndex + ec_icdf(d, pitch_lag_low_icdf(ch), 8));
+            ix.lagIndex = (int16_t)(silk_icdf(d, tab::kPitchLagIcdf, 32) * (ch->fs_kHz >> 1));
+            ix.lagIndex = (int16_t)(ix.lagIndex + silk_icdf(d, pitch_lag_low_icdf(ch), ch->fs_kHz >> 1));
         }
         ch->ec_prevLagIndex = ix.lagIndex;
-        ix.contourIndex = (int8_t)ec_icdf(d, pitch_contour_icdf(ch), 8);
+        ix.contourIndex = (int8_t)silk_icdf(d, pitch_contour_icdf(ch), pitch_contour_n(ch));
         ix.PERIndex = (int8_t)ec_icdf(d, tab::kLtpPerIndexIcdf, 8);
-        for (int k = 0; k < ch->nb_subfr; k++) ix.LTPIndex[k] = (int8_t)ec_icdf(d, ltp_gain_icdf(ix.PERIndex), 8);
+        for (int k = 0; k < ch->nb_subfr; k++)
+            ix.LTPIndex[k] = (int8_t)silk_icdf(d, ltp_gain_icdf(ix.PERIndex), 8 << ix.PERIndex);
         if (condCoding == CODE_INDEP) ix.LTP_scaleIndex = (int8_t)ec_icdf(d, tab::kLtpScaleIcdf, 8);
         else ix.LTP_scaleIndex = 0;
     }
