@@ -1038,6 +1038,31 @@ CUO_FI unsigned qp_leaf(BandCtx *ctx, int16_t *fX, int N, int b, int B, int16_t 
     int q;
     {
         CUO_PROF_SCOPE(kProfQpLeafBits);
+#if CUO_LANE_PATHS
+        // The cache row (at most 40 entries after its count) one entry per
+        // lane, read once: bits2pulses' six-step bisection ends at the lowest
+        // entry index >= bits (its count when there is none) and the one
+        // below it -- the same pair for every row and every bits value
+        // (checked over all rows and bit counts) -- and pulses2bits is a
+        // lane read.
+        const uint8_t *cache = pulse_cache(ctx->band, LM + 1);
+        const int mx = (int)ld_u8(cache, 0);
+        const int j = lane_id();
+        const int cj = (int)cache[j <= mx ? j : mx];
+        const int bits = b - 1;
+        const uint64_t m = lanes_s<kCmpSge>(cj, bits) & lanes_s<kCmpSge>(j, 1) & lanes_s<kCmpSle>(j, mx);
+        const int hi = m ? __builtin_ctzll(m) : mx;
+        const int lo = hi - 1;
+        q = bits - (lo == 0 ? -1 : lane_get(cj, lo)) <= lane_get(cj, hi) - bits ? lo : hi;
+        int curr_bits = q ? lane_get(cj, q) + 1 : 0;
+        ctx->remaining_bits -= curr_bits;
+        while (ctx->remaining_bits < 0 && q > 0) {
+            ctx->remaining_bits += curr_bits;
+            q--;
+            curr_bits = q ? lane_get(cj, q) + 1 : 0;
+            ctx->remaining_bits -= curr_bits;
+        }
+#else
         q = bits2pulses(ctx->band, LM, b);
         int curr_bits = pulses2bits(ctx->band, LM, q);
         ctx->remaining_bits -= curr_bits;
@@ -1047,6 +1072,7 @@ CUO_FI unsigned qp_leaf(BandCtx *ctx, int16_t *fX, int N, int b, int B, int16_t 
             curr_bits = pulses2bits(ctx->band, LM, q);
             ctx->remaining_bits -= curr_bits;
         }
+#endif
     }
     unsigned cm = 0;
     if (q != 0) {
