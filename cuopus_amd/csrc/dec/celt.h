@@ -2042,9 +2042,13 @@ CUO_FN int celt_decode_lost(CeltState *st, int N, int LM, CeltScratch *sc)
 // ------------------------------------------------------------------ frame decode
 // celt_decode_with_ec (celt_decoder.c:970-1369).  data == nullptr or len <= 1
 // conceals.  ec may be nullptr (then a decoder is opened on data).  Returns the
-// number of samples (frame_size) or a negative error.
-CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size, RangeDec *dec,
-                       int accum, CeltScratch *sc)
+// number of samples (frame_size) or a negative error.  celt_decode_frame is
+// the body, inlined where a caller decodes runs of frames; celt_decode is its
+// out-of-line form for every other call site.
+// MayConceal false: the caller guarantees data != nullptr and len > 1.
+template <bool MayConceal>
+CUO_FI int celt_decode_frame(CeltState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size,
+                             RangeDec *dec, int accum, CeltScratch *sc)
 {
     CUO_PROF_SCOPE(kProfCelt);
     st = lds_uniform(st);
@@ -2064,7 +2068,7 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     int16_t *oldBandE = st->oldBandE, *oldLogE = st->oldLogE, *oldLogE2 = st->oldLogE2;
 
     const int CC = celt_cc(st);
-    if (data == nullptr || len <= 1) {
+    if (MayConceal && (data == nullptr || len <= 1)) {
         if (celt_decode_lost(st, N, LM, sc) < 0) return -3;
         out_syn = st->mem + kDecBuf - N;
         for (int c = 0; c < CC; c++)
@@ -2246,6 +2250,12 @@ CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm
     if (ec_tell(dec) > 8 * len) return -3;  // OPUS_INTERNAL_ERROR
     if (dec->error) st->error = 1;
     return frame_size;
+}
+
+CUO_FN int celt_decode(CeltState *st, const uint8_t *data, int len, int16_t *pcm, int frame_size, RangeDec *dec,
+                       int accum, CeltScratch *sc)
+{
+    return celt_decode_frame<true>(st, data, len, pcm, frame_size, dec, accum, sc);
 }
 
 }  // namespace CUO_NS

@@ -137,7 +137,16 @@ __host__ __device__ inline int32_t ld_s16(const int16_t *t, int i) { return t[i]
 // On the host there is one lane.
 #if defined(__HIP_DEVICE_COMPILE__)
 constexpr int kLanes = 64;
-__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+// The lane index is formed where it is used (two VALU instructions): as a
+// volatile asm it is not hoisted out of the chain loop, where the per-lane
+// offsets derived from it would otherwise stay live across the whole inlined
+// frame decoder and be spilled.
+__device__ __forceinline__ int lane_id()
+{
+    int l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
 #else
 constexpr int kLanes = 1;
 __host__ __device__ inline int lane_id() { return 0; }

@@ -555,4 +555,60 @@ CUO_FN int opus_decode_packet(OpusDecState *st, const uint8_t *data, int len, Ou
     return decode_fec ? frame_size : nb_samples;
 }
 
+// A plain CELT packet: one coded channel, CELT-only, decoded after a CELT-only
+// packet or first after a reset, every frame longer than one byte.  For such
+// a packet opus_decode_frame (opus_decoder.c:222-653) reduces to the CELT
+// decoder alone: no mode transition, no redundancy, no SILK, no CELT reset,
+// no accumulation.  Decodes it as opus_decode_packet would (same state
+// updates, same outputs) with the CELT frame decoder inlined, so a caller that
+// loops over such packets pays no call per frame.  Returns the packet's
+// samples, a negative decoder error, or kNotPlain (nothing touched) when the
+// packet is not of this kind.
+constexpr int kNotPlain = -0x7fff;
+template <class Out>
+CUO_FI int celt_plain_packet(OpusDecState *st, const uint8_t *data, int len, Out &out, DecScratch *sc)
+{
+    if (kMaxCh != 1 || len <= 1) return kNotPlain;
+    const uint8_t toc = data[0];
+    if (toc_mode(toc) != MODE_CELT_ONLY || (toc & 0x4)) return kNotPlain;
+    if (st->prev_mode != 0 && st->prev_mode != MODE_CELT_ONLY) return kNotPlain;
+    int16_t *size = lds_uniform(sc->frame_sizes);
+    int offset;
+    const int count = packet_parse(data, len, size, &offset);
+    if (count <= 0) return kNotPlain;  // the general path reports the error
+    for (int i = 0; i < count; i++)
+        if (size[i] <= 1) return kNotPlain;
+    const int fs = toc_samples_per_frame(toc);
+    const int bw = toc_bandwidth(toc);
+    st->mode = MODE_CELT_ONLY;
+    st->bandwidth = bw;
+    st->frame_size = fs;
+    st->stream_channels = 1;
+    const int endband = bw == BW_NB ? 13 : (bw == BW_MB || bw == BW_WB ? 17 : (bw == BW_SWB ? 19 : 21));
+    data += offset;
+    int nb_samples = 0;
+    for (int i = 0; i < count; i++) {
+        RangeDec &dec = sc->dec;
+        const int n = size[i];
+        ec_init(&dec, data, (uint32_t)n);
+        st->celt.end = endband;
+        st->celt.stream_channels = 1;
+        st->celt.start = 0;
+        const int r = celt_decode_frame<false>(&st->celt, data, n, out.buf, fs, &dec, 0, &sc->celt);
+        if (st->decode_gain) {
+            const int32_t gain = celt_exp2((int16_t)mul16_16_p15(kGainLog2Q25, st->decode_gain));
+            CUO_PFOR(j, 0, fs) out.buf[j] = (int16_t)saturate(mul16_32_p16(out.buf[j], gain), 32767);
+        }
+        st->rangeFinal = dec.rng;
+        st->prev_mode = MODE_CELT_ONLY;
+        st->prev_redundancy = 0;
+        if (r < 0) return r;
+        out.emit(fs);
+        data += n;
+        nb_samples += fs;
+    }
+    st->last_packet_duration = nb_samples;
+    return nb_samples;
+}
+
 }  // namespace CUO_NS

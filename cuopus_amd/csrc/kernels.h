@@ -43,6 +43,25 @@ struct ChainResult {
     uint32_t t0, t1;   // constant 100 MHz clock (low 32 bits) when the chain started and ended
 };
 
+// Frame sink of a chain: each decoded frame's samples that fall inside the
+// chain's keep window are copied by the 64 lanes in parallel from the LDS
+// frame buffer straight into the compacted output.
+struct ChainOut {
+    int16_t *buf;
+    int16_t *out;
+    uint64_t dst, lo, hi, pos;  // in sample frames (one sample per output channel)
+    int cc;                     // output channels, interleaved in buf and out
+    __device__ void emit(int n)
+    {
+        CUO_PROF_SCOPE(kProfEmit);
+        const uint64_t C = kMaxCh == 1 ? 1 : (uint64_t)cc;
+        uint64_t a = pos > lo ? pos : lo;
+        uint64_t b = pos + (uint64_t)n < hi ? pos + (uint64_t)n : hi;
+        for (uint64_t e = C * a + (uint64_t)lane_id(); e < C * b; e += 64) out[C * (dst - lo) + e] = buf[e - C * pos];
+        pos += (uint64_t)n;
+    }
+};
+
 // Decoder memory of one wavefront.  The chain's state and per-call scratch
 // (WaveState, under 10 KB) sit in LDS, so four wavefronts fit per SIMD; the
 // CELT history and synthesis spectrum, the output, SILK, transition and
@@ -51,9 +70,22 @@ struct ChainResult {
 // per-wavefront HBM slot, where only lane-parallel loops and the rare
 // concealment touch them.  Chains restart from a fresh decoder, so nothing is
 // carried between chains.
+// The chain loop's own state.  It lives in LDS and is read where it is used,
+// so the inlined CELT frame decoder has every scalar register to itself.
+struct ChainWalk {
+    const uint8_t *data;
+    const uint32_t *off, *len;
+    const uint8_t *lost;
+    int32_t *pkt_nsamp;
+    uint32_t *pkt_rng;
+    uint32_t first, npkts, rec_end, k;
+};
+
 struct WaveState {
     OpusDecState st;
     DecScratch sc;
+    ChainOut out;
+    ChainWalk walk;
 };
 static_assert(sizeof(WaveState) <= kWaveLdsBytes, "wavefront decoder memory exceeds its LDS block");
 // Resident wavefronts per SIMD: four for the mono build (10 KB of LDS, <= 128
@@ -77,59 +109,71 @@ struct DecodeArgs {
     uint32_t *queue;            // work-queue counter (persistent variant)
 };
 
-// Frame sink of a chain: each decoded frame's samples that fall inside the
-// chain's keep window are copied by the 64 lanes in parallel from the LDS
-// frame buffer straight into the compacted output.
-struct ChainOut {
-    int16_t *buf;
-    int16_t *out;
-    uint64_t dst, lo, hi, pos;  // in sample frames (one sample per output channel)
-    int lane;
-    int cc;                     // output channels, interleaved in buf and out
-    __device__ void emit(int n)
-    {
-        CUO_PROF_SCOPE(kProfEmit);
-        const uint64_t C = kMaxCh == 1 ? 1 : (uint64_t)cc;
-        uint64_t a = pos > lo ? pos : lo;
-        uint64_t b = pos + (uint64_t)n < hi ? pos + (uint64_t)n : hi;
-        for (uint64_t e = C * a + (uint64_t)lane; e < C * b; e += 64) out[C * (dst - lo) + e] = buf[e - C * pos];
-        pos += (uint64_t)n;
-    }
-};
+__device__ __forceinline__ const uint8_t *uniform_ptr(const uint8_t *p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (const uint8_t *)(((uint64_t)hi << 32) | lo);
+}
 
-// Decode one chain on the calling wavefront.
-// Kept out of line: the persistent loop around it then stays a small loop
-// whose every iteration runs with all 64 lanes together.
-__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, DecHeap *heap,
-                                              int lane)
+// Every packet the plain CELT path does not take (lost packets, SILK and
+// hybrid packets, mode transitions, stereo): the general packet decoder, out
+// of line, so the chain loop below holds the inlined CELT frame decoder
+// without the general path's registers.  data may be nullptr (concealment).
+__device__ __noinline__ int decode_packet_general(WaveState *ws, const uint8_t *data, int len, int frame_size,
+                                                  int fec)
+{
+    ws = lds_uniform(ws);
+    return opus_decode_packet(&ws->st, uniform_ptr(data), uniform(len), ws->out, uniform(frame_size), uniform(fec),
+                              &ws->sc);
+}
+
+// Decode one chain on the calling wavefront.  Plain CELT packets (the whole
+// of a CELT-only stream without loss) decode in this loop with the CELT frame
+// decoder inlined, so no call is made per frame; every other packet goes
+// through decode_packet_general.  Kept out of line: the persistent loop
+// around it then stays a small loop whose every iteration runs with all 64
+// lanes together.
+__device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, WaveState *ws, DecHeap *heap)
 {
     prof_reset();
     const uint32_t t_start = (uint32_t)__builtin_amdgcn_s_memrealtime();
     {
     CUO_PROF_SCOPE(kProfChain);
     const ChainDesc ch = a.chains[c];
-    const uint8_t *data = a.data;
-    const uint32_t *off = a.off, *len = a.len;
-    const uint8_t *lost = a.lost;
-    int32_t *pkt_nsamp = a.pkt_nsamp;
-    uint32_t *pkt_rng = a.pkt_rng;
+    ws = lds_uniform(ws);
     OpusDecState *st = lds_uniform(&ws->st);
     DecScratch *sc = lds_uniform(&ws->sc);
+    ChainOut *out = lds_uniform(&ws->out);
+    ChainWalk *w = lds_uniform(&ws->walk);
     dec_bind(st, sc, heap);
     opus_dec_init(st, kMaxCh == 1 ? 1 : ch.channels);
     st->decode_gain = ch.gain;
-    ChainOut out;
-    out.buf = sc->frame_out;
-    out.out = a.out;
-    out.dst = ch.dst;
-    out.lo = ch.trim;
-    out.hi = (uint64_t)ch.trim + ch.keep;
-    out.pos = 0;
-    out.lane = lane;
-    out.cc = st->channels;
+    out->buf = sc->frame_out;
+    out->out = a.out;
+    out->dst = ch.dst;
+    out->lo = ch.trim;
+    out->hi = (uint64_t)ch.trim + ch.keep;
+    out->pos = 0;
+    out->cc = st->channels;
+    w->data = a.data;
+    w->off = a.off;
+    w->len = a.len;
+    w->lost = a.lost;
+    w->pkt_nsamp = a.pkt_nsamp;
+    w->pkt_rng = a.pkt_rng;
+    w->first = ch.first;
+    w->npkts = ch.npkts;
+    w->rec_end = ch.rec_end;
+    w->k = 0;
     int32_t stop = 0;
-    for (uint32_t k = 0; k < ch.npkts; k++) {
-        const uint32_t p = ch.first + k;
+    while (w->k < w->npkts) {
+        const uint32_t k = w->k;
+        const uint32_t p = w->first + k;
+        const uint8_t *data = w->data;
+        const uint32_t *off = w->off, *len = w->len;
+        const uint8_t *lost = w->lost;
         const uint8_t *pk = data + off[p];
         int r;
         if (lost && lost[p]) {
@@ -138,26 +182,29 @@ __device__ __noinline__ void decode_one_chain(const DecodeArgs &a, uint32_t c, W
             int dur = packet_nb_samples(pk, (int)len[p]);
             if (dur <= 0) { stop = dur ? dur : OPUS_INVALID_PACKET; break; }
             bool fec = false;
-            if (k + 1 < ch.npkts && !lost[p + 1])
+            if (k + 1 < w->npkts && !lost[p + 1])
                 fec = packet_has_lbrr(data + off[p + 1], (int)len[p + 1]) > 0;
-            if (fec) r = opus_decode_packet(st, data + off[p + 1], (int)len[p + 1], out, dur, 1, sc);
-            else r = opus_decode_packet(st, nullptr, 0, out, dur, 0, sc);
+            if (fec) r = decode_packet_general(ws, data + off[p + 1], (int)len[p + 1], dur, 1);
+            else r = decode_packet_general(ws, nullptr, 0, dur, 0);
         } else {
-            r = opus_decode_packet(st, pk, (int)len[p], out, kMaxPktSamples, 0, sc);
+            r = celt_plain_packet(st, pk, (int)len[p], *out, sc);
+            if (r == kNotPlain) r = decode_packet_general(ws, w->data + w->off[p], (int)w->len[p], kMaxPktSamples, 0);
         }
         if (r < 0) { stop = r; break; }
         // a packet held by several chains (preroll overlap) is recorded by the
         // last chain that holds it, which owns packets below the next chain's
         // first packet
-        if (pkt_nsamp && lane == 0 && p < ch.rec_end) {
-            pkt_nsamp[p] = r;
-            pkt_rng[p] = st->rangeFinal;
+        const uint32_t q = w->first + w->k;
+        if (w->pkt_nsamp && lane_id() == 0 && q < w->rec_end) {
+            w->pkt_nsamp[q] = r;
+            w->pkt_rng[q] = st->rangeFinal;
         }
+        w->k = w->k + 1;
     }
     const uint32_t t_end = (uint32_t)__builtin_amdgcn_s_memrealtime();
-    if (lane == 0) {
+    if (lane_id() == 0) {
         a.res[c].stop = stop;
-        a.res[c].samples = (uint32_t)out.pos;
+        a.res[c].samples = (uint32_t)out->pos;
         a.res[c].t0 = t_start;
         a.res[c].t1 = t_end;
     }
@@ -180,7 +227,7 @@ __device__ __noinline__ void plain_worker(const DecodeArgs *ap, uint32_t g, int 
 {
     if (g >= ap->n_chains) return;
     const uint32_t c = ap->order ? ap->order[g] : g;
-    decode_one_chain(*ap, c, (WaveState *)g_wave_lds, ap->heap + g, lane);
+    decode_one_chain(*ap, c, (WaveState *)g_wave_lds, ap->heap + g);
 }
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void k_decode_chains(DecodeArgs)
 {
@@ -202,7 +249,7 @@ __device__ __noinline__ void persistent_worker(const DecodeArgs *ap, DecHeap *he
         w = (uint32_t)__builtin_amdgcn_readfirstlane((int)w);
         if (w >= ap->n_chains) break;
         const uint32_t c = ap->order ? ap->order[w] : w;
-        decode_one_chain(*ap, c, ws, heap, lane);
+        decode_one_chain(*ap, c, ws, heap);
     }
 }
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWavesPerSimd))) void k_decode_persistent(DecodeArgs)

@@ -57,7 +57,9 @@ extern "C" long hg_decode_chain_cfg(const uint8_t *data, const uint32_t *off, co
             else
                 r = opus_decode_packet(st, nullptr, 0, out, dur, 0, sc);
         } else {
-            r = opus_decode_packet(st, p, (int)len[k], out, 5760, 0, sc);
+            // as the kernel's chain loop: plain CELT packets first
+            r = celt_plain_packet(st, p, (int)len[k], out, sc);
+            if (r == kNotPlain) r = opus_decode_packet(st, p, (int)len[k], out, 5760, 0, sc);
         }
         if (r < 0) { ret = r; break; }
         if (nsamp) nsamp[k] = r;
@@ -145,7 +147,8 @@ extern "C" void hg_decode_chains_reuse(const uint8_t *data, const uint32_t *off,
                 else
                     n = opus_decode_packet(st, nullptr, 0, out, dur, 0, sc);
             } else {
-                n = opus_decode_packet(st, pk, (int)len[p], out, 5760, 0, sc);
+                n = celt_plain_packet(st, pk, (int)len[p], out, sc);
+                if (n == kNotPlain) n = opus_decode_packet(st, pk, (int)len[p], out, 5760, 0, sc);
             }
             if (n < 0) { r = n; break; }
         }
