@@ -790,8 +790,8 @@ CUO_FN void exp_rotation1(int16_t *X, int len, int stride, int16_t c, int16_t s)
                 const int32_t p = mul16_16(c, x2) + 16384;
                 int32_t x1 = 0;
                 serial_lanes(nb, [&](auto k) {
-                    x1 = lane_set(x1, k, carry);
-                    carry = (int16_t)((lane_get(p, k) + mul16_16(s, carry)) >> 15);
+                    x1 = lane_set_s(x1, k, carry);
+                    carry = shr15_to16(lane_get(p, k) + mul16_16(s, carry));
                 });
                 if (l < nb) X[i0 + l] = (int16_t)pshr32(mac16_16(mul16_16(c, x1), ms, x2), 15);
             }
@@ -806,8 +806,8 @@ CUO_FN void exp_rotation1(int16_t *X, int len, int stride, int16_t c, int16_t s)
                 const int32_t q = mul16_16(c, x1) + 16384;
                 int32_t x2 = 0;
                 serial_lanes(nb, [&](auto k) {
-                    x2 = lane_set(x2, k, carry);
-                    carry = (int16_t)((lane_get(q, k) + mul16_16(ms, carry)) >> 15);
+                    x2 = lane_set_s(x2, k, carry);
+                    carry = shr15_to16(lane_get(q, k) + mul16_16(ms, carry));
                 });
                 if (l < nb) X[ih - l + 1] = (int16_t)pshr32(mac16_16(mul16_16(c, x2), s, x1), 15);
             }
@@ -1962,7 +1962,7 @@ CUO_FN void deemphasis(const int32_t *x, int16_t *pcm, int N, int32_t *mem, int 
         serial_lanes(nb, [&](auto k) {
             int32_t tmp = lane_get(v, k) + m;
             m = mul16_32_q15(coef0, tmp);
-            t = lane_set(t, k, tmp);
+            t = lane_set_s(t, k, tmp);
         });
         if (l < nb) {
             int16_t y = sig2word16(t);

@@ -191,6 +191,17 @@ __device__ __forceinline__ int32_t lane_set(int32_t v, LaneK<K>, int32_t x)
     asm("v_writelane_b32 %0, %1, %2" : "=v"(v) : "s"(__builtin_amdgcn_readfirstlane(x)), "i"(K), "0"(v));
     return v;
 }
+// lane_set for a value the compiler already holds in a scalar register (no
+// readfirstlane in front of the write, which it would otherwise move into
+// the value's computation and duplicate part of it).  A value it cannot
+// prove uniform fails to compile here.
+template <int K>
+__device__ __forceinline__ int32_t lane_set_s(int32_t v, LaneK<K>, int32_t x)
+{
+    asm("v_writelane_b32 %0, %1, %2" : "=v"(v) : "s"(x), "i"(K), "0"(v));
+    return v;
+}
+__device__ __forceinline__ int32_t lane_set_s(int32_t v, int k, int32_t x) { return lane_id() == k ? x : v; }
 template <int K, int E, class F>
 __device__ __forceinline__ void serial_unrolled(F &f)
 {
@@ -218,6 +229,7 @@ __device__ __forceinline__ void serial_lanes(int nb, F &&f)
 #else
 __host__ __device__ inline int32_t lane_get(int32_t v, int) { return v; }
 __host__ __device__ inline int32_t lane_set(int32_t, int, int32_t x) { return x; }
+__host__ __device__ inline int32_t lane_set_s(int32_t, int, int32_t x) { return x; }
 template <class F>
 inline void serial_lanes(int nb, F &&f)
 {
@@ -373,6 +385,17 @@ CUO_FN int32_t add32_ovf(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + 
 CUO_FN int32_t sub32_ovf(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
 CUO_FN int32_t neg32_ovf(int32_t a) { return (int32_t)(0u - (uint32_t)a); }
 CUO_FN int32_t saturate(int32_t x, int32_t a) { return x > a ? a : (x < -a ? -a : x); }
+// (int16_t)(x >> 15) as an int32: on the device one bit-field extract (bits
+// 15..30 sign-extended), so a serial recurrence that feeds the value both to
+// the next step and to a lane write forms it once.
+CUO_FI int32_t shr15_to16(int32_t x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sbfe(x, 15, 16);
+#else
+    return (int16_t)(x >> 15);
+#endif
+}
 CUO_FN int16_t sat16(int32_t x) { return (int16_t)(x > 32767 ? 32767 : (x < -32768 ? -32768 : x)); }
 
 CUO_FN int32_t mul16_16(int32_t a, int32_t b) { return (int32_t)(int16_t)a * (int32_t)(int16_t)b; }

@@ -554,7 +554,7 @@ CUO_FN void silk_up2_hq(int32_t *S, int16_t *out, const int16_t *in, int len)
             X = silk_smlawb(Y, Y, h02);
             o1 = s2 + X;
             s2 = o2 + X;
-            oe = lane_set(oe, k, silk_sat16(silk_rshift_round(o1, 10)));
+            oe = lane_set_s(oe, k, silk_sat16(silk_rshift_round(o1, 10)));
             Y = in32 - s3;
             X = silk_smulwb(Y, h10);
             o1 = s3 + X;
@@ -567,7 +567,7 @@ CUO_FN void silk_up2_hq(int32_t *S, int16_t *out, const int16_t *in, int len)
             X = silk_smlawb(Y, Y, h12);
             o1 = s5 + X;
             s5 = o2 + X;
-            oo = lane_set(oo, k, silk_sat16(silk_rshift_round(o1, 10)));
+            oo = lane_set_s(oo, k, silk_sat16(silk_rshift_round(o1, 10)));
         }
         CUO_PFOR(k, 0, cnt) {
             out[2 * (k0 + k)] = (int16_t)oe;
@@ -1854,7 +1854,7 @@ CUO_FN void silk_cng(SilkChannel *ch, const SilkCtrl *ctrl, int16_t *frame, int 
             int32_t idxv = 0;
             serial_lanes(nb, [&](auto k) {
                 seed = silk_rand(seed);
-                idxv = lane_set(idxv, k, (seed >> 24) & exc_mask);
+                idxv = lane_set_s(idxv, k, (seed >> 24) & exc_mask);
             });
             if (lane_id() < nb) sig[kSilkMaxOrder + i0 + lane_id()] = exc_buf[idxv];
         }
