@@ -636,12 +636,20 @@ CUO_FN int32_t cwrsi(int n, int k, uint32_t i, int32_t *y)
 // the highest passing lane is the walk's result.  Needs k + 1 < 64; lanes
 // beyond the ones a step reads are masked out of every ballot, and the
 // gather index is clamped into the table.
-__device__ __forceinline__ uint32_t pvq_row_lanes(int n, int row_lane, int row_n)
+using PvqTable = const __attribute__((address_space(1))) uint32_t *;
+__device__ __forceinline__ uint32_t pvq_row_lanes(PvqTable U, int j, int n, int row_lane, int row_n)
 {
-    const int j = lane_id();
     int idx = j < n ? row_lane + n : row_n + j;
     idx = idx < 0 ? 0 : (idx > 1271 ? 1271 : idx);
-    return tab::kPvqU[idx];
+    return U[idx];
+}
+// The U table's global address in scalar registers, held there across a
+// loop (rather than re-formed from the program counter at every gather).
+__device__ __forceinline__ PvqTable pvq_u_table()
+{
+    uint64_t a = (uint64_t)(uintptr_t)tab::kPvqU;
+    asm("" : "+s"(a));  // opaque: kept in its registers, not re-formed
+    return (PvqTable)a;
 }
 __device__ __forceinline__ int pvq_row_off(int r) { return tab::kPvqURow[r < 14 ? r : 14]; }
 
@@ -700,10 +708,11 @@ __device__ __noinline__ int32_t cwrsi_lanes(int n, int k, uint32_t i, int32_t *y
     i = uniform(i);
     y = lds_uniform(y);
     const int l = lane_id();
+    const PvqTable U = pvq_u_table();
     int32_t yy = 0;
     while (n > 2) {
         // the next dimension's row, consumed one step later
-        const uint32_t next = n > 3 ? pvq_row_lanes(n - 1, row_lane, lane_get(row_lane, imin(n - 1, 14))) : 0u;
+        const uint32_t next = n > 3 ? pvq_row_lanes(U, l, n - 1, row_lane, lane_get(row_lane, imin(n - 1, 14))) : 0u;
         uint32_t p;
         int s, k0;
         int16_t val;
@@ -728,7 +737,7 @@ __device__ __noinline__ int32_t cwrsi_lanes(int n, int k, uint32_t i, int32_t *y
                     const int run = pvq_zero_run(n, k, i, y);
                     y += run;
                     n -= run;
-                    if (n > 2) row = pvq_row_lanes(n, row_lane, lane_get(row_lane, imin(n, 14)));
+                    if (n > 2) row = pvq_row_lanes(U, l, n, row_lane, lane_get(row_lane, imin(n, 14)));
                     continue;
                 }
                 i -= p;
@@ -870,7 +879,7 @@ CUO_FI unsigned alg_unquant(int16_t *X, int N, int K, int spread, int B, RangeDe
     if (K + 1 < 64 && N > 2) {
         // V(N, K) = U(N, K) + U(N, K + 1) from the first dimension's row
         const int row_lane = pvq_row_off(lane_id());
-        const uint32_t row = pvq_row_lanes(N, row_lane, lane_get(row_lane, imin(N, 14)));
+        const uint32_t row = pvq_row_lanes(pvq_u_table(), lane_id(), N, row_lane, lane_get(row_lane, imin(N, 14)));
         const uint32_t v = (uint32_t)lane_get((int32_t)row, K) + (uint32_t)lane_get((int32_t)row, K + 1);
         {
             CUO_PROF_SCOPE(kProfEcUint);
